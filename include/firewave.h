@@ -1,0 +1,151 @@
+/* firewave — C ABI of the MI355X (gfx950) Tier-3 spread_3d time step.
+ *
+ * Every entry point is `extern "C"`, takes plain device pointers plus sizes
+ * and scalars, and returns 0 on success or a negative FW_E_* code; the
+ * message of the last failure on the calling thread is fw_last_error().
+ * Field pointers address dense (Nz, Ny, Nx) C-order device arrays of the
+ * grid's storage type (fw_grid.dtype: FW_F32 or FW_F64).  `stream` is a
+ * hipStream_t (NULL = the default stream).  No entry point allocates device
+ * memory unless its name says so (*_create); scratch comes from a handle or
+ * from caller-supplied work pointers.  Output arrays never alias inputs
+ * unless stated.
+ *
+ * Reference call surface replaced (paths relative to the FireWebApp tree):
+ * src/model_outdoor/physics_3d/*.py module functions and the three stateful
+ * classes ProjectionSolver3D, DOMRadiationSolver, LevelSetFront3D.
+ */
+#ifndef FIREWAVE_H
+#define FIREWAVE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FW_OK = 0,
+  FW_E_CONFIG = -1,    /* bad argument / configuration  (reference: ValueError) */
+  FW_E_NOCONV = -2,    /* Krylov or DOM did not converge (reference: RuntimeError) */
+  FW_E_NONFINITE = -3, /* NaN/Inf in state             (reference: RuntimeError) */
+  FW_E_OVERFLOW = -4,  /* particle buffer overflow      (reference: ValueError) */
+  FW_E_HIP = -5        /* HIP runtime failure */
+};
+
+enum { FW_F32 = 0, FW_F64 = 1 };
+
+/* Grid metrics (Grid3D, spread_3d.py:160-424).  The 1-D arrays are device
+ * pointers to float64 arrays: dz, d_above, d_below of length nz, z_face of
+ * length nz+1.  y is periodic (the only y boundary of the hot path). */
+typedef struct fw_grid {
+  int32_t nx, ny, nz;
+  int32_t dtype;   /* FW_F32 or FW_F64: storage and arithmetic type of fields */
+  double dx, dy;
+  const double* dz;
+  const double* d_above;
+  const double* d_below;
+  const double* z_face;
+  int32_t n_z_bed;
+  int32_t reserved;
+} fw_grid;
+
+const char* fw_last_error(void);
+int fw_version(void);
+
+/* ---- transport stencils ------------------------------------------------- */
+
+/* muscl_3d.advect_3d_scalar_muscl (muscl_3d.py:67-171): rhs -= (u.grad)phi.
+ * rhs is updated in place (read-modify-write). */
+int fw_advect_scalar_muscl(const fw_grid* g, const void* phi, const void* u,
+                           const void* v, const void* w, void* rhs,
+                           double phi_inlet, void* stream);
+
+/* species_3d.step_species_transport (species_3d.py:40-175), out of place:
+ * Y_out = clip(Y + dt*(adv + D lap Y + S/rho), 0, 1), then clip to y_max.
+ * y_max = 1 reproduces the kernel; the driver passes 0.232 for O2 to fold the
+ * caller-side clip (spread_3d.py:3059-3060). */
+int fw_species_transport(const fw_grid* g, const void* Y, void* Y_out,
+                         const void* rho, const void* u, const void* v,
+                         const void* w, const void* S, double dt, double D,
+                         double Y_inlet, double y_max, void* stream);
+
+/* _adv_gas_energy (spread_3d.py:3827-3897): MUSCL advection with inlet ghost
+ * T_amb + alpha_th FV diffusion, then clamp to [T_amb, 1e4]; out of place. */
+int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
+                  const void* u, const void* v, const void* w, double dt,
+                  double alpha_th, double T_amb, void* stream);
+
+/* One sub-step of turbulence_3d.apply_turbulent_diffusion
+ * (turbulence_3d.py:920-949): f_out = f + nu_t/sc_t * lap(f) * dt_sub with
+ * zero-flux ghosts.  If clip_lo <= clip_hi the result is clipped to that
+ * range (the caller-side clips of spread_3d.py:3079-3084); pass lo > hi for
+ * the bare reference kernel. */
+int fw_turbulent_diffusion_pass(const fw_grid* g, const void* f, void* f_out,
+                                const void* nu_t, double sc_t, double dt_sub,
+                                double clip_lo, double clip_hi, void* stream);
+
+/* max(nu_t) over interior cells (turbulence_3d.py:890-897); synchronises the
+ * stream and writes the host scalar.  work: FW_REDUCE_BLOCKS (256) doubles. */
+int fw_nu_t_max_interior(const fw_grid* g, const void* nu_t, double* work,
+                         double* out_host, void* stream);
+
+/* solid_conduction_3d.step_solid_conduction_vertical
+ * (solid_conduction_3d.py:56-127), out of place; optional clamp as above
+ * (the T_s clamps of spread_3d.py:3195-3196). */
+int fw_solid_conduction_vertical(const fw_grid* g, const void* T_s,
+                                 void* T_out, const void* alpha_s,
+                                 double k_solid, double rho_solid,
+                                 double cp_solid, double dt, double clip_lo,
+                                 double clip_hi, void* stream);
+
+/* drag_3d.step_drag_force (drag_3d.py:99-144). */
+int fw_drag_force(const fw_grid* g, const void* u, const void* v,
+                  const void* w, const void* rho, const void* alpha_s,
+                  double sigma_sav, double C_D, void* Fx, void* Fy, void* Fz,
+                  void* stream);
+
+/* momentum_3d.step_tentative_velocity (momentum_3d.py:41-276), out of place.
+ * u_inlet/v_inlet/w_inlet: (Nz, Ny). */
+int fw_tentative_velocity(const fw_grid* g, const void* u, const void* v,
+                          const void* w, void* u_out, void* v_out,
+                          void* w_out, const void* rho, const void* T_g,
+                          const void* Fx, const void* Fy, const void* Fz,
+                          const void* u_inlet, const void* v_inlet,
+                          const void* w_inlet, double dt, double T_amb,
+                          void* stream);
+
+/* momentum_3d.apply_outflow_sponge (momentum_3d.py:292-317), in place.
+ * u_target: (Nz, Ny); sigma_x: (Nx). */
+int fw_outflow_sponge(const fw_grid* g, void* u, const void* u_target,
+                      const void* sigma_x, const void* Y_F, double Y_F_skip,
+                      double dt, void* stream);
+
+/* rho = P0 / (R_air max(T_g, T_amb))  (spread_3d.py:2042, :3440). */
+int fw_eos(const fw_grid* g, const void* T_g, void* rho, double T_amb,
+           void* stream);
+
+/* soil_3d.step_soil_conduction (soil_3d.py:105-153), in place, Gauss-Seidel
+ * in k.  T_soil: (n_soil, Ny, Nx); q_in_surface: (Ny, Nx); soil_* float64
+ * device arrays of length n_soil. */
+int fw_soil_conduction(const fw_grid* g, int n_soil, void* T_soil,
+                       const void* q_in_surface, const double* soil_dz,
+                       const double* soil_d_above, const double* soil_d_below,
+                       double dt, double k_s, double rho_s, double cp_s,
+                       double eps_s, double T_amb, void* stream);
+
+/* ---- turbulence -------------------------------------------------------- */
+
+/* turbulence_3d.step_k_epsilon (turbulence_3d.py:481-858) with
+ * _strain_and_vorticity_squared (:391-478) fused; out of place.
+ * u_inlet: (Nz, Ny); k/eps wall ghosts: (Ny, Nx). */
+int fw_k_epsilon(const fw_grid* g, const void* k_turb, const void* eps_turb,
+                 void* k_out, void* eps_out, void* nu_t_out, const void* u,
+                 const void* v, const void* w, const void* T_g,
+                 const void* alpha_s, const void* u_inlet,
+                 const void* k_wall_ghost, const void* eps_wall_ghost,
+                 double sigma_sav, double dt, double T_amb, double beta_p,
+                 double beta_d, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FIREWAVE_H */
