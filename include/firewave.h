@@ -145,6 +145,23 @@ int fw_k_epsilon(const fw_grid* g, const void* k_turb, const void* eps_turb,
                  double sigma_sav, double dt, double T_amb, double beta_p,
                  double beta_d, void* stream);
 
+/* ---- combustion ---------------------------------------------------------- */
+
+/* Number of int32 workspace entries fw_edc needs for grid g. */
+int fw_edc_workspace_ints(const fw_grid* g);
+
+/* chemistry_closures.run('edc') -> step_chemistry_ode_edc
+ * (chemistry_closures/edc.py:133-376; dispatch __init__.py:46-62).
+ * In place on T_g, Y_fuel, Y_O2; writes omega_out (time-averaged rate).
+ * Reproduces the reference's carried wet-bulb flag (edc.py:223) exactly
+ * (deterministic prefix scan in (k, j, i) order).  work: device int32 array
+ * of fw_edc_workspace_ints(g) entries. */
+int fw_edc(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel,
+           void* Y_O2, const void* k_turb, const void* eps_turb,
+           const void* Y_H2O, void* omega_out, double chi_rad, double cp_g,
+           double dt, int n_substeps, int extinction_enable, double s_stoich,
+           double hoc_J, int* work, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
