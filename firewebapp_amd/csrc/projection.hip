@@ -1,0 +1,904 @@
+// Pressure projection (projection_3d.py:65-175, :677-892 of the reference),
+// fft_pcg path, re-designed for the GPU:
+//
+// * the variable-density 7-point operator is applied matrix-free: the face
+//   coefficients 0.5*(1/rho_a + 1/rho_b) are recomputed from rho in
+//   registers, so a matvec streams rho and x once instead of 7 coefficient
+//   arrays;
+// * the separable constant-coefficient preconditioner (fft_poisson_3d.py)
+//   is three dense 1-D transforms: x (DCT-IV eigenbasis, Nx x Nx), y (real
+//   periodic eigenbasis, Ny x Ny; identical operator to the reference's
+//   FFT), z (eigenbasis of the sqrt(dz)-symmetrised L_z with sqrt(dz) folded
+//   into the matrices), each a batched LDS-tiled GEMM; the spectral divide
+//   is fused into the epilogue of the forward z transform;
+// * the Krylov method is PCG on the symmetrised system -W A p = -W b with
+//   W = diag(dz) (W A is symmetric because d_above[k] = d_below[k+1]); with
+//   the preconditioner above every W-inner product reduces to
+//   sum dz_k a b over the original residual;
+// * mixed precision: the solution, the right-hand side and the refinement
+//   residual b - A x are float64; the inner PCG runs in the grid's storage
+//   type; all dot products accumulate in float64 with a fixed-order
+//   (deterministic) two-level reduction.
+#include <string.h>
+#include <vector>
+#include "fw_common.h"
+
+namespace fw {
+
+#define PJ_RED 512  // partial-sum workgroups of the projection reductions
+
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_NS };
+
+// ---------------------------------------------------------------------------
+// operator
+// ---------------------------------------------------------------------------
+template <typename C, typename TR>
+__device__ __forceinline__ C inv_rho_at(const TR* rho, size_t c) {
+  C r = C(rho[c]);
+  return C(1) / (r > C(0.01) ? r : C(0.01));
+}
+
+// y = A x at cell (k, j, i), computed in precision C from rho (type TR).
+template <typename C, typename TX, typename TR>
+__device__ __forceinline__ C apply_A(const GridArgs& g, const TR* __restrict__ rho,
+                                     const TX* __restrict__ x, int k, int j,
+                                     int i, size_t c, double eps_reg) {
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  const size_t sy = Nx, sz = (size_t)Nx * Ny;
+  const C irc = inv_rho_at<C>(rho, c);
+  const C xc = C(x[c]);
+  const C dx2 = C(g.dx) * C(g.dx), dy2 = C(g.dy) * C(g.dy);
+  C diag = C(eps_reg);
+  C acc = C(0);
+  if (Nx > 1) {
+    if (i < Nx - 1) {
+      C cxp = (C(0.5) * (irc + inv_rho_at<C>(rho, c + 1))) / dx2;
+      diag -= cxp;
+      acc += cxp * C(x[c + 1]);
+    } else {
+      diag -= C(2) * irc / dx2;
+    }
+    if (i > 0) {
+      C cxm = (C(0.5) * (inv_rho_at<C>(rho, c - 1) + irc)) / dx2;
+      diag -= cxm;
+      acc += cxm * C(x[c - 1]);
+    }
+  }
+  if (Ny > 1) {
+    const size_t cp = c + (size_t)(wrap(j + 1, Ny) - j) * sy;
+    const size_t cm = c + (size_t)(wrap(j - 1, Ny) - j) * sy;
+    C cyp = (C(0.5) * (irc + inv_rho_at<C>(rho, cp))) / dy2;
+    C cym = (C(0.5) * (inv_rho_at<C>(rho, cm) + irc)) / dy2;
+    diag -= cyp + cym;
+    acc += cyp * C(x[cp]) + cym * C(x[cm]);
+  }
+  if (Nz > 1) {
+    const C inv_dz = C(1.0 / g.dz[k]);
+    if (k < Nz - 1) {
+      C czp = (C(0.5) * (irc + inv_rho_at<C>(rho, c + sz))) * inv_dz / C(g.d_above[k]);
+      diag -= czp;
+      acc += czp * C(x[c + sz]);
+    } else {
+      diag -= C(2) * irc * inv_dz * inv_dz;
+    }
+    if (k > 0) {
+      C czm = (C(0.5) * (inv_rho_at<C>(rho, c - sz) + irc)) * inv_dz / C(g.d_below[k]);
+      diag -= czm;
+      acc += czm * C(x[c - sz]);
+    }
+  }
+  return diag * xc + acc;
+}
+
+// Block reduction of one double over 256 threads into partial[blockIdx.x].
+__device__ __forceinline__ void block_sum_store(double v, double* sh, double* out) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x] = sh[0];
+}
+
+__device__ __forceinline__ void block_max_store(double v, double* sh, double* out) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s && sh[t + s] > sh[t]) sh[t] = sh[t + s];
+    __syncthreads();
+  }
+  if (t == 0) out[blockIdx.x] = sh[0];
+}
+
+struct Cell {
+  int i, j, k;
+};
+__device__ __forceinline__ Cell cell_of(const GridArgs& g, size_t c) {
+  Cell r;
+  r.i = (int)(c % g.nx);
+  size_t t = c / g.nx;
+  r.j = (int)(t % g.ny);
+  r.k = (int)(t / g.ny);
+  return r;
+}
+
+// q = A p (precision T), partial sum of dz_k * p * q
+template <typename T, typename TR>
+__global__ void __launch_bounds__(256)
+k_matvec_pq(GridArgs g, size_t n, const TR* __restrict__ rho,
+            const T* __restrict__ p, T* __restrict__ q, double eps_reg,
+            double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256) {
+    Cell e = cell_of(g, c);
+    T y = apply_A<T, T, TR>(g, rho, p, e.k, e.j, e.i, c, eps_reg);
+    q[c] = y;
+    acc += g.dz[e.k] * (double)p[c] * (double)y;
+  }
+  block_sum_store(acc, sh, part);
+}
+
+// r64 = b64 - A64 x64 (float64 operator from rho), r_t = (T) r64,
+// partial sum of r64^2
+template <typename T, typename TR>
+__global__ void __launch_bounds__(256)
+k_residual64(GridArgs g, size_t n, const TR* __restrict__ rho,
+             const double* __restrict__ b, const double* __restrict__ x,
+             double* __restrict__ r64, T* __restrict__ r_t, double eps_reg,
+             double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256) {
+    Cell e = cell_of(g, c);
+    double ax = apply_A<double, double, TR>(g, rho, x, e.k, e.j, e.i, c, eps_reg);
+    double r = b[c] - ax;
+    r64[c] = r;
+    r_t[c] = (T)r;
+    acc += r * r;
+  }
+  block_sum_store(acc, sh, part);
+}
+
+// d += alpha p ; r -= alpha q ; partial sum r^2
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_update_xr(size_t n, T* __restrict__ d, T* __restrict__ r,
+            const T* __restrict__ p, const T* __restrict__ q,
+            const double* __restrict__ S, double* __restrict__ part) {
+  __shared__ double sh[256];
+  const T alpha = (T)S[S_ALPHA];
+  double acc = 0.0;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256) {
+    d[c] += alpha * p[c];
+    T rv = r[c] - alpha * q[c];
+    r[c] = rv;
+    acc += (double)rv * (double)rv;
+  }
+  block_sum_store(acc, sh, part);
+}
+
+// partial sum of dz_k a b
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dot_w(GridArgs g, size_t n, const T* __restrict__ a, const T* __restrict__ b,
+        double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  const size_t plane = (size_t)g.nx * g.ny;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256)
+    acc += g.dz[c / plane] * (double)a[c] * (double)b[c];
+  block_sum_store(acc, sh, part);
+}
+
+// partial sum of a^2 (unweighted)
+__global__ void __launch_bounds__(256)
+k_sumsq(size_t n, const double* __restrict__ a, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256)
+    acc += a[c] * a[c];
+  block_sum_store(acc, sh, part);
+}
+
+// p = z + beta p   (beta = 0 on the first iteration)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_p_update(size_t n, T* __restrict__ p, const T* __restrict__ z,
+           const double* __restrict__ S, int first) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  p[c] = first ? z[c] : z[c] + (T)S[S_BETA] * p[c];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_x64_add(size_t n, double* __restrict__ x, const T* __restrict__ d) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c < n) x[c] += (double)d[c];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_zero(size_t n, T* __restrict__ a) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c < n) a[c] = T(0);
+}
+
+// Fixed-order final sum of PJ_RED partials; mode selects what is stored.
+enum { F_STORE = 0, F_ALPHA = 1, F_BETA = 2 };
+__global__ void __launch_bounds__(256)
+k_finalize(const double* __restrict__ part, int np, double* __restrict__ S,
+           int slot, int mode) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  double a = 0.0;
+  for (int b = t; b < np; b += 256) a += part[b];
+  sh[t] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  if (t == 0) {
+    const double v = sh[0];
+    if (mode == F_ALPHA) {          // v = <p, Ap>_W
+      S[S_PQ] = v;
+      S[S_ALPHA] = S[S_RZ] / v;
+    } else if (mode == F_BETA) {    // v = <r, z>_W (new)
+      S[S_BETA] = v / S[S_RZ];
+      S[S_RZ] = v;
+    } else {
+      S[slot] = v;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_finalize_max(const double* __restrict__ part, int np, double* __restrict__ S,
+               int slot) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  double a = 0.0;
+  for (int b = t; b < np; b += 256) a = part[b] > a ? part[b] : a;
+  sh[t] = a;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s && sh[t + s] > sh[t]) sh[t] = sh[t + s];
+    __syncthreads();
+  }
+  if (t == 0) S[slot] = sh[0];
+}
+
+// ---------------------------------------------------------------------------
+// compatible divergence (projection_3d.py:847-892), velocity correction
+// (:818-842) and the post-correction divergence check
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ double div_at(const GridArgs& g, const T* u,
+                                         const T* v, const T* w,
+                                         const T* u_inlet, int k, int j, int i,
+                                         size_t c) {
+  const int Nx = g.nx, Ny = g.ny;
+  const size_t sy = Nx, sz = (size_t)Nx * Ny;
+  double d = 0.0;
+  if (i > 0) d += ((double)u[c] - (double)u[c - 1]) / g.dx;
+  else d += ((double)u[c] - (double)u_inlet[(size_t)k * Ny + j]) / g.dx;
+  if (Ny > 1) {
+    const size_t cm = c + (size_t)(wrap(j - 1, Ny) - j) * sy;
+    d += ((double)v[c] - (double)v[cm]) / g.dy;
+  }
+  if (k > 0) d += ((double)w[c] - (double)w[c - sz]) / g.dz[k];
+  else d += ((double)w[c] - 0.0) / g.dz[0];
+  return d;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_div_rhs(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict__ v,
+          const T* __restrict__ w, const T* __restrict__ u_inlet,
+          const T* __restrict__ target, double inv_dt, double* __restrict__ b,
+          double* __restrict__ part) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256) {
+    Cell e = cell_of(g, c);
+    double d = div_at<T>(g, u, v, w, u_inlet, e.k, e.j, e.i, c);
+    if (target) d = d - (double)target[c];
+    double r = d * inv_dt;
+    if (b) b[c] = r;
+    acc += r * r;
+  }
+  block_sum_store(acc, sh, part);
+}
+
+// plain divergence into an output array (ProjectionSolver3D.divergence)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_divergence(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict__ v,
+             const T* __restrict__ w, const T* __restrict__ u_inlet,
+             T* __restrict__ out) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  Cell e = cell_of(g, c);
+  out[c] = (T)div_at<T>(g, u, v, w, u_inlet, e.k, e.j, e.i, c);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_div_check(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict__ v,
+            const T* __restrict__ w, const T* __restrict__ u_inlet,
+            const T* __restrict__ target, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double m = 0.0;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256) {
+    Cell e = cell_of(g, c);
+    double d = div_at<T>(g, u, v, w, u_inlet, e.k, e.j, e.i, c);
+    if (target) d -= (double)target[c];
+    d = fabs(d);
+    if (d > m || d != d) m = (d != d) ? __builtin_inf() : d;
+  }
+  block_max_store(m, sh, part);
+}
+
+// Velocity correction: reads u, v, w (old) and writes the corrected fields
+// to uo, vo, wo (may alias u, v, w: each cell writes only itself).
+template <typename T, typename TR>
+__global__ void __launch_bounds__(256)
+k_correct(GridArgs g, size_t n, T* u, T* v, T* w, const double* __restrict__ p,
+          const TR* __restrict__ rho, double dt) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  Cell e = cell_of(g, c);
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  const size_t sy = Nx, sz = (size_t)Nx * Ny;
+  const double ir = inv_rho_at<double>(rho, c);
+  if (Nx > 2) {
+    double uu = (double)u[c];
+    if (e.i < Nx - 1) {
+      double irx = 0.5 * (ir + inv_rho_at<double>(rho, c + 1));
+      uu -= dt * (p[c + 1] - p[c]) / g.dx * irx;
+    } else {
+      uu += 2.0 * dt * p[c] / g.dx * ir;
+    }
+    u[c] = (T)uu;
+  }
+  if (Ny > 1) {
+    const size_t cp = c + (size_t)(wrap(e.j + 1, Ny) - e.j) * sy;
+    double iry = 0.5 * (ir + inv_rho_at<double>(rho, cp));
+    v[c] = (T)((double)v[c] - dt * (p[cp] - p[c]) / g.dy * iry);
+  }
+  if (Nz > 2) {
+    double ww = (double)w[c];
+    if (e.k < Nz - 1) {
+      double irz = 0.5 * (ir + inv_rho_at<double>(rho, c + sz));
+      ww -= dt * (p[c + sz] - p[c]) / g.d_above[e.k] * irz;
+    } else {
+      ww += 2.0 * dt * p[c] / g.dz[Nz - 1] * ir;
+    }
+    w[c] = (T)ww;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// batched strided GEMM  C[b][m][n] = sum_k A[b][m][k] B[b][k][n]
+// 32x32 output tile per workgroup of 256 lanes (2x2 per lane), K in steps of
+// 16 staged through LDS.  Optional epilogue: multiply by the inverse
+// spectral eigenvalue 1/(lz[m] + ly[j] + lx[i] - eps), n = j*Nx + i.
+// ---------------------------------------------------------------------------
+struct GemmArgs {
+  int M, N, K;
+  long sAb, sAm, sAk, sBb, sBk, sBn, sCb, sCm, sCn;
+  const double* lz;
+  const double* ly;
+  const double* lx;
+  int nx;
+  double eps;
+};
+
+#define GT 32
+#define GK 16
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm(GemmArgs a, const T* __restrict__ A, const T* __restrict__ B,
+       T* __restrict__ C) {
+  __shared__ T As[GK][GT + 1];
+  __shared__ T Bs[GK][GT + 1];
+  const int tid = threadIdx.x;
+  const int tx = tid % 16, ty = tid / 16;
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  const long b = blockIdx.z;
+  const T* Ab = A + b * a.sAb;
+  const T* Bb = B + b * a.sBb;
+  T acc[2][2] = {{T(0), T(0)}, {T(0), T(0)}};
+  const bool a_kc = (a.sAk == 1);
+  const bool b_nc = (a.sBn == 1);
+  for (int k0 = 0; k0 < a.K; k0 += GK) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int e = tid + 256 * r;  // 512 elements per tile
+      int mm, kk;
+      if (a_kc) { kk = e % GK; mm = e / GK; } else { mm = e % GT; kk = e / GT; }
+      const int gm = m0 + mm, gk = k0 + kk;
+      As[kk][mm] = (gm < a.M && gk < a.K) ? Ab[gm * a.sAm + gk * a.sAk] : T(0);
+      int nn, kb;
+      if (b_nc) { nn = e % GT; kb = e / GT; } else { kb = e % GK; nn = e / GK; }
+      const int gn = n0 + nn, gkb = k0 + kb;
+      Bs[kb][nn] = (gn < a.N && gkb < a.K) ? Bb[gkb * a.sBk + gn * a.sBn] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; ++kk) {
+      const T a0 = As[kk][ty], a1 = As[kk][ty + 16];
+      const T b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
+      acc[0][0] += a0 * b0;
+      acc[0][1] += a0 * b1;
+      acc[1][0] += a1 * b0;
+      acc[1][1] += a1 * b1;
+    }
+    __syncthreads();
+  }
+  T* Cb = C + b * a.sCb;
+#pragma unroll
+  for (int ii = 0; ii < 2; ++ii) {
+    const int gm = m0 + ty + 16 * ii;
+    if (gm >= a.M) continue;
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) {
+      const int gn = n0 + tx + 16 * jj;
+      if (gn >= a.N) continue;
+      T v = acc[ii][jj];
+      if (a.lz) {
+        const int j = gn / a.nx, i = gn % a.nx;
+        v = v * T(1.0 / (a.lz[gm] + a.ly[j] + a.lx[i] - a.eps));
+      }
+      Cb[gm * a.sCm + gn * a.sCn] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// handle
+// ---------------------------------------------------------------------------
+struct ProjHandle {
+  fw_grid g;
+  GridArgs ga;
+  size_t n;
+  int dtype;
+  double eps_reg, rtol;
+  int maxiter;
+  size_t esz;  // element size of T
+  // preconditioner (type T): Ux (Nx*Nx), Yf (Ny*Ny forward = basis^T),
+  // Yi (Ny*Ny inverse = basis), Zf (Nz*Nz, D folded), Zi (Nz*Nz, D^-1 folded)
+  void *Ux, *UxT, *Yf, *Yi, *Zf, *Zi;
+  double *lx, *ly, *lz;  // eigenvalues (device, float64)
+  // vectors
+  double *x64, *b64, *r64;
+  void *r, *z, *p, *q, *d, *w1, *w2;
+  double *part, *S, *S_host;
+  bool have_prev;
+  int last_iters, last_outer;
+  double last_rel, last_div_max;
+  int precond_enabled;
+};
+
+template <typename T>
+static void launch_gemm(ProjHandle* h, const T* A, const T* B, T* C, int M,
+                        int N, int K, int batch, long sAb, long sAm, long sAk,
+                        long sBb, long sBk, long sBn, long sCb, long sCm,
+                        long sCn, bool spectral, hipStream_t s) {
+  GemmArgs a;
+  a.M = M; a.N = N; a.K = K;
+  a.sAb = sAb; a.sAm = sAm; a.sAk = sAk;
+  a.sBb = sBb; a.sBk = sBk; a.sBn = sBn;
+  a.sCb = sCb; a.sCm = sCm; a.sCn = sCn;
+  a.lz = spectral ? h->lz : nullptr;
+  a.ly = h->ly; a.lx = h->lx; a.nx = h->g.nx; a.eps = h->eps_reg;
+  dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, batch);
+  hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, s, a, A, B, C);
+}
+
+// z = M^{-1} r  (separable solve of fft_poisson_3d.py:184-228)
+template <typename T>
+static int precond_apply(ProjHandle* h, const T* r, T* z, hipStream_t s) {
+  const int Nx = h->g.nx, Ny = h->g.ny, Nz = h->g.nz;
+  const long NP = (long)Nx * Ny;
+  T* w1 = (T*)h->w1;
+  T* w2 = (T*)h->w2;
+  // x forward: w1[r][i'] = sum_i r[r][i] Ux[i][i']
+  launch_gemm<T>(h, r, (const T*)h->Ux, w1, Nz * Ny, Nx, Nx, 1, 0, Nx, 1, 0, Nx,
+                 1, 0, Nx, 1, false, s);
+  const T* cur = w1;
+  T* other = w2;
+  if (Ny > 1) {  // y forward per k-plane: w2_k = Yf (Ny x Ny) * w1_k (Ny x Nx)
+    launch_gemm<T>(h, (const T*)h->Yf, w1, w2, Ny, Nx, Ny, Nz, 0, Ny, 1, NP, Nx,
+                   1, NP, Nx, 1, false, s);
+    cur = w2;
+    other = w1;
+  }
+  // z forward with sqrt(dz) folded in, spectral divide in the epilogue
+  launch_gemm<T>(h, (const T*)h->Zf, cur, other, Nz, (int)NP, Nz, 1, 0, Nz, 1, 0,
+                 NP, 1, 0, NP, 1, true, s);
+  // z inverse with 1/sqrt(dz) folded in
+  T* t2 = (other == w1) ? w2 : w1;
+  launch_gemm<T>(h, (const T*)h->Zi, other, t2, Nz, (int)NP, Nz, 1, 0, Nz, 1, 0,
+                 NP, 1, 0, NP, 1, false, s);
+  const T* c2 = t2;
+  if (Ny > 1) {
+    T* t3 = (t2 == w1) ? w2 : w1;
+    launch_gemm<T>(h, (const T*)h->Yi, t2, t3, Ny, Nx, Ny, Nz, 0, Ny, 1, NP, Nx,
+                   1, NP, Nx, 1, false, s);
+    c2 = t3;
+  }
+  // x inverse: z[r][i] = sum_i' c2[r][i'] Ux[i][i']  (B = Ux^T)
+  launch_gemm<T>(h, c2, (const T*)h->UxT, z, Nz * Ny, Nx, Nx, 1, 0, Nx, 1, 0, Nx,
+                 1, 0, Nx, 1, false, s);
+  FW_CHECK_LAUNCH("precond gemm");
+  return FW_OK;
+}
+
+static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+static int read_scalars(ProjHandle* h, hipStream_t s) {
+  FW_CHECK_HIP(hipMemcpyAsync(h->S_host, h->S, S_NS * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+  FW_CHECK_HIP(hipStreamSynchronize(s));
+  return FW_OK;
+}
+
+// Inner PCG on A d = r (type T); r holds the right-hand side on entry and
+// the recursively updated residual on exit.  Stops when ||r||_2 <= tol_abs.
+template <typename T, typename TR>
+static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
+                     int* iters, hipStream_t s) {
+  const size_t n = h->n;
+  T* r = (T*)h->r;
+  T* z = (T*)h->z;
+  T* p = (T*)h->p;
+  T* q = (T*)h->q;
+  T* d = (T*)h->d;
+  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n)), dim3(256), 0, s, n, d);
+  int rc;
+  if (h->precond_enabled) {
+    if ((rc = precond_apply<T>(h, r, z, s))) return rc;
+  } else {
+    FW_CHECK_HIP(hipMemcpyAsync(z, r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  }
+  hipLaunchKernelGGL(k_dot_w<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n, r, z, h->part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_RZ, F_STORE);
+  hipLaunchKernelGGL(k_p_update<T>, dim3(nblk(n)), dim3(256), 0, s, n, p, z, h->S, 1);
+  FW_CHECK_LAUNCH("pcg setup");
+  int it = 0;
+  for (; it < maxit;) {
+    hipLaunchKernelGGL((k_matvec_pq<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga,
+                       n, rho, p, q, h->eps_reg, h->part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0, F_ALPHA);
+    hipLaunchKernelGGL(k_update_xr<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
+                       h->S, h->part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_RR, F_STORE);
+    ++it;
+    if (h->precond_enabled) {
+      if ((rc = precond_apply<T>(h, r, z, s))) return rc;
+    } else {
+      FW_CHECK_HIP(hipMemcpyAsync(z, r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+    hipLaunchKernelGGL(k_dot_w<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n, r, z, h->part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0, F_BETA);
+    FW_CHECK_LAUNCH("pcg iteration");
+    if ((rc = read_scalars(h, s))) return rc;
+    const double rr = h->S_host[S_RR];
+    if (!(rr == rr)) return fw_set_error(FW_E_NONFINITE, "projection: NaN residual");
+    if (sqrt(rr) <= tol_abs) break;
+    hipLaunchKernelGGL(k_p_update<T>, dim3(nblk(n)), dim3(256), 0, s, n, p, z, h->S, 0);
+  }
+  *iters = it;
+  return FW_OK;
+}
+
+// Solve A x = b: x64 (warm start on entry), b64 given; ||b - A x||_2 <=
+// rtol ||b||_2 with the residual recomputed in float64 after each inner solve.
+template <typename T, typename TR>
+static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm,
+                         hipStream_t s) {
+  const size_t n = h->n;
+  const double tol_abs = rtol * bnorm;
+  const double inner_floor = (sizeof(T) == 4) ? 2e-5 : 0.0;
+  int total = 0, outer = 0, rc;
+  double rnorm = 0.0;
+  for (;;) {
+    hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga,
+                       n, rho, h->b64, h->x64, h->r64, (T*)h->r, h->eps_reg, h->part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP, F_STORE);
+    FW_CHECK_LAUNCH("residual64");
+    if ((rc = read_scalars(h, s))) return rc;
+    rnorm = sqrt(h->S_host[S_TMP]);
+    if (!(rnorm == rnorm)) return fw_set_error(FW_E_NONFINITE, "projection: NaN residual");
+    if (rnorm <= tol_abs || bnorm == 0.0) break;
+    if (total >= h->maxiter || outer >= 20) {
+      h->last_iters = total;
+      return fw_set_error(FW_E_NOCONV,
+                          "projection PCG did not converge: ||r||/||b|| = %.3e after "
+                          "%d iterations (rtol %.1e)", rnorm / bnorm, total, rtol);
+    }
+    double inner_rel = tol_abs / rnorm;
+    if (inner_rel < inner_floor) inner_rel = inner_floor;
+    int it = 0;
+    if ((rc = pcg_inner<T, TR>(h, rho, inner_rel * rnorm, h->maxiter - total, &it, s)))
+      return rc;
+    total += it;
+    ++outer;
+    hipLaunchKernelGGL(k_x64_add<T>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64,
+                       (const T*)h->d);
+    FW_CHECK_LAUNCH("x64_add");
+  }
+  h->last_iters = total;
+  h->last_outer = outer;
+  h->last_rel = bnorm > 0.0 ? rnorm / bnorm : 0.0;
+  return FW_OK;
+}
+
+template <typename T>
+static int project_impl(ProjHandle* h, void* u, void* v, void* w,
+                        const void* rho, double dt, const void* target,
+                        const void* u_inlet, hipStream_t s) {
+  const size_t n = h->n;
+  hipLaunchKernelGGL(k_div_rhs<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
+                     (const T*)u, (const T*)v, (const T*)w, (const T*)u_inlet,
+                     (const T*)target, 1.0 / dt, h->b64, h->part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_BB, F_STORE);
+  FW_CHECK_LAUNCH("div_rhs");
+  if (!h->have_prev) {
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64);
+    h->have_prev = true;
+  }
+  int rc = read_scalars(h, s);
+  if (rc) return rc;
+  const double bnorm = sqrt(h->S_host[S_BB]);
+  if (!(bnorm == bnorm)) return fw_set_error(FW_E_NONFINITE, "projection: NaN divergence");
+  if (bnorm == 0.0) {
+    // scipy's bicgstab returns x = 0 for b = 0
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64);
+    h->last_iters = 0;
+    h->last_rel = 0.0;
+  } else {
+    rc = solve_refined<T, T>(h, (const T*)rho, h->rtol, bnorm, s);
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL((k_correct<T, T>), dim3(nblk(n)), dim3(256), 0, s, h->ga, n,
+                     (T*)u, (T*)v, (T*)w, (const double*)h->x64, (const T*)rho, dt);
+  hipLaunchKernelGGL(k_div_check<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
+                     (const T*)u, (const T*)v, (const T*)w, (const T*)u_inlet,
+                     (const T*)target, h->part);
+  hipLaunchKernelGGL(k_finalize_max, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP);
+  FW_CHECK_LAUNCH("correct");
+  if ((rc = read_scalars(h, s))) return rc;
+  h->last_div_max = h->S_host[S_TMP];
+  return FW_OK;
+}
+
+template <typename T>
+static int upload_cast(void** dst, const double* src, size_t n) {
+  std::vector<T> tmp(n);
+  for (size_t i = 0; i < n; ++i) tmp[i] = (T)src[i];
+  FW_CHECK_HIP(hipMalloc(dst, n * sizeof(T)));
+  FW_CHECK_HIP(hipMemcpy(*dst, tmp.data(), n * sizeof(T), hipMemcpyHostToDevice));
+  return FW_OK;
+}
+
+}  // namespace fw
+
+using namespace fw;
+
+extern "C" {
+
+int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
+                   const double* Uz, const double* lam_x, const double* lam_y,
+                   const double* lam_z, double eps_reg, double cg_rtol,
+                   int maxiter, void** handle) {
+  FW_REQUIRE(g && handle && Ux && Uy && Uz && lam_x && lam_y && lam_z,
+             "proj_create: null argument");
+  FW_REQUIRE(g->nx >= 2 && g->nz >= 2 && g->ny >= 1, "proj_create: needs Nx, Nz >= 2");
+  ProjHandle* h = new ProjHandle();
+  memset(h, 0, sizeof(*h));
+  h->g = *g;
+  h->ga = grid_args(g);
+  h->n = (size_t)g->nx * g->ny * g->nz;
+  h->dtype = g->dtype;
+  h->eps_reg = eps_reg;
+  h->rtol = cg_rtol;
+  h->maxiter = maxiter > 0 ? maxiter : 400;
+  h->esz = g->dtype == FW_F64 ? 8 : 4;
+  h->precond_enabled = 1;
+  const int Nx = g->nx, Ny = g->ny, Nz = g->nz;
+  // host copies of the dz metric for folding sqrt(dz) into the z matrices
+  std::vector<double> dz(Nz);
+  FW_CHECK_HIP(hipMemcpy(dz.data(), g->dz, Nz * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<double> UxT((size_t)Nx * Nx), Yf((size_t)Ny * Ny), Zf((size_t)Nz * Nz),
+      Zi((size_t)Nz * Nz);
+  for (int a = 0; a < Nx; ++a)
+    for (int b = 0; b < Nx; ++b) UxT[(size_t)a * Nx + b] = Ux[(size_t)b * Nx + a];
+  for (int a = 0; a < Ny; ++a)
+    for (int b = 0; b < Ny; ++b) Yf[(size_t)a * Ny + b] = Uy[(size_t)b * Ny + a];
+  // forward: out[k'] = sum_k Uz[k][k'] sqrt(dz_k) in[k]
+  // inverse: out[k]  = (1/sqrt(dz_k)) sum_k' Uz[k][k'] in[k']
+  for (int kp = 0; kp < Nz; ++kp)
+    for (int k = 0; k < Nz; ++k) {
+      Zf[(size_t)kp * Nz + k] = Uz[(size_t)k * Nz + kp] * sqrt(dz[k]);
+      Zi[(size_t)k * Nz + kp] = Uz[(size_t)k * Nz + kp] / sqrt(dz[k]);
+    }
+  int rc;
+  if (g->dtype == FW_F64) {
+    if ((rc = upload_cast<double>(&h->Ux, Ux, (size_t)Nx * Nx)) ||
+        (rc = upload_cast<double>(&h->UxT, UxT.data(), (size_t)Nx * Nx)) ||
+        (rc = upload_cast<double>(&h->Yf, Yf.data(), (size_t)Ny * Ny)) ||
+        (rc = upload_cast<double>(&h->Yi, Uy, (size_t)Ny * Ny)) ||
+        (rc = upload_cast<double>(&h->Zf, Zf.data(), (size_t)Nz * Nz)) ||
+        (rc = upload_cast<double>(&h->Zi, Zi.data(), (size_t)Nz * Nz)))
+      return rc;
+  } else {
+    if ((rc = upload_cast<float>(&h->Ux, Ux, (size_t)Nx * Nx)) ||
+        (rc = upload_cast<float>(&h->UxT, UxT.data(), (size_t)Nx * Nx)) ||
+        (rc = upload_cast<float>(&h->Yf, Yf.data(), (size_t)Ny * Ny)) ||
+        (rc = upload_cast<float>(&h->Yi, Uy, (size_t)Ny * Ny)) ||
+        (rc = upload_cast<float>(&h->Zf, Zf.data(), (size_t)Nz * Nz)) ||
+        (rc = upload_cast<float>(&h->Zi, Zi.data(), (size_t)Nz * Nz)))
+      return rc;
+  }
+  FW_CHECK_HIP(hipMalloc((void**)&h->lx, Nx * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->ly, Ny * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->lz, Nz * sizeof(double)));
+  FW_CHECK_HIP(hipMemcpy(h->lx, lam_x, Nx * sizeof(double), hipMemcpyHostToDevice));
+  FW_CHECK_HIP(hipMemcpy(h->ly, lam_y, Ny * sizeof(double), hipMemcpyHostToDevice));
+  FW_CHECK_HIP(hipMemcpy(h->lz, lam_z, Nz * sizeof(double), hipMemcpyHostToDevice));
+  const size_t n = h->n;
+  FW_CHECK_HIP(hipMalloc((void**)&h->x64, n * 8));
+  FW_CHECK_HIP(hipMalloc((void**)&h->b64, n * 8));
+  FW_CHECK_HIP(hipMalloc((void**)&h->r64, n * 8));
+  void** vecs[] = {&h->r, &h->z, &h->p, &h->q, &h->d, &h->w1, &h->w2};
+  for (void** v : vecs) FW_CHECK_HIP(hipMalloc(v, n * h->esz));
+  FW_CHECK_HIP(hipMalloc((void**)&h->part, PJ_RED * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->S, S_NS * sizeof(double)));
+  FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, S_NS * sizeof(double)));
+  FW_CHECK_HIP(hipMemset(h->x64, 0, n * 8));
+  *handle = h;
+  return FW_OK;
+}
+
+int fw_proj_destroy(void* handle) {
+  ProjHandle* h = (ProjHandle*)handle;
+  if (!h) return FW_OK;
+  void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
+                  h->x64, h->b64, h->r64, h->r, h->z, h->p, h->q, h->d, h->w1,
+                  h->w2, h->part, h->S};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (h->S_host) (void)hipHostFree(h->S_host);
+  delete h;
+  return FW_OK;
+}
+
+int fw_proj_project(void* handle, void* u, void* v, void* w, const void* rho,
+                    double dt, const void* div_target, const void* u_inlet,
+                    void* stream) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && u && v && w && rho && u_inlet && dt > 0.0,
+             "proj_project: bad argument");
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == FW_F64
+             ? project_impl<double>(h, u, v, w, rho, dt, div_target, u_inlet, s)
+             : project_impl<float>(h, u, v, w, rho, dt, div_target, u_inlet, s);
+}
+
+int fw_proj_stats(void* handle, int* iters, double* rel_resid, double* div_max) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h, "proj_stats: null handle");
+  if (iters) *iters = h->last_iters;
+  if (rel_resid) *rel_resid = h->last_rel;
+  if (div_max) *div_max = h->last_div_max;
+  return FW_OK;
+}
+
+int fw_proj_pressure(void* handle, double* p_out, void* stream) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && p_out, "proj_pressure: null argument");
+  FW_CHECK_HIP(hipMemcpyAsync(p_out, h->x64, h->n * 8, hipMemcpyDeviceToDevice,
+                              (hipStream_t)stream));
+  return FW_OK;
+}
+
+int fw_proj_set_pressure(void* handle, const double* p_in, void* stream) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && p_in, "proj_set_pressure: null argument");
+  FW_CHECK_HIP(hipMemcpyAsync(h->x64, p_in, h->n * 8, hipMemcpyDeviceToDevice,
+                              (hipStream_t)stream));
+  h->have_prev = true;
+  return FW_OK;
+}
+
+int fw_proj_divergence(void* handle, const void* u, const void* v,
+                       const void* w, const void* u_inlet, void* div_out,
+                       void* stream) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && u && v && w && u_inlet && div_out, "proj_divergence: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == FW_F64)
+    hipLaunchKernelGGL(k_divergence<double>, dim3(nblk(h->n)), dim3(256), 0, s, h->ga,
+                       h->n, (const double*)u, (const double*)v, (const double*)w,
+                       (const double*)u_inlet, (double*)div_out);
+  else
+    hipLaunchKernelGGL(k_divergence<float>, dim3(nblk(h->n)), dim3(256), 0, s, h->ga,
+                       h->n, (const float*)u, (const float*)v, (const float*)w,
+                       (const float*)u_inlet, (float*)div_out);
+  FW_CHECK_LAUNCH("divergence");
+  return FW_OK;
+}
+
+int fw_proj_matvec(void* handle, const void* rho, const void* x, void* y,
+                   void* stream) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && rho && x && y, "proj_matvec: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->dtype == FW_F64)
+    hipLaunchKernelGGL((k_matvec_pq<double, double>), dim3(PJ_RED), dim3(256), 0, s,
+                       h->ga, h->n, (const double*)rho, (const double*)x, (double*)y,
+                       h->eps_reg, h->part);
+  else
+    hipLaunchKernelGGL((k_matvec_pq<float, float>), dim3(PJ_RED), dim3(256), 0, s,
+                       h->ga, h->n, (const float*)rho, (const float*)x, (float*)y,
+                       h->eps_reg, h->part);
+  FW_CHECK_LAUNCH("matvec");
+  return FW_OK;
+}
+
+int fw_proj_precondition(void* handle, const void* r, void* z, void* stream) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && r && z, "proj_precondition: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  return h->dtype == FW_F64 ? precond_apply<double>(h, (const double*)r, (double*)z, s)
+                            : precond_apply<float>(h, (const float*)r, (float*)z, s);
+}
+
+int fw_proj_set_preconditioner(void* handle, int enabled) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h, "proj_set_preconditioner: null handle");
+  h->precond_enabled = enabled ? 1 : 0;
+  return FW_OK;
+}
+
+int fw_proj_solve(void* handle, const void* rho, const double* b, double* x,
+                  double rtol, int* iters, double* rel_resid, void* stream) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && rho && b && x, "proj_solve: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  FW_CHECK_HIP(hipMemcpyAsync(h->b64, b, h->n * 8, hipMemcpyDeviceToDevice, s));
+  FW_CHECK_HIP(hipMemcpyAsync(h->x64, x, h->n * 8, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_sumsq, dim3(PJ_RED), dim3(256), 0, s, h->n,
+                     (const double*)h->b64, h->part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
+                     S_BB, F_STORE);
+  FW_CHECK_LAUNCH("solve setup");
+  int rc0 = read_scalars(h, s);
+  if (rc0) return rc0;
+  const double bnorm = sqrt(h->S_host[S_BB]);
+  h->have_prev = true;
+  int rc = h->dtype == FW_F64
+               ? solve_refined<double, double>(h, (const double*)rho, rtol, bnorm, s)
+               : solve_refined<float, float>(h, (const float*)rho, rtol, bnorm, s);
+  if (iters) *iters = h->last_iters;
+  if (rel_resid) *rel_resid = h->last_rel;
+  if (rc) return rc;
+  FW_CHECK_HIP(hipMemcpyAsync(x, h->x64, h->n * 8, hipMemcpyDeviceToDevice, s));
+  FW_CHECK_HIP(hipStreamSynchronize(s));
+  return FW_OK;
+}
+
+}  // extern "C"

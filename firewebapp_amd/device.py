@@ -115,14 +115,21 @@ class Marshal:
         else:
             self.dtype = torch.float64 if np.asarray(like).dtype == np.float64 else torch.float32
         self._back = []
+        # Device copies must outlive the kernel launch that reads them: a
+        # temporary freed after `ptr()` would let the caching allocator hand
+        # its block to the next copy before the kernel has run.
+        self._keep = []
 
     def dev(self, a, dtype=None):
         dt = dtype or self.dtype
         if isinstance(a, torch.Tensor):
             if a.is_cuda and a.dtype == dt and a.is_contiguous():
                 return a
-            return a.to(device=device(), dtype=dt).contiguous()
-        return torch.as_tensor(np.ascontiguousarray(a), device=device()).to(dt).contiguous()
+            t = a.to(device=device(), dtype=dt).contiguous()
+        else:
+            t = torch.as_tensor(np.ascontiguousarray(a), device=device()).to(dt).contiguous()
+        self._keep.append(t)
+        return t
 
     def out(self, a, dtype=None):
         """Device buffer for an in/out argument, written back on writeback()."""
@@ -139,6 +146,7 @@ class Marshal:
             else:
                 a[...] = t.cpu().numpy().astype(a.dtype, copy=False)
         self._back.clear()
+        self._keep.clear()
 
 
 def grid_for(m: Marshal, shape, dx, dy, dz_arr, d_face_above, d_face_below,

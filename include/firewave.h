@@ -145,6 +145,61 @@ int fw_k_epsilon(const fw_grid* g, const void* k_turb, const void* eps_turb,
                  double sigma_sav, double dt, double T_amb, double beta_p,
                  double beta_d, void* stream);
 
+/* ---- pressure projection (ProjectionSolver3D, projection_3d.py:214-948;
+ *      SeparableLaplacian3D, fft_poisson_3d.py:45-228) -------------------- */
+
+/* Create a projection handle (replaces ProjectionSolver3D.__init__ with
+ * method='fft_pcg', projection_3d.py:214-280, and SeparableLaplacian3D.__init__,
+ * fft_poisson_3d.py:70-125).  Host float64 inputs: Ux (Nx x Nx, row-major,
+ * columns = x eigenvectors of the Neumann/Dirichlet L_x), Uy (Ny x Ny, columns
+ * = real orthonormal periodic eigenvectors), Uz (Nz x Nz, columns =
+ * eigenvectors of sqrt(dz) L_z sqrt(dz)^-1), and their eigenvalues.  Allocates
+ * the solver state (warm-start pressure, Krylov vectors). */
+int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
+                   const double* Uz, const double* lam_x, const double* lam_y,
+                   const double* lam_z, double eps_reg, double cg_rtol,
+                   int maxiter, void** handle);
+int fw_proj_destroy(void* handle);
+
+/* ProjectionSolver3D.project (projection_3d.py:677-844): rhs = (div(u) -
+ * div_target)/dt in float64, solve A p = rhs to ||r||_2 <= cg_rtol ||rhs||_2
+ * (PCG with the separable preconditioner, float64 refinement residual,
+ * warm-started from the previous p), then correct u, v, w in place.
+ * div_target may be NULL.  u_inlet: (Nz, Ny).  FW_E_NOCONV if the Krylov
+ * solve fails (reference: RuntimeError, projection_3d.py:785-790). */
+int fw_proj_project(void* handle, void* u, void* v, void* w, const void* rho,
+                    double dt, const void* div_target, const void* u_inlet,
+                    void* stream);
+
+/* Iterations and final relative residual of the last solve, and
+ * max|div(u_new) - div_target| after the last correction (float64). */
+int fw_proj_stats(void* handle, int* iters, double* rel_resid, double* div_max);
+
+/* Copy the solver's float64 pressure (the warm start, N values) out / in. */
+int fw_proj_pressure(void* handle, double* p_out, void* stream);
+int fw_proj_set_pressure(void* handle, const double* p_in, void* stream);
+
+/* ProjectionSolver3D.divergence (projection_3d.py:847-948), alpha_g = None. */
+int fw_proj_divergence(void* handle, const void* u, const void* v,
+                       const void* w, const void* u_inlet, void* div_out,
+                       void* stream);
+
+/* y = A x for the operator of _fill_var_density_data (projection_3d.py:65-175)
+ * built from rho (matrix-free). */
+int fw_proj_matvec(void* handle, const void* rho, const void* x, void* y,
+                   void* stream);
+
+/* z = SeparableLaplacian3D.solve(r) (fft_poisson_3d.py:184-228). */
+int fw_proj_precondition(void* handle, const void* r, void* z, void* stream);
+
+/* 1 = separable preconditioner (default), 0 = none (identity). */
+int fw_proj_set_preconditioner(void* handle, int enabled);
+
+/* Solve A x = b (float64 b, x; x is the warm start) to ||b-Ax||_2 <=
+ * rtol ||b||_2 with the handle's precision policy; rho in the grid's type. */
+int fw_proj_solve(void* handle, const void* rho, const double* b, double* x,
+                  double rtol, int* iters, double* rel_resid, void* stream);
+
 /* ---- combustion ---------------------------------------------------------- */
 
 /* Number of int32 workspace entries fw_edc needs for grid g. */
