@@ -120,8 +120,13 @@ class Marshal:
         # its block to the next copy before the kernel has run.
         self._keep = []
 
-    def dev(self, a, dtype=None):
+    def dev(self, a, dtype=None, shape=None):
+        """Device tensor for `a`; with `shape`, a size mismatch raises
+        ValueError before any kernel can read past the buffer."""
         dt = dtype or self.dtype
+        if shape is not None and tuple(a.shape) != tuple(shape):
+            raise ValueError(f"array of shape {tuple(a.shape)} where {tuple(shape)} "
+                             "is required")
         if isinstance(a, torch.Tensor):
             if a.is_cuda and a.dtype == dt and a.is_contiguous():
                 return a
@@ -131,9 +136,9 @@ class Marshal:
         self._keep.append(t)
         return t
 
-    def out(self, a, dtype=None):
+    def out(self, a, dtype=None, shape=None):
         """Device buffer for an in/out argument, written back on writeback()."""
-        t = self.dev(a, dtype)
+        t = self.dev(a, dtype, shape)
         if not (isinstance(a, torch.Tensor) and t.data_ptr() == a.data_ptr()):
             self._back.append((a, t))
         return t

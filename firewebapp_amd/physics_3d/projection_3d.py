@@ -74,10 +74,11 @@ class ProjectionSolver3D:
     def project(self, u, v, w, rho, dt, div_target=None):
         m = Marshal(u)
         h = self._handle(m.dtype)
-        U, V, W = m.out(u), m.out(v), m.out(w)
-        R = m.dev(rho)
-        T = m.dev(div_target) if div_target is not None else None
-        UI = m.dev(self._u_inlet)
+        sh = (self.Nz, self.Ny, self.Nx)
+        U, V, W = m.out(u, shape=sh), m.out(v, shape=sh), m.out(w, shape=sh)
+        R = m.dev(rho, shape=sh)
+        T = m.dev(div_target, shape=sh) if div_target is not None else None
+        UI = m.dev(self._u_inlet, shape=(self.Nz, self.Ny))
         _lib.call("fw_proj_project", h, ptr(U), ptr(V), ptr(W), ptr(R), float(dt),
                   ptr(T), ptr(UI), stream_ptr())
         it, rel, dmax = C.c_int(), C.c_double(), C.c_double()
@@ -93,10 +94,12 @@ class ProjectionSolver3D:
     def divergence(self, u, v, w):
         m = Marshal(u)
         h = self._handle(m.dtype)
-        U, V, W = m.dev(u), m.dev(v), m.dev(w)
+        sh = (self.Nz, self.Ny, self.Nx)
+        U, V, W = m.dev(u, shape=sh), m.dev(v, shape=sh), m.dev(w, shape=sh)
         out = torch.empty_like(U)
         _lib.call("fw_proj_divergence", h, ptr(U), ptr(V), ptr(W),
-                  ptr(m.dev(self._u_inlet)), ptr(out), stream_ptr())
+                  ptr(m.dev(self._u_inlet, shape=(self.Nz, self.Ny))), ptr(out),
+                  stream_ptr())
         torch.cuda.current_stream().synchronize()
         return out.cpu().numpy() if isinstance(u, np.ndarray) else out
 
@@ -108,7 +111,8 @@ class ProjectionSolver3D:
         X = m.dev(np.asarray(x).reshape(self.Nz, self.Ny, self.Nx)
                   if isinstance(x, np.ndarray) else x.reshape(self.Nz, self.Ny, self.Nx))
         Y = torch.empty_like(X)
-        _lib.call("fw_proj_matvec", h, ptr(m.dev(rho)), ptr(X), ptr(Y), stream_ptr())
+        _lib.call("fw_proj_matvec", h, ptr(m.dev(rho, shape=(self.Nz, self.Ny, self.Nx))),
+                  ptr(X), ptr(Y), stream_ptr())
         torch.cuda.current_stream().synchronize()
         return Y.cpu().numpy().reshape(-1) if isinstance(x, np.ndarray) else Y
 
