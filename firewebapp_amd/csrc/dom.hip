@@ -1,0 +1,467 @@
+// Discrete-ordinates radiation (dom_3d.py:98-473 of the reference).
+//
+// Sweep design.  For ordinate n the upwind step-differencing update
+//   I(i) = (kB + ax I(i-1) + ay I_y + az I_z) / (k + ax + ay + az)
+// is a first-order linear recurrence along x once the y and z inflows of the
+// row are known.  A row (n, j, k) is therefore one workgroup: each lane folds
+// its run of cells into an affine map x -> a x + b, a workgroup-wide
+// Hillis-Steele scan composes the maps, and each lane then re-evaluates its
+// cells with the reference formula from the exact upwind value.  Rows whose
+// sweep-ordered (j', k') lie on the same anti-diagonal j' + k' = d are
+// independent, so one launch processes diagonal d for all M ordinates at
+// once; Ny + Nz - 1 launches make one full source iteration.
+//
+// Periodic y uses the reference's stale wrap (dom_3d.py:189-202): the
+// inflow of row j' = 0 is row j' = Ny-1 of the same plane, which lies on a
+// later diagonal and still holds the previous iteration's intensities (for
+// Ny = 1 it is the row itself, read before it is written).  I_set persists
+// across solves (warm start, dom_3d.py:282).
+#include <math.h>
+#include <string.h>
+#include <vector>
+#include "fw_common.h"
+
+namespace fw {
+
+#define DOM_RED 256
+
+struct DomHandle {
+  fw_grid g;
+  GridArgs ga;
+  int M, maxiter;
+  double eps_ground, tol, relax, kappa_gas_max;
+  size_t n, esz;
+  double* Om;  // device M x 3
+  double* w;   // device M
+  std::vector<double> Om_h, w_h;
+  void *I, *G, *Gp, *kap, *kap_s, *kap_safe, *B;
+  void *Ig_out, *Ig_emit, *q_in;  // (Ny, Nx)
+  double *part, *part2, *S, *S_host;
+  int last_iters;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dom_props(size_t n, const T* __restrict__ Ts, const T* __restrict__ Tg,
+            const T* __restrict__ alpha, const T* __restrict__ omega,
+            const T* __restrict__ Yw, const T* __restrict__ rho,
+            const T* __restrict__ Mloc, T sigma_sav, T T_amb, T kgmax,
+            T* __restrict__ kap, T* __restrict__ kap_s, T* __restrict__ kap_safe,
+            T* __restrict__ B) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  T ks = sigma_sav * alpha[c];
+  if (Mloc) ks = ks * (T(1) + T(5.0) * Mloc[c]);
+  T of = omega[c] / T(1.0e-3);
+  of = of < T(0) ? T(0) : (of > T(1) ? T(1) : of);
+  T tf = (Tg[c] - T(600.0)) / T(400.0);
+  tf = tf < T(0) ? T(0) : (tf > T(1) ? T(1) : tf);
+  T kg = kgmax * (of > tf ? of : tf);
+  if (Yw && rho) kg = kg + T(30.0) * rho[c] * Yw[c];
+  const T k = ks + kg;
+  const T ksafe = k > T(1.0e-3) ? k : T(1.0e-3);
+  T Tr4;
+  if (k > T(1.0e-9)) {
+    const T a = Ts[c], b = Tg[c];
+    Tr4 = (ks * ((a * a) * (a * a)) + kg * ((b * b) * (b * b))) / ksafe;
+  } else {
+    Tr4 = (T_amb * T_amb) * (T_amb * T_amb);
+  }
+  kap[c] = k;
+  kap_s[c] = ks;
+  kap_safe[c] = ksafe;
+  B[c] = T(5.67e-8) * Tr4 / T(M_PI);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dom_ground_init(int ncol, const T* __restrict__ Tsoil, T eps, T I_amb,
+                  T* __restrict__ emit, T* __restrict__ out) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncol) return;
+  T e;
+  if (Tsoil) {
+    const T t = Tsoil[c];
+    e = eps * T(5.67e-8) * ((t * t) * (t * t)) / T(M_PI);
+  } else {
+    e = eps * I_amb;
+  }
+  emit[c] = e;
+  out[c] = e;
+}
+
+// One anti-diagonal d of the row wavefront, all ordinates.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dom_sweep_diag(GridArgs g, int d, int jlo, const double* __restrict__ Om,
+                 T* __restrict__ Iset, const T* __restrict__ kap,
+                 const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb) {
+  __shared__ T sA[256];
+  __shared__ T sB[256];
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  const int n = blockIdx.y;
+  const int jp = jlo + blockIdx.x;
+  const int kp = d - jp;
+  const double xi = Om[3 * n], eta = Om[3 * n + 1], mu = Om[3 * n + 2];
+  const int j = eta >= 0.0 ? jp : Ny - 1 - jp;
+  const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
+  const size_t plane = (size_t)Nx * Ny;
+  T* In = Iset + (size_t)n * plane * Nz;
+  const T ax = T(fabs(xi) / g.dx);
+  const T ay = T(fabs(eta) / g.dy);
+  const T az = T(fabs(mu) / g.dz[k]);
+  const int jy = eta >= 0.0 ? (j > 0 ? j - 1 : Ny - 1) : (j < Ny - 1 ? j + 1 : 0);
+  const bool zb = mu >= 0.0 ? (k == 0) : (k == Nz - 1);
+  const int kz = mu >= 0.0 ? k - 1 : k + 1;
+  const size_t row = ((size_t)k * Ny + j) * Nx;
+  const size_t rowy = ((size_t)k * Ny + jy) * Nx;
+  const size_t rowz = zb ? 0 : ((size_t)kz * Ny + j) * Nx;
+  const bool fwd = xi >= 0.0;
+  const int per = (Nx + 255) / 256;
+  const int t = threadIdx.x;
+  // pass 1: each lane folds its cells (sweep order) into x -> A x + Bc and
+  // keeps their y and z inflows for the exact re-evaluation in pass 2
+  constexpr int MAXPER = 8;
+  T Iys[MAXPER], Izs[MAXPER];
+  T A = T(1), Bc = T(0);
+  for (int q = 0; q < per && q < MAXPER; ++q) {
+    const int s = t * per + q;
+    if (s >= Nx) break;
+    const int i = fwd ? s : Nx - 1 - s;
+    const size_t c = row + i;
+    const T kc = kap[c];
+    const T Iy = In[rowy + i];
+    const T Iz = zb ? (mu >= 0.0 ? Ig_out[(size_t)j * Nx + i] : I_amb) : In[rowz + i];
+    const T den = kc + ax + ay + az;
+    const T a = ax / den;
+    const T b = (kc * B[c] + ay * Iy + az * Iz) / den;
+    Bc = a * Bc + b;  // compose: x -> a (A x + Bc) + b
+    A = a * A;
+    Iys[q] = Iy;
+    Izs[q] = Iz;
+  }
+  sA[t] = A;
+  sB[t] = Bc;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    T pa = T(1), pb = T(0);
+    if (t >= off) { pa = sA[t - off]; pb = sB[t - off]; }
+    __syncthreads();
+    if (t >= off) {
+      // (A_t, B_t) o (pa, pb): x -> A_t (pa x + pb) + B_t
+      sB[t] = sA[t] * pb + sB[t];
+      sA[t] = sA[t] * pa;
+    }
+    __syncthreads();
+  }
+  // x inflow: I_left = I_right = I_amb (dom_3d.py:376, :424)
+  T Iprev = (t == 0) ? I_amb : sA[t - 1] * I_amb + sB[t - 1];
+  for (int q = 0; q < per && q < MAXPER; ++q) {
+    const int s = t * per + q;
+    if (s >= Nx) break;
+    const int i = fwd ? s : Nx - 1 - s;
+    const size_t c = row + i;
+    const T kc = kap[c];
+    const T num = kc * B[c] + ax * Iprev + ay * Iys[q] + az * Izs[q];
+    const T den = kc + ax + ay + az;
+    const T v = num / den;
+    In[c] = v;
+    Iprev = v;
+  }
+}
+
+// G = sum_n w_n I_n ; partial max |G - G_prev| and max G
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ w,
+        T* __restrict__ G, const T* __restrict__ Gp, double* __restrict__ pdiff,
+        double* __restrict__ pmax) {
+  __shared__ double sh1[256], sh2[256];
+  double md = 0.0, mg = -1e300;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256) {
+    T s = T(0);
+    for (int q = 0; q < M; ++q) s += T(w[q]) * Iset[(size_t)q * n + c];
+    G[c] = s;
+    double dd = fabs((double)s - (double)Gp[c]);
+    if (dd > md || dd != dd) md = dd;
+    if ((double)s > mg) mg = (double)s;
+  }
+  const int t = threadIdx.x;
+  sh1[t] = md;
+  sh2[t] = mg;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      if (sh1[t + s] > sh1[t] || sh1[t + s] != sh1[t + s]) sh1[t] = sh1[t + s];
+      if (sh2[t + s] > sh2[t]) sh2[t] = sh2[t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) { pdiff[blockIdx.x] = sh1[0]; pmax[blockIdx.x] = sh2[0]; }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dom_ground(int ncol, size_t n, int M, const double* __restrict__ Om,
+             const double* __restrict__ w, const T* __restrict__ Iset,
+             const T* __restrict__ emit, T* __restrict__ Ig_out,
+             T* __restrict__ q_in, T eps, T relax) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncol) return;
+  T q = T(0);
+  for (int m = 0; m < M; ++m) {
+    const double mu = Om[3 * m + 2];
+    if (mu < 0.0) q += T(w[m]) * T(fabs(mu)) * Iset[(size_t)m * n + c];
+  }
+  q_in[c] = q;
+  const T Inew = emit[c] + (T(1) - eps) / T(M_PI) * q;
+  Ig_out[c] = relax * Inew + (T(1) - relax) * Ig_out[c];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dom_out(GridArgs g, size_t n, const T* __restrict__ G, const T* __restrict__ kap,
+          const T* __restrict__ kap_s, const T* __restrict__ kap_safe,
+          const T* __restrict__ B, T* __restrict__ qs, T* __restrict__ qg) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const int k = (int)(c / ((size_t)g.nx * g.ny));
+  const T k_ = kap[c];
+  const T net = k_ * (G[c] - T(4.0 * M_PI) * B[c]);
+  const T net_h = net * T(g.dz[k]);
+  T fs = T(0), fg = T(0);
+  if (k_ > T(1.0e-9)) {
+    fs = kap_s[c] / kap_safe[c];
+    fg = (k_ - kap_s[c]) / kap_safe[c];
+  }
+  qs[c] = net_h * fs;
+  qg[c] = net_h * fg;
+}
+
+template <typename T>
+__global__ void k_scale_copy(int ncol, const T* __restrict__ a, T s,
+                             T* __restrict__ out) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < ncol) out[c] = s * a[c];
+}
+
+__global__ void __launch_bounds__(256)
+k_dom_fin(const double* __restrict__ pd, const double* __restrict__ pm, int np,
+          double* __restrict__ S) {
+  __shared__ double a[256], b[256];
+  const int t = threadIdx.x;
+  double x = 0.0, y = -1e300;
+  for (int i = t; i < np; i += 256) {
+    if (pd[i] > x || pd[i] != pd[i]) x = pd[i];
+    if (pm[i] > y) y = pm[i];
+  }
+  a[t] = x;
+  b[t] = y;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      if (a[t + s] > a[t] || a[t + s] != a[t + s]) a[t] = a[t + s];
+      if (b[t + s] > b[t]) b[t] = b[t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0) { S[0] = a[0]; S[1] = b[0]; }
+}
+
+static inline unsigned nb(size_t n) { return (unsigned)((n + 255) / 256); }
+
+template <typename T>
+static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
+                          const void* alpha, const void* omega, double sigma_sav,
+                          double T_amb, void* qs, void* qg, const void* Tsoil,
+                          void* q_soil, const void* Yw, const void* rho,
+                          const void* Mloc, hipStream_t s) {
+  const size_t n = h->n;
+  const int Nx = h->g.nx, Ny = h->g.ny, Nz = h->g.nz, M = h->M;
+  const int ncol = Nx * Ny;
+  const T I_amb = T(5.67e-8 * (T_amb * T_amb) * (T_amb * T_amb) / M_PI);
+  hipLaunchKernelGGL(k_dom_props<T>, dim3(nb(n)), dim3(256), 0, s, n, (const T*)Ts,
+                     (const T*)Tg, (const T*)alpha, (const T*)omega, (const T*)Yw,
+                     (const T*)rho, (const T*)Mloc, T(sigma_sav), T(T_amb),
+                     T(h->kappa_gas_max), (T*)h->kap, (T*)h->kap_s,
+                     (T*)h->kap_safe, (T*)h->B);
+  hipLaunchKernelGGL(k_dom_ground_init<T>, dim3(nb(ncol)), dim3(256), 0, s, ncol,
+                     (const T*)Tsoil, T(h->eps_ground), I_amb, (T*)h->Ig_emit,
+                     (T*)h->Ig_out);
+  FW_CHECK_LAUNCH("dom_props");
+  int it = 0;
+  for (; it < h->maxiter; ++it) {
+    // G_prev <- G (swap)
+    void* tmp = h->G;
+    h->G = h->Gp;
+    h->Gp = tmp;
+    for (int d = 0; d <= Ny + Nz - 2; ++d) {
+      const int jlo = d - (Nz - 1) > 0 ? d - (Nz - 1) : 0;
+      const int jhi = d < Ny - 1 ? d : Ny - 1;
+      const int nrows = jhi - jlo + 1;
+      hipLaunchKernelGGL(k_dom_sweep_diag<T>, dim3(nrows, M), dim3(256), 0, s,
+                         h->ga, d, jlo, (const double*)h->Om, (T*)h->I,
+                         (const T*)h->kap_safe, (const T*)h->B,
+                         (const T*)h->Ig_out, I_amb);
+    }
+    FW_CHECK_LAUNCH("dom_sweep");
+    hipLaunchKernelGGL(k_dom_G<T>, dim3(DOM_RED), dim3(256), 0, s, n, M,
+                       (const T*)h->I, (const double*)h->w, (T*)h->G,
+                       (const T*)h->Gp, h->part, h->part2);
+    hipLaunchKernelGGL(k_dom_ground<T>, dim3(nb(ncol)), dim3(256), 0, s, ncol, n, M,
+                       (const double*)h->Om, (const double*)h->w, (const T*)h->I,
+                       (const T*)h->Ig_emit, (T*)h->Ig_out, (T*)h->q_in,
+                       T(h->eps_ground), T(h->relax));
+    FW_CHECK_LAUNCH("dom_G");
+    if (it > 0) {
+      hipLaunchKernelGGL(k_dom_fin, dim3(1), dim3(256), 0, s, h->part, h->part2,
+                         DOM_RED, h->S);
+      FW_CHECK_HIP(hipMemcpyAsync(h->S_host, h->S, 2 * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+      FW_CHECK_HIP(hipStreamSynchronize(s));
+      double gmax = h->S_host[1] > 1e-12 ? h->S_host[1] : 1e-12;
+      double err = h->S_host[0] / gmax;
+      if (!(err == err)) return fw_set_error(FW_E_NONFINITE, "dom: non-finite G");
+      if (err < h->tol) { ++it; break; }
+    }
+  }
+  h->last_iters = it;
+  hipLaunchKernelGGL(k_dom_out<T>, dim3(nb(n)), dim3(256), 0, s, h->ga, n,
+                     (const T*)h->G, (const T*)h->kap, (const T*)h->kap_s,
+                     (const T*)h->kap_safe, (const T*)h->B, (T*)qs, (T*)qg);
+  if (q_soil)
+    hipLaunchKernelGGL(k_scale_copy<T>, dim3(nb(ncol)), dim3(256), 0, s, ncol,
+                       (const T*)h->q_in, T(h->eps_ground), (T*)q_soil);
+  FW_CHECK_LAUNCH("dom_out");
+  return FW_OK;
+}
+
+// Sum over forward ordinates (xi > 0) of w |xi| I_n in the masked cells
+// (flame_front_3d.py:594-630); mask is uint8 (Nz, Ny, Nx).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dom_fwd(size_t n, int M, const double* __restrict__ Om, const double* __restrict__ w,
+          const T* __restrict__ Iset, const unsigned char* __restrict__ mask,
+          T* __restrict__ out) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  T s = T(0);
+  if (!mask || mask[c]) {
+    for (int q = 0; q < M; ++q) {
+      const double xi = Om[3 * q];
+      if (xi > 0.0) s += T(w[q]) * T(xi) * Iset[(size_t)q * n + c];
+    }
+  }
+  out[c] = s;
+}
+
+}  // namespace fw
+
+using namespace fw;
+
+extern "C" {
+
+int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* weights,
+                  double eps_w_ground, int max_source_iter, double tol_source_iter,
+                  double omega_relax, double kappa_gas_max, void** handle) {
+  FW_REQUIRE(g && Omega && weights && handle && M > 0, "dom_create: bad argument");
+  FW_REQUIRE(g->nx <= 256 * 8, "dom_create: Nx > 2048 not supported by the row scan");
+  DomHandle* h = new DomHandle();
+  h->g = *g;
+  h->ga = grid_args(g);
+  h->M = M;
+  h->maxiter = max_source_iter;
+  h->eps_ground = eps_w_ground;
+  h->tol = tol_source_iter;
+  h->relax = omega_relax;
+  h->kappa_gas_max = kappa_gas_max;
+  h->n = (size_t)g->nx * g->ny * g->nz;
+  h->esz = g->dtype == FW_F64 ? 8 : 4;
+  h->Om_h.assign(Omega, Omega + 3 * M);
+  h->w_h.assign(weights, weights + M);
+  FW_CHECK_HIP(hipMalloc((void**)&h->Om, 3 * M * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->w, M * sizeof(double)));
+  FW_CHECK_HIP(hipMemcpy(h->Om, Omega, 3 * M * sizeof(double), hipMemcpyHostToDevice));
+  FW_CHECK_HIP(hipMemcpy(h->w, weights, M * sizeof(double), hipMemcpyHostToDevice));
+  const size_t n = h->n, ncol = (size_t)g->nx * g->ny;
+  FW_CHECK_HIP(hipMalloc(&h->I, (size_t)M * n * h->esz));
+  FW_CHECK_HIP(hipMemset(h->I, 0, (size_t)M * n * h->esz));
+  void** f[] = {&h->G, &h->Gp, &h->kap, &h->kap_s, &h->kap_safe, &h->B};
+  for (void** p : f) {
+    FW_CHECK_HIP(hipMalloc(p, n * h->esz));
+    FW_CHECK_HIP(hipMemset(*p, 0, n * h->esz));
+  }
+  void** c2[] = {&h->Ig_out, &h->Ig_emit, &h->q_in};
+  for (void** p : c2) FW_CHECK_HIP(hipMalloc(p, ncol * h->esz));
+  FW_CHECK_HIP(hipMalloc((void**)&h->part, DOM_RED * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->part2, DOM_RED * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->S, 4 * sizeof(double)));
+  FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, 4 * sizeof(double)));
+  *handle = h;
+  return FW_OK;
+}
+
+int fw_dom_destroy(void* handle) {
+  DomHandle* h = (DomHandle*)handle;
+  if (!h) return FW_OK;
+  void* b[] = {h->Om, h->w, h->I, h->G, h->Gp, h->kap, h->kap_s, h->kap_safe, h->B,
+               h->Ig_out, h->Ig_emit, h->q_in, h->part, h->part2, h->S};
+  for (void* p : b)
+    if (p) (void)hipFree(p);
+  if (h->S_host) (void)hipHostFree(h->S_host);
+  delete h;
+  return FW_OK;
+}
+
+int fw_dom_solve(void* handle, const void* T_s, const void* T_g, const void* alpha_s,
+                 const void* omega_comb, double sigma_sav, double T_amb,
+                 void* q_rad_solid_out, void* q_rad_gas_out, const void* T_soil_surface,
+                 void* q_in_soil_out, const void* Y_H2O, const void* rho,
+                 const void* bed_moisture, void* stream) {
+  DomHandle* h = (DomHandle*)handle;
+  FW_REQUIRE(h && T_s && T_g && alpha_s && omega_comb && q_rad_solid_out && q_rad_gas_out,
+             "dom_solve: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  const void* Yw = (Y_H2O && rho) ? Y_H2O : nullptr;
+  const void* rh = (Y_H2O && rho) ? rho : nullptr;
+  return h->g.dtype == FW_F64
+             ? dom_solve_impl<double>(h, T_s, T_g, alpha_s, omega_comb, sigma_sav, T_amb,
+                                      q_rad_solid_out, q_rad_gas_out, T_soil_surface,
+                                      q_in_soil_out, Yw, rh, bed_moisture, s)
+             : dom_solve_impl<float>(h, T_s, T_g, alpha_s, omega_comb, sigma_sav, T_amb,
+                                     q_rad_solid_out, q_rad_gas_out, T_soil_surface,
+                                     q_in_soil_out, Yw, rh, bed_moisture, s);
+}
+
+int fw_dom_stats(void* handle, int* iters) {
+  DomHandle* h = (DomHandle*)handle;
+  FW_REQUIRE(h, "dom_stats: null handle");
+  if (iters) *iters = h->last_iters;
+  return FW_OK;
+}
+
+int fw_dom_intensity(void* handle, void** I_set) {
+  DomHandle* h = (DomHandle*)handle;
+  FW_REQUIRE(h && I_set, "dom_intensity: null argument");
+  *I_set = h->I;
+  return FW_OK;
+}
+
+int fw_dom_forward_flux(void* handle, const unsigned char* mask, void* out, void* stream) {
+  DomHandle* h = (DomHandle*)handle;
+  FW_REQUIRE(h && out, "dom_forward_flux: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  if (h->g.dtype == FW_F64)
+    hipLaunchKernelGGL(k_dom_fwd<double>, dim3(nb(h->n)), dim3(256), 0, s, h->n, h->M,
+                       (const double*)h->Om, (const double*)h->w, (const double*)h->I,
+                       mask, (double*)out);
+  else
+    hipLaunchKernelGGL(k_dom_fwd<float>, dim3(nb(h->n)), dim3(256), 0, s, h->n, h->M,
+                       (const double*)h->Om, (const double*)h->w, (const float*)h->I,
+                       mask, (float*)out);
+  FW_CHECK_LAUNCH("dom_forward_flux");
+  return FW_OK;
+}
+
+}  // extern "C"

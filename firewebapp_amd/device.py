@@ -159,3 +159,17 @@ def grid_for(m: Marshal, shape, dx, dy, dz_arr, d_face_above, d_face_below,
     nz, ny, nx = shape
     return DeviceGrid(nx, ny, nz, dx, dy, dz_arr, d_face_above, d_face_below,
                       z_face=z_face, n_z_bed=n_z_bed, dtype=m.dtype)
+
+
+class _CudaArray:
+    def __init__(self, addr, shape, dtype):
+        typestr = {torch.float32: "<f4", torch.float64: "<f8",
+                   torch.int32: "<i4", torch.uint8: "|u1"}[dtype]
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": typestr,
+                                         "data": (int(addr), False), "version": 2,
+                                         "strides": None}
+
+
+def device_view(addr, shape, dtype):
+    """Zero-copy torch view of device memory owned by a libfirewave handle."""
+    return torch.as_tensor(_CudaArray(addr, shape, dtype), device=device())

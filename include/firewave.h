@@ -200,6 +200,39 @@ int fw_proj_set_preconditioner(void* handle, int enabled);
 int fw_proj_solve(void* handle, const void* rho, const double* b, double* x,
                   double rtol, int* iters, double* rel_resid, void* stream);
 
+/* ---- radiation (DOMRadiationSolver, dom_3d.py:227-473) ------------------ */
+
+/* Replaces DOMRadiationSolver.__init__ (dom_3d.py:235-284): M ordinates
+ * Omega (host, M x 3) and weights (host, M); allocates I_set (M x N, zero:
+ * the reference's initial warm start), G and the property scratch. */
+int fw_dom_create(const fw_grid* g, int M, const double* Omega,
+                  const double* weights, double eps_w_ground,
+                  int max_source_iter, double tol_source_iter,
+                  double omega_relax, double kappa_gas_max, void** handle);
+int fw_dom_destroy(void* handle);
+
+/* DOMRadiationSolver.solve (dom_3d.py:286-473).  Y_H2O/rho, T_soil_surface,
+ * q_in_soil_out and bed_moisture may be NULL (reference: None). */
+int fw_dom_solve(void* handle, const void* T_s, const void* T_g,
+                 const void* alpha_s, const void* omega_comb, double sigma_sav,
+                 double T_amb, void* q_rad_solid_out, void* q_rad_gas_out,
+                 const void* T_soil_surface, void* q_in_soil_out,
+                 const void* Y_H2O, const void* rho, const void* bed_moisture,
+                 void* stream);
+
+/* Source iterations of the last solve. */
+int fw_dom_stats(void* handle, int* iters);
+
+/* Device pointer of I_set (M, Nz, Ny, Nx) in the grid's type (read by
+ * flame_front_3d.compute_q_dom_fwd_at_band, flame_front_3d.py:594-630). */
+int fw_dom_intensity(void* handle, void** I_set);
+
+/* out = sum over ordinates with xi > 0 of w |xi| I_n where mask != 0
+ * (flame_front_3d.compute_q_dom_fwd_at_band, flame_front_3d.py:594-630);
+ * mask: uint8 field or NULL (all cells). */
+int fw_dom_forward_flux(void* handle, const unsigned char* mask, void* out,
+                        void* stream);
+
 /* ---- combustion ---------------------------------------------------------- */
 
 /* Number of int32 workspace entries fw_edc needs for grid g. */

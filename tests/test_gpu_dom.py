@@ -1,0 +1,80 @@
+"""GPU parity of the DOM solve through the C ABI.
+
+FP64: the golden vectors (S4) at 1e-10 relative on q_rad_solid, q_rad_gas
+and q_in_soil (the row scan re-associates the x recurrence, so agreement is
+to rounding, not bits).  S8 has no reference: the GPU solver is compared
+with the oracle's literal-order sweep at 1e-10 (FP64) on a seeded Ny > 1
+case with warm-started repeat solves.  FP32 at 2e-4 relative (max norm).
+"""
+import numpy as np
+import pytest
+
+from oracle import dom as D
+from tests.golden_io import arr, case_ids, cases, field, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(*a, **k):
+    from firewebapp_amd.physics_3d import dom_3d
+    return dom_3d.DOMRadiationSolver(*a, **k)
+
+
+def _run_case(c, dtype):
+    import torch
+    sh = (c["nz"], c["ny"], c["nx"])
+    dz = arr(c, "dz_arr")
+    s = _solver(*sh, c["dy"], c["dx"], dz, dz, dz, "periodic", 4)
+    cv = (lambda a: a) if dtype == np.float64 else \
+        (lambda a: torch.as_tensor(a.astype(np.float32), device="cuda"))
+    qs, qg = cv(np.zeros(sh)), cv(np.zeros(sh))
+    qsoil = cv(np.zeros((c["ny"], c["nx"])))
+    YH = cv(field(c, "Y_H2O")) if c["with_h2o"] else None
+    rho = cv(field(c, "rho")) if c["with_h2o"] else None
+    s.solve(cv(field(c, "T_s")), cv(field(c, "T_g")), cv(field(c, "alpha_s")),
+            cv(field(c, "omega_comb")), c["sigma_sav"], c["T_amb"], qs, qg,
+            cv(field(c, "T_soil", (c["ny"], c["nx"]))), qsoil, YH, rho,
+            cv(field(c, "bed_moisture")))
+    back = (lambda a: a) if dtype == np.float64 else (lambda a: a.double().cpu().numpy())
+    return back(qs), back(qg), back(qsoil)
+
+
+@pytest.mark.parametrize("c", cases("dom"), ids=case_ids("dom"))
+def test_dom_golden_fp64(c):
+    qs, qg, qsoil = _run_case(c, np.float64)
+    assert relerr(qs, field(c, "q_rad_solid")) <= 1e-10
+    assert relerr(qg, field(c, "q_rad_gas")) <= 1e-10
+    assert relerr(qsoil, field(c, "q_in_soil", (c["ny"], c["nx"]))) <= 1e-10
+
+
+@pytest.mark.parametrize("c", cases("dom"), ids=case_ids("dom"))
+def test_dom_golden_fp32(c):
+    qs, qg, qsoil = _run_case(c, np.float32)
+    assert relerr(qs, field(c, "q_rad_solid")) <= 2e-4
+    assert relerr(qg, field(c, "q_rad_gas")) <= 2e-4
+    assert relerr(qsoil, field(c, "q_in_soil", (c["ny"], c["nx"]))) <= 2e-4
+
+
+@pytest.mark.parametrize("N", [4, 8])
+def test_dom_vs_oracle_warm_repeat(N):
+    rng = np.random.default_rng(77)
+    sh = (9, 5, 37)
+    dz = np.concatenate([np.full(3, 0.042), np.full(6, 0.1)])
+    alpha = np.zeros(sh)
+    alpha[:3] = 0.002 + 0.01 * rng.random((3, 5, 37))
+    T_s = 300.0 + 800.0 * rng.random(sh) * (alpha > 0)
+    T_g = 300.0 + 1200.0 * rng.random(sh)
+    om = 5e-3 * rng.random(sh)
+    Tsoil = 300.0 + 50.0 * rng.random((5, 37))
+    ref = D.DOMRadiationSolver(*sh, 0.1, 0.1, dz, N_quadrature=N)
+    gpu = _solver(*sh, 0.1, 0.1, dz, dz, dz, "periodic", N)
+    for rep in range(2):
+        T_g = T_g * (1.0 + 0.05 * rep)
+        out = []
+        for s in (ref, gpu):
+            qs, qg, qsoil = np.zeros(sh), np.zeros(sh), np.zeros((5, 37))
+            s.solve(T_s, T_g, alpha, om, 2000.0, 300.0, qs, qg, Tsoil, qsoil)
+            out.append((qs, qg, qsoil))
+        assert gpu.last_iters == ref.last_iters
+        for a, b in zip(out[1], out[0]):
+            assert relerr(a, b) <= 1e-10
