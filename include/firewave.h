@@ -233,6 +233,79 @@ int fw_dom_intensity(void* handle, void** I_set);
 int fw_dom_forward_flux(void* handle, const unsigned char* mask, void* out,
                         void* stream);
 
+/* ---- Lagrangian bed (lagrangian_bed_3d.py:193-1070) --------------------- */
+
+/* The particle buffer dict of allocate_bed_particle_buffers
+ * (lagrangian_bed_3d.py:193-218) as device float64 SoA + int8 alive. */
+typedef struct fw_bed_particles {
+  double *x, *y, *z, *m_solid, *m_water, *m_char, *T_s, *m_water_0, *sav,
+      *m_char_max;
+  int8_t* alive;
+  int32_t n;
+  int32_t reserved;
+} fw_bed_particles;
+
+/* Particle -> cell map and its inverse: cell[p] (flat (k,j,i) index or -1)
+ * and, for cells 0..n_seg_cells-1, the particles of each cell in slot order
+ * (seg_idx[seg_ptr[c] .. seg_ptr[c+1]-1]).  Cells >= n_seg_cells hold no
+ * particle. */
+typedef struct fw_bed_index {
+  const int32_t* cell;
+  const int32_t* seg_ptr;
+  const int32_t* seg_idx;
+  int32_t n_seg_cells;
+  int32_t reserved;
+} fw_bed_index;
+
+/* Scalar arguments of step_bed_particles (lagrangian_bed_3d.py:593-676). */
+typedef struct fw_bed_params {
+  double dt, h_conv, rho_solid_true, cp_solid, eps_solid, T_amb, view_factor,
+      h_bed, kappa_bed_eff, char_ox_flux_cap, char_ox_ash_exp;
+  int32_t n_per_cell_for_split, view_factor_geometric, do_drying,
+      do_pyrolysis, do_char_ox, do_smolder, drying_mode, reserved;
+} fw_bed_params;
+
+/* Cell of every particle, float64, in both of the reference's conventions:
+ * cell_step = locate_cell (lagrangian_particles_3d.py:111-138, x < 0 -> out
+ * before truncation), cell_agg = int() then range test (the aggregators and
+ * the conduction scatter, lagrangian_bed_3d.py:490-497, :528-534). */
+int fw_bed_locate(const fw_grid* g, int n, const double* x, const double* y,
+                  const double* z, int32_t* cell_step, int32_t* cell_agg,
+                  void* stream);
+
+/* step_bed_particles (lagrangian_bed_3d.py:593-1070).  index: built from
+ * cell_step.  sources[8]: S_pyro, S_drying, Q_pyro, Q_drying, Y_F_source,
+ * Q_char, Q_smold, Q_g_conv (grid type); cells 0..n_seg_cells-1 are written,
+ * the caller keeps the rest zero.  contrib_work: 10*n doubles.
+ * counters: device int[2] = n_alive, n_burned.  diag_work (15*n doubles) and
+ * diag_out (device double[16]) may be NULL; when given, diag_out receives
+ * the reference's diag_max_out. */
+int fw_bed_step(const fw_grid* g, const fw_bed_particles* particles,
+                const fw_bed_index* index, const fw_bed_params* params,
+                const void* T_g, const void* Y_O2, const void* Q_solid_ext,
+                void* const* sources, double* contrib_work, int32_t* counters,
+                double* diag_work, double* diag_out, void* stream);
+
+/* aggregate_particles_to_T_s_grid (lagrangian_bed_3d.py:500-546) and/or
+ * aggregate_particles_to_M_local_grid (:549-590); either output may be NULL.
+ * agg_index: built from cell_agg. */
+int fw_bed_aggregate(const fw_grid* g, const fw_bed_particles* particles,
+                     const fw_bed_index* agg_index, void* T_s_grid,
+                     void* M_local_grid, void* stream);
+
+/* step_horizontal_solid_conduction_scatter (lagrangian_bed_3d.py:427-497):
+ * T_s_grid (in/out) updated on alpha_s > 0 bed cells from T_s_old (a copy
+ * of T_s_grid on entry); every alive particle of a bed cell gets the cell's
+ * dT.  dT_work: N doubles. */
+int fw_bed_horizontal_conduction(const fw_grid* g,
+                                 const fw_bed_particles* particles,
+                                 const fw_bed_index* agg_index,
+                                 const void* T_s_old, void* T_s_grid,
+                                 const void* alpha_s, double k_solid,
+                                 double rho_solid_true, double cp_solid,
+                                 int n_z_bed, double dt, double* dT_work,
+                                 void* stream);
+
 /* ---- combustion ---------------------------------------------------------- */
 
 /* Number of int32 workspace entries fw_edc needs for grid g. */

@@ -124,9 +124,11 @@ def test_projection_fire_case_vs_oracle(dtype):
     it1 = sol._last_fft_pcg_iters
     assert 1 <= it1 <= 60
     p = p.cpu().numpy()
-    # same pressure up to the rtol ball, same velocity up to storage precision
+    # same pressure up to the rtol ball; velocities differ by dt grad(dp)/rho
+    # with ||A dp|| <= 1e-6 ||b|| (the oracle is solved to 1e-10), plus
+    # float32 storage rounding in the FP32 case
     assert relerr(p, p_ref) <= 1e-4
-    tol_u = 1e-9 if dtype == np.float64 else 2e-6
+    tol_u = 2e-6 if dtype == np.float64 else 6e-6
     for a, b in ((U, ur), (V, vr), (W, wr)):
         assert relerr(a.double().cpu().numpy(), b) <= tol_u
     # warm start: projecting the corrected field again needs ~no work
