@@ -70,7 +70,8 @@ __global__ void __launch_bounds__(256)
 k_species(GridArgs g, const T* __restrict__ Y, T* __restrict__ Yo,
           const T* __restrict__ rho, const T* __restrict__ u,
           const T* __restrict__ v, const T* __restrict__ w,
-          const T* __restrict__ S, T dt, T D, T Y_inlet, T y_max) {
+          const T* __restrict__ S, const T* __restrict__ Sp,
+          const T* __restrict__ Sd, int kind, T dt, T D, T Y_inlet, T y_max) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k = blockIdx.z;
   if (i >= g.nx) return;
@@ -116,7 +117,17 @@ k_species(GridArgs g, const T* __restrict__ Y, T* __restrict__ Yo,
   T d2y = (Yjp1 - T(2) * Yi + Yjm1) * inv_dy2;
   T d2z = ((YR_z - Yi) * inv_da - (Yi - YL_z) * inv_db) * inv_dz;
   T diff = D * (d2x + d2y + d2z);
-  T src = S[c] / rho[c];
+  // effective source: given, or the dilution form of spread_3d.py:2990-3004
+  // (S_tot = Sp + Sd; fuel Sp - Y S_tot, O2 -Y S_tot, H2O Sd - Y S_tot)
+  T Se;
+  if (kind == 0) {
+    Se = S[c];
+  } else {
+    const T sp = Sp[c], sd = Sd[c];
+    const T st = sp + sd;
+    Se = kind == 1 ? sp - Yi * st : (kind == 2 ? -Yi * st : sd - Yi * st);
+  }
+  T src = Se / rho[c];
   T dY = (adv + diff + src) * dt;
   T nv = Yi + dY;
   if (nv < T(0)) nv = T(0);
@@ -485,13 +496,14 @@ static int advect_impl(const fw_grid* g, const void* phi, const void* u,
 template <typename T>
 static int species_impl(const fw_grid* g, const void* Y, void* Yo,
                         const void* rho, const void* u, const void* v,
-                        const void* w, const void* S, double dt, double D,
+                        const void* w, const void* S, const void* Sp,
+                        const void* Sd, int kind, double dt, double D,
                         double Y_inlet, double y_max, hipStream_t s) {
   Launch L = cell_launch(g);
   hipLaunchKernelGGL(k_species<T>, L.grid, L.block, 0, s, grid_args(g),
                      (const T*)Y, (T*)Yo, (const T*)rho, (const T*)u,
-                     (const T*)v, (const T*)w, (const T*)S, T(dt), T(D),
-                     T(Y_inlet), T(y_max));
+                     (const T*)v, (const T*)w, (const T*)S, (const T*)Sp,
+                     (const T*)Sd, kind, T(dt), T(D), T(Y_inlet), T(y_max));
   FW_CHECK_LAUNCH("species_transport");
   return FW_OK;
 }
@@ -627,9 +639,22 @@ int fw_species_transport(const fw_grid* g, const void* Y, void* Y_out,
                          const void* rho, const void* u, const void* v,
                          const void* w, const void* S, double dt, double D,
                          double Y_inlet, double y_max, void* stream) {
-  FW_REQUIRE(g && Y != Y_out, "species_transport: Y_out must not alias Y");
-  return FW_DISPATCH(g, species_impl, g, Y, Y_out, rho, u, v, w, S, dt, D,
-                     Y_inlet, y_max, (hipStream_t)stream);
+  FW_REQUIRE(g && Y != Y_out && S, "species_transport: Y_out must not alias Y");
+  return FW_DISPATCH(g, species_impl, g, Y, Y_out, rho, u, v, w, S, nullptr,
+                     nullptr, 0, dt, D, Y_inlet, y_max, (hipStream_t)stream);
+}
+
+int fw_species_transport_dilution(const fw_grid* g, const void* Y, void* Y_out,
+                                  const void* rho, const void* u, const void* v,
+                                  const void* w, const void* S_pyro_part,
+                                  const void* S_dry_part, int kind, double dt,
+                                  double D, double Y_inlet, double y_max,
+                                  void* stream) {
+  FW_REQUIRE(g && Y != Y_out && S_pyro_part && S_dry_part && kind >= 1 && kind <= 3,
+             "species_transport_dilution: bad argument");
+  return FW_DISPATCH(g, species_impl, g, Y, Y_out, rho, u, v, w, nullptr,
+                     S_pyro_part, S_dry_part, kind, dt, D, Y_inlet, y_max,
+                     (hipStream_t)stream);
 }
 
 int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,

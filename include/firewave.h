@@ -68,6 +68,17 @@ int fw_species_transport(const fw_grid* g, const void* Y, void* Y_out,
                          const void* w, const void* S, double dt, double D,
                          double Y_inlet, double y_max, void* stream);
 
+/* step_species_transport with the multi-species dilution source of the
+ * time step (spread_3d.py:2990-3038) computed in the kernel: with
+ * S_tot = S_pyro_part + S_dry_part, kind 1 (fuel) S = S_pyro_part - Y S_tot,
+ * kind 2 (O2) S = -Y S_tot, kind 3 (H2O) S = S_dry_part - Y S_tot. */
+int fw_species_transport_dilution(const fw_grid* g, const void* Y, void* Y_out,
+                                  const void* rho, const void* u, const void* v,
+                                  const void* w, const void* S_pyro_part,
+                                  const void* S_dry_part, int kind, double dt,
+                                  double D, double Y_inlet, double y_max,
+                                  void* stream);
+
 /* _adv_gas_energy (spread_3d.py:3827-3897): MUSCL advection with inlet ghost
  * T_amb + alpha_th FV diffusion, then clamp to [T_amb, 1e4]; out of place. */
 int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
@@ -305,6 +316,68 @@ int fw_bed_horizontal_conduction(const fw_grid* g,
                                  double rho_solid_true, double cp_solid,
                                  int n_z_bed, double dt, double* dT_work,
                                  void* stream);
+
+/* ---- time-step glue (spread_3d.py:1897-3683) ----------------------------- */
+
+/* rho_prev <- rho; rho <- P0/(R max(T_g, T_amb)); Y_O2[:, :, 0] <- 0.232
+ * (spread_3d.py:1984-1987, :2042, :2049-2050). */
+int fw_step_prologue(const fw_grid* g, const void* T_g, void* rho,
+                     void* rho_prev, void* Y_O2, double T_amb, void* stream);
+
+/* Q_solid_ext on bed layers k < n_z_bed (spread_3d.py:2078-2103): drip_W_m3
+ * on columns [i_src0, i_src1) plus clip(q_rad/dz, +-q_rad_max). */
+int fw_bed_external_heat(const fw_grid* g, const void* q_rad, void* Q_ext,
+                         int n_z_bed, int i_src0, int i_src1, double drip_W_m3,
+                         double q_rad_max, void* stream);
+
+/* S_pyro = S_pyro_part + S_dry_part; div_target = (S_pyro - (rho -
+ * rho_prev)/dt)/max(rho, 0.1)  (spread_3d.py:2147-2154, :2300-2317). */
+int fw_div_target(const fw_grid* g, const void* S_pyro_part,
+                  const void* S_dry_part, const void* rho, const void* rho_prev,
+                  double dt, void* S_pyro, void* div_target, void* stream);
+
+/* Sub-step gas energy update of the particle path (spread_3d.py:2916-2957,
+ * :3104-3105, :3129-3146), in place on T_g; Q_comb_out may be NULL. */
+int fw_gas_energy_update(const fw_grid* g, void* T_g, const void* T_s,
+                         const void* rho, const void* Y_H2O,
+                         const void* S_pyro_part, const void* S_dry_part,
+                         const void* Q_g_conv, const void* q_rad_gas,
+                         void* Q_comb_out, int n_z_bed, int i_src0, int i_src1,
+                         double drip_W_m3, double dt_sub, double T_amb,
+                         void* stream);
+
+/* field[k, :, i0:i1] += add  (ignition pulse, spread_3d.py:2528-2532). */
+int fw_add_patch(const fw_grid* g, void* field, int k, int i0, int i1,
+                 double add, void* stream);
+
+/* Flame-body mask = (omega > 1e-3) | (T_g > 1000 & Y_F > 1e-3), identical to
+ * compute_phi_flame_from_state(...) <= 0 (flame_front_3d.py:756-813); and
+ * update_cell_age (flame_front_3d.py:816-835) when cell_age != NULL.
+ * mask: uint8 field or NULL. */
+int fw_flame_mask_age(const fw_grid* g, const void* omega, const void* T_g,
+                      const void* Y_fuel, void* cell_age, uint8_t* mask,
+                      double dt, void* stream);
+
+/* All per-step scalars in one pass (dt inputs, NaN check, diagnostics of
+ * spread_3d.py:1906-1941, :3503-3512, front tracking :3743-3777).
+ * out (device double[14]): max|u|, max|v|, max|w|, max nu_t, max T_g,
+ * T_g non-finite flag, max T_s, max S_pyro, max Q_comb, max omega,
+ * max Y_fuel, count(T_s >= T_ign), count(flame_mask), last burning column i
+ * (T_s >= T_front on alpha_s > 0 in k < n_z_bed; -1 if none).  S_pyro,
+ * Q_comb, flame_mask may be NULL.  work: 14 * 256 doubles. */
+int fw_step_reduce(const fw_grid* g, const void* u, const void* v,
+                   const void* w, const void* nu_t, const void* T_g,
+                   const void* T_s, const void* S_pyro, const void* Q_comb,
+                   const void* omega, const void* Y_fuel, const void* alpha_s,
+                   const uint8_t* flame_mask, double T_ign, double T_front,
+                   int n_z_bed, double* work, double* out, void* stream);
+
+/* LevelSetFront3D (flame_front_3d.py:113-360): mode 0 = evolve,
+ * phi_out = phi - step v_n |grad phi|_Godunov; mode 1 = one Sussman reinit
+ * substep, phi_out = phi - step sign(phi0)(|grad phi| - 1). */
+int fw_levelset_update(const fw_grid* g, const void* phi, const void* phi0,
+                       const void* v_n, void* phi_out, double step, int mode,
+                       void* stream);
 
 /* ---- combustion ---------------------------------------------------------- */
 
