@@ -123,8 +123,8 @@ k_flame_mask_age(size_t n, const T* __restrict__ om, const T* __restrict__ Tg,
 // Per-step reductions in one pass.  Output slots:
 //  0 max|u|  1 max|v|  2 max|w|  3 max nu_t  4 max T_g  5 T_g non-finite (0/1)
 //  6 max T_s  7 max S_pyro  8 max Q_comb  9 max omega  10 max Y_fuel
-// 11 count(T_s >= T_ign)  12 count(flame mask)  13 last i with a burning bed
-// column (T_s >= T_front on alpha_s > 0, k < nzb), -1 if none
+// 11 count(T_s >= T_ign)  12 count of flame-active cells  13 last i with a
+// burning bed column (T_s >= T_front on alpha_s > 0, k < nzb), -1 if none
 #define NRED 14
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -133,8 +133,8 @@ k_step_reduce(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict
               const T* __restrict__ Tg, const T* __restrict__ Ts,
               const T* __restrict__ Sp, const T* __restrict__ Qc,
               const T* __restrict__ om, const T* __restrict__ YF,
-              const T* __restrict__ alpha, const unsigned char* __restrict__ mask,
-              double T_ign, double T_front, int nzb, double* __restrict__ part) {
+              const T* __restrict__ alpha, double T_ign, double T_front, int nzb,
+              double* __restrict__ part) {
   __shared__ double sh[NRED][256];
   double r[NRED];
   for (int q = 0; q < NRED; ++q) r[q] = -1e300;
@@ -160,7 +160,7 @@ k_step_reduce(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict
     const double y = (double)YF[c];
     r[10] = y > r[10] ? y : r[10];
     if (ts >= T_ign) r[11] += 1.0;
-    if (mask && mask[c]) r[12] += 1.0;
+    if (o > 1.0e-3 || (tg > 1000.0 && y > 1.0e-3)) r[12] += 1.0;
     const int k = (int)(c / plane);
     if (k < nzb && ts >= T_front && (double)alpha[c] > 0.0) {
       const double i = (double)(c % g.nx);
@@ -401,8 +401,8 @@ int fw_flame_mask_age(const fw_grid* g, const void* omega, const void* T_g,
 int fw_step_reduce(const fw_grid* g, const void* u, const void* v, const void* w,
                    const void* nu_t, const void* T_g, const void* T_s,
                    const void* S_pyro, const void* Q_comb, const void* omega,
-                   const void* Y_fuel, const void* alpha_s, const uint8_t* flame_mask,
-                   double T_ign, double T_front, int n_z_bed, double* work,
+                   const void* Y_fuel, const void* alpha_s, double T_ign,
+                   double T_front, int n_z_bed, double* work,
                    double* out, void* stream) {
   FW_REQUIRE(g && u && v && w && nu_t && T_g && T_s && omega && Y_fuel && alpha_s &&
                  work && out,
@@ -416,14 +416,14 @@ int fw_step_reduce(const fw_grid* g, const void* u, const void* v, const void* w
                        (const double*)nu_t, (const double*)T_g, (const double*)T_s,
                        (const double*)S_pyro, (const double*)Q_comb,
                        (const double*)omega, (const double*)Y_fuel,
-                       (const double*)alpha_s, flame_mask, T_ign, T_front, n_z_bed, work);
+                       (const double*)alpha_s, T_ign, T_front, n_z_bed, work);
   else
     hipLaunchKernelGGL(k_step_reduce<float>, dim3(nb), dim3(256), 0, s, grid_args(g), n,
                        (const float*)u, (const float*)v, (const float*)w,
                        (const float*)nu_t, (const float*)T_g, (const float*)T_s,
                        (const float*)S_pyro, (const float*)Q_comb, (const float*)omega,
-                       (const float*)Y_fuel, (const float*)alpha_s, flame_mask, T_ign,
-                       T_front, n_z_bed, work);
+                       (const float*)Y_fuel, (const float*)alpha_s, T_ign, T_front,
+                       n_z_bed, work);
   hipLaunchKernelGGL(k_step_reduce_fin, dim3(NRED), dim3(256), 0, s, (const double*)work,
                      nb, out);
   FW_CHECK_LAUNCH("step_reduce");

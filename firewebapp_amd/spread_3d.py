@@ -1,12 +1,68 @@
-"""GPU time-step driver of the Tier-3 spread (spread_3d.py of the reference).
+"""GPU driver of the Tier-3 spread time step: ``run_3d_spread`` of the
+reference (spread_3d.py:563-3729) for the Lagrangian-bed / k-epsilon / EDC /
+DOM / passive-level-set configuration, with every field resident in device
+memory.
 
-Helpers with reference names live here too (``_adv_gas_energy``).
+Precision: the grid fields are float32 by default (deck key
+``outdoor.dtype = float64`` selects the float64 parity build); bed particles,
+the level set, the pressure warm start and every reduction stay float64.
+
+Each operation is one call into libfirewave.  The host keeps only the
+reference's scalar decisions (dt, the projection loop, the turbulent
+diffusion sub-step count, exits), read back from one fused reduction per
+step plus the projection and nu_t scalars.
+
+Work the reference computes but never consumes on this path is skipped; the
+skip is exact because nothing downstream reads it: the O2-supply limiter and
+omega_max_T (ignored by the EDC closure), tau_mix, burning_mask, Q_pyro,
+q_dom_fwd / q_in_at_front (passive level set), update_cell_age (read only by
+the ebu_bootstrap closure), the level-set evolve with v_n = 0 (the
+identity), and the distance transforms of compute_phi_flame_from_state: the
+flame-body mask with band 0 is exactly the `active` predicate
+(flame_front_3d.py:756-813), and its count for step n is the count of the
+state the previous step ended with (omega, T_g and Y_fuel are not written
+between the end of a step and that point of the next one).
 """
 from __future__ import annotations
 
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
 import torch
 
-from .physics_3d._common import Marshal, call, grid_for, ptr, stream_ptr
+from . import _lib
+from .deck import load_text_input, outdoor_env_from_dict
+from .device import DeviceGrid, Marshal, device, grid_for, ptr, stream_ptr
+from .grid import Grid3D, build_u_inlet, midflame_wind_speed
+from .physics_3d import dom_3d as _dom
+from .physics_3d import lagrangian_bed_3d as _lb
+from .physics_3d import turbulence_3d as _turb
+from .physics_3d.fft_poisson_3d import create_handle as _proj_handle
+from .physics_3d.soil_3d import (CP_SOIL_DEFAULT, EPS_SOIL_DEFAULT, K_SOIL_DEFAULT,
+                                 RHO_SOIL_DEFAULT, build_soil_grid)
+
+_P0 = 101_325.0
+_R_AIR = 8.314 / 0.029
+_G = 9.81
+_RHO_PARTICLE = 500.0
+_RHO_SOLID = 500.0          # coupling_3d.py:50
+_CP_SOLID = 1300.0          # coupling_3d.py:51
+K_SOLID_GRASS = 0.20        # solid_conduction_3d.py:52
+Q_DRIP_PER_AREA = 30_000.0  # spread_3d.py:2088, :2945
+F_DRIP_TO_SOLID = 0.80      # spread_3d.py:2090, :2974
+Q_RAD_MAX = 1.0e5           # spread_3d.py:2113
+Q_IGNITION_PULSE = 240_000.0  # spread_3d.py:2531
+N_SUB = 10                  # spread_3d.py:2678
+T_CAP = 10_000.0            # spread_3d.py:2650-2651
+L_BURNOUT_M = 0.50          # flame_front_3d.py:62
+LEVELSET_REINIT_INTERVAL = 10
+REINIT_SUBSTEPS = 5
+DT_REINIT_FRAC = 0.5
+T_BOOTSTRAP_S = 2.0         # flame_front_3d.py:523
 
 
 def _adv_gas_energy(T_g, u, v, w, dt, dx, dy, dz_arr, d_face_above,
@@ -22,7 +78,746 @@ def _adv_gas_energy(T_g, u, v, w, dt, dx, dy, dz_arr, d_face_above,
     g = grid_for(m, tuple(T_g.shape), dx, dy, dz_arr, d_face_above, d_face_below)
     Tg = m.out(T_g)
     To = torch.empty_like(Tg)
-    call("fw_gas_energy", g.ref, ptr(Tg), ptr(To), ptr(m.dev(u)), ptr(m.dev(v)),
-         ptr(m.dev(w)), float(dt), float(alpha_th), float(T_amb), stream_ptr())
+    _lib.call("fw_gas_energy", g.ref, ptr(Tg), ptr(To), ptr(m.dev(u)), ptr(m.dev(v)),
+              ptr(m.dev(w)), float(dt), float(alpha_th), float(T_amb), stream_ptr())
     Tg.copy_(To)
     m.writeback()
+
+
+# ---------------------------------------------------------------------------
+# configuration (spread_3d.py:959-1125)
+# ---------------------------------------------------------------------------
+
+# deck switches whose other values select a branch this driver does not
+# implement: (deck key, the one accepted value, reference default)
+_REQUIRED = (
+    ("combustion_closure", "edc", "edc"),
+    ("radiation_solver", "dom", "dom"),
+    ("boundary_condition_kind", "outdoor_wind", "outdoor_wind"),
+    ("chemistry_family", "biomass", "biomass"),
+    ("lagrangian_bed_enable", True, False),
+    ("level_set_passive", True, False),
+    ("ignition_T_pin_enable", False, False),
+    ("edc_extinction_enable", False, False),
+    ("finney_tendril_enable", False, False),
+    ("empirical_ros_enable", False, False),
+    ("moisture_jump_enable", False, False),
+)
+_OUTDOOR_OFF = ("sem_enable", "fuel_pert_enable", "volume_weighted_projection",
+                "rothermel_packing_enable", "free_burning_mode")
+
+
+@dataclass
+class Spread3DConfig:
+    """Resolved parameters of one run.  Mesh keys resolve keyword -> deck ->
+    default (`_pick`, spread_3d.py:972-1000); case-spec keys deck -> keyword
+    -> default (`_deck_first`, :1034-1085)."""
+    wind_speed_m_s: float
+    Lx: float
+    Ly: float
+    Lz: float
+    dx: float
+    dy: float
+    n_z_bed: int
+    mesh_kwargs: dict
+    bed_x_start: float
+    bed_x_end: float
+    cfl_factor: float
+    proj_max_iter: int
+    proj_div_tol: float
+    h_bed: float
+    rho_b: float
+    T_amb: float
+    sigma_sav: float
+    moisture: float
+    ignition_duration_s: float
+    terrain: str
+    canopy_C_d: float
+    canopy_beta_p: float
+    canopy_beta_d: float
+    dom_subcycle_every: int
+    dom_quadrature: int
+    n_per_cell: int
+    drying_mode: int
+    h_conv: float
+    rho_solid_true: float
+    cp_solid: float
+    eps_solid: float
+    view_factor: float
+    view_factor_geometric: bool
+    char_ox_flux_cap: float
+    char_ox_ash_exp: float
+    solid_phase_ignition: bool
+    solid_phase_T_s_K: float
+    ignition_q_mult: float
+    ignition_width_mult: float
+    min_dt_s: float
+    max_wall_time_s: float
+    cg_rtol: float
+    n_steps: Optional[int]
+    dtype: torch.dtype
+    particle_jitter: float
+    particle_jitter_seed: int
+    spin_up_s: float
+
+    @classmethod
+    def resolve(cls, ri_or_path, wind_speed_m_s=0.0, **kw):
+        ri = load_text_input(ri_or_path) if isinstance(ri_or_path, (str, Path)) else ri_or_path
+        ovr = ri.outdoor_overrides
+        ocfg = outdoor_env_from_dict(ovr)
+        ocfg.wind_speed_m_s = wind_speed_m_s
+
+        def deck(key):
+            for k in (key, key.lower()):
+                if k in ovr and ovr[k] is not None:
+                    return ovr[k]
+            return None
+
+        def pick(name, default):
+            v = kw.pop(name, None)
+            if v is not None:
+                return v
+            d = deck(name)
+            return default if d is None else d
+
+        def deck_first(name, default):
+            v = kw.pop(name, None)
+            d = deck(name)
+            if d is not None:
+                return d
+            return default if v is None else v
+
+        Lx = float(pick("Lx", 10.0))
+        Ly = float(pick("Ly", 2.0))
+        Lz = float(pick("Lz", 1.5))
+        dx = float(pick("dx", 0.05))
+        n_z_bed = int(pick("n_z_bed", 4))
+        mesh = dict(
+            dz_expansion=float(pick("dz_expansion", 1.0)),
+            bl_growth=float(pick("bl_growth", 1.0)),
+            wall_bl_N=int(pick("wall_bl_N", 0)),
+            wall_bl_first_dz=float(pick("wall_bl_first_dz", 0.0)),
+            wall_bl_growth=float(pick("wall_bl_growth", 1.3)),
+            bed_top_inner_bl_N=int(pick("bed_top_inner_bl_N", 0)),
+            bed_top_inner_bl_first_dz=float(pick("bed_top_inner_bl_first_dz", 0.0)),
+            bed_top_inner_bl_growth=float(pick("bed_top_inner_bl_growth", 1.3)),
+            bed_top_outer_bl_N=int(pick("bed_top_outer_bl_N", 0)),
+            bed_top_outer_bl_first_dz=float(pick("bed_top_outer_bl_first_dz", 0.0)),
+            bed_top_outer_bl_growth=float(pick("bed_top_outer_bl_growth", 1.3)),
+            atm_growth=float(pick("atm_growth", 1.3)),
+        )
+        for opt in ("dz_first", "atm_max_dz", "atm_uniform_dz"):
+            v = pick(opt, None)
+            mesh[opt] = float(v) if v is not None else None
+        cfl = float(pick("cfl_factor", 0.4))
+        if bool(pick("wall_function", False)):
+            raise ValueError("wall_function=true is not implemented on the GPU step")
+        pmi, pdt = deck("proj_max_iter"), deck("proj_div_tol")
+        proj_max_iter = int(pmi) if pmi is not None else 50
+        proj_div_tol = float(pdt) if pdt is not None else 1.0e-3
+        bed_x_start = float(pick("bed_x_start", 0.0))
+        bed_x_end = float(pick("bed_x_end", Lx))
+        dy_kw = kw.pop("dy", None)
+        dy_d = deck("dy")
+        dy = float(dy_d if dy_d is not None else (dy_kw if dy_kw is not None else dx))
+        n_z_bed = int(deck_first("n_z_bed", n_z_bed))
+        for key, want, default in _REQUIRED:
+            got = deck_first(key, default)
+            got = bool(got) if type(want) is bool else str(got)
+            if got != want:
+                raise ValueError(f"{key}={got!r}: only {want!r} is implemented on the "
+                                 f"GPU step")
+        if str(deck_first("turbulence_model", "laminar")) == "smagorinsky":
+            raise ValueError("turbulence_model='smagorinsky' is not implemented on the "
+                             "GPU step (every other value runs k-epsilon, "
+                             "spread_3d.py:2394-2433)")
+        # every projection_method solves the same linear system; the GPU solve
+        # is PCG to projection_cg_rtol
+        deck_first("projection_method", "pardiso")
+        if float(deck_first("nu_t_multiplier", 1.0)) != 1.0:
+            raise ValueError("nu_t_multiplier != 1 is not implemented on the GPU step")
+        for key in _OUTDOOR_OFF:
+            if bool(getattr(ocfg, key, False)):
+                raise ValueError(f"outdoor.{key}=true is not implemented on the GPU step")
+        dmode = str(deck_first("lagrangian_bed_drying_mode", "arrhenius"))
+        dmap = {"arrhenius": 0, "equilibrium": 1, "combined": 2}
+        if dmode not in dmap:
+            raise ValueError(f"lagrangian_bed_drying_mode must be one of {sorted(dmap)} "
+                             f"(got {dmode!r})")
+        dt_key = str(deck_first("dtype", "float32"))
+        if dt_key not in ("float32", "float64"):
+            raise ValueError("dtype must be float32 or float64")
+        moisture = float(getattr(ocfg, "initial_moisture_frac", 0.0) or 0.0)
+        mmin, mmax = deck("moisture_min"), deck("moisture_max")
+        if mmin is not None and mmax is not None:
+            seed = int(deck("moisture_seed") or 0)
+            moisture = float(np.random.default_rng(seed).uniform(float(mmin), float(mmax)))
+        n_steps = deck_first("n_steps", None)
+        quad = int(deck_first("dom_quadrature", 4))
+        if quad not in (4, 8):
+            raise ValueError("dom_quadrature must be 4 or 8")
+        sigma_sav = float(getattr(ocfg, "sav_ratio_1_m", 6562.0))
+        sav_bed = kw.pop("lagrangian_bed_sav_1_m", None)
+        if sav_bed is not None and float(sav_bed) != sigma_sav:
+            raise ValueError("lagrangian_bed_sav_1_m different from the deck SAV is not "
+                             "implemented on the GPU step")
+        cg = kw.pop("projection_cg_rtol", None)
+        if cg is None:
+            cg = deck("projection_cg_rtol")
+        y_bc = kw.pop("y_bc", "periodic")
+        if y_bc != "periodic":
+            raise ValueError("only y_bc='periodic' is implemented on the GPU step")
+        out = cls(
+            wind_speed_m_s=float(wind_speed_m_s), Lx=Lx, Ly=Ly, Lz=Lz, dx=dx, dy=dy,
+            n_z_bed=n_z_bed, mesh_kwargs=mesh, bed_x_start=bed_x_start,
+            bed_x_end=bed_x_end, cfl_factor=cfl, proj_max_iter=proj_max_iter,
+            proj_div_tol=proj_div_tol, h_bed=float(ocfg.fuel_depth_m),
+            rho_b=float(ocfg.bulk_density_kg_m3), T_amb=float(ocfg.ambient_T_K),
+            sigma_sav=sigma_sav, moisture=moisture,
+            ignition_duration_s=float(ocfg.ignition_duration_s), terrain=str(ocfg.terrain),
+            canopy_C_d=float(ocfg.canopy_C_d), canopy_beta_p=float(ocfg.canopy_beta_p),
+            canopy_beta_d=float(ocfg.canopy_beta_d),
+            dom_subcycle_every=int(deck_first("dom_subcycle_every", 1)),
+            dom_quadrature=quad,
+            n_per_cell=int(deck_first("lagrangian_bed_N_per_cell", 50)),
+            drying_mode=dmap[dmode],
+            h_conv=float(deck_first("lagrangian_bed_h_conv", 25.0)),
+            rho_solid_true=float(kw.pop("lagrangian_bed_rho_solid_true", 380.0)),
+            cp_solid=float(kw.pop("lagrangian_bed_cp_solid", 1500.0)),
+            eps_solid=float(kw.pop("lagrangian_bed_eps_solid", 0.9)),
+            view_factor=float(kw.pop("lagrangian_bed_view_factor", 1.0)),
+            view_factor_geometric=bool(kw.pop("lagrangian_bed_view_factor_geometric", True)),
+            char_ox_flux_cap=float(deck_first("char_ox_flux_cap_W_m2", 1.0e5)),
+            char_ox_ash_exp=float(deck_first("char_ox_ash_exp", 0.0)),
+            solid_phase_ignition=bool(deck_first("solid_phase_ignition_enable", False)),
+            solid_phase_T_s_K=float(deck_first("solid_phase_ignition_T_s_K", 800.0)),
+            ignition_q_mult=float(deck_first("ignition_q_mult", 1.0)),
+            ignition_width_mult=float(deck_first("ignition_width_mult", 1.0)),
+            min_dt_s=float(deck_first("min_dt_s", 1.0e-4)),
+            max_wall_time_s=float(deck_first("max_wall_time_s", 60.0)),
+            cg_rtol=float(cg) if cg is not None else 1.0e-6,
+            n_steps=int(n_steps) if n_steps is not None else None,
+            dtype=torch.float64 if dt_key == "float64" else torch.float32,
+            particle_jitter=float(deck_first("particle_jitter", 0.0)),
+            particle_jitter_seed=int(deck_first("particle_jitter_seed", 0)),
+            spin_up_s=float(ocfg.spin_up_s),
+        )
+        if kw:
+            raise TypeError(f"unsupported keyword arguments: {sorted(kw)}")
+        return out
+
+
+def particle_mass_scale(cfg: Spread3DConfig, n: int):
+    """Seeded per-particle load factor 1 + jitter * U(-1, 1) (PCG64), or None."""
+    if cfg.particle_jitter <= 0.0:
+        return None
+    rng = np.random.default_rng(cfg.particle_jitter_seed)
+    return 1.0 + cfg.particle_jitter * rng.uniform(-1.0, 1.0, n)
+
+
+@dataclass
+class Spread3DResult:
+    """The Spread3DResult fields the step produces (spread_3d.py:428-559)."""
+    ros_m_s: float
+    n_cells_ignited: int
+    front_t: list
+    front_x: list
+    grid: Grid3D
+    state_final: dict
+    diag_t: list = field(default_factory=list)
+    diag_Tg_max: list = field(default_factory=list)
+    diag_Ts_max: list = field(default_factory=list)
+    diag_Sp_max: list = field(default_factory=list)
+    diag_Qc_max: list = field(default_factory=list)
+    diag_omega_max: list = field(default_factory=list)
+    diag_Y_max: list = field(default_factory=list)
+    diag_n_ign: list = field(default_factory=list)
+    diag_proj_div_max: list = field(default_factory=list)
+    diag_proj_n_iter: list = field(default_factory=list)
+    n_steps: int = 0
+    stop_reason: Optional[str] = None
+
+
+# ---------------------------------------------------------------------------
+# solver
+# ---------------------------------------------------------------------------
+
+class Spread3DSolver:
+    """Device-resident state and one outer step of the reference loop."""
+
+    FIELDS = ("rho", "rho_prev", "u", "v", "w", "T_g", "T_s", "Y_fuel", "Y_O2",
+              "Y_H2O", "alpha_s", "Fx", "Fy", "Fz", "k", "eps", "nu_t", "omega",
+              "S_pyro", "div_target", "Q_comb", "q_rad", "q_rad_gas", "Q_ext", "Sp",
+              "Sd", "Qp", "Qd", "YF", "Qch", "Qsm", "Qgc", "M_local", "T_s_old",
+              "tmp1", "tmp2", "tmp3")
+
+    def __init__(self, cfg: Spread3DConfig, grid: Optional[Grid3D] = None):
+        self.cfg = cfg
+        self.proj = None
+        self.dev = device()
+        self.dtype = cfg.dtype
+        g = grid or Grid3D.build(Lx=cfg.Lx, Ly=cfg.Ly, Lz=cfg.Lz, dx=cfg.dx, dy=cfg.dy,
+                                 h_bed=cfg.h_bed, n_z_bed=cfg.n_z_bed, **cfg.mesh_kwargs)
+        if g.n_z_bed <= 0:
+            raise ValueError("the GPU step needs a fuel bed (n_z_bed > 0)")
+        self.grid = g
+        self.dg = DeviceGrid(g.Nx, g.Ny, g.Nz, g.dx, g.dy, g.dz_arr, g.d_face_above,
+                             g.d_face_below, z_face=g.z_face, n_z_bed=g.n_z_bed,
+                             dtype=self.dtype)
+        self.dg64 = (self.dg if self.dtype == torch.float64
+                     else self.dg.with_dtype(torch.float64))
+        self.shape = (g.Nz, g.Ny, g.Nx)
+        self.N = g.Nx * g.Ny * g.Nz
+        self.f = {n: torch.zeros(self.shape, dtype=self.dtype, device=self.dev)
+                  for n in self.FIELDS}
+        self._setup()
+
+    # -- setup (spread_3d.py:1126-1896) --------------------------------------
+    def _setup(self):
+        cfg, g, f, dev = self.cfg, self.grid, self.f, self.dev
+        T_amb = cfg.T_amb
+        f64 = dict(dtype=torch.float64, device=dev)
+        f["rho"].fill_(_P0 / (_R_AIR * T_amb))
+        f["T_g"].fill_(T_amb)
+        f["T_s"].fill_(T_amb)
+        f["Y_O2"].fill_(0.232)
+        i0 = int(np.searchsorted(g.x_mid, cfg.bed_x_start, side="left"))
+        i1 = int(np.searchsorted(g.x_mid, cfg.bed_x_end, side="right"))
+        if i0 >= i1:
+            raise ValueError(f"bed_x_start={cfg.bed_x_start} >= bed_x_end={cfg.bed_x_end} "
+                             f"after grid snapping (i_bed_start={i0}, i_bed_end={i1})")
+        self.i_bed = (i0, i1)
+        alpha = np.zeros(self.shape)
+        alpha[:g.n_z_bed, :, i0:i1] = cfg.rho_b / _RHO_PARTICLE
+        f["alpha_s"].copy_(torch.as_tensor(alpha))
+        # bed particles (spread_3d.py:1296-1330)
+        n_cells = int((alpha[:g.n_z_bed, :, i0:i1] > 0).sum())
+        buf = _lb.allocate_bed_particle_buffers(n_cells * cfg.n_per_cell)
+        n_alloc = _lb.initialize_bed_particles_from_alpha_s(
+            buf, alpha, cfg.rho_b, cfg.moisture, T_amb, g.dx, g.dy, g.dz_arr,
+            g.n_z_bed, cfg.n_per_cell, sav=cfg.sigma_sav, i_lo=i0, i_hi=i1,
+            mass_scale=particle_mass_scale(cfg, n_cells * cfg.n_per_cell))
+        if n_alloc <= 0:
+            raise ValueError("no bed particles were allocated")
+        self.n_particles = n_alloc
+        # ignition patch and solid-phase seeding (:1486-1526)
+        n_src = max(1, int(round(0.5 * cfg.ignition_width_mult / g.dx)))
+        s0, s1 = i0, min(i1, i0 + n_src)
+        self.i_src = (s0, s1)
+        if cfg.solid_phase_ignition:
+            x_lo = g.x_mid[s0] - 0.5 * g.dx
+            x_hi = g.x_mid[s1 - 1] + 0.5 * g.dx
+            x = buf["x"][:n_alloc]
+            sel = (buf["alive"][:n_alloc] != 0) & (x_lo <= x) & (x < x_hi)
+            buf["T_s"][:n_alloc][sel] = cfg.solid_phase_T_s_K
+        self.P = _lb.DeviceParticles(buf, n_alloc)
+        cs, ca = _lb._locate(self.dg, self.P)
+        if not torch.equal(cs, ca):
+            raise RuntimeError("particle cell maps of the step and the aggregates differ")
+        if int(cs.min()) < 0:
+            raise RuntimeError("bed particle outside the grid")
+        self.ix = _lb.DeviceIndex(cs)
+        self.contrib = torch.empty(10 * n_alloc, **f64)
+        self.counters = torch.zeros(2, dtype=torch.int32, device=dev)
+        self.dT_work = torch.zeros(self.N, **f64)
+        self.src_arr = (C.c_void_p * 8)(*[f[k].data_ptr() for k in
+                                          ("Sp", "Sd", "Qp", "Qd", "YF", "Qch", "Qsm", "Qgc")])
+        self.bed_params = _lb.FwBedParams(
+            0.0, cfg.h_conv, cfg.rho_solid_true, cfg.cp_solid, cfg.eps_solid, T_amb,
+            cfg.view_factor, cfg.h_bed, cfg.sigma_sav * cfg.rho_b / cfg.rho_solid_true,
+            cfg.char_ox_flux_cap, cfg.char_ox_ash_exp, cfg.n_per_cell,
+            int(cfg.view_factor_geometric), 1, 1, 1, 1, cfg.drying_mode, 0)
+        # inflow, sponge, turbulence (:1391-1580)
+        self.U_mf = midflame_wind_speed(cfg.wind_speed_m_s, cfg.terrain)
+        u_in = build_u_inlet(g, cfg.wind_speed_m_s)
+        self.u_inlet = torch.as_tensor(u_in, dtype=self.dtype, device=dev)
+        self.v_inlet = torch.zeros_like(self.u_inlet)
+        self.w_inlet = torch.zeros_like(self.u_inlet)
+        f["u"].copy_(self.u_inlet[:, :, None].expand(self.shape))
+        Ns = max(3, min(int(round(0.5 / g.dx)), g.Nx // 4))
+        sig = np.zeros(g.Nx)
+        for i in range(g.Nx - Ns, g.Nx):
+            fr = (i - (g.Nx - Ns)) / Ns
+            sig[i] = 5.0 * fr * fr
+        self.sigma_x = torch.as_tensor(sig, dtype=self.dtype, device=dev)
+        k_init = max(1.5 * (0.10 * max(self.U_mf, 0.5)) ** 2, 1.0e-4)
+        eps_init = _turb.C_MU * k_init * k_init / 0.01
+        f["k"].fill_(k_init)
+        f["eps"].fill_(eps_init)
+        f["nu_t"].fill_(0.01)
+        self.k_wall = torch.full((g.Ny, g.Nx), _turb.K_MIN, dtype=self.dtype, device=dev)
+        self.e_wall = torch.full((g.Ny, g.Nx), _turb.EPS_MIN, dtype=self.dtype, device=dev)
+        # soil (:1545-1551)
+        sdz, sda, sdb, _ = build_soil_grid(6, 0.001, 1.5)
+        self.soil_dz = torch.as_tensor(sdz, **f64)
+        self.soil_da = torch.as_tensor(sda, **f64)
+        self.soil_db = torch.as_tensor(sdb, **f64)
+        self.T_soil = torch.full((6, g.Ny, g.Nx), T_amb, dtype=self.dtype, device=dev)
+        self.q_in_soil = torch.zeros((g.Ny, g.Nx), dtype=self.dtype, device=dev)
+        # level set, float64 (flame_front_3d.py:304-330)
+        x_front = g.x_mid[s1 - 1] + 0.5 * g.dx
+        d = g.x_mid - x_front
+        row = np.where(d > 0.0, d, np.maximum(d, -L_BURNOUT_M / 2.0))
+        phi = np.broadcast_to(row[None, None, :], self.shape).copy()
+        phi[:g.n_z_bed, :, s0:s1] = -L_BURNOUT_M / 2.0
+        self.phi = torch.as_tensor(phi, **f64)
+        self.phi_tmp = torch.empty_like(self.phi)
+        self.phi0 = torch.empty_like(self.phi)
+        self.lset_step = 0
+        # projection and DOM handles (:1719-1780)
+        self.proj = _proj_handle(self.dg, 1.0e-6, cfg.cg_rtol)
+        self.dom = _dom.DOMRadiationSolver(g.Nz, g.Ny, g.Nx, g.dy, g.dx, g.dz_arr,
+                                           g.d_face_above, g.d_face_below, "periodic",
+                                           cfg.dom_quadrature)
+        self.dom_h = self.dom._handle(self.dtype)
+        # time step and histories (:1782-1830)
+        self.T_ign = T_amb + 300.0
+        self.dz_min = float(g.dz_arr.min())
+        U_char = max(self.U_mf, math.sqrt(_G * cfg.h_bed * 0.5), 0.1)
+        dt_adv = cfg.cfl_factor * min(g.dx, g.dy, self.dz_min) / U_char
+        dt_diff = 0.25 * self.dz_min * self.dz_min / max(0.01, 1.0e-3)
+        self.dt = min(dt_adv, dt_diff, 0.05)
+        self.t = 0.0
+        self.step_no = 0
+        self.red_work = torch.empty(14 * _lib.FW_REDUCE_BLOCKS, **f64)
+        self.red_out = torch.zeros(14, **f64)
+        self.red_host = torch.zeros(14, dtype=torch.float64).pin_memory()
+        self.nu_work = torch.empty(_lib.FW_REDUCE_BLOCKS, **f64)
+        self.edc_work = torch.empty(max(_lib.lib().fw_edc_workspace_ints(self.dg.ref), 1),
+                                    dtype=torch.int32, device=dev)
+        self.front_t = [0.0]
+        self.front_x = [float(x_front)]
+        self._refresh_front()
+        self.diag = {k: [] for k in ("t", "Tg", "Ts", "Sp", "Qc", "om", "Y", "n_ign",
+                                     "div", "iters")}
+        self._reduce(with_sources=False)
+        self._append_diag(0.0, self.red_host, 0.0, 0, sources=False)
+        self.extinct_count = 0
+        self.last_front = 0.0
+        self.last_advance_t = 0.0
+        self.stop_reason = None
+        self.last_pcg_iters = 0
+
+    # -- helpers ---------------------------------------------------------------
+    @staticmethod
+    def _c(name, *args):
+        _lib.call(name, *args)
+
+    def _refresh_front(self):
+        """LevelSetFront3D.front_x (flame_front_3d.py:391-402) at the two rows
+        the loop reads: (k=1, j=Ny//2) for the exit test and
+        (k=n_z_bed//2, j=Ny//2) for front tracking."""
+        g = self.grid
+        rows = self.phi[[min(1, g.Nz - 1), g.n_z_bed // 2], g.Ny // 2].cpu().numpy()
+        out = []
+        for row in rows:
+            x = math.inf
+            for i in range(g.Nx - 1):
+                p0, p1 = row[i], row[i + 1]
+                if p0 < 0.0 < p1 or p1 < 0.0 < p0:
+                    x = (i + (-p0 / (p1 - p0)) + 0.5) * g.dx
+                    break
+            out.append(x)
+        self.front_exit_x, self.front_mid_x = out
+
+    def _reduce(self, with_sources=True):
+        f = self.f
+        self._c("fw_step_reduce", self.dg.ref, ptr(f["u"]), ptr(f["v"]), ptr(f["w"]),
+                ptr(f["nu_t"]), ptr(f["T_g"]), ptr(f["T_s"]),
+                ptr(f["S_pyro"]) if with_sources else None,
+                ptr(f["Q_comb"]) if with_sources else None, ptr(f["omega"]),
+                ptr(f["Y_fuel"]), ptr(f["alpha_s"]), float(self.T_ign), float(self.T_ign),
+                int(self.grid.n_z_bed), ptr(self.red_work), ptr(self.red_out), stream_ptr())
+        self.red_host.copy_(self.red_out, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+
+    def _append_diag(self, t, r, div, iters, sources=True):
+        d = self.diag
+        d["t"].append(t)
+        d["Tg"].append(float(r[4]))
+        d["Ts"].append(float(r[6]))
+        d["Sp"].append(float(r[7]) if sources else 0.0)
+        d["Qc"].append(float(r[8]) if sources else 0.0)
+        d["om"].append(float(r[9]) if sources else 0.0)
+        d["Y"].append(float(r[10]))
+        d["n_ign"].append(int(r[11]) if sources else 0)
+        d["div"].append(div)
+        d["iters"].append(iters)
+
+    def _swap(self, a, b):
+        self.f[a], self.f[b] = self.f[b], self.f[a]
+
+    # -- one outer step (spread_3d.py:1897-3683) --------------------------------
+    def step(self):
+        """Advance one outer step; returns the stop reason or None."""
+        cfg, g, f = self.cfg, self.grid, self.f
+        G, s = self.dg.ref, stream_ptr()
+        T_amb = cfg.T_amb
+        nzb = g.n_z_bed
+        s0, s1 = self.i_src
+        r = self.red_host
+        self.step_no += 1
+        t = self.t
+        # adaptive dt and the NaN halt (:1906-1941)
+        u_now = max(float(r[0]), float(r[1]), float(r[2]), self.U_mf,
+                    math.sqrt(_G * cfg.h_bed * 0.5), 0.1)
+        nu_now = max(float(r[3]), 1.0e-3)
+        dt = min(cfg.cfl_factor * min(g.dx, g.dy, self.dz_min) / u_now,
+                 0.25 * self.dz_min * self.dz_min / nu_now, 0.05)
+        dt = min(dt, self.dt * 1.5)
+        if dt < cfg.min_dt_s:
+            dt = cfg.min_dt_s
+        if float(r[5]) != 0.0:
+            raise RuntimeError(f"State NaN/Inf at t={t:.3f}s -- simulation exploded.")
+        self.dt = dt
+        flame_n = int(r[12])            # flame-body count of this step (module doc)
+        combustion = t >= cfg.spin_up_s
+        ign_on = t < cfg.ignition_duration_s
+        Q_drip = Q_DRIP_PER_AREA / max(cfg.h_bed, 1e-3)
+        # rho_prev, EoS, inlet O2 (:1984-2050)
+        self._c("fw_step_prologue", G, ptr(f["T_g"]), ptr(f["rho"]), ptr(f["rho_prev"]),
+                ptr(f["Y_O2"]), T_amb, s)
+        # bed particles (:2064-2182); the sources stay zero until ignition
+        if combustion:
+            self._c("fw_bed_external_heat", G, ptr(f["q_rad"]), ptr(f["Q_ext"]), nzb,
+                    s0, s1, F_DRIP_TO_SOLID * Q_drip if ign_on else 0.0, Q_RAD_MAX, s)
+            self.bed_params.dt = dt
+            self._c("fw_bed_step", G, C.byref(self.P.c), C.byref(self.ix.c),
+                    C.byref(self.bed_params), ptr(f["T_g"]), ptr(f["Y_O2"]),
+                    ptr(f["Q_ext"]), self.src_arr, ptr(self.contrib), ptr(self.counters),
+                    None, None, s)
+            self._c("fw_bed_aggregate", G, C.byref(self.P.c), C.byref(self.ix.c),
+                    ptr(f["T_s"]), None, s)
+            f["T_s_old"].copy_(f["T_s"])
+            self._c("fw_bed_horizontal_conduction", G, C.byref(self.P.c),
+                    C.byref(self.ix.c), ptr(f["T_s_old"]), ptr(f["T_s"]),
+                    ptr(f["alpha_s"]), K_SOLID_GRASS, cfg.rho_solid_true, cfg.cp_solid,
+                    nzb, dt, ptr(self.dT_work), s)
+        # drag, tentative momentum (:2217-2277)
+        self._c("fw_drag_force", G, ptr(f["u"]), ptr(f["v"]), ptr(f["w"]), ptr(f["rho"]),
+                ptr(f["alpha_s"]), cfg.sigma_sav, cfg.canopy_C_d, ptr(f["Fx"]),
+                ptr(f["Fy"]), ptr(f["Fz"]), s)
+        self._c("fw_tentative_velocity", G, ptr(f["u"]), ptr(f["v"]), ptr(f["w"]),
+                ptr(f["tmp1"]), ptr(f["tmp2"]), ptr(f["tmp3"]), ptr(f["rho"]),
+                ptr(f["T_g"]), ptr(f["Fx"]), ptr(f["Fy"]), ptr(f["Fz"]),
+                ptr(self.u_inlet), ptr(self.v_inlet), ptr(self.w_inlet), dt, T_amb, s)
+        self._swap("u", "tmp1")
+        self._swap("v", "tmp2")
+        self._swap("w", "tmp3")
+        # S_pyro, div_target and the projection loop (:2147, :2293-2360)
+        self._c("fw_div_target", G, ptr(f["Sp"]), ptr(f["Sd"]), ptr(f["rho"]),
+                ptr(f["rho_prev"]), dt, ptr(f["S_pyro"]), ptr(f["div_target"]), s)
+        proj_n, proj_div = 0, 0.0
+        ni, rel, dmax = C.c_int(), C.c_double(), C.c_double()
+        for it in range(cfg.proj_max_iter):
+            self._c("fw_proj_project", self.proj, ptr(f["u"]), ptr(f["v"]), ptr(f["w"]),
+                    ptr(f["rho"]), dt, ptr(f["div_target"]), ptr(self.u_inlet), s)
+            self._c("fw_proj_stats", self.proj, C.byref(ni), C.byref(rel), C.byref(dmax))
+            self.last_pcg_iters = ni.value
+            proj_n, proj_div = it + 1, dmax.value
+            if proj_div < cfg.proj_div_tol:
+                break
+        # outflow sponge (:2368-2372)
+        self._c("fw_outflow_sponge", G, ptr(f["u"]), ptr(self.u_inlet), ptr(self.sigma_x),
+                ptr(f["Y_fuel"]), 1.0e-3, dt, s)
+        # k-epsilon (:2408-2433)
+        self._c("fw_k_epsilon", G, ptr(f["k"]), ptr(f["eps"]), ptr(f["tmp1"]),
+                ptr(f["tmp2"]), ptr(f["nu_t"]), ptr(f["u"]), ptr(f["v"]), ptr(f["w"]),
+                ptr(f["T_g"]), ptr(f["alpha_s"]), ptr(self.u_inlet), ptr(self.k_wall),
+                ptr(self.e_wall), cfg.sigma_sav, dt, T_amb, cfg.canopy_beta_p,
+                cfg.canopy_beta_d, s)
+        self._swap("k", "tmp1")
+        self._swap("eps", "tmp2")
+        # DOM every dom_subcycle_every steps, soil every step (:2455-2510)
+        if self.step_no % max(cfg.dom_subcycle_every, 1) == 0:
+            self._c("fw_bed_aggregate", G, C.byref(self.P.c), C.byref(self.ix.c), None,
+                    ptr(f["M_local"]), s)
+            self._c("fw_dom_solve", self.dom_h, ptr(f["T_s"]), ptr(f["T_g"]),
+                    ptr(f["alpha_s"]), ptr(f["omega"]), cfg.sigma_sav, T_amb,
+                    ptr(f["q_rad"]), ptr(f["q_rad_gas"]), ptr(self.T_soil[0]),
+                    ptr(self.q_in_soil), ptr(f["Y_H2O"]), ptr(f["rho"]),
+                    ptr(f["M_local"]), s)
+        self._c("fw_soil_conduction", G, 6, ptr(self.T_soil), ptr(self.q_in_soil),
+                ptr(self.soil_dz), ptr(self.soil_da), ptr(self.soil_db), dt,
+                K_SOIL_DEFAULT, RHO_SOIL_DEFAULT, CP_SOIL_DEFAULT, EPS_SOIL_DEFAULT,
+                T_amb, s)
+        # ignition pulse on the cached q_rad (:2528-2532)
+        if ign_on:
+            self._c("fw_add_patch", G, ptr(f["q_rad"]), nzb - 1, s0, s1,
+                    Q_IGNITION_PULSE * cfg.ignition_q_mult, s)
+        # gas-energy advection (:2613-2622)
+        self._c("fw_gas_energy", G, ptr(f["T_g"]), ptr(f["tmp1"]), ptr(f["u"]), ptr(f["v"]),
+                ptr(f["w"]), dt, 2.0e-5, T_amb, s)
+        self._swap("T_g", "tmp1")
+        # N_SUB chemistry / transport / coupling sub-steps (:2678-3200)
+        nu_max = C.c_double()
+        self._c("fw_nu_t_max_interior", G, ptr(f["nu_t"]), ptr(self.nu_work),
+                C.byref(nu_max), s)
+        dt_sub = dt / N_SUB
+        nsc, dsc = _turb.turbulent_diffusion_substeps(nu_max.value, _turb.SC_T, dt_sub,
+                                                      g.dx, g.dy, self.dz_min)
+        npr, dpr = _turb.turbulent_diffusion_substeps(nu_max.value, _turb.PR_T, dt_sub,
+                                                      g.dx, g.dy, self.dz_min)
+        drip_gas = (1.0 - F_DRIP_TO_SOLID) * Q_drip if (combustion and ign_on) else 0.0
+        species = (("Y_fuel", 1, 0.0, 1.0), ("Y_O2", 2, 0.232, 0.232),
+                   ("Y_H2O", 3, 0.0, 1.0))
+        for sub in range(N_SUB):
+            self._c("fw_edc", G, ptr(f["rho"]), ptr(f["T_g"]), ptr(f["Y_fuel"]),
+                    ptr(f["Y_O2"]), ptr(f["k"]), ptr(f["eps"]), ptr(f["Y_H2O"]),
+                    ptr(f["omega"]), 0.34, 1100.0, dt_sub, 1, 0, 1.3, 17_000_000.0,
+                    ptr(self.edc_work), s)
+            # species with the dilution sources; the Y_O2 cap of :3059 rides
+            # in y_max, the [0, 1] clip is the kernel's own
+            for name, kind, y_in, ymax in species:
+                self._c("fw_species_transport_dilution", G, ptr(f[name]), ptr(f["tmp1"]),
+                        ptr(f["rho"]), ptr(f["u"]), ptr(f["v"]), ptr(f["w"]), ptr(f["Sp"]),
+                        ptr(f["Sd"]), kind, dt_sub, 1.0e-5, y_in, ymax, s)
+                self._swap(name, "tmp1")
+            # turbulent diffusion; the clips of :3082-3084 fold into the last
+            # pass (with no pass they are the identity: every field is already
+            # in range after transport)
+            for name, _, _, ymax in species:
+                for q in range(nsc):
+                    lo, hi = (0.0, ymax) if q == nsc - 1 else (1.0, 0.0)
+                    self._c("fw_turbulent_diffusion_pass", G, ptr(f[name]), ptr(f["tmp1"]),
+                            ptr(f["nu_t"]), _turb.SC_T, dsc, lo, hi, s)
+                    self._swap(name, "tmp1")
+            last = sub == N_SUB - 1
+            self._c("fw_gas_energy_update", G, ptr(f["T_g"]), ptr(f["T_s"]), ptr(f["rho"]),
+                    ptr(f["Y_H2O"]), ptr(f["Sp"]), ptr(f["Sd"]), ptr(f["Qgc"]),
+                    ptr(f["q_rad_gas"]), ptr(f["Q_comb"]) if last else None, nzb, s0, s1,
+                    drip_gas, dt_sub, T_amb, s)
+            # vertical solid conduction with the T_s clamps of :3195-3196
+            self._c("fw_solid_conduction_vertical", G, ptr(f["T_s"]), ptr(f["tmp1"]),
+                    ptr(f["alpha_s"]), K_SOLID_GRASS, _RHO_SOLID, _CP_SOLID, dt_sub,
+                    T_amb, T_CAP, s)
+            self._swap("T_s", "tmp1")
+            # T_g turbulent diffusion with the clamps of :3193-3194 folded into
+            # the last pass (T_g >= T_amb already holds when no pass runs)
+            for q in range(npr):
+                lo, hi = (T_amb, T_CAP) if q == npr - 1 else (1.0, 0.0)
+                self._c("fw_turbulent_diffusion_pass", G, ptr(f["T_g"]), ptr(f["tmp1"]),
+                        ptr(f["nu_t"]), _turb.PR_T, dpr, lo, hi, s)
+                self._swap("T_g", "tmp1")
+            if npr == 0:
+                f["T_g"].clamp_(max=T_CAP)
+        # passive level set: reinitialise every 10th step (:3204-3262)
+        self.lset_step += 1
+        if self.lset_step % LEVELSET_REINIT_INTERVAL == 0:
+            self._reinitialize()
+        # exits (:3414-3436), before the EoS as in the reference
+        if flame_n == 0 and t > cfg.ignition_duration_s + T_BOOTSTRAP_S:
+            self.extinct_count += 1
+            if self.extinct_count > 50:
+                self.stop_reason = "extinct"
+                return self.stop_reason
+        else:
+            self.extinct_count = 0
+        if self.front_exit_x > cfg.bed_x_end - g.dx:
+            self.stop_reason = "front_exit"
+            return self.stop_reason
+        # EoS, reductions, front tracking, diagnostics (:3440-3512)
+        self._c("fw_eos", G, ptr(f["T_g"]), ptr(f["rho"]), T_amb, s)
+        self._reduce()
+        r = self.red_host
+        fl = self.front_mid_x
+        if math.isfinite(fl):
+            if fl > (self.front_x[-1] if self.front_x else 0.0):
+                self.front_t.append(t)
+                self.front_x.append(fl)
+            new_front = fl
+        else:
+            ib = int(r[13])
+            new_front = (float(g.x_mid[ib]) if ib >= 0
+                         else (self.front_x[-1] if self.front_x else 0.0))
+        if new_front > self.last_front:
+            self.last_front = new_front
+            self.last_advance_t = t
+        self.t = t + dt
+        self._append_diag(self.t, r, proj_div, proj_n)
+        if self.front_x[-1] > g.Lx * 0.85:
+            self.stop_reason = "domain_end"
+            return self.stop_reason
+        return None
+
+    def _reinitialize(self):
+        """LevelSetFront3D.reinitialize (flame_front_3d.py:339-355), float64."""
+        g = self.grid
+        self.phi0.copy_(self.phi)
+        dtau = DT_REINIT_FRAC * min(g.dx, g.dy, self.dz_min)
+        for _ in range(REINIT_SUBSTEPS):
+            self._c("fw_levelset_update", self.dg64.ref, ptr(self.phi), ptr(self.phi0),
+                    None, ptr(self.phi_tmp), dtau, 1, stream_ptr())
+            self.phi, self.phi_tmp = self.phi_tmp, self.phi
+        self._refresh_front()
+
+    # -- whole run -------------------------------------------------------------
+    def run(self, n_steps=None):
+        cfg = self.cfg
+        n_target = n_steps if n_steps is not None else cfg.n_steps
+        while True:
+            if n_target is not None:
+                if self.step_no >= n_target:
+                    break
+            elif self.t >= cfg.max_wall_time_s:
+                break
+            if self.step() is not None:
+                break
+        return self
+
+    def state_numpy(self):
+        torch.cuda.current_stream().synchronize()
+        out = {k: self.f[k].double().cpu().numpy() for k in
+               ("rho", "u", "v", "w", "T_g", "Y_fuel", "Y_O2", "Y_H2O", "T_s", "alpha_s",
+                "k", "eps", "nu_t", "omega", "q_rad", "q_rad_gas", "S_pyro", "Q_comb")}
+        out["phi"] = self.phi.cpu().numpy()
+        out["T_soil"] = self.T_soil.double().cpu().numpy()
+        for k in ("x", "y", "z", "T_s", "m_solid", "m_water", "m_char", "alive"):
+            out["p_" + k] = self.P.t[k].cpu().numpy().copy()
+        return out
+
+    def ros(self):
+        """_compute_steady_ros (spread_3d.py:3796-3824)."""
+        g = self.grid
+        ft, fx = self.front_t, self.front_x
+        if len(ft) < 2:
+            return 0.0
+        n_src = self.i_src[1] - self.i_src[0]
+        if fx[-1] - n_src * g.dx < 0.1:
+            return 0.0
+        if self.t - ft[-1] > 30.0 and not fx[-1] > g.Lx * 0.7:
+            return 0.0
+        return (fx[-1] - fx[0]) / (self.t - ft[0]) if self.t > ft[0] else 0.0
+
+    def result(self):
+        st = self.state_numpy()
+        d = self.diag
+        return Spread3DResult(
+            ros_m_s=self.ros(), n_cells_ignited=int(np.sum(st["T_s"] >= self.T_ign)),
+            front_t=list(self.front_t), front_x=list(self.front_x), grid=self.grid,
+            state_final=st, diag_t=d["t"], diag_Tg_max=d["Tg"], diag_Ts_max=d["Ts"],
+            diag_Sp_max=d["Sp"], diag_Qc_max=d["Qc"], diag_omega_max=d["om"],
+            diag_Y_max=d["Y"], diag_n_ign=d["n_ign"], diag_proj_div_max=d["div"],
+            diag_proj_n_iter=d["iters"], n_steps=self.step_no,
+            stop_reason=self.stop_reason)
+
+    def close(self):
+        if self.proj is not None:
+            _lib.lib().fw_proj_destroy(self.proj)
+            self.proj = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_3d_spread(ri_or_path, *, wind_speed_m_s=0.0, **kwargs) -> Spread3DResult:
+    """Drop-in for run_3d_spread (spread_3d.py:563-3729): deck + keyword
+    overrides, run until max_wall_time_s (or the new deck key n_steps), and
+    return the result summary."""
+    cfg = Spread3DConfig.resolve(ri_or_path, wind_speed_m_s, **kwargs)
+    solver = Spread3DSolver(cfg)
+    solver.run()
+    return solver.result()

@@ -362,15 +362,17 @@ int fw_flame_mask_age(const fw_grid* g, const void* omega, const void* T_g,
  * spread_3d.py:1906-1941, :3503-3512, front tracking :3743-3777).
  * out (device double[14]): max|u|, max|v|, max|w|, max nu_t, max T_g,
  * T_g non-finite flag, max T_s, max S_pyro, max Q_comb, max omega,
- * max Y_fuel, count(T_s >= T_ign), count(flame_mask), last burning column i
- * (T_s >= T_front on alpha_s > 0 in k < n_z_bed; -1 if none).  S_pyro,
- * Q_comb, flame_mask may be NULL.  work: 14 * 256 doubles. */
+ * max Y_fuel, count(T_s >= T_ign), count of flame-active cells
+ * ((omega > 1e-3) | (T_g > 1000 & Y_F > 1e-3): the flame-body mask of the
+ * next step, flame_front_3d.py:756-813), last burning column i (T_s >=
+ * T_front on alpha_s > 0 in k < n_z_bed; -1 if none).  S_pyro and Q_comb
+ * may be NULL.  work: 14 * 256 doubles. */
 int fw_step_reduce(const fw_grid* g, const void* u, const void* v,
                    const void* w, const void* nu_t, const void* T_g,
                    const void* T_s, const void* S_pyro, const void* Q_comb,
                    const void* omega, const void* Y_fuel, const void* alpha_s,
-                   const uint8_t* flame_mask, double T_ign, double T_front,
-                   int n_z_bed, double* work, double* out, void* stream);
+                   double T_ign, double T_front, int n_z_bed, double* work,
+                   double* out, void* stream);
 
 /* LevelSetFront3D (flame_front_3d.py:113-360): mode 0 = evolve,
  * phi_out = phi - step v_n |grad phi|_Godunov; mode 1 = one Sussman reinit
