@@ -1,0 +1,231 @@
+"""Throughput of the Tier-3 spread outer step on one MI355X.
+
+Workload (default): 256 x 64 x 48 cells at dx = dy = 0.1 m (25.6 x 6.4 x
+4.51 m), float32 grid fields, grass bed 0.21 m deep in 5 layers from
+x = 1 m (bulk density 1.4762 kg/m^3 = 0.31 kg/m^2, SAV 9770 1/m), moisture
+drawn U(2 %, 10 %) from a seeded PCG64, 4 particles per bed cell with seeded
++-10 % load jitter (314,880 particles), log-law inflow U10 = 4 m/s, k-epsilon,
+EDC, DOM S8 (80 directions) every 5th step, fft-PCG projection (rtol 1e-6,
+outer divergence tolerance 1e-3), passive level set, solid-phase ignition.
+
+One "step" is one outer step of the reference loop (spread_3d.py:1897-3683)
+with its 10 chemistry/transport sub-steps.  ``value`` = cells x steps / wall
+seconds over exactly K timed steps after W warm-up steps, inputs resident in
+HBM.  The JSON line also carries:
+* roofline: the dominant kernel's algorithmic bytes per launch / its mean
+  launch time, both measured here with HIP events on the stream the kernel
+  is launched on (torch's current stream), against 8 TB/s;
+* cpu_baseline: the oracle's float64 NumPy step on one host core, timed on a
+  bounded sample of the same workload (rank 0, N = 1 only).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "3D cell-steps/s at 256x64x48; learned vs plain PCG time-to-1e-8 residual"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # 256 x 64 x 48: n_z_bed 5 (dz 0.042) + 43 air cells of 0.1 m
+    "spread3d_256x64x48_s8": dict(
+        lx=25.6, ly=6.4, lz=4.51, dx=0.1, dy=0.1, n_z_bed=5, wall_bl_n=1,
+        wall_bl_first_dz=0.042, wall_bl_growth=1.0, atm_uniform_dz=0.1,
+        dom_quadrature=8, moisture_min=0.02, moisture_max=0.10, moisture_seed=1,
+        particle_jitter=0.10, particle_jitter_seed=7),
+    # 128 x 32 x 32: n_z_bed 2 (dz 0.105) + 30 air cells of 0.1 m
+    "spread3d_128x32x32_s4": dict(
+        lx=12.8, ly=3.2, lz=3.2, dx=0.1, dy=0.1, n_z_bed=2, wall_bl_n=1,
+        wall_bl_first_dz=0.105, wall_bl_growth=1.0, atm_uniform_dz=0.1,
+        dom_quadrature=4, initial_moisture_frac=0.04),
+}
+COMMON = dict(
+    fuel_depth_m=0.21, bulk_density_kg_m3=1.4762, sav_ratio_1_m=9770.0,
+    initial_moisture_frac=0.04, bed_x_start=1.0, turbulence_model="k_epsilon",
+    combustion_closure="edc", radiation_solver="dom", dom_subcycle_every=5,
+    projection_method="fft_pcg", lagrangian_bed_enable=True,
+    lagrangian_bed_n_per_cell=4, lagrangian_bed_drying_mode="combined",
+    lagrangian_bed_h_conv=250.0, level_set_passive=True,
+    solid_phase_ignition_enable=True, solid_phase_ignition_t_s_k=1000.0,
+    ignition_duration_s=3.0, ignition_q_mult=3.0, ignition_width_mult=3.0,
+    min_dt_s=1.0e-4, max_wall_time_s=1.0e9)
+WIND = 4.0
+
+# dominant-kernel table: entry point -> (kernel, algorithmic bytes per cell
+# per launch in units of the grid element size).  fw_species_transport_dilution
+# reads Y, rho, u, v, w, S_pyro part, S_dry part and writes Y_out: 8 fields.
+ROOFLINE_KERNELS = {
+    "fw_species_transport_dilution": ("k_species", 8),
+}
+
+
+def deck(name, dtype):
+    from firewebapp_amd.deck import RomInputs
+    ovr = dict(COMMON)
+    ovr.update(WORKLOADS[name])
+    ovr["dtype"] = dtype
+    return RomInputs(outdoor_overrides=ovr)
+
+
+class EntryTimer:
+    """HIP-event timing of one entry point (one kernel launch per call) on
+    torch's current stream, the stream the call launches on.  A run of
+    back-to-back calls (the three species launches of a sub-step) shares one
+    event pair, so the event overhead is spread over the run."""
+
+    def __init__(self, name):
+        import torch
+        from firewebapp_amd import _lib
+        self.name, self.torch, self.lib = name, torch, _lib
+        self.groups = []          # (start event, end event, launches)
+        self.launches = 0
+        self.on = False
+        self._open = None
+        self._orig = _lib.call
+
+        def call(n, *args):
+            if not self.on:
+                return self._orig(n, *args)
+            if n == self.name:
+                if self._open is None:
+                    a = torch.cuda.Event(enable_timing=True)
+                    a.record()
+                    self._open = [a, 0]
+                self._orig(n, *args)
+                self._open[1] += 1
+                return
+            self._close()
+            return self._orig(n, *args)
+        _lib.call = call
+
+    def _close(self):
+        if self._open is not None:
+            b = self.torch.cuda.Event(enable_timing=True)
+            b.record()
+            self.groups.append((self._open[0], b, self._open[1]))
+            self.launches += self._open[1]
+            self._open = None
+
+    def stop(self):
+        self._close()
+        self.on = False
+
+    def mean_ms(self):
+        self.torch.cuda.synchronize()
+        if not self.launches:
+            return float("nan")
+        return sum(a.elapsed_time(b) for a, b, _ in self.groups) / self.launches
+
+    def restore(self):
+        self.lib.call = self._orig
+
+
+def cpu_baseline(name, budget_s):
+    """Oracle float64 step on one core over a bounded sample of the workload."""
+    from threadpoolctl import threadpool_limits
+    from firewebapp_amd.grid import Grid3D
+    from firewebapp_amd.spread_3d import Spread3DConfig, particle_mass_scale
+    from oracle.step import OracleSpread3D
+    cfg = Spread3DConfig.resolve(deck(name, "float64"), wind_speed_m_s=WIND)
+    g = Grid3D.build(Lx=cfg.Lx, Ly=cfg.Ly, Lz=cfg.Lz, dx=cfg.dx, dy=cfg.dy,
+                     h_bed=cfg.h_bed, n_z_bed=cfg.n_z_bed, **cfg.mesh_kwargs)
+    nb = int(g.n_z_bed * g.Ny * (np.sum((g.x_mid >= cfg.bed_x_start) &
+                                        (g.x_mid <= cfg.bed_x_end))))
+    o = OracleSpread3D(cfg, g, particle_mass_scale(cfg, nb * cfg.n_per_cell))
+    n = 0
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        while True:
+            o.step()
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s or el * (n + 1) / n > 2.0 * budget_s:
+                break
+    cells = g.Nx * g.Ny * g.Nz
+    return {"value": cells * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle float64 NumPy step, outer steps 1-{n} of {name} "
+                      f"({el:.1f} s; no DOM solve in these steps, it runs every 5th)"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="spread3d_256x64x48_s8", choices=sorted(WORKLOADS))
+    ap.add_argument("--dtype", default="float32", choices=("float32", "float64"))
+    ap.add_argument("--roofline-kernel", default="fw_species_transport_dilution",
+                    choices=sorted(ROOFLINE_KERNELS))
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of oracle CPU work for cpu_baseline (0 = skip)")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the step is "
+              f"single-GPU in this build", file=sys.stderr)
+        return 2
+    import torch
+    from firewebapp_amd.spread_3d import Spread3DConfig, Spread3DSolver
+    torch.cuda.init()
+    cfg = Spread3DConfig.resolve(deck(a.workload, a.dtype), wind_speed_m_s=WIND)
+    sv = Spread3DSolver(cfg)
+    g = sv.grid
+    cells = g.Nx * g.Ny * g.Nz
+    print(f"# workload {a.workload}: {g.Nx}x{g.Ny}x{g.Nz} cells, {sv.n_particles} "
+          f"particles, moisture {cfg.moisture:.4f}", file=sys.stderr, flush=True)
+    for i in range(a.warmup):
+        sv.step()
+    print(f"# warmup {a.warmup} steps done, t = {sv.t:.4f} s", file=sys.stderr, flush=True)
+    timer = EntryTimer(a.roofline_kernel)
+    timer.on = True
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        if sv.step() is not None:
+            raise RuntimeError(f"run stopped early: {sv.stop_reason}")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    timer.stop()
+    timer.restore()
+    print(f"# timed {a.steps} steps in {wall:.3f} s, t = {sv.t:.4f} s", file=sys.stderr,
+          flush=True)
+    kern, fields = ROOFLINE_KERNELS[a.roofline_kernel]
+    esz = 4 if a.dtype == "float32" else 8
+    bytes_launch = fields * esz * cells
+    ms = timer.mean_ms()
+    achieved = bytes_launch / (ms * 1e-3) / 1e9
+    line = {
+        "metric": METRIC, "value": cells * a.steps / wall, "unit": "cell-steps/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32" if a.dtype == "float32" else "f64",
+        "data": "synthetic deck (seeded moisture and particle load jitter)",
+        "config": {"workload": a.workload, "cells": [g.Nx, g.Ny, g.Nz],
+                   "particles": sv.n_particles, "dom": f"S{cfg.dom_quadrature}",
+                   "dom_every": cfg.dom_subcycle_every, "n_sub": 10,
+                   "projection": "fft-PCG rtol 1e-6", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": bytes_launch,
+                     "mean_launch_ms": ms, "launches": timer.launches},
+        "cpu_baseline": None,
+    }
+    del sv
+    torch.cuda.empty_cache()
+    if a.cpu_budget > 0:
+        print("# timing the oracle on the host", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_budget)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -71,10 +71,10 @@ def allocate_bed_particle_buffers(N_max):
 def initialize_bed_particles_from_alpha_s(buf, alpha_s, rho_b_dry, moisture_frac,
                                           T_amb, dx, dy, dz_arr, n_z_bed,
                                           n_per_cell, sav=SAV_GRASS_DEFAULT,
-                                          rho_b_scale=None, m_scale=None):
+                                          m_scale=None):
     """lagrangian_bed_3d.py:221-329.  Optional `m_scale` (per-slot factor on
-    the dry mass, new) supports seeded per-particle load jitter; None keeps
-    the reference exactly."""
+    the dry and water mass, new) supports seeded per-particle load jitter;
+    None keeps the reference exactly."""
     Nz, Ny, Nx = alpha_s.shape
     N_max = buf["alive"].shape[0]
     nb = int(np.count_nonzero(alpha_s[:min(n_z_bed, Nz)] > 0.0))
@@ -91,8 +91,6 @@ def initialize_bed_particles_from_alpha_s(buf, alpha_s, rho_b_dry, moisture_frac
         m_solid_cell = rho_b_dry * V_cell
         m_solid_per_p = m_solid_cell / n_per_cell
         m_water_per_p = moisture_frac * m_solid_per_p
-        for p in range(n_per_cell):
-            pass
         nc = ks.shape[0]
         p = np.arange(n_per_cell)
         fx = ((p * 13) % n_per_cell + 0.5) / n_per_cell
@@ -107,11 +105,16 @@ def initialize_bed_particles_from_alpha_s(buf, alpha_s, rho_b_dry, moisture_frac
         for f in ("u", "v", "w", "age", "m_char"):
             buf[f][sl] = 0.0
         buf["alive"][sl] = 1
-        buf["m_solid"][sl] = m_solid_per_p
-        buf["m_water"][sl] = m_water_per_p
+        if m_scale is None:
+            buf["m_solid"][sl] = m_solid_per_p
+            buf["m_water"][sl] = m_water_per_p
+        else:
+            sc = np.asarray(m_scale)[sl]
+            buf["m_solid"][sl] = m_solid_per_p * sc
+            buf["m_water"][sl] = m_water_per_p * sc
         buf["T_s"][sl] = T_amb
-        buf["m_solid_0"][sl] = m_solid_per_p
-        buf["m_water_0"][sl] = m_water_per_p
+        buf["m_solid_0"][sl] = buf["m_solid"][sl]
+        buf["m_water_0"][sl] = buf["m_water"][sl]
         buf["sav"][sl] = sav
         slot += nc * n_per_cell
     return slot
