@@ -60,12 +60,13 @@ COMMON = dict(
     min_dt_s=1.0e-4, max_wall_time_s=1.0e9)
 WIND = 4.0
 
-# dominant-kernel table: entry point -> (kernel, algorithmic bytes per cell
-# per launch in units of the grid element size).  fw_species_transport_dilution
-# reads Y, rho, u, v, w, S_pyro part, S_dry part and writes Y_out: 8 fields.
-ROOFLINE_KERNELS = {
-    "fw_species_transport_dilution": ("k_species", 8),
-}
+# the roofline kernel and its algorithmic fields per cell per launch:
+# k_species reads Y, rho, u, v, w, S_pyro part, S_dry part and writes Y_out
+ROOFLINE_KERNEL = ("k_species", 8)
+# HBM bytes per k_species launch at the default workload (float32), from the
+# FETCH_SIZE + WRITE_SIZE counters scaled by a 256 MiB copy:
+# profiles/step_256x64x48_s8_hbm_traffic.md
+ROOFLINE_TRAFFIC = {("spread3d_256x64x48_s8", "float32"): (31.27 + 3.00) * 2**20}
 
 
 def deck(name, dtype):
@@ -76,57 +77,19 @@ def deck(name, dtype):
     return RomInputs(outdoor_overrides=ovr)
 
 
-class EntryTimer:
-    """HIP-event timing of one entry point (one kernel launch per call) on
-    torch's current stream, the stream the call launches on.  A run of
-    back-to-back calls (the three species launches of a sub-step) shares one
-    event pair, so the event overhead is spread over the run."""
+def species_timer(on):
+    """Start (reset) or stop the step driver's HIP-event timing of the k_species
+    launches; the events are recorded on the stream the kernels run on."""
+    from firewebapp_amd import _lib
+    _lib.call("fw_spread_timer", 1 if on else 0)
 
-    def __init__(self, name):
-        import torch
-        from firewebapp_amd import _lib
-        self.name, self.torch, self.lib = name, torch, _lib
-        self.groups = []          # (start event, end event, launches)
-        self.launches = 0
-        self.on = False
-        self._open = None
-        self._orig = _lib.call
 
-        def call(n, *args):
-            if not self.on:
-                return self._orig(n, *args)
-            if n == self.name:
-                if self._open is None:
-                    a = torch.cuda.Event(enable_timing=True)
-                    a.record()
-                    self._open = [a, 0]
-                self._orig(n, *args)
-                self._open[1] += 1
-                return
-            self._close()
-            return self._orig(n, *args)
-        _lib.call = call
-
-    def _close(self):
-        if self._open is not None:
-            b = self.torch.cuda.Event(enable_timing=True)
-            b.record()
-            self.groups.append((self._open[0], b, self._open[1]))
-            self.launches += self._open[1]
-            self._open = None
-
-    def stop(self):
-        self._close()
-        self.on = False
-
-    def mean_ms(self):
-        self.torch.cuda.synchronize()
-        if not self.launches:
-            return float("nan")
-        return sum(a.elapsed_time(b) for a, b, _ in self.groups) / self.launches
-
-    def restore(self):
-        self.lib.call = self._orig
+def species_timer_read():
+    import ctypes as C
+    from firewebapp_amd import _lib
+    ms, n = C.c_double(), C.c_int()
+    _lib.call("fw_spread_timer_read", C.byref(ms), C.byref(n))
+    return ms.value, n.value
 
 
 def cpu_baseline(name, budget_s):
@@ -163,10 +126,11 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="spread3d_256x64x48_s8", choices=sorted(WORKLOADS))
     ap.add_argument("--dtype", default="float32", choices=("float32", "float64"))
-    ap.add_argument("--roofline-kernel", default="fw_species_transport_dilution",
-                    choices=sorted(ROOFLINE_KERNELS))
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of oracle CPU work for cpu_baseline (0 = skip)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (the "
+                         "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the step is "
@@ -175,6 +139,13 @@ def main(argv=None):
     import torch
     from firewebapp_amd.spread_3d import Spread3DConfig, Spread3DSolver
     torch.cuda.init()
+    if a.calibrate_copy_mib > 0:
+        n = a.calibrate_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cfg = Spread3DConfig.resolve(deck(a.workload, a.dtype), wind_speed_m_s=WIND)
     sv = Spread3DSolver(cfg)
     g = sv.grid
@@ -184,8 +155,7 @@ def main(argv=None):
     for i in range(a.warmup):
         sv.step()
     print(f"# warmup {a.warmup} steps done, t = {sv.t:.4f} s", file=sys.stderr, flush=True)
-    timer = EntryTimer(a.roofline_kernel)
-    timer.on = True
+    species_timer(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -193,14 +163,14 @@ def main(argv=None):
             raise RuntimeError(f"run stopped early: {sv.stop_reason}")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    timer.stop()
-    timer.restore()
+    species_timer(False)
     print(f"# timed {a.steps} steps in {wall:.3f} s, t = {sv.t:.4f} s", file=sys.stderr,
           flush=True)
-    kern, fields = ROOFLINE_KERNELS[a.roofline_kernel]
+    kern, fields = ROOFLINE_KERNEL
     esz = 4 if a.dtype == "float32" else 8
     bytes_launch = fields * esz * cells
-    ms = timer.mean_ms()
+    tot_ms, launches = species_timer_read()
+    ms = tot_ms / max(launches, 1)
     achieved = bytes_launch / (ms * 1e-3) / 1e9
     line = {
         "metric": METRIC, "value": cells * a.steps / wall, "unit": "cell-steps/s",
@@ -214,8 +184,8 @@ def main(argv=None):
                    "projection": "fft-PCG rtol 1e-6", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bytes_launch,
-                     "mean_launch_ms": ms, "launches": timer.launches},
+                     "traffic": ROOFLINE_TRAFFIC.get((a.workload, a.dtype)), "bytes_per_launch": bytes_launch,
+                     "mean_launch_ms": ms, "launches": launches},
         "cpu_baseline": None,
     }
     del sv

@@ -1,0 +1,377 @@
+// Native driver of one outer time step (spread_3d.py:1897-3683 of the
+// reference), for the Lagrangian-bed / k-epsilon / EDC / DOM / passive
+// level-set configuration.
+//
+// The whole step is issued from here, so the host cost per kernel is one
+// hipLaunchKernelGGL instead of an interpreter round trip.  Host-side
+// decisions are the reference's scalar ones: dt from the previous step's
+// fused reduction, the projection's divergence loop, the turbulent
+// diffusion sub-step count from max nu_t, the level-set front rows after
+// each reinitialisation, and the exits.  Ping-pong fields are swapped by
+// exchanging pointers inside fw_spread_fields; the caller re-reads them.
+#include <math.h>
+#include <string.h>
+
+#include "fw_common.h"
+
+namespace fw {
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_clamp_max(size_t n, T* __restrict__ f, T hi) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const T v = f[c];
+  if (v > hi) f[c] = hi;   // np.minimum(T_g, 1e4) (NaN passes through)
+}
+
+// apply_turbulent_diffusion's sub-step count (turbulence_3d.py:879-918);
+// 0 = skip.
+static int turb_substeps(double nu_t_max, double sc_t, double dt, double dx, double dy,
+                         double dz_min, double* dt_sub) {
+  const double h_min2 = fmin(dx * dx, fmin(dy * dy, dz_min * dz_min));
+  const double D = nu_t_max / sc_t;
+  *dt_sub = 0.0;
+  if (D <= 1.0e-12) return 0;
+  if (!isfinite(D)) return 0;
+  const double dt_max = 0.4 * h_min2 / D;
+  if (dt_max <= 0.0) return 0;
+  const double target = dt / dt_max;
+  if (target > 1000.0) return 0;
+  int n = (int)ceil(target);
+  if (n < 1) n = 1;
+  *dt_sub = dt / n;
+  return n;
+}
+
+#define SWAP(a, b)     \
+  do {                 \
+    void* _t = (a);    \
+    (a) = (b);         \
+    (b) = _t;          \
+  } while (0)
+
+#define TRY(call)            \
+  do {                       \
+    int _rc = (call);        \
+    if (_rc) return _rc;     \
+  } while (0)
+
+static const double SC_T = 0.7, PR_T = 0.85;
+static const double K_SOLID_GRASS = 0.20;     // solid_conduction_3d.py:52
+static const double RHO_SOLID = 500.0, CP_SOLID = 1300.0;  // coupling_3d.py:50-51
+static const double Q_DRIP_PER_AREA = 30000.0, F_DRIP_TO_SOLID = 0.80;
+static const double Q_RAD_MAX = 1.0e5, Q_PULSE = 240000.0, T_CAP = 10000.0;
+static const double T_BOOTSTRAP_S = 2.0;
+static const int N_SUB = 10, REINIT_INTERVAL = 10, REINIT_SUBSTEPS = 5;
+
+// Optional HIP-event timing of the species launches (three back-to-back
+// launches of k_species per sub-step share one event pair), for the
+// roofline figure of the benchmark.
+struct SpeciesTimer {
+  bool on = false;
+  int used = 0, launches = 0;
+  static const int CAP = 4096;
+  hipEvent_t ev[2 * CAP];
+  bool made = false;
+};
+static SpeciesTimer g_timer;
+
+// LevelSetFront3D.front_x (flame_front_3d.py:391-402) on one host row.
+static double front_x_row(const double* row, int Nx, double dx) {
+  for (int i = 0; i < Nx - 1; ++i) {
+    const double p0 = row[i], p1 = row[i + 1];
+    if ((p0 < 0.0 && 0.0 < p1) || (p1 < 0.0 && 0.0 < p0))
+      return (i + (-p0 / (p1 - p0)) + 0.5) * dx;
+  }
+  return INFINITY;
+}
+
+}  // namespace fw
+
+using namespace fw;
+
+extern "C" {
+
+int fw_spread_timer(int enable) {
+  if (enable && !g_timer.made) {
+    for (int q = 0; q < 2 * SpeciesTimer::CAP; ++q)
+      FW_CHECK_HIP(hipEventCreate(&g_timer.ev[q]));
+    g_timer.made = true;
+  }
+  if (enable) {
+    g_timer.used = 0;
+    g_timer.launches = 0;
+  }
+  g_timer.on = enable != 0;
+  return FW_OK;
+}
+
+int fw_spread_timer_read(double* total_ms, int* launches) {
+  FW_REQUIRE(total_ms && launches, "spread_timer_read: null argument");
+  double tot = 0.0;
+  for (int q = 0; q < g_timer.used; ++q) {
+    FW_CHECK_HIP(hipEventSynchronize(g_timer.ev[2 * q + 1]));
+    float ms = 0.f;
+    FW_CHECK_HIP(hipEventElapsedTime(&ms, g_timer.ev[2 * q], g_timer.ev[2 * q + 1]));
+    tot += ms;
+  }
+  *total_ms = tot;
+  *launches = g_timer.launches;
+  return FW_OK;
+}
+
+int fw_spread_front(const fw_grid* g, const fw_spread_fields* f, fw_spread_state* st,
+                    void* stream) {
+  FW_REQUIRE(g && f && st && f->phi, "spread_front: null argument");
+  const int Nx = g->nx, Ny = g->ny, Nz = g->nz;
+  const int k_exit = Nz > 1 ? 1 : 0, k_mid = g->n_z_bed / 2;
+  const size_t row_exit = ((size_t)k_exit * Ny + Ny / 2) * Nx;
+  const size_t row_mid = ((size_t)k_mid * Ny + Ny / 2) * Nx;
+  double* h = (double*)malloc(2 * (size_t)Nx * sizeof(double));
+  FW_REQUIRE(h, "spread_front: out of host memory");
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e1 = hipMemcpyAsync(h, f->phi + row_exit, Nx * sizeof(double),
+                                 hipMemcpyDeviceToHost, s);
+  hipError_t e2 = hipMemcpyAsync(h + Nx, f->phi + row_mid, Nx * sizeof(double),
+                                 hipMemcpyDeviceToHost, s);
+  hipError_t e3 = hipStreamSynchronize(s);
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    free(h);
+    return fw_set_error(FW_E_HIP, "spread_front: %s",
+                        hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3)));
+  }
+  st->front_exit_x = front_x_row(h, Nx, g->dx);
+  st->front_mid_x = front_x_row(h + Nx, Nx, g->dx);
+  free(h);
+  return FW_OK;
+}
+
+int fw_spread_reduce(const fw_grid* g, const fw_spread_fields* f,
+                     const fw_spread_params* p, int with_sources, fw_spread_state* st,
+                     void* stream) {
+  FW_REQUIRE(g && f && p && st, "spread_reduce: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  TRY(fw_step_reduce(g, f->u, f->v, f->w, f->nu_t, f->T_g, f->T_s,
+                     with_sources ? f->S_pyro : nullptr, with_sources ? f->Q_comb : nullptr,
+                     f->omega, f->Y_fuel, f->alpha_s, p->T_ign, p->T_ign, g->n_z_bed,
+                     f->red_work, f->red_out, stream));
+  FW_CHECK_HIP(hipMemcpyAsync(f->red_host, f->red_out, 14 * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+  FW_CHECK_HIP(hipStreamSynchronize(s));
+  memcpy(st->red, f->red_host, sizeof(st->red));
+  return FW_OK;
+}
+
+int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
+                   const fw_spread_params* p, fw_spread_state* st, void* proj, void* dom,
+                   const fw_bed_particles* particles, const fw_bed_index* index,
+                   fw_bed_params* bed, void* stream) {
+  FW_REQUIRE(g && g64 && f && p && st && proj && dom && particles && index && bed,
+             "spread_step: null argument");
+  FW_REQUIRE(g->n_z_bed > 0 && p->i_src1 > p->i_src0, "spread_step: needs a fuel bed");
+  hipStream_t s = (hipStream_t)stream;
+  const int nzb = g->n_z_bed;
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  const size_t esz = g->dtype == FW_F64 ? 8 : 4;
+  const double* r = st->red;
+  st->step_no += 1;
+  st->front_appended = 0;
+  const double t = st->t;
+  // adaptive dt and the NaN halt (spread_3d.py:1906-1941)
+  double u_now = fmax(fmax(r[0], r[1]), r[2]);
+  u_now = fmax(fmax(u_now, p->U_mf), fmax(sqrt(9.81 * p->h_bed * 0.5), 0.1));
+  const double nu_now = fmax(r[3], 1.0e-3);
+  const double hmin = fmin(fmin(g->dx, g->dy), p->dz_min);
+  double dt = fmin(fmin(p->cfl_factor * hmin / u_now,
+                        0.25 * p->dz_min * p->dz_min / nu_now), 0.05);
+  dt = fmin(dt, st->dt * 1.5);
+  if (dt < p->min_dt) dt = p->min_dt;
+  if (r[5] != 0.0)
+    return fw_set_error(FW_E_NONFINITE, "State NaN/Inf at t=%.3fs -- simulation exploded.",
+                        t);
+  st->dt = dt;
+  const int flame_n = (int)r[12];
+  const bool combustion = t >= p->spin_up;
+  const bool ign_on = t < p->ignition_duration;
+  const double Q_drip = Q_DRIP_PER_AREA / fmax(p->h_bed, 1e-3);
+  const int s0 = p->i_src0, s1 = p->i_src1;
+
+  // rho_prev, EoS, inlet O2 (:1984-2050)
+  TRY(fw_step_prologue(g, f->T_g, f->rho, f->rho_prev, f->Y_O2, p->T_amb, stream));
+  // bed particles (:2064-2182); the sources stay zero until ignition
+  if (combustion) {
+    TRY(fw_bed_external_heat(g, f->q_rad, f->Q_ext, nzb, s0, s1,
+                             ign_on ? F_DRIP_TO_SOLID * Q_drip : 0.0, Q_RAD_MAX, stream));
+    bed->dt = dt;
+    TRY(fw_bed_step(g, particles, index, bed, f->T_g, f->Y_O2, f->Q_ext, f->src, f->contrib,
+                    f->counters, nullptr, nullptr, stream));
+    TRY(fw_bed_aggregate(g, particles, index, f->T_s, nullptr, stream));
+    FW_CHECK_HIP(hipMemcpyAsync(f->T_s_old, f->T_s, n * esz, hipMemcpyDeviceToDevice, s));
+    TRY(fw_bed_horizontal_conduction(g, particles, index, f->T_s_old, f->T_s, f->alpha_s,
+                                     K_SOLID_GRASS, p->rho_solid_true, p->cp_solid, nzb, dt,
+                                     f->dT_work, stream));
+  }
+  // drag, tentative momentum (:2217-2277)
+  TRY(fw_drag_force(g, f->u, f->v, f->w, f->rho, f->alpha_s, p->sigma_sav, p->canopy_C_d,
+                    f->Fx, f->Fy, f->Fz, stream));
+  TRY(fw_tentative_velocity(g, f->u, f->v, f->w, f->tmp1, f->tmp2, f->tmp3, f->rho, f->T_g,
+                            f->Fx, f->Fy, f->Fz, f->u_inlet, f->v_inlet, f->w_inlet, dt,
+                            p->T_amb, stream));
+  SWAP(f->u, f->tmp1);
+  SWAP(f->v, f->tmp2);
+  SWAP(f->w, f->tmp3);
+  // S_pyro, div_target and the projection loop (:2147, :2293-2360)
+  TRY(fw_div_target(g, f->src[0], f->src[1], f->rho, f->rho_prev, dt, f->S_pyro,
+                    f->div_target, stream));
+  st->proj_n = 0;
+  st->proj_div = 0.0;
+  for (int it = 0; it < p->proj_max_iter; ++it) {
+    TRY(fw_proj_project(proj, f->u, f->v, f->w, f->rho, dt, f->div_target, f->u_inlet,
+                        stream));
+    int ni = 0;
+    double rel = 0.0, dmax = 0.0;
+    TRY(fw_proj_stats(proj, &ni, &rel, &dmax));
+    st->pcg_iters = ni;
+    st->proj_n = it + 1;
+    st->proj_div = dmax;
+    if (dmax < p->proj_div_tol) break;
+  }
+  // outflow sponge (:2368-2372)
+  TRY(fw_outflow_sponge(g, f->u, f->u_inlet, f->sigma_x, f->Y_fuel, 1.0e-3, dt, stream));
+  // k-epsilon (:2408-2433)
+  TRY(fw_k_epsilon(g, f->k, f->eps, f->tmp1, f->tmp2, f->nu_t, f->u, f->v, f->w, f->T_g,
+                   f->alpha_s, f->u_inlet, f->k_wall, f->eps_wall, p->sigma_sav, dt, p->T_amb,
+                   p->canopy_beta_p, p->canopy_beta_d, stream));
+  SWAP(f->k, f->tmp1);
+  SWAP(f->eps, f->tmp2);
+  // DOM every dom_every steps, soil every step (:2455-2510)
+  if (st->step_no % (p->dom_every > 0 ? p->dom_every : 1) == 0) {
+    TRY(fw_bed_aggregate(g, particles, index, nullptr, f->M_local, stream));
+    TRY(fw_dom_solve(dom, f->T_s, f->T_g, f->alpha_s, f->omega, p->sigma_sav, p->T_amb,
+                     f->q_rad, f->q_rad_gas, f->T_soil, f->q_in_soil, f->Y_H2O, f->rho,
+                     f->M_local, stream));
+  }
+  TRY(fw_soil_conduction(g, p->n_soil, f->T_soil, f->q_in_soil, f->soil_dz,
+                         f->soil_d_above, f->soil_d_below, dt, p->soil_k, p->soil_rho,
+                         p->soil_cp, p->soil_eps, p->T_amb, stream));
+  // ignition pulse on the cached q_rad (:2528-2532)
+  if (ign_on)
+    TRY(fw_add_patch(g, f->q_rad, nzb - 1, s0, s1, Q_PULSE * p->ignition_q_mult, stream));
+  // gas-energy advection (:2613-2622)
+  TRY(fw_gas_energy(g, f->T_g, f->tmp1, f->u, f->v, f->w, dt, 2.0e-5, p->T_amb, stream));
+  SWAP(f->T_g, f->tmp1);
+  // N_SUB chemistry / transport / coupling sub-steps (:2678-3200)
+  double nu_max = 0.0;
+  TRY(fw_nu_t_max_interior(g, f->nu_t, f->nu_work, &nu_max, stream));
+  const double dt_sub = dt / N_SUB;
+  double dsc = 0.0, dpr = 0.0;
+  const int nsc = turb_substeps(nu_max, SC_T, dt_sub, g->dx, g->dy, p->dz_min, &dsc);
+  const int npr = turb_substeps(nu_max, PR_T, dt_sub, g->dx, g->dy, p->dz_min, &dpr);
+  const double drip_gas = (combustion && ign_on) ? (1.0 - F_DRIP_TO_SOLID) * Q_drip : 0.0;
+  void** Y[3] = {&f->Y_fuel, &f->Y_O2, &f->Y_H2O};
+  const double y_in[3] = {0.0, 0.232, 0.0}, y_max[3] = {1.0, 0.232, 1.0};
+  for (int sub = 0; sub < N_SUB; ++sub) {
+    TRY(fw_edc(g, f->rho, f->T_g, f->Y_fuel, f->Y_O2, f->k, f->eps, f->Y_H2O, f->omega,
+               0.34, 1100.0, dt_sub, 1, 0, 1.3, 17.0e6, f->edc_work, stream));
+    // species with the dilution sources; the Y_O2 cap of :3059 rides in
+    // y_max, the [0, 1] clip is the kernel's own
+    const bool timed = g_timer.on && g_timer.used < SpeciesTimer::CAP;
+    if (timed) FW_CHECK_HIP(hipEventRecord(g_timer.ev[2 * g_timer.used], s));
+    for (int q = 0; q < 3; ++q) {
+      TRY(fw_species_transport_dilution(g, *Y[q], f->tmp1, f->rho, f->u, f->v, f->w,
+                                        f->src[0], f->src[1], q + 1, dt_sub, 1.0e-5, y_in[q],
+                                        y_max[q], stream));
+      SWAP(*Y[q], f->tmp1);
+    }
+    if (timed) {
+      FW_CHECK_HIP(hipEventRecord(g_timer.ev[2 * g_timer.used + 1], s));
+      g_timer.used += 1;
+      g_timer.launches += 3;
+    }
+    // turbulent diffusion; the clips of :3082-3084 fold into the last pass
+    // (with no pass they are the identity: each field is already in range)
+    for (int q = 0; q < 3; ++q)
+      for (int m = 0; m < nsc; ++m) {
+        const bool last = m == nsc - 1;
+        TRY(fw_turbulent_diffusion_pass(g, *Y[q], f->tmp1, f->nu_t, SC_T, dsc,
+                                        last ? 0.0 : 1.0, last ? y_max[q] : 0.0, stream));
+        SWAP(*Y[q], f->tmp1);
+      }
+    TRY(fw_gas_energy_update(g, f->T_g, f->T_s, f->rho, f->Y_H2O, f->src[0], f->src[1],
+                             f->src[7], f->q_rad_gas, sub == N_SUB - 1 ? f->Q_comb : nullptr,
+                             nzb, s0, s1, drip_gas, dt_sub, p->T_amb, stream));
+    // vertical solid conduction with the T_s clamps of :3195-3196
+    TRY(fw_solid_conduction_vertical(g, f->T_s, f->tmp1, f->alpha_s, K_SOLID_GRASS, RHO_SOLID,
+                                     CP_SOLID, dt_sub, p->T_amb, T_CAP, stream));
+    SWAP(f->T_s, f->tmp1);
+    // T_g turbulent diffusion, the clamps of :3193-3194 in the last pass
+    for (int m = 0; m < npr; ++m) {
+      const bool last = m == npr - 1;
+      TRY(fw_turbulent_diffusion_pass(g, f->T_g, f->tmp1, f->nu_t, PR_T, dpr,
+                                      last ? p->T_amb : 1.0, last ? T_CAP : 0.0, stream));
+      SWAP(f->T_g, f->tmp1);
+    }
+    if (npr == 0) {
+      if (g->dtype == FW_F64)
+        hipLaunchKernelGGL(k_clamp_max<double>, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                           (double*)f->T_g, T_CAP);
+      else
+        hipLaunchKernelGGL(k_clamp_max<float>, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                           (float*)f->T_g, (float)T_CAP);
+      FW_CHECK_LAUNCH("clamp");
+    }
+  }
+  // passive level set: reinitialise every 10th step, float64 (:3204-3262)
+  st->lset_step += 1;
+  if (st->lset_step % REINIT_INTERVAL == 0) {
+    const size_t nb = n * sizeof(double);
+    FW_CHECK_HIP(hipMemcpyAsync(f->phi0, f->phi, nb, hipMemcpyDeviceToDevice, s));
+    const double dtau = 0.5 * fmin(fmin(g->dx, g->dy), p->dz_min);
+    for (int m = 0; m < REINIT_SUBSTEPS; ++m) {
+      TRY(fw_levelset_update(g64, f->phi, f->phi0, nullptr, f->phi_tmp, dtau, 1, stream));
+      double* tp = f->phi;
+      f->phi = f->phi_tmp;
+      f->phi_tmp = tp;
+    }
+    TRY(fw_spread_front(g64, f, st, stream));
+  }
+  // exits (:3414-3436), before the EoS as in the reference
+  if (flame_n == 0 && t > p->ignition_duration + T_BOOTSTRAP_S) {
+    st->extinct_count += 1;
+    if (st->extinct_count > 50) {
+      st->stop = 1;
+      return FW_OK;
+    }
+  } else {
+    st->extinct_count = 0;
+  }
+  if (st->front_exit_x > p->bed_x_end - g->dx) {
+    st->stop = 2;
+    return FW_OK;
+  }
+  // EoS, reductions, front tracking, diagnostics (:3440-3512)
+  TRY(fw_eos(g, f->T_g, f->rho, p->T_amb, stream));
+  TRY(fw_spread_reduce(g, f, p, 1, st, stream));
+  double new_front;
+  if (isfinite(st->front_mid_x)) {
+    if (st->front_mid_x > st->front_x_last) {
+      st->front_appended = 1;
+      st->front_t_new = t;
+      st->front_x_new = st->front_mid_x;
+      st->front_x_last = st->front_mid_x;
+    }
+    new_front = st->front_mid_x;
+  } else {
+    const int ib = (int)st->red[13];
+    new_front = ib >= 0 ? (ib + 0.5) * p->x_step : st->front_x_last;
+  }
+  if (new_front > st->last_front) {
+    st->last_front = new_front;
+    st->last_advance_t = t;
+  }
+  st->t = t + dt;
+  if (st->front_x_last > p->Lx * 0.85) st->stop = 3;
+  return FW_OK;
+}
+
+}  // extern "C"

@@ -49,20 +49,50 @@ _P0 = 101_325.0
 _R_AIR = 8.314 / 0.029
 _G = 9.81
 _RHO_PARTICLE = 500.0
-_RHO_SOLID = 500.0          # coupling_3d.py:50
-_CP_SOLID = 1300.0          # coupling_3d.py:51
-K_SOLID_GRASS = 0.20        # solid_conduction_3d.py:52
-Q_DRIP_PER_AREA = 30_000.0  # spread_3d.py:2088, :2945
-F_DRIP_TO_SOLID = 0.80      # spread_3d.py:2090, :2974
-Q_RAD_MAX = 1.0e5           # spread_3d.py:2113
-Q_IGNITION_PULSE = 240_000.0  # spread_3d.py:2531
-N_SUB = 10                  # spread_3d.py:2678
-T_CAP = 10_000.0            # spread_3d.py:2650-2651
 L_BURNOUT_M = 0.50          # flame_front_3d.py:62
-LEVELSET_REINIT_INTERVAL = 10
-REINIT_SUBSTEPS = 5
-DT_REINIT_FRAC = 0.5
-T_BOOTSTRAP_S = 2.0         # flame_front_3d.py:523
+
+
+class FwSpreadFields(C.Structure):
+    """fw_spread_fields (include/firewave.h)."""
+    _fields_ = ([(n, C.c_void_p) for n in (
+        "rho", "rho_prev", "u", "v", "w", "T_g", "T_s", "Y_fuel", "Y_O2", "Y_H2O",
+        "alpha_s", "Fx", "Fy", "Fz", "k", "eps", "nu_t", "omega", "S_pyro", "div_target",
+        "Q_comb", "q_rad", "q_rad_gas", "Q_ext", "M_local", "T_s_old", "tmp1", "tmp2",
+        "tmp3")] + [("src", C.c_void_p * 8)] + [(n, C.c_void_p) for n in (
+            "u_inlet", "v_inlet", "w_inlet", "sigma_x", "k_wall", "eps_wall", "T_soil",
+            "q_in_soil", "soil_dz", "soil_d_above", "soil_d_below", "phi", "phi_tmp",
+            "phi0", "red_work", "red_out", "red_host", "nu_work", "contrib", "dT_work",
+            "counters", "edc_work")])
+
+
+class FwSpreadParams(C.Structure):
+    """fw_spread_params (include/firewave.h)."""
+    _fields_ = [(n, C.c_double) for n in (
+        "T_amb", "h_bed", "sigma_sav", "canopy_C_d", "canopy_beta_p", "canopy_beta_d",
+        "cfl_factor", "min_dt", "U_mf", "dz_min", "T_ign", "ignition_duration", "spin_up",
+        "ignition_q_mult", "bed_x_end", "Lx", "x_step", "proj_div_tol", "rho_solid_true",
+        "cp_solid", "soil_k", "soil_rho", "soil_cp", "soil_eps")] + [
+        (n, C.c_int32) for n in ("n_soil", "i_src0", "i_src1", "proj_max_iter",
+                                 "dom_every", "reserved")]
+
+
+class FwSpreadState(C.Structure):
+    """fw_spread_state (include/firewave.h)."""
+    _fields_ = [(n, C.c_double) for n in (
+        "t", "dt", "front_exit_x", "front_mid_x", "last_front", "last_advance_t",
+        "front_x_last", "front_t_new", "front_x_new", "proj_div")] + [
+        ("red", C.c_double * 14)] + [(n, C.c_int32) for n in (
+            "step_no", "lset_step", "extinct_count", "stop", "front_appended", "proj_n",
+            "pcg_iters", "reserved")]
+
+
+# grid-type fields the native step reads, writes and swaps
+_GRID_SLOTS = ("rho", "rho_prev", "u", "v", "w", "T_g", "T_s", "Y_fuel", "Y_O2", "Y_H2O",
+               "alpha_s", "Fx", "Fy", "Fz", "k", "eps", "nu_t", "omega", "S_pyro",
+               "div_target", "Q_comb", "q_rad", "q_rad_gas", "Q_ext", "M_local", "T_s_old",
+               "tmp1", "tmp2", "tmp3")
+_SRC_NAMES = ("Sp", "Sd", "Qp", "Qd", "YF", "Qch", "Qsm", "Qgc")
+_STOPS = {1: "extinct", 2: "front_exit", 3: "domain_end"}
 
 
 def _adv_gas_energy(T_g, u, v, w, dt, dx, dy, dz_arr, d_face_above,
@@ -476,60 +506,76 @@ class Spread3DSolver:
         U_char = max(self.U_mf, math.sqrt(_G * cfg.h_bed * 0.5), 0.1)
         dt_adv = cfg.cfl_factor * min(g.dx, g.dy, self.dz_min) / U_char
         dt_diff = 0.25 * self.dz_min * self.dz_min / max(0.01, 1.0e-3)
-        self.dt = min(dt_adv, dt_diff, 0.05)
-        self.t = 0.0
-        self.step_no = 0
         self.red_work = torch.empty(14 * _lib.FW_REDUCE_BLOCKS, **f64)
         self.red_out = torch.zeros(14, **f64)
         self.red_host = torch.zeros(14, dtype=torch.float64).pin_memory()
         self.nu_work = torch.empty(_lib.FW_REDUCE_BLOCKS, **f64)
         self.edc_work = torch.empty(max(_lib.lib().fw_edc_workspace_ints(self.dg.ref), 1),
                                     dtype=torch.int32, device=dev)
+        # the native step's views of all of the above
+        self.params = FwSpreadParams(
+            T_amb, cfg.h_bed, cfg.sigma_sav, cfg.canopy_C_d, cfg.canopy_beta_p,
+            cfg.canopy_beta_d, cfg.cfl_factor, cfg.min_dt_s, self.U_mf, self.dz_min,
+            self.T_ign, cfg.ignition_duration_s, cfg.spin_up_s, cfg.ignition_q_mult,
+            cfg.bed_x_end, g.Lx, g.Lx / g.Nx, cfg.proj_div_tol, cfg.rho_solid_true,
+            cfg.cp_solid, K_SOIL_DEFAULT, RHO_SOIL_DEFAULT, CP_SOIL_DEFAULT, EPS_SOIL_DEFAULT,
+            6, s0, s1, cfg.proj_max_iter, max(cfg.dom_subcycle_every, 1), 0)
+        self.fields = FwSpreadFields()
+        fs = self.fields
+        for name in _GRID_SLOTS:
+            setattr(fs, name, f[name].data_ptr())
+        for q, name in enumerate(_SRC_NAMES):
+            fs.src[q] = f[name].data_ptr()
+        for name, t in (("u_inlet", self.u_inlet), ("v_inlet", self.v_inlet),
+                        ("w_inlet", self.w_inlet), ("sigma_x", self.sigma_x),
+                        ("k_wall", self.k_wall), ("eps_wall", self.e_wall),
+                        ("T_soil", self.T_soil), ("q_in_soil", self.q_in_soil),
+                        ("soil_dz", self.soil_dz), ("soil_d_above", self.soil_da),
+                        ("soil_d_below", self.soil_db), ("phi", self.phi),
+                        ("phi_tmp", self.phi_tmp), ("phi0", self.phi0),
+                        ("red_work", self.red_work), ("red_out", self.red_out),
+                        ("red_host", self.red_host), ("nu_work", self.nu_work),
+                        ("contrib", self.contrib), ("dT_work", self.dT_work),
+                        ("counters", self.counters), ("edc_work", self.edc_work)):
+            setattr(fs, name, t.data_ptr())
+        self._by_ptr = {t.data_ptr(): t for t in list(f.values()) +
+                        [self.phi, self.phi_tmp, self.phi0]}
+        self.state = FwSpreadState()
+        st = self.state
+        st.t = 0.0
+        st.dt = min(dt_adv, dt_diff, 0.05)
+        st.front_x_last = float(x_front)
         self.front_t = [0.0]
         self.front_x = [float(x_front)]
-        self._refresh_front()
+        s = stream_ptr()
+        self._c("fw_spread_front", self.dg64.ref, C.byref(fs), C.byref(st), s)
+        self._c("fw_spread_reduce", self.dg.ref, C.byref(fs), C.byref(self.params), 0,
+                C.byref(st), s)
         self.diag = {k: [] for k in ("t", "Tg", "Ts", "Sp", "Qc", "om", "Y", "n_ign",
                                      "div", "iters")}
-        self._reduce(with_sources=False)
-        self._append_diag(0.0, self.red_host, 0.0, 0, sources=False)
-        self.extinct_count = 0
-        self.last_front = 0.0
-        self.last_advance_t = 0.0
+        self._append_diag(0.0, st.red, 0.0, 0, sources=False)
         self.stop_reason = None
-        self.last_pcg_iters = 0
 
     # -- helpers ---------------------------------------------------------------
     @staticmethod
     def _c(name, *args):
         _lib.call(name, *args)
 
-    def _refresh_front(self):
-        """LevelSetFront3D.front_x (flame_front_3d.py:391-402) at the two rows
-        the loop reads: (k=1, j=Ny//2) for the exit test and
-        (k=n_z_bed//2, j=Ny//2) for front tracking."""
-        g = self.grid
-        rows = self.phi[[min(1, g.Nz - 1), g.n_z_bed // 2], g.Ny // 2].cpu().numpy()
-        out = []
-        for row in rows:
-            x = math.inf
-            for i in range(g.Nx - 1):
-                p0, p1 = row[i], row[i + 1]
-                if p0 < 0.0 < p1 or p1 < 0.0 < p0:
-                    x = (i + (-p0 / (p1 - p0)) + 0.5) * g.dx
-                    break
-            out.append(x)
-        self.front_exit_x, self.front_mid_x = out
+    @property
+    def t(self):
+        return self.state.t
 
-    def _reduce(self, with_sources=True):
-        f = self.f
-        self._c("fw_step_reduce", self.dg.ref, ptr(f["u"]), ptr(f["v"]), ptr(f["w"]),
-                ptr(f["nu_t"]), ptr(f["T_g"]), ptr(f["T_s"]),
-                ptr(f["S_pyro"]) if with_sources else None,
-                ptr(f["Q_comb"]) if with_sources else None, ptr(f["omega"]),
-                ptr(f["Y_fuel"]), ptr(f["alpha_s"]), float(self.T_ign), float(self.T_ign),
-                int(self.grid.n_z_bed), ptr(self.red_work), ptr(self.red_out), stream_ptr())
-        self.red_host.copy_(self.red_out, non_blocking=True)
-        torch.cuda.current_stream().synchronize()
+    @property
+    def dt(self):
+        return self.state.dt
+
+    @property
+    def step_no(self):
+        return self.state.step_no
+
+    @property
+    def last_pcg_iters(self):
+        return self.state.pcg_iters
 
     def _append_diag(self, t, r, div, iters, sources=True):
         d = self.diag
@@ -544,212 +590,38 @@ class Spread3DSolver:
         d["div"].append(div)
         d["iters"].append(iters)
 
-    def _swap(self, a, b):
-        self.f[a], self.f[b] = self.f[b], self.f[a]
+    def _remap(self):
+        """Re-read which buffer holds which field after the step's swaps."""
+        fs = self.fields
+        for name in _GRID_SLOTS:
+            self.f[name] = self._by_ptr[getattr(fs, name)]
+        self.phi = self._by_ptr[fs.phi]
+        self.phi_tmp = self._by_ptr[fs.phi_tmp]
 
     # -- one outer step (spread_3d.py:1897-3683) --------------------------------
     def step(self):
-        """Advance one outer step; returns the stop reason or None."""
-        cfg, g, f = self.cfg, self.grid, self.f
-        G, s = self.dg.ref, stream_ptr()
-        T_amb = cfg.T_amb
-        nzb = g.n_z_bed
-        s0, s1 = self.i_src
-        r = self.red_host
-        self.step_no += 1
-        t = self.t
-        # adaptive dt and the NaN halt (:1906-1941)
-        u_now = max(float(r[0]), float(r[1]), float(r[2]), self.U_mf,
-                    math.sqrt(_G * cfg.h_bed * 0.5), 0.1)
-        nu_now = max(float(r[3]), 1.0e-3)
-        dt = min(cfg.cfl_factor * min(g.dx, g.dy, self.dz_min) / u_now,
-                 0.25 * self.dz_min * self.dz_min / nu_now, 0.05)
-        dt = min(dt, self.dt * 1.5)
-        if dt < cfg.min_dt_s:
-            dt = cfg.min_dt_s
-        if float(r[5]) != 0.0:
-            raise RuntimeError(f"State NaN/Inf at t={t:.3f}s -- simulation exploded.")
-        self.dt = dt
-        flame_n = int(r[12])            # flame-body count of this step (module doc)
-        combustion = t >= cfg.spin_up_s
-        ign_on = t < cfg.ignition_duration_s
-        Q_drip = Q_DRIP_PER_AREA / max(cfg.h_bed, 1e-3)
-        # rho_prev, EoS, inlet O2 (:1984-2050)
-        self._c("fw_step_prologue", G, ptr(f["T_g"]), ptr(f["rho"]), ptr(f["rho_prev"]),
-                ptr(f["Y_O2"]), T_amb, s)
-        # bed particles (:2064-2182); the sources stay zero until ignition
-        if combustion:
-            self._c("fw_bed_external_heat", G, ptr(f["q_rad"]), ptr(f["Q_ext"]), nzb,
-                    s0, s1, F_DRIP_TO_SOLID * Q_drip if ign_on else 0.0, Q_RAD_MAX, s)
-            self.bed_params.dt = dt
-            self._c("fw_bed_step", G, C.byref(self.P.c), C.byref(self.ix.c),
-                    C.byref(self.bed_params), ptr(f["T_g"]), ptr(f["Y_O2"]),
-                    ptr(f["Q_ext"]), self.src_arr, ptr(self.contrib), ptr(self.counters),
-                    None, None, s)
-            self._c("fw_bed_aggregate", G, C.byref(self.P.c), C.byref(self.ix.c),
-                    ptr(f["T_s"]), None, s)
-            f["T_s_old"].copy_(f["T_s"])
-            self._c("fw_bed_horizontal_conduction", G, C.byref(self.P.c),
-                    C.byref(self.ix.c), ptr(f["T_s_old"]), ptr(f["T_s"]),
-                    ptr(f["alpha_s"]), K_SOLID_GRASS, cfg.rho_solid_true, cfg.cp_solid,
-                    nzb, dt, ptr(self.dT_work), s)
-        # drag, tentative momentum (:2217-2277)
-        self._c("fw_drag_force", G, ptr(f["u"]), ptr(f["v"]), ptr(f["w"]), ptr(f["rho"]),
-                ptr(f["alpha_s"]), cfg.sigma_sav, cfg.canopy_C_d, ptr(f["Fx"]),
-                ptr(f["Fy"]), ptr(f["Fz"]), s)
-        self._c("fw_tentative_velocity", G, ptr(f["u"]), ptr(f["v"]), ptr(f["w"]),
-                ptr(f["tmp1"]), ptr(f["tmp2"]), ptr(f["tmp3"]), ptr(f["rho"]),
-                ptr(f["T_g"]), ptr(f["Fx"]), ptr(f["Fy"]), ptr(f["Fz"]),
-                ptr(self.u_inlet), ptr(self.v_inlet), ptr(self.w_inlet), dt, T_amb, s)
-        self._swap("u", "tmp1")
-        self._swap("v", "tmp2")
-        self._swap("w", "tmp3")
-        # S_pyro, div_target and the projection loop (:2147, :2293-2360)
-        self._c("fw_div_target", G, ptr(f["Sp"]), ptr(f["Sd"]), ptr(f["rho"]),
-                ptr(f["rho_prev"]), dt, ptr(f["S_pyro"]), ptr(f["div_target"]), s)
-        proj_n, proj_div = 0, 0.0
-        ni, rel, dmax = C.c_int(), C.c_double(), C.c_double()
-        for it in range(cfg.proj_max_iter):
-            self._c("fw_proj_project", self.proj, ptr(f["u"]), ptr(f["v"]), ptr(f["w"]),
-                    ptr(f["rho"]), dt, ptr(f["div_target"]), ptr(self.u_inlet), s)
-            self._c("fw_proj_stats", self.proj, C.byref(ni), C.byref(rel), C.byref(dmax))
-            self.last_pcg_iters = ni.value
-            proj_n, proj_div = it + 1, dmax.value
-            if proj_div < cfg.proj_div_tol:
-                break
-        # outflow sponge (:2368-2372)
-        self._c("fw_outflow_sponge", G, ptr(f["u"]), ptr(self.u_inlet), ptr(self.sigma_x),
-                ptr(f["Y_fuel"]), 1.0e-3, dt, s)
-        # k-epsilon (:2408-2433)
-        self._c("fw_k_epsilon", G, ptr(f["k"]), ptr(f["eps"]), ptr(f["tmp1"]),
-                ptr(f["tmp2"]), ptr(f["nu_t"]), ptr(f["u"]), ptr(f["v"]), ptr(f["w"]),
-                ptr(f["T_g"]), ptr(f["alpha_s"]), ptr(self.u_inlet), ptr(self.k_wall),
-                ptr(self.e_wall), cfg.sigma_sav, dt, T_amb, cfg.canopy_beta_p,
-                cfg.canopy_beta_d, s)
-        self._swap("k", "tmp1")
-        self._swap("eps", "tmp2")
-        # DOM every dom_subcycle_every steps, soil every step (:2455-2510)
-        if self.step_no % max(cfg.dom_subcycle_every, 1) == 0:
-            self._c("fw_bed_aggregate", G, C.byref(self.P.c), C.byref(self.ix.c), None,
-                    ptr(f["M_local"]), s)
-            self._c("fw_dom_solve", self.dom_h, ptr(f["T_s"]), ptr(f["T_g"]),
-                    ptr(f["alpha_s"]), ptr(f["omega"]), cfg.sigma_sav, T_amb,
-                    ptr(f["q_rad"]), ptr(f["q_rad_gas"]), ptr(self.T_soil[0]),
-                    ptr(self.q_in_soil), ptr(f["Y_H2O"]), ptr(f["rho"]),
-                    ptr(f["M_local"]), s)
-        self._c("fw_soil_conduction", G, 6, ptr(self.T_soil), ptr(self.q_in_soil),
-                ptr(self.soil_dz), ptr(self.soil_da), ptr(self.soil_db), dt,
-                K_SOIL_DEFAULT, RHO_SOIL_DEFAULT, CP_SOIL_DEFAULT, EPS_SOIL_DEFAULT,
-                T_amb, s)
-        # ignition pulse on the cached q_rad (:2528-2532)
-        if ign_on:
-            self._c("fw_add_patch", G, ptr(f["q_rad"]), nzb - 1, s0, s1,
-                    Q_IGNITION_PULSE * cfg.ignition_q_mult, s)
-        # gas-energy advection (:2613-2622)
-        self._c("fw_gas_energy", G, ptr(f["T_g"]), ptr(f["tmp1"]), ptr(f["u"]), ptr(f["v"]),
-                ptr(f["w"]), dt, 2.0e-5, T_amb, s)
-        self._swap("T_g", "tmp1")
-        # N_SUB chemistry / transport / coupling sub-steps (:2678-3200)
-        nu_max = C.c_double()
-        self._c("fw_nu_t_max_interior", G, ptr(f["nu_t"]), ptr(self.nu_work),
-                C.byref(nu_max), s)
-        dt_sub = dt / N_SUB
-        nsc, dsc = _turb.turbulent_diffusion_substeps(nu_max.value, _turb.SC_T, dt_sub,
-                                                      g.dx, g.dy, self.dz_min)
-        npr, dpr = _turb.turbulent_diffusion_substeps(nu_max.value, _turb.PR_T, dt_sub,
-                                                      g.dx, g.dy, self.dz_min)
-        drip_gas = (1.0 - F_DRIP_TO_SOLID) * Q_drip if (combustion and ign_on) else 0.0
-        species = (("Y_fuel", 1, 0.0, 1.0), ("Y_O2", 2, 0.232, 0.232),
-                   ("Y_H2O", 3, 0.0, 1.0))
-        for sub in range(N_SUB):
-            self._c("fw_edc", G, ptr(f["rho"]), ptr(f["T_g"]), ptr(f["Y_fuel"]),
-                    ptr(f["Y_O2"]), ptr(f["k"]), ptr(f["eps"]), ptr(f["Y_H2O"]),
-                    ptr(f["omega"]), 0.34, 1100.0, dt_sub, 1, 0, 1.3, 17_000_000.0,
-                    ptr(self.edc_work), s)
-            # species with the dilution sources; the Y_O2 cap of :3059 rides
-            # in y_max, the [0, 1] clip is the kernel's own
-            for name, kind, y_in, ymax in species:
-                self._c("fw_species_transport_dilution", G, ptr(f[name]), ptr(f["tmp1"]),
-                        ptr(f["rho"]), ptr(f["u"]), ptr(f["v"]), ptr(f["w"]), ptr(f["Sp"]),
-                        ptr(f["Sd"]), kind, dt_sub, 1.0e-5, y_in, ymax, s)
-                self._swap(name, "tmp1")
-            # turbulent diffusion; the clips of :3082-3084 fold into the last
-            # pass (with no pass they are the identity: every field is already
-            # in range after transport)
-            for name, _, _, ymax in species:
-                for q in range(nsc):
-                    lo, hi = (0.0, ymax) if q == nsc - 1 else (1.0, 0.0)
-                    self._c("fw_turbulent_diffusion_pass", G, ptr(f[name]), ptr(f["tmp1"]),
-                            ptr(f["nu_t"]), _turb.SC_T, dsc, lo, hi, s)
-                    self._swap(name, "tmp1")
-            last = sub == N_SUB - 1
-            self._c("fw_gas_energy_update", G, ptr(f["T_g"]), ptr(f["T_s"]), ptr(f["rho"]),
-                    ptr(f["Y_H2O"]), ptr(f["Sp"]), ptr(f["Sd"]), ptr(f["Qgc"]),
-                    ptr(f["q_rad_gas"]), ptr(f["Q_comb"]) if last else None, nzb, s0, s1,
-                    drip_gas, dt_sub, T_amb, s)
-            # vertical solid conduction with the T_s clamps of :3195-3196
-            self._c("fw_solid_conduction_vertical", G, ptr(f["T_s"]), ptr(f["tmp1"]),
-                    ptr(f["alpha_s"]), K_SOLID_GRASS, _RHO_SOLID, _CP_SOLID, dt_sub,
-                    T_amb, T_CAP, s)
-            self._swap("T_s", "tmp1")
-            # T_g turbulent diffusion with the clamps of :3193-3194 folded into
-            # the last pass (T_g >= T_amb already holds when no pass runs)
-            for q in range(npr):
-                lo, hi = (T_amb, T_CAP) if q == npr - 1 else (1.0, 0.0)
-                self._c("fw_turbulent_diffusion_pass", G, ptr(f["T_g"]), ptr(f["tmp1"]),
-                        ptr(f["nu_t"]), _turb.PR_T, dpr, lo, hi, s)
-                self._swap("T_g", "tmp1")
-            if npr == 0:
-                f["T_g"].clamp_(max=T_CAP)
-        # passive level set: reinitialise every 10th step (:3204-3262)
-        self.lset_step += 1
-        if self.lset_step % LEVELSET_REINIT_INTERVAL == 0:
-            self._reinitialize()
-        # exits (:3414-3436), before the EoS as in the reference
-        if flame_n == 0 and t > cfg.ignition_duration_s + T_BOOTSTRAP_S:
-            self.extinct_count += 1
-            if self.extinct_count > 50:
-                self.stop_reason = "extinct"
-                return self.stop_reason
-        else:
-            self.extinct_count = 0
-        if self.front_exit_x > cfg.bed_x_end - g.dx:
-            self.stop_reason = "front_exit"
+        """Advance one outer step in the native driver; returns the stop reason
+        (``extinct``, ``front_exit``, ``domain_end``) or None."""
+        st = self.state
+        rc = _lib.lib().fw_spread_step(
+            self.dg.ref, self.dg64.ref, C.byref(self.fields), C.byref(self.params),
+            C.byref(st), self.proj, self.dom_h, C.byref(self.P.c), C.byref(self.ix.c),
+            C.byref(self.bed_params), stream_ptr())
+        self._remap()
+        if rc == _lib.FW_E_NONFINITE:
+            raise RuntimeError(_lib.lib().fw_last_error().decode())
+        _lib.check(rc, "fw_spread_step")
+        if st.stop in (1, 2):
+            self.stop_reason = _STOPS[st.stop]
             return self.stop_reason
-        # EoS, reductions, front tracking, diagnostics (:3440-3512)
-        self._c("fw_eos", G, ptr(f["T_g"]), ptr(f["rho"]), T_amb, s)
-        self._reduce()
-        r = self.red_host
-        fl = self.front_mid_x
-        if math.isfinite(fl):
-            if fl > (self.front_x[-1] if self.front_x else 0.0):
-                self.front_t.append(t)
-                self.front_x.append(fl)
-            new_front = fl
-        else:
-            ib = int(r[13])
-            new_front = (float(g.x_mid[ib]) if ib >= 0
-                         else (self.front_x[-1] if self.front_x else 0.0))
-        if new_front > self.last_front:
-            self.last_front = new_front
-            self.last_advance_t = t
-        self.t = t + dt
-        self._append_diag(self.t, r, proj_div, proj_n)
-        if self.front_x[-1] > g.Lx * 0.85:
-            self.stop_reason = "domain_end"
+        if st.front_appended:
+            self.front_t.append(st.front_t_new)
+            self.front_x.append(st.front_x_new)
+        self._append_diag(st.t, st.red, st.proj_div, st.proj_n)
+        if st.stop == 3:
+            self.stop_reason = _STOPS[3]
             return self.stop_reason
         return None
-
-    def _reinitialize(self):
-        """LevelSetFront3D.reinitialize (flame_front_3d.py:339-355), float64."""
-        g = self.grid
-        self.phi0.copy_(self.phi)
-        dtau = DT_REINIT_FRAC * min(g.dx, g.dy, self.dz_min)
-        for _ in range(REINIT_SUBSTEPS):
-            self._c("fw_levelset_update", self.dg64.ref, ptr(self.phi), ptr(self.phi0),
-                    None, ptr(self.phi_tmp), dtau, 1, stream_ptr())
-            self.phi, self.phi_tmp = self.phi_tmp, self.phi
-        self._refresh_front()
 
     # -- whole run -------------------------------------------------------------
     def run(self, n_steps=None):

@@ -398,6 +398,88 @@ int fw_edc(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel,
            double dt, int n_substeps, int extinction_enable, double s_stoich,
            double hoc_J, int* work, void* stream);
 
+/* ---- the outer time step (run_3d_spread loop, spread_3d.py:1897-3683) -- */
+
+/* Device state of one run.  Grid-type (Nz, Ny, Nx) fields unless noted.
+ * fw_spread_step exchanges ping-pong pointers (u/v/w, k/eps, T_g, T_s, the
+ * species, phi) with the tmp* fields in place; read the struct back after
+ * each step to know which buffer holds which field. */
+typedef struct fw_spread_fields {
+  void *rho, *rho_prev, *u, *v, *w, *T_g, *T_s, *Y_fuel, *Y_O2, *Y_H2O, *alpha_s;
+  void *Fx, *Fy, *Fz, *k, *eps, *nu_t, *omega, *S_pyro, *div_target, *Q_comb;
+  void *q_rad, *q_rad_gas, *Q_ext, *M_local, *T_s_old, *tmp1, *tmp2, *tmp3;
+  /* per-cell particle sources: S_pyro, S_drying, Q_pyro, Q_drying,
+   * Y_F_source, Q_char, Q_smold, Q_g_conv */
+  void* src[8];
+  void *u_inlet, *v_inlet, *w_inlet; /* (Nz, Ny) */
+  void* sigma_x;                     /* (Nx) outflow sponge profile */
+  void *k_wall, *eps_wall;           /* (Ny, Nx) wall ghosts */
+  void *T_soil, *q_in_soil;          /* (n_soil, Ny, Nx), (Ny, Nx) */
+  const double *soil_dz, *soil_d_above, *soil_d_below; /* (n_soil) */
+  double *phi, *phi_tmp, *phi0;      /* float64 level set (Nz, Ny, Nx) */
+  double* red_work;                  /* 14 * 256 */
+  double* red_out;                   /* 14, device */
+  double* red_host;                  /* 14, pinned host */
+  double* nu_work;                   /* 256 */
+  double* contrib;                   /* 10 * n_particles */
+  double* dT_work;                   /* N */
+  int32_t* counters;                 /* 2 */
+  int32_t* edc_work;                 /* fw_edc_workspace_ints(g) */
+} fw_spread_fields;
+
+/* Run constants resolved from the deck (spread_3d.py:959-1896). */
+typedef struct fw_spread_params {
+  double T_amb, h_bed, sigma_sav, canopy_C_d, canopy_beta_p, canopy_beta_d;
+  double cfl_factor, min_dt, U_mf, dz_min, T_ign, ignition_duration, spin_up;
+  double ignition_q_mult, bed_x_end, Lx, x_step, proj_div_tol;
+  double rho_solid_true, cp_solid;   /* bed particles (horizontal conduction) */
+  double soil_k, soil_rho, soil_cp, soil_eps;
+  int32_t n_soil, i_src0, i_src1, proj_max_iter, dom_every, reserved;
+} fw_spread_params;
+
+/* Host-side scalars carried from step to step. */
+typedef struct fw_spread_state {
+  double t, dt;
+  double front_exit_x, front_mid_x;  /* level-set front rows (k=1 / k=n_z_bed/2, j=Ny/2) */
+  double last_front, last_advance_t, front_x_last;
+  double front_t_new, front_x_new;   /* the front point appended this step */
+  double proj_div;                   /* max|div(u) - target| after the projection loop */
+  double red[14];                    /* end-of-step fw_step_reduce slots */
+  int32_t step_no, lset_step, extinct_count;
+  int32_t stop;                      /* 0 running, 1 extinct, 2 front exit, 3 domain end */
+  int32_t front_appended, proj_n, pcg_iters, reserved;
+} fw_spread_state;
+
+/* One outer step of run_3d_spread for the Lagrangian-bed / k-epsilon / EDC /
+ * DOM / passive-level-set configuration.  g: the fields' grid; g64: the same
+ * grid with dtype FW_F64 (level set).  proj / dom: handles from
+ * fw_proj_create / fw_dom_create.  The dt, NaN halt, projection loop, exits,
+ * front tracking and diagnostics of the reference live in `st`.  Returns
+ * FW_E_NONFINITE where the reference raises on NaN/Inf. */
+int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
+                   const fw_spread_params* p, fw_spread_state* st, void* proj,
+                   void* dom, const fw_bed_particles* particles,
+                   const fw_bed_index* index, fw_bed_params* bed, void* stream);
+
+/* The end-of-step reduction into st->red (with_sources = 0 skips S_pyro and
+ * Q_comb: the state before the first step). */
+int fw_spread_reduce(const fw_grid* g, const fw_spread_fields* f,
+                     const fw_spread_params* p, int with_sources,
+                     fw_spread_state* st, void* stream);
+
+/* Benchmark timing: while enabled, fw_spread_step records a HIP event pair
+ * on its stream around each sub-step's three k_species launches (at most
+ * 4096 pairs).  fw_spread_timer(1) resets and starts, fw_spread_timer(0)
+ * stops; fw_spread_timer_read waits for the events and returns the summed
+ * milliseconds and the number of launches covered. */
+int fw_spread_timer(int enable);
+int fw_spread_timer_read(double* total_ms, int* launches);
+
+/* LevelSetFront3D.front_x (flame_front_3d.py:391-402) at (k=1, j=Ny/2) and
+ * (k=n_z_bed/2, j=Ny/2) into st->front_exit_x / st->front_mid_x. */
+int fw_spread_front(const fw_grid* g64, const fw_spread_fields* f,
+                    fw_spread_state* st, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
