@@ -367,8 +367,8 @@ __global__ void __launch_bounds__(256)
 k_bed_hcond(GridArgs g, int nzb, const T* __restrict__ To, T* __restrict__ Tn,
             const T* __restrict__ alpha, double diff, double dt,
             double* __restrict__ dTc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx || k >= nzb) return;
   const int Nx = g.nx, Ny = g.ny;
   const size_t c = cidx(g, k, j, i);

@@ -3,10 +3,10 @@
 // Sweep design.  For ordinate n the upwind step-differencing update
 //   I(i) = (kB + ax I(i-1) + ay I_y + az I_z) / (k + ax + ay + az)
 // is a first-order linear recurrence along x once the y and z inflows of the
-// row are known.  A row (n, j, k) is therefore one workgroup: each lane folds
-// its run of cells into an affine map x -> a x + b, a workgroup-wide
-// Hillis-Steele scan composes the maps, and each lane then re-evaluates its
-// cells with the reference formula from the exact upwind value.  Rows whose
+// row are known.  A row (n, j, k) is therefore one wavefront: each lane folds
+// its run of cells into an affine map x -> a x + b, a shuffle scan across the
+// 64 lanes composes the maps, and each lane then re-evaluates its cells with
+// the reference formula from the exact upwind value.  Rows whose
 // sweep-ordered (j', k') lie on the same anti-diagonal j' + k' = d are
 // independent, so one launch processes diagonal d for all M ordinates at
 // once; Ny + Nz - 1 launches make one full source iteration.
@@ -23,7 +23,6 @@
 
 namespace fw {
 
-#define DOM_RED 256
 
 struct DomHandle {
   fw_grid g;
@@ -90,16 +89,22 @@ k_dom_ground_init(int ncol, const T* __restrict__ Tsoil, T eps, T I_amb,
   out[c] = e;
 }
 
-// One anti-diagonal d of the row wavefront, all ordinates.
-template <typename T>
+// One anti-diagonal d of the row wavefront, all ordinates.  One wavefront
+// per (row, ordinate), four ordinates per workgroup.  Lane l owns the PER
+// consecutive cells [l PER, (l+1) PER) of the row in sweep order: it folds
+// them into an affine map x -> A x + Bc, the maps are composed across the
+// wavefront with a shuffle scan (no LDS, no barriers), and each lane then
+// re-evaluates its cells with the reference formula from the exact upwind
+// value.
+template <typename T, int PER>
 __global__ void __launch_bounds__(256)
-k_dom_sweep_diag(GridArgs g, int d, int jlo, const double* __restrict__ Om,
+k_dom_sweep_diag(GridArgs g, int d, int jlo, int M, const double* __restrict__ Om,
                  T* __restrict__ Iset, const T* __restrict__ kap,
                  const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb) {
-  __shared__ T sA[256];
-  __shared__ T sB[256];
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
-  const int n = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int n = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (n >= M) return;                       // whole wavefront exits together
   const int jp = jlo + blockIdx.x;
   const int kp = d - jp;
   const double xi = Om[3 * n], eta = Om[3 * n + 1], mu = Om[3 * n + 2];
@@ -117,60 +122,168 @@ k_dom_sweep_diag(GridArgs g, int d, int jlo, const double* __restrict__ Om,
   const size_t rowy = ((size_t)k * Ny + jy) * Nx;
   const size_t rowz = zb ? 0 : ((size_t)kz * Ny + j) * Nx;
   const bool fwd = xi >= 0.0;
-  const int per = (Nx + 255) / 256;
-  const int t = threadIdx.x;
-  // pass 1: each lane folds its cells (sweep order) into x -> A x + Bc and
-  // keeps their y and z inflows for the exact re-evaluation in pass 2
-  constexpr int MAXPER = 8;
-  T Iys[MAXPER], Izs[MAXPER];
+  // pass 1: fold the lane's cells (sweep order) into x -> A x + Bc; keep the
+  // y and z inflows for the exact re-evaluation in pass 2
+  T Iys[PER], Izs[PER];
   T A = T(1), Bc = T(0);
-  for (int q = 0; q < per && q < MAXPER; ++q) {
-    const int s = t * per + q;
-    if (s >= Nx) break;
-    const int i = fwd ? s : Nx - 1 - s;
-    const size_t c = row + i;
-    const T kc = kap[c];
-    const T Iy = In[rowy + i];
-    const T Iz = zb ? (mu >= 0.0 ? Ig_out[(size_t)j * Nx + i] : I_amb) : In[rowz + i];
-    const T den = kc + ax + ay + az;
-    const T a = ax / den;
-    const T b = (kc * B[c] + ay * Iy + az * Iz) / den;
-    Bc = a * Bc + b;  // compose: x -> a (A x + Bc) + b
-    A = a * A;
-    Iys[q] = Iy;
-    Izs[q] = Iz;
-  }
-  sA[t] = A;
-  sB[t] = Bc;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    T pa = T(1), pb = T(0);
-    if (t >= off) { pa = sA[t - off]; pb = sB[t - off]; }
-    __syncthreads();
-    if (t >= off) {
-      // (A_t, B_t) o (pa, pb): x -> A_t (pa x + pb) + B_t
-      sB[t] = sA[t] * pb + sB[t];
-      sA[t] = sA[t] * pa;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int s = lane * PER + q;
+    if (s < Nx) {
+      const int i = fwd ? s : Nx - 1 - s;
+      const size_t c = row + i;
+      const T kc = kap[c];
+      const T Iy = In[rowy + i];
+      const T Iz = zb ? (mu >= 0.0 ? Ig_out[(size_t)j * Nx + i] : I_amb) : In[rowz + i];
+      const T den = kc + ax + ay + az;
+      const T a = ax / den;
+      const T b = (kc * B[c] + ay * Iy + az * Iz) / den;
+      Bc = a * Bc + b;  // compose: x -> a (A x + Bc) + b
+      A = a * A;
+      Iys[q] = Iy;
+      Izs[q] = Iz;
     }
-    __syncthreads();
   }
-  // x inflow: I_left = I_right = I_amb (dom_3d.py:376, :424)
-  T Iprev = (t == 0) ? I_amb : sA[t - 1] * I_amb + sB[t - 1];
-  for (int q = 0; q < per && q < MAXPER; ++q) {
-    const int s = t * per + q;
-    if (s >= Nx) break;
-    const int i = fwd ? s : Nx - 1 - s;
-    const size_t c = row + i;
-    const T kc = kap[c];
-    const T num = kc * B[c] + ax * Iprev + ay * Iys[q] + az * Izs[q];
-    const T den = kc + ax + ay + az;
-    const T v = num / den;
-    In[c] = v;
-    Iprev = v;
+  // inclusive scan of the maps over the wavefront
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const T pa = __shfl_up(A, off, 64);
+    const T pb = __shfl_up(Bc, off, 64);
+    if (lane >= off) {
+      Bc = A * pb + Bc;   // (A, Bc) o (pa, pb): x -> A (pa x + pb) + Bc
+      A = A * pa;
+    }
+  }
+  // exclusive prefix applied to the x inflow I_left = I_right = I_amb
+  // (dom_3d.py:376, :424)
+  const T eA = __shfl_up(A, 1, 64), eB = __shfl_up(Bc, 1, 64);
+  T Iprev = lane == 0 ? I_amb : eA * I_amb + eB;
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int s = lane * PER + q;
+    if (s < Nx) {
+      const int i = fwd ? s : Nx - 1 - s;
+      const size_t c = row + i;
+      const T kc = kap[c];
+      const T num = kc * B[c] + ax * Iprev + ay * Iys[q] + az * Izs[q];
+      const T den = kc + ax + ay + az;
+      const T v = num / den;
+      In[c] = v;
+      Iprev = v;
+    }
   }
 }
 
-// G = sum_n w_n I_n ; partial max |G - G_prev| and max G
+// Tiled wavefront: one workgroup sweeps a 4 x 4 block of rows (sweep-ordered
+// j' in [4 tj, 4 tj + 4), k' in [4 tk, 4 tk + 4)) of one ordinate, one
+// wavefront per row.  Blocks on the block anti-diagonal tj + tk = D are
+// independent, so Ny/4 + Nz/4 - 1 launches make a source iteration.  Inside
+// the block the seven row diagonals run in order; a row's y and z inflows
+// come from LDS when they lie in the block and from global memory (written by
+// an earlier launch, or the stale periodic wrap / boundary) otherwise.  All
+// global inputs are read before any row of the block is written, and the
+// rows are stored at the end, so the stale wrap of small Ny still reads the
+// previous iteration.
+template <typename T, int PER>
+__global__ void __launch_bounds__(1024)
+k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ Om,
+                 T* __restrict__ Iset, const T* __restrict__ kap,
+                 const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb) {
+  extern __shared__ unsigned char smem_raw[];
+  T* sh = reinterpret_cast<T*>(smem_raw);   // 16 rows of Nx
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int jj = wv & 3, kk = wv >> 2;
+  const int n = blockIdx.y;
+  const int tj = tjlo + blockIdx.x, tk = D - tj;
+  const int jp = 4 * tj + jj, kp = 4 * tk + kk;
+  const bool active = jp < Ny && kp < Nz;     // uniform per wavefront
+  const double xi = Om[3 * n], eta = Om[3 * n + 1], mu = Om[3 * n + 2];
+  const int j = eta >= 0.0 ? jp : Ny - 1 - jp;
+  const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
+  const size_t plane = (size_t)Nx * Ny;
+  T* In = Iset + (size_t)n * plane * Nz;
+  const bool fwd = xi >= 0.0;
+  T kv[PER], bv[PER], Iy[PER], Iz[PER], out[PER];
+  T ax = T(0), ay = T(0), az = T(0);
+  size_t row = 0;
+  if (active) {
+    ax = T(fabs(xi) / g.dx);
+    ay = T(fabs(eta) / g.dy);
+    az = T(fabs(mu) / g.dz[k]);
+    const int jy = eta >= 0.0 ? (j > 0 ? j - 1 : Ny - 1) : (j < Ny - 1 ? j + 1 : 0);
+    const bool zb = mu >= 0.0 ? (k == 0) : (k == Nz - 1);
+    const int kz = mu >= 0.0 ? k - 1 : k + 1;
+    row = ((size_t)k * Ny + j) * Nx;
+    const size_t rowy = ((size_t)k * Ny + jy) * Nx;
+    const size_t rowz = zb ? 0 : ((size_t)kz * Ny + j) * Nx;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int s = lane * PER + q;
+      if (s < Nx) {
+        const int i = fwd ? s : Nx - 1 - s;
+        kv[q] = kap[row + i];
+        bv[q] = B[row + i];
+        if (jj == 0) Iy[q] = In[rowy + i];
+        if (kk == 0)
+          Iz[q] = zb ? (mu >= 0.0 ? Ig_out[(size_t)j * Nx + i] : I_amb) : In[rowz + i];
+      }
+    }
+  }
+  for (int st = 0; st < 7; ++st) {
+    if (active && jj + kk == st) {
+      T A = T(1), Bc = T(0);
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int s = lane * PER + q;
+        if (s < Nx) {
+          const int i = fwd ? s : Nx - 1 - s;
+          if (jj > 0) Iy[q] = sh[(wv - 1) * Nx + i];
+          if (kk > 0) Iz[q] = sh[(wv - 4) * Nx + i];
+          const T den = kv[q] + ax + ay + az;
+          const T a = ax / den;
+          const T b = (kv[q] * bv[q] + ay * Iy[q] + az * Iz[q]) / den;
+          Bc = a * Bc + b;
+          A = a * A;
+        }
+      }
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const T pa = __shfl_up(A, off, 64);
+        const T pb = __shfl_up(Bc, off, 64);
+        if (lane >= off) {
+          Bc = A * pb + Bc;
+          A = A * pa;
+        }
+      }
+      const T eA = __shfl_up(A, 1, 64), eB = __shfl_up(Bc, 1, 64);
+      T Iprev = lane == 0 ? I_amb : eA * I_amb + eB;
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int s = lane * PER + q;
+        if (s < Nx) {
+          const int i = fwd ? s : Nx - 1 - s;
+          const T num = kv[q] * bv[q] + ax * Iprev + ay * Iy[q] + az * Iz[q];
+          const T v = num / (kv[q] + ax + ay + az);
+          out[q] = v;
+          sh[wv * Nx + i] = v;
+          Iprev = v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (active) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int s = lane * PER + q;
+      if (s < Nx) In[row + (fwd ? s : Nx - 1 - s)] = out[q];
+    }
+  }
+}
+
+// G = sum_n w_n I_n (ordinates summed in order), one cell per thread; per
+// workgroup partial max |G - G_prev| and max G
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ w,
@@ -178,12 +291,22 @@ k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ 
         double* __restrict__ pmax) {
   __shared__ double sh1[256], sh2[256];
   double md = 0.0, mg = -1e300;
-  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * 256) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c < n) {
     T s = T(0);
-    for (int q = 0; q < M; ++q) s += T(w[q]) * Iset[(size_t)q * n + c];
+    const T* p = Iset + c;
+    int q = 0;
+    for (; q + 4 <= M; q += 4) {
+      const T i0 = p[(size_t)q * n], i1 = p[(size_t)(q + 1) * n];
+      const T i2 = p[(size_t)(q + 2) * n], i3 = p[(size_t)(q + 3) * n];
+      s += T(w[q]) * i0;
+      s += T(w[q + 1]) * i1;
+      s += T(w[q + 2]) * i2;
+      s += T(w[q + 3]) * i3;
+    }
+    for (; q < M; ++q) s += T(w[q]) * p[(size_t)q * n];
     G[c] = s;
-    double dd = fabs((double)s - (double)Gp[c]);
+    const double dd = fabs((double)s - (double)Gp[c]);
     if (dd > md || dd != dd) md = dd;
     if ((double)s > mg) mg = (double)s;
   }
@@ -191,10 +314,10 @@ k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ 
   sh1[t] = md;
   sh2[t] = mg;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s) {
-      if (sh1[t + s] > sh1[t] || sh1[t + s] != sh1[t + s]) sh1[t] = sh1[t + s];
-      if (sh2[t + s] > sh2[t]) sh2[t] = sh2[t + s];
+  for (int st = 128; st > 0; st >>= 1) {
+    if (t < st) {
+      if (sh1[t + st] > sh1[t] || sh1[t + st] != sh1[t + st]) sh1[t] = sh1[t + st];
+      if (sh2[t + st] > sh2[t]) sh2[t] = sh2[t + st];
     }
     __syncthreads();
   }
@@ -271,6 +394,69 @@ k_dom_fin(const double* __restrict__ pd, const double* __restrict__ pm, int np,
 
 static inline unsigned nb(size_t n) { return (unsigned)((n + 255) / 256); }
 
+#define TRY_DOM(call)        \
+  do {                       \
+    int _rc = (call);        \
+    if (_rc) return _rc;     \
+  } while (0)
+
+// Tiled sweep launch for block diagonal D (16 rows of LDS per workgroup).
+template <typename T, int PER>
+static void launch_tile(DomHandle* h, int D, int tjlo, int ntiles, T I_amb, hipStream_t s) {
+  const size_t shm = 16 * (size_t)h->g.nx * sizeof(T);
+  hipLaunchKernelGGL((k_dom_sweep_tile<T, PER>), dim3(ntiles, h->M), dim3(1024), shm, s,
+                     h->ga, D, tjlo, h->M, (const double*)h->Om, (T*)h->I,
+                     (const T*)h->kap_safe, (const T*)h->B, (const T*)h->Ig_out, I_amb);
+}
+
+template <typename T>
+static bool tile_fits(const DomHandle* h) {
+  return 16 * (size_t)h->g.nx * sizeof(T) <= 65536;
+}
+
+// One source iteration of the tiled sweep (cells per lane as below).
+template <typename T>
+static int sweep_tiled(DomHandle* h, T I_amb, hipStream_t s) {
+  const int Nx = h->g.nx;
+  const int TJ = (h->g.ny + 3) / 4, TK = (h->g.nz + 3) / 4;
+  for (int D = 0; D <= TJ + TK - 2; ++D) {
+    const int tjlo = D - (TK - 1) > 0 ? D - (TK - 1) : 0;
+    const int tjhi = D < TJ - 1 ? D : TJ - 1;
+    const int nt = tjhi - tjlo + 1;
+    if (Nx <= 64) launch_tile<T, 1>(h, D, tjlo, nt, I_amb, s);
+    else if (Nx <= 128) launch_tile<T, 2>(h, D, tjlo, nt, I_amb, s);
+    else if (Nx <= 256) launch_tile<T, 4>(h, D, tjlo, nt, I_amb, s);
+    else if (Nx <= 512) launch_tile<T, 8>(h, D, tjlo, nt, I_amb, s);
+    else launch_tile<T, 16>(h, D, tjlo, nt, I_amb, s);
+  }
+  return FW_OK;
+}
+
+// Cells per lane: ceil(Nx / 64) rounded up to a power of two (Nx <= 2048).
+template <typename T>
+static int launch_sweep(DomHandle* h, int d, int jlo, int nrows, T I_amb, hipStream_t s) {
+  const int Nx = h->g.nx, M = h->M;
+  const dim3 grid(nrows, (M + 3) / 4), blk(256);
+  const double* Om = (const double*)h->Om;
+  T* I = (T*)h->I;
+  const T* kap = (const T*)h->kap_safe;
+  const T* B = (const T*)h->B;
+  const T* Ig = (const T*)h->Ig_out;
+  if (Nx <= 64)
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 1>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+  else if (Nx <= 128)
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 2>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+  else if (Nx <= 256)
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 4>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+  else if (Nx <= 512)
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 8>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+  else if (Nx <= 1024)
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 16>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+  else
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 32>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+  return FW_OK;
+}
+
 template <typename T>
 static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
                           const void* alpha, const void* omega, double sigma_sav,
@@ -296,17 +482,18 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
     void* tmp = h->G;
     h->G = h->Gp;
     h->Gp = tmp;
-    for (int d = 0; d <= Ny + Nz - 2; ++d) {
-      const int jlo = d - (Nz - 1) > 0 ? d - (Nz - 1) : 0;
-      const int jhi = d < Ny - 1 ? d : Ny - 1;
-      const int nrows = jhi - jlo + 1;
-      hipLaunchKernelGGL(k_dom_sweep_diag<T>, dim3(nrows, M), dim3(256), 0, s,
-                         h->ga, d, jlo, (const double*)h->Om, (T*)h->I,
-                         (const T*)h->kap_safe, (const T*)h->B,
-                         (const T*)h->Ig_out, I_amb);
+    if (tile_fits<T>(h)) {
+      TRY_DOM(sweep_tiled<T>(h, I_amb, s));
+    } else {
+      for (int d = 0; d <= Ny + Nz - 2; ++d) {
+        const int jlo = d - (Nz - 1) > 0 ? d - (Nz - 1) : 0;
+        const int jhi = d < Ny - 1 ? d : Ny - 1;
+        const int nrows = jhi - jlo + 1;
+        TRY_DOM(launch_sweep<T>(h, d, jlo, nrows, I_amb, s));
+      }
     }
     FW_CHECK_LAUNCH("dom_sweep");
-    hipLaunchKernelGGL(k_dom_G<T>, dim3(DOM_RED), dim3(256), 0, s, n, M,
+    hipLaunchKernelGGL(k_dom_G<T>, dim3(nb(n)), dim3(256), 0, s, n, M,
                        (const T*)h->I, (const double*)h->w, (T*)h->G,
                        (const T*)h->Gp, h->part, h->part2);
     hipLaunchKernelGGL(k_dom_ground<T>, dim3(nb(ncol)), dim3(256), 0, s, ncol, n, M,
@@ -316,7 +503,7 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
     FW_CHECK_LAUNCH("dom_G");
     if (it > 0) {
       hipLaunchKernelGGL(k_dom_fin, dim3(1), dim3(256), 0, s, h->part, h->part2,
-                         DOM_RED, h->S);
+                         (int)nb(n), h->S);
       FW_CHECK_HIP(hipMemcpyAsync(h->S_host, h->S, 2 * sizeof(double),
                                   hipMemcpyDeviceToHost, s));
       FW_CHECK_HIP(hipStreamSynchronize(s));
@@ -366,7 +553,7 @@ int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* we
                   double eps_w_ground, int max_source_iter, double tol_source_iter,
                   double omega_relax, double kappa_gas_max, void** handle) {
   FW_REQUIRE(g && Omega && weights && handle && M > 0, "dom_create: bad argument");
-  FW_REQUIRE(g->nx <= 256 * 8, "dom_create: Nx > 2048 not supported by the row scan");
+  FW_REQUIRE(g->nx <= 64 * 32, "dom_create: Nx > 2048 not supported by the row scan");
   DomHandle* h = new DomHandle();
   h->g = *g;
   h->ga = grid_args(g);
@@ -394,8 +581,9 @@ int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* we
   }
   void** c2[] = {&h->Ig_out, &h->Ig_emit, &h->q_in};
   for (void** p : c2) FW_CHECK_HIP(hipMalloc(p, ncol * h->esz));
-  FW_CHECK_HIP(hipMalloc((void**)&h->part, DOM_RED * sizeof(double)));
-  FW_CHECK_HIP(hipMalloc((void**)&h->part2, DOM_RED * sizeof(double)));
+  const size_t npart = (n + 255) / 256;
+  FW_CHECK_HIP(hipMalloc((void**)&h->part, npart * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->part2, npart * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->S, 4 * sizeof(double)));
   FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, 4 * sizeof(double)));
   *handle = h;

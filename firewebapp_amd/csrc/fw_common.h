@@ -50,6 +50,26 @@ inline GridArgs grid_args(const fw_grid* g) {
   return a;
 }
 
+// Cell coordinates of the calling thread for a cell_launch grid, with an
+// XCD-aware block order.  Workgroups are handed to the 8 XCDs round-robin by
+// linear id, so in launch order consecutive rows (j, k) would sit in
+// different L2 caches and every y/z halo row would be fetched from HBM once
+// per XCD.  The remap gives each XCD a contiguous run of rows instead, so
+// neighbouring rows share one L2.  It is a bijection on the block ids, so
+// correctness does not depend on the dispatch order.
+__device__ __forceinline__ void cell_ijk(int& i, int& j, int& k) {
+  const unsigned bx = gridDim.x, by = gridDim.y;
+  const unsigned nb = bx * by * gridDim.z;
+  const unsigned lin = blockIdx.x + bx * (blockIdx.y + by * blockIdx.z);
+  const unsigned q = nb >> 3, r = nb & 7u;
+  const unsigned xcd = lin & 7u, pos = lin >> 3;
+  const unsigned m = xcd * q + (xcd < r ? xcd : r) + pos;
+  const unsigned t = m / bx;
+  i = (int)((m - t * bx) * blockDim.x + threadIdx.x);
+  j = (int)(t % by);
+  k = (int)(t / by);
+}
+
 __device__ __forceinline__ int wrap(int j, int n) {
   j %= n;
   return j < 0 ? j + n : j;

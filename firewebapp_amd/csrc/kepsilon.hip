@@ -26,8 +26,8 @@ k_kepsilon(GridArgs g, const T* __restrict__ kt, const T* __restrict__ et,
            const T* __restrict__ alpha, const T* __restrict__ u_inlet,
            const T* __restrict__ k_wall, const T* __restrict__ e_wall,
            T sigma_sav, T dt, T T_amb, T beta_p, T beta_d) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t c = cidx(g, k, j, i);

@@ -27,7 +27,11 @@ namespace fw {
 
 #define PJ_RED 512  // partial-sum workgroups of the projection reductions
 
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_NS };
+// S_DONE: 0 while the inner PCG runs, 1 once ||r|| <= S_TOL, 2 on a NaN
+// residual; S_IT counts the iterations taken.  The iteration kernels leave
+// the Krylov state untouched once S_DONE is set, so iterations can be
+// enqueued in batches and the host reads S_DONE once per batch.
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_DONE, S_IT, S_TOL, S_NS };
 
 // ---------------------------------------------------------------------------
 // operator
@@ -172,6 +176,7 @@ k_update_xr(size_t n, T* __restrict__ d, T* __restrict__ r,
             const T* __restrict__ p, const T* __restrict__ q,
             const double* __restrict__ S, double* __restrict__ part) {
   __shared__ double sh[256];
+  if (S[S_DONE] != 0.0) return;   // converged: leave d, r and the partials
   const T alpha = (T)S[S_ALPHA];
   double acc = 0.0;
   for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
@@ -215,7 +220,7 @@ __global__ void __launch_bounds__(256)
 k_p_update(size_t n, T* __restrict__ p, const T* __restrict__ z,
            const double* __restrict__ S, int first) {
   const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= n) return;
+  if (c >= n || S[S_DONE] != 0.0) return;
   p[c] = first ? z[c] : z[c] + (T)S[S_BETA] * p[c];
 }
 
@@ -226,6 +231,14 @@ k_x64_add(size_t n, double* __restrict__ x, const T* __restrict__ d) {
   if (c < n) x[c] += (double)d[c];
 }
 
+__global__ void k_pcg_init(double* __restrict__ S, double tol) {
+  if (threadIdx.x == 0) {
+    S[S_DONE] = 0.0;
+    S[S_IT] = 0.0;
+    S[S_TOL] = tol;
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_zero(size_t n, T* __restrict__ a) {
   const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -233,7 +246,7 @@ __global__ void __launch_bounds__(256) k_zero(size_t n, T* __restrict__ a) {
 }
 
 // Fixed-order final sum of PJ_RED partials; mode selects what is stored.
-enum { F_STORE = 0, F_ALPHA = 1, F_BETA = 2 };
+enum { F_STORE = 0, F_ALPHA = 1, F_BETA = 2, F_RR = 3 };
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ part, int np, double* __restrict__ S,
            int slot, int mode) {
@@ -249,12 +262,24 @@ k_finalize(const double* __restrict__ part, int np, double* __restrict__ S,
   }
   if (t == 0) {
     const double v = sh[0];
+    const bool done = S[S_DONE] != 0.0;
     if (mode == F_ALPHA) {          // v = <p, Ap>_W
-      S[S_PQ] = v;
-      S[S_ALPHA] = S[S_RZ] / v;
+      if (!done) {
+        S[S_PQ] = v;
+        S[S_ALPHA] = S[S_RZ] / v;
+      }
     } else if (mode == F_BETA) {    // v = <r, z>_W (new)
-      S[S_BETA] = v / S[S_RZ];
-      S[S_RZ] = v;
+      if (!done) {
+        S[S_BETA] = v / S[S_RZ];
+        S[S_RZ] = v;
+      }
+    } else if (mode == F_RR) {      // v = ||r||^2 after an iteration
+      if (!done) {
+        S[S_RR] = v;
+        S[S_IT] += 1.0;
+        if (!(v == v)) S[S_DONE] = 2.0;
+        else if (sqrt(v) <= S[S_TOL]) S[S_DONE] = 1.0;
+      }
     } else {
       S[slot] = v;
     }
@@ -466,6 +491,72 @@ k_gemm(GemmArgs a, const T* __restrict__ A, const T* __restrict__ B,
   }
 }
 
+// Orthonormal DCT-IV of each row (the x eigenbasis of the Neumann-left /
+// Dirichlet-right operator) through an M-point complex FFT, N = 2M, M a
+// power of two:
+//   v[n] = (x[2n] + i x[N-1-2n]) e^{-i pi (4n+1)/(4N)},  V = FFT_M(v),
+//   W[k] = V[k] e^{-i pi k/N},  X[2k] = sqrt(2/N) Re W[k],
+//   X[N-1-2k] = -sqrt(2/N) Im W[k].
+// One wavefront per row, four rows per workgroup, radix-2 decimation in time
+// in LDS.  tw: pre-twiddles (M complex), post-twiddles (M complex), FFT
+// twiddles e^{-2 pi i k/M} (M/2 complex), re/im interleaved.  O(N log N)
+// instead of the dense N x N product.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dct4_rows(int nrows, int N, int logM, const T* __restrict__ tw,
+            const T* __restrict__ in, T* __restrict__ out) {
+  extern __shared__ unsigned char smem_raw[];
+  const int M = N >> 1;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int row = blockIdx.x * 4 + wv;
+  const bool act = row < nrows;
+  T* a = reinterpret_cast<T*>(smem_raw) + (size_t)wv * 2 * M;
+  const T* pre = tw;
+  const T* post = tw + 2 * M;
+  const T* W = tw + 4 * M;
+  if (act) {
+    const T* x = in + (size_t)row * N;
+    for (int n = lane; n < M; n += 64) {
+      const T re = x[2 * n], im = x[N - 1 - 2 * n];
+      const T pr = pre[2 * n], pi = pre[2 * n + 1];
+      const int r = (int)(__brev((unsigned)n) >> (32 - logM));
+      a[2 * r] = re * pr - im * pi;
+      a[2 * r + 1] = re * pi + im * pr;
+    }
+  }
+  __syncthreads();
+  for (int st = 0; st < logM; ++st) {
+    const int half = 1 << st;
+    if (act) {
+      for (int b = lane; b < (M >> 1); b += 64) {
+        const int pos = b & (half - 1);
+        const int i1 = ((b >> st) << (st + 1)) + pos;
+        const int i2 = i1 + half;
+        const int t = pos << (logM - 1 - st);
+        const T wr = W[2 * t], wi = W[2 * t + 1];
+        const T xr = a[2 * i2], xi = a[2 * i2 + 1];
+        const T tr = wr * xr - wi * xi, ti = wr * xi + wi * xr;
+        const T ur = a[2 * i1], ui = a[2 * i1 + 1];
+        a[2 * i1] = ur + tr;
+        a[2 * i1 + 1] = ui + ti;
+        a[2 * i2] = ur - tr;
+        a[2 * i2 + 1] = ui - ti;
+      }
+    }
+    __syncthreads();
+  }
+  if (act) {
+    T* y = out + (size_t)row * N;
+    const T sc = T(sqrt(2.0 / N));
+    for (int kk = lane; kk < M; kk += 64) {
+      const T vr = a[2 * kk], vi = a[2 * kk + 1];
+      const T pr = post[2 * kk], pi = post[2 * kk + 1];
+      y[2 * kk] = (vr * pr - vi * pi) * sc;
+      y[N - 1 - 2 * kk] = -((vr * pi + vi * pr) * sc);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
@@ -480,6 +571,13 @@ struct ProjHandle {
   // preconditioner (type T): Ux (Nx*Nx), Yf (Ny*Ny forward = basis^T),
   // Yi (Ny*Ny inverse = basis), Zf (Nz*Nz, D folded), Zi (Nz*Nz, D^-1 folded)
   void *Ux, *UxT, *Yf, *Yi, *Zf, *Zi;
+  // x transform through k_dct4_rows when Ux is the orthonormal DCT-IV basis
+  // and Nx is a power of two (twx: its twiddle table, type T)
+  bool fft_x;
+  int logMx;
+  void* twx;
+  int last_inner[8];  // iterations of the previous solve's inner PCG, per round
+  int round;          // refinement round of the running solve
   double *lx, *ly, *lz;  // eigenvalues (device, float64)
   // vectors
   double *x64, *b64, *r64;
@@ -507,6 +605,24 @@ static void launch_gemm(ProjHandle* h, const T* A, const T* B, T* C, int M,
   hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, s, a, A, B, C);
 }
 
+// x transform of every row: DCT-IV by FFT when available, else the GEMM
+// with B = Ux (forward; Ux is symmetric for the DCT-IV basis) or Ux^T.
+template <typename T>
+static void x_transform(ProjHandle* h, const T* in, T* out, bool inverse, hipStream_t s) {
+  const int Nx = h->g.nx, rows = h->g.nz * h->g.ny;
+  if (h->fft_x) {
+    const size_t shm = 4 * (size_t)Nx * sizeof(T);
+    hipLaunchKernelGGL(k_dct4_rows<T>, dim3((rows + 3) / 4), dim3(256), shm, s, rows, Nx,
+                       h->logMx, (const T*)h->twx, in, out);
+  } else if (!inverse) {
+    launch_gemm<T>(h, in, (const T*)h->Ux, out, rows, Nx, Nx, 1, 0, Nx, 1, 0, Nx, 1, 0, Nx,
+                   1, false, s);
+  } else {
+    launch_gemm<T>(h, in, (const T*)h->UxT, out, rows, Nx, Nx, 1, 0, Nx, 1, 0, Nx, 1, 0, Nx,
+                   1, false, s);
+  }
+}
+
 // z = M^{-1} r  (separable solve of fft_poisson_3d.py:184-228)
 template <typename T>
 static int precond_apply(ProjHandle* h, const T* r, T* z, hipStream_t s) {
@@ -515,8 +631,7 @@ static int precond_apply(ProjHandle* h, const T* r, T* z, hipStream_t s) {
   T* w1 = (T*)h->w1;
   T* w2 = (T*)h->w2;
   // x forward: w1[r][i'] = sum_i r[r][i] Ux[i][i']
-  launch_gemm<T>(h, r, (const T*)h->Ux, w1, Nz * Ny, Nx, Nx, 1, 0, Nx, 1, 0, Nx,
-                 1, 0, Nx, 1, false, s);
+  x_transform<T>(h, r, w1, false, s);
   const T* cur = w1;
   T* other = w2;
   if (Ny > 1) {  // y forward per k-plane: w2_k = Yf (Ny x Ny) * w1_k (Ny x Nx)
@@ -540,8 +655,7 @@ static int precond_apply(ProjHandle* h, const T* r, T* z, hipStream_t s) {
     c2 = t3;
   }
   // x inverse: z[r][i] = sum_i' c2[r][i'] Ux[i][i']  (B = Ux^T)
-  launch_gemm<T>(h, c2, (const T*)h->UxT, z, Nz * Ny, Nx, Nx, 1, 0, Nx, 1, 0, Nx,
-                 1, 0, Nx, 1, false, s);
+  x_transform<T>(h, c2, z, true, s);
   FW_CHECK_LAUNCH("precond gemm");
   return FW_OK;
 }
@@ -567,6 +681,7 @@ static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
   T* q = (T*)h->q;
   T* d = (T*)h->d;
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n)), dim3(256), 0, s, n, d);
+  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, h->S, tol_abs);
   int rc;
   if (h->precond_enabled) {
     if ((rc = precond_apply<T>(h, r, z, s))) return rc;
@@ -577,50 +692,71 @@ static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_RZ, F_STORE);
   hipLaunchKernelGGL(k_p_update<T>, dim3(nblk(n)), dim3(256), 0, s, n, p, z, h->S, 1);
   FW_CHECK_LAUNCH("pcg setup");
-  int it = 0;
-  for (; it < maxit;) {
-    hipLaunchKernelGGL((k_matvec_pq<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga,
-                       n, rho, p, q, h->eps_reg, h->part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0, F_ALPHA);
-    hipLaunchKernelGGL(k_update_xr<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
-                       h->S, h->part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_RR, F_STORE);
-    ++it;
-    if (h->precond_enabled) {
-      if ((rc = precond_apply<T>(h, r, z, s))) return rc;
-    } else {
-      FW_CHECK_HIP(hipMemcpyAsync(z, r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  // iterations in batches with one host read per batch; the device flag stops
+  // the Krylov updates at the first converged iteration, so the result is
+  // that of a one-by-one loop.  The first batch is the previous inner
+  // solve's iteration count (they change slowly from step to step), then
+  // one iteration at a time.
+  int queued = 0;
+  const int rd = h->round < 8 ? h->round : 7;
+  int batch = h->last_inner[rd] > 0 ? h->last_inner[rd] : 1;
+  for (;;) {
+    const int nb_it = maxit - queued < batch ? maxit - queued : batch;
+    batch = 1;
+    for (int b = 0; b < nb_it; ++b) {
+      hipLaunchKernelGGL((k_matvec_pq<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga,
+                         n, rho, p, q, h->eps_reg, h->part);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0,
+                         F_ALPHA);
+      hipLaunchKernelGGL(k_update_xr<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
+                         h->S, h->part);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0,
+                         F_RR);
+      if (h->precond_enabled) {
+        if ((rc = precond_apply<T>(h, r, z, s))) return rc;
+      } else {
+        FW_CHECK_HIP(hipMemcpyAsync(z, r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+      }
+      hipLaunchKernelGGL(k_dot_w<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n, r, z, h->part);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0,
+                         F_BETA);
+      hipLaunchKernelGGL(k_p_update<T>, dim3(nblk(n)), dim3(256), 0, s, n, p, z, h->S, 0);
     }
-    hipLaunchKernelGGL(k_dot_w<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n, r, z, h->part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0, F_BETA);
+    queued += nb_it;
     FW_CHECK_LAUNCH("pcg iteration");
     if ((rc = read_scalars(h, s))) return rc;
-    const double rr = h->S_host[S_RR];
-    if (!(rr == rr)) return fw_set_error(FW_E_NONFINITE, "projection: NaN residual");
-    if (sqrt(rr) <= tol_abs) break;
-    hipLaunchKernelGGL(k_p_update<T>, dim3(nblk(n)), dim3(256), 0, s, n, p, z, h->S, 0);
+    if (h->S_host[S_DONE] == 2.0) return fw_set_error(FW_E_NONFINITE, "projection: NaN residual");
+    if (h->S_host[S_DONE] != 0.0 || queued >= maxit) break;
   }
-  *iters = it;
+  *iters = (int)h->S_host[S_IT];
+  h->last_inner[rd] = *iters;
   return FW_OK;
 }
 
 // Solve A x = b: x64 (warm start on entry), b64 given; ||b - A x||_2 <=
 // rtol ||b||_2 with the residual recomputed in float64 after each inner solve.
+// r0 >= 0: ||b - A x|| of the warm start, already computed with r64 and r
+// in place (saves one host read); r0 < 0: compute it here.
 template <typename T, typename TR>
 static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm,
-                         hipStream_t s) {
+                         double r0, hipStream_t s) {
   const size_t n = h->n;
   const double tol_abs = rtol * bnorm;
   const double inner_floor = (sizeof(T) == 4) ? 2e-5 : 0.0;
   int total = 0, outer = 0, rc;
   double rnorm = 0.0;
   for (;;) {
-    hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga,
-                       n, rho, h->b64, h->x64, h->r64, (T*)h->r, h->eps_reg, h->part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP, F_STORE);
-    FW_CHECK_LAUNCH("residual64");
-    if ((rc = read_scalars(h, s))) return rc;
-    rnorm = sqrt(h->S_host[S_TMP]);
+    if (outer == 0 && r0 >= 0.0) {
+      rnorm = r0;
+    } else {
+      hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga,
+                         n, rho, h->b64, h->x64, h->r64, (T*)h->r, h->eps_reg, h->part);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP,
+                         F_STORE);
+      FW_CHECK_LAUNCH("residual64");
+      if ((rc = read_scalars(h, s))) return rc;
+      rnorm = sqrt(h->S_host[S_TMP]);
+    }
     if (!(rnorm == rnorm)) return fw_set_error(FW_E_NONFINITE, "projection: NaN residual");
     if (rnorm <= tol_abs || bnorm == 0.0) break;
     if (total >= h->maxiter || outer >= 20) {
@@ -632,6 +768,7 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm
     double inner_rel = tol_abs / rnorm;
     if (inner_rel < inner_floor) inner_rel = inner_floor;
     int it = 0;
+    h->round = outer;
     if ((rc = pcg_inner<T, TR>(h, rho, inner_rel * rnorm, h->maxiter - total, &it, s)))
       return rc;
     total += it;
@@ -660,9 +797,16 @@ static int project_impl(ProjHandle* h, void* u, void* v, void* w,
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64);
     h->have_prev = true;
   }
+  // the warm start's residual, read together with ||b||
+  hipLaunchKernelGGL((k_residual64<T, T>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
+                     (const T*)rho, h->b64, h->x64, h->r64, (T*)h->r, h->eps_reg, h->part);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP,
+                     F_STORE);
+  FW_CHECK_LAUNCH("residual64");
   int rc = read_scalars(h, s);
   if (rc) return rc;
   const double bnorm = sqrt(h->S_host[S_BB]);
+  const double r0 = sqrt(h->S_host[S_TMP]);
   if (!(bnorm == bnorm)) return fw_set_error(FW_E_NONFINITE, "projection: NaN divergence");
   if (bnorm == 0.0) {
     // scipy's bicgstab returns x = 0 for b = 0
@@ -670,7 +814,7 @@ static int project_impl(ProjHandle* h, void* u, void* v, void* w,
     h->last_iters = 0;
     h->last_rel = 0.0;
   } else {
-    rc = solve_refined<T, T>(h, (const T*)rho, h->rtol, bnorm, s);
+    rc = solve_refined<T, T>(h, (const T*)rho, h->rtol, bnorm, r0, s);
     if (rc) return rc;
   }
   hipLaunchKernelGGL((k_correct<T, T>), dim3(nblk(n)), dim3(256), 0, s, h->ga, n,
@@ -753,6 +897,41 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
         (rc = upload_cast<float>(&h->Zi, Zi.data(), (size_t)Nz * Nz)))
       return rc;
   }
+  // FFT path for x when Ux is the orthonormal DCT-IV matrix and Nx = 2^p >= 4
+  {
+    bool pow2 = Nx >= 4 && (Nx & (Nx - 1)) == 0;
+    double dmax = 0.0;
+    for (int a = 0; pow2 && a < Nx; ++a)
+      for (int b = 0; b < Nx; ++b) {
+        const double e = sqrt(2.0 / Nx) * cos(M_PI * (a + 0.5) * (b + 0.5) / Nx);
+        const double d = fabs(e - Ux[(size_t)a * Nx + b]);
+        dmax = d > dmax ? d : dmax;
+      }
+    h->fft_x = pow2 && dmax < 1e-12 && 4 * (size_t)Nx * h->esz <= 65536;
+    if (h->fft_x) {
+      const int Mx = Nx / 2;
+      int lg = 0;
+      while ((1 << lg) < Mx) ++lg;
+      h->logMx = lg;
+      std::vector<double> tw(2 * Mx + 2 * Mx + Mx);
+      for (int q = 0; q < Mx; ++q) {
+        const double a1 = -M_PI * (4.0 * q + 1.0) / (4.0 * Nx);
+        const double a2 = -M_PI * q / (double)Nx;
+        tw[2 * q] = cos(a1);
+        tw[2 * q + 1] = sin(a1);
+        tw[2 * Mx + 2 * q] = cos(a2);
+        tw[2 * Mx + 2 * q + 1] = sin(a2);
+      }
+      for (int q = 0; q < Mx / 2; ++q) {
+        const double a3 = -2.0 * M_PI * q / (double)Mx;
+        tw[4 * Mx + 2 * q] = cos(a3);
+        tw[4 * Mx + 2 * q + 1] = sin(a3);
+      }
+      if (g->dtype == FW_F64) rc = upload_cast<double>(&h->twx, tw.data(), tw.size());
+      else rc = upload_cast<float>(&h->twx, tw.data(), tw.size());
+      if (rc) return rc;
+    }
+  }
   FW_CHECK_HIP(hipMalloc((void**)&h->lx, Nx * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->ly, Ny * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->lz, Nz * sizeof(double)));
@@ -778,7 +957,7 @@ int fw_proj_destroy(void* handle) {
   if (!h) return FW_OK;
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
                   h->x64, h->b64, h->r64, h->r, h->z, h->p, h->q, h->d, h->w1,
-                  h->w2, h->part, h->S};
+                  h->w2, h->part, h->S, h->twx};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (h->S_host) (void)hipHostFree(h->S_host);
@@ -891,8 +1070,8 @@ int fw_proj_solve(void* handle, const void* rho, const double* b, double* x,
   const double bnorm = sqrt(h->S_host[S_BB]);
   h->have_prev = true;
   int rc = h->dtype == FW_F64
-               ? solve_refined<double, double>(h, (const double*)rho, rtol, bnorm, s)
-               : solve_refined<float, float>(h, (const float*)rho, rtol, bnorm, s);
+               ? solve_refined<double, double>(h, (const double*)rho, rtol, bnorm, -1.0, s)
+               : solve_refined<float, float>(h, (const float*)rho, rtol, bnorm, -1.0, s);
   if (iters) *iters = h->last_iters;
   if (rel_resid) *rel_resid = h->last_rel;
   if (rc) return rc;

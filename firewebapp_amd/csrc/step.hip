@@ -12,9 +12,10 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_prologue(GridArgs g, const T* __restrict__ Tg, T* __restrict__ rho,
            T* __restrict__ rho_prev, T* __restrict__ YO2, T T_amb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
-  const size_t c = cidx(g, blockIdx.z, blockIdx.y, i);
+  const size_t c = cidx(g, k, j, i);
   rho_prev[c] = rho[c];
   const T t = Tg[c];
   const T tm = (t > T_amb || t != t) ? t : T_amb;
@@ -28,8 +29,8 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_bed_qext(GridArgs g, int nzb, const T* __restrict__ q_rad, T* __restrict__ Qext,
            int i0, int i1, double drip, double qmax) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx || k >= nzb) return;
   const size_t c = cidx(g, k, j, i);
   double v = 0.0;
@@ -68,8 +69,8 @@ k_gas_update(GridArgs g, T* __restrict__ Tg, const T* __restrict__ Ts,
              const T* __restrict__ Qgc, const T* __restrict__ qrg,
              T* __restrict__ Qcomb_out, int nzb, int i0, int i1, T drip, T dt_sub,
              T T_amb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const size_t c = cidx(g, k, j, i);
   T Q = T(0);
@@ -229,8 +230,8 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_lset(GridArgs g, const T* __restrict__ phi, const T* __restrict__ phi0,
        const T* __restrict__ vn, T* __restrict__ out, T step, int mode) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t c = cidx(g, k, j, i);

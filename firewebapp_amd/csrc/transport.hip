@@ -21,8 +21,8 @@ __global__ void __launch_bounds__(256)
 k_advect_muscl(GridArgs g, const T* __restrict__ phi, const T* __restrict__ u,
                const T* __restrict__ v, const T* __restrict__ w,
                T* __restrict__ rhs, T phi_inlet) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t c = cidx(g, k, j, i);
@@ -72,8 +72,8 @@ k_species(GridArgs g, const T* __restrict__ Y, T* __restrict__ Yo,
           const T* __restrict__ v, const T* __restrict__ w,
           const T* __restrict__ S, const T* __restrict__ Sp,
           const T* __restrict__ Sd, int kind, T dt, T D, T Y_inlet, T y_max) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t c = cidx(g, k, j, i);
@@ -145,8 +145,8 @@ __global__ void __launch_bounds__(256)
 k_gas_energy(GridArgs g, const T* __restrict__ Tg, T* __restrict__ To,
              const T* __restrict__ u, const T* __restrict__ v,
              const T* __restrict__ w, T dt, T alpha_th, T T_amb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t c = cidx(g, k, j, i);
@@ -214,8 +214,8 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_turb_diff(GridArgs g, const T* __restrict__ f, T* __restrict__ fo,
             const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo, T hi) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t c = cidx(g, k, j, i);
@@ -281,8 +281,8 @@ __global__ void __launch_bounds__(256)
 k_solid_cond(GridArgs g, const T* __restrict__ Ts, T* __restrict__ To,
              const T* __restrict__ alpha, T k_solid, T inv_rho_cp, T dt,
              T lo, T hi) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const int Nz = g.nz;
   const size_t c = cidx(g, k, j, i);
@@ -326,9 +326,10 @@ k_drag(GridArgs g, const T* __restrict__ u, const T* __restrict__ v,
        const T* __restrict__ w, const T* __restrict__ rho,
        const T* __restrict__ alpha, T sigma_sav, T C_D, T* __restrict__ Fx,
        T* __restrict__ Fy, T* __restrict__ Fz) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
-  const size_t c = cidx(g, blockIdx.z, blockIdx.y, i);
+  const size_t c = cidx(g, k, j, i);
   const T a = alpha[c];
   if (a <= T(0)) {
     Fx[c] = T(0); Fy[c] = T(0); Fz[c] = T(0);
@@ -359,8 +360,8 @@ k_momentum(GridArgs g, const T* __restrict__ u, const T* __restrict__ v,
            const T* __restrict__ Fy, const T* __restrict__ Fz,
            const T* __restrict__ u_in, const T* __restrict__ v_in,
            const T* __restrict__ w_in, T dt, T T_amb) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t c = cidx(g, k, j, i);
@@ -427,8 +428,8 @@ __global__ void __launch_bounds__(256)
 k_sponge(GridArgs g, T* __restrict__ u, const T* __restrict__ u_target,
          const T* __restrict__ sigma_x, const T* __restrict__ Y_F, T Y_F_skip,
          T dt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
+  int i, j, k;
+  cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const T sig = sigma_x[i];
   if (!(sig > T(0))) return;

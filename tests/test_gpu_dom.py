@@ -78,3 +78,37 @@ def test_dom_vs_oracle_warm_repeat(N):
         assert gpu.last_iters == ref.last_iters
         for a, b in zip(out[1], out[0]):
             assert relerr(a, b) <= 1e-10
+
+
+@pytest.mark.parametrize("shape,dtype", [((7, 6, 70), np.float64),    # 4x4 blocks, ragged
+                                         ((5, 3, 520), np.float32),    # blocks, 16 cells/lane
+                                         ((3, 2, 600), np.float64)])   # row kernel (LDS > 64 KB)
+def test_dom_sweep_shapes_vs_oracle(shape, dtype):
+    """Block edges (Ny, Nz not multiples of 4), partial lanes, many cells per
+    lane, the small-Ny stale wrap inside one block, and the per-row fallback
+    used when a block's rows do not fit in 64 KB of LDS."""
+    import torch
+    rng = np.random.default_rng(5)
+    nz, ny, nx = shape
+    dz = 0.03 + 0.07 * rng.random(nz)
+    alpha = np.zeros(shape)
+    alpha[:2] = 0.002 + 0.01 * rng.random((2, ny, nx))
+    T_s = 300.0 + 800.0 * rng.random(shape) * (alpha > 0)
+    T_g = 300.0 + 1200.0 * rng.random(shape)
+    om = 5e-3 * rng.random(shape)
+    Tsoil = 300.0 + 50.0 * rng.random((ny, nx))
+    # (dy, dx) = (0.12, 0.1) in both constructors
+    ref = D.DOMRadiationSolver(*shape, 0.12, 0.1, dz, N_quadrature=8)
+    gpu = _solver(*shape, 0.12, 0.1, dz, dz, dz, "periodic", 8)
+    qs, qg, qsoil = np.zeros(shape), np.zeros(shape), np.zeros((ny, nx))
+    ref.solve(T_s, T_g, alpha, om, 2000.0, 300.0, qs, qg, Tsoil, qsoil)
+    cv = (lambda a: a) if dtype == np.float64 else \
+        (lambda a: torch.as_tensor(a.astype(np.float32), device="cuda"))
+    gs, gg, gsoil = cv(np.zeros(shape)), cv(np.zeros(shape)), cv(np.zeros((ny, nx)))
+    gpu.solve(cv(T_s), cv(T_g), cv(alpha), cv(om), 2000.0, 300.0, gs, gg, cv(Tsoil), gsoil)
+    back = (lambda a: a) if dtype == np.float64 else (lambda a: a.double().cpu().numpy())
+    tol = 1e-10 if dtype == np.float64 else 2e-4
+    if dtype == np.float64:
+        assert gpu.last_iters == ref.last_iters
+    for a, b in zip((back(gs), back(gg), back(gsoil)), (qs, qg, qsoil)):
+        assert relerr(a, b) <= tol
