@@ -13,8 +13,10 @@ with its 10 chemistry/transport sub-steps.  ``value`` = cells x steps / wall
 seconds over exactly K timed steps after W warm-up steps, inputs resident in
 HBM.  The JSON line also carries:
 * roofline: the dominant kernel's algorithmic bytes per launch / its mean
-  launch time, both measured here with HIP events on the stream the kernel
-  is launched on (torch's current stream), against 8 TB/s;
+  launch time over the timed steps, measured with HIP events that the launch
+  itself records (hipExtLaunchKernel, on the stream the kernel is launched
+  on: torch's current stream), against 8 TB/s; traffic is the HBM bytes per
+  launch from the FETCH_SIZE and WRITE_SIZE counters;
 * cpu_baseline: the oracle's float64 NumPy step on one host core, timed on a
   bounded sample of the same workload (rank 0, N = 1 only).
 """
@@ -61,12 +63,13 @@ COMMON = dict(
 WIND = 4.0
 
 # the roofline kernel and its algorithmic fields per cell per launch:
-# k_species reads Y, rho, u, v, w, S_pyro part, S_dry part and writes Y_out
-ROOFLINE_KERNEL = ("k_species", 8)
-# HBM bytes per k_species launch at the default workload (float32), from the
+# k_species3 reads Y_fuel, Y_O2, Y_H2O, rho, u, v, w, S_pyro part, S_dry part
+# and writes the three species
+ROOFLINE_KERNEL = ("k_species3", 12)
+# HBM bytes per k_species3 launch at the default workload (float32), from the
 # FETCH_SIZE + WRITE_SIZE counters scaled by a 256 MiB copy:
 # profiles/step_256x64x48_s8_hbm_traffic.md
-ROOFLINE_TRAFFIC = {("spread3d_256x64x48_s8", "float32"): (31.27 + 3.00) * 2**20}
+ROOFLINE_TRAFFIC = {("spread3d_256x64x48_s8", "float32"): 43337925}
 
 
 def deck(name, dtype):

@@ -46,6 +46,7 @@ class FwGrid(C.Structure):
         ("dz", C.c_void_p), ("d_above", C.c_void_p), ("d_below", C.c_void_p),
         ("z_face", C.c_void_p),
         ("n_z_bed", C.c_int32), ("reserved", C.c_int32),
+        ("inv_dz", C.c_void_p), ("inv_d_above", C.c_void_p), ("inv_d_below", C.c_void_p),
     ]
 
 

@@ -376,7 +376,7 @@ k_bed_hcond(GridArgs g, int nzb, const T* __restrict__ To, T* __restrict__ Tn,
     dTc[c] = 0.0;
     return;
   }
-  const double inv_dx2 = 1.0 / (g.dx * g.dx), inv_dy2 = 1.0 / (g.dy * g.dy);
+  const double inv_dx2 = g.inv_dx2, inv_dy2 = g.inv_dy2;
   const double Tc = (double)To[c];
   const double Txp = (i + 1 < Nx && (double)alpha[c + 1] > 0.0) ? (double)To[c + 1] : Tc;
   const double Txm = (i - 1 >= 0 && (double)alpha[c - 1] > 0.0) ? (double)To[c - 1] : Tc;
@@ -484,6 +484,7 @@ extern "C" {
 
 int fw_bed_locate(const fw_grid* g, int n, const double* x, const double* y,
                   const double* z, int* cell_step, int* cell_agg, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && n >= 0, "bed_locate: bad argument");
   if (n == 0) return FW_OK;
   hipLaunchKernelGGL(k_bed_locate, dim3((n + 255) / 256), dim3(256), 0,
@@ -497,6 +498,7 @@ int fw_bed_step(const fw_grid* g, const fw_bed_particles* particles,
                 const void* T_g, const void* Y_O2, const void* Q_solid_ext,
                 void* const* sources, double* contrib_work, int* counters,
                 double* diag_work, double* diag_out, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && particles && index && params && T_g && Y_O2 && Q_solid_ext && sources &&
                  contrib_work && counters,
              "bed_step: null argument");
@@ -509,6 +511,7 @@ int fw_bed_step(const fw_grid* g, const fw_bed_particles* particles,
 int fw_bed_aggregate(const fw_grid* g, const fw_bed_particles* particles,
                      const fw_bed_index* agg_index, void* T_s_grid,
                      void* M_local_grid, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && particles && agg_index, "bed_aggregate: null argument");
   if (agg_index->n_seg_cells <= 0) return FW_OK;
   BedArgs B{particles->x, particles->y, particles->z, particles->m_solid,
@@ -533,6 +536,7 @@ int fw_bed_horizontal_conduction(const fw_grid* g, const fw_bed_particles* parti
                                  void* T_s_grid, const void* alpha_s, double k_solid,
                                  double rho_solid_true, double cp_solid, int n_z_bed,
                                  double dt, double* dT_work, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && particles && agg_index && T_s_old && T_s_grid && alpha_s && dT_work,
              "bed_horizontal_conduction: null argument");
   FW_REQUIRE(T_s_old != T_s_grid, "bed_horizontal_conduction: T_s_old must not alias");

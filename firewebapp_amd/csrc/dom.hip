@@ -89,6 +89,51 @@ k_dom_ground_init(int ncol, const T* __restrict__ Tsoil, T eps, T I_amb,
   out[c] = e;
 }
 
+// Cross-lane moves for the wavefront scans, as DPP modifiers of a VALU move
+// (no LDS round trip).  Lanes whose source lies outside the pattern, or
+// whose row is masked off, keep `old`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_mov(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v),
+                                                    CTRL, ROWS, 0xF, false));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_mov(double old, double v) {
+  const unsigned long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)x,
+                                                            CTRL, ROWS, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32),
+                                                            (int)(unsigned)(x >> 32), CTRL,
+                                                            ROWS, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// (A, Bc) o (pa, pb): x -> A (pa x + pb) + Bc, with the lane CTRL names as
+// the earlier map; lanes without one compose with the identity (1, 0).
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ void compose_from(T& A, T& Bc) {
+  const T pa = dpp_mov<CTRL, ROWS>(T(1), A);
+  const T pb = dpp_mov<CTRL, ROWS>(T(0), Bc);
+  Bc = A * pb + Bc;
+  A = A * pa;
+}
+
+// Inclusive scan of affine maps over the 64 lanes: shifts of 1, 2, 4, 8
+// inside each row of 16 lanes, then lane 15 of rows 0 and 2 into rows 1 and
+// 3, then lane 31 into rows 2 and 3.  On return (eA, eB) is the exclusive
+// prefix (the identity on lane 0).
+template <typename T>
+__device__ __forceinline__ void affine_scan64(T& A, T& Bc, T& eA, T& eB) {
+  compose_from<0x111, 0xF>(A, Bc);   // row_shr:1
+  compose_from<0x112, 0xF>(A, Bc);   // row_shr:2
+  compose_from<0x114, 0xF>(A, Bc);   // row_shr:4
+  compose_from<0x118, 0xF>(A, Bc);   // row_shr:8
+  compose_from<0x142, 0xA>(A, Bc);   // row_bcast:15 into rows 1, 3
+  compose_from<0x143, 0xC>(A, Bc);   // row_bcast:31 into rows 2, 3
+  eA = dpp_mov<0x138, 0xF>(T(1), A); // wave_shr:1
+  eB = dpp_mov<0x138, 0xF>(T(0), Bc);
+}
+
 // One anti-diagonal d of the row wavefront, all ordinates.  One wavefront
 // per (row, ordinate), four ordinates per workgroup.  Lane l owns the PER
 // consecutive cells [l PER, (l+1) PER) of the row in sweep order: it folds
@@ -144,20 +189,11 @@ k_dom_sweep_diag(GridArgs g, int d, int jlo, int M, const double* __restrict__ O
       Izs[q] = Iz;
     }
   }
-  // inclusive scan of the maps over the wavefront
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const T pa = __shfl_up(A, off, 64);
-    const T pb = __shfl_up(Bc, off, 64);
-    if (lane >= off) {
-      Bc = A * pb + Bc;   // (A, Bc) o (pa, pb): x -> A (pa x + pb) + Bc
-      A = A * pa;
-    }
-  }
-  // exclusive prefix applied to the x inflow I_left = I_right = I_amb
-  // (dom_3d.py:376, :424)
-  const T eA = __shfl_up(A, 1, 64), eB = __shfl_up(Bc, 1, 64);
-  T Iprev = lane == 0 ? I_amb : eA * I_amb + eB;
+  // scan of the maps over the wavefront; the exclusive prefix applied to
+  // the x inflow I_left = I_right = I_amb (dom_3d.py:376, :424)
+  T eA, eB;
+  affine_scan64(A, Bc, eA, eB);
+  T Iprev = eA * I_amb + eB;
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
     const int s = lane * PER + q;
@@ -175,110 +211,134 @@ k_dom_sweep_diag(GridArgs g, int d, int jlo, int M, const double* __restrict__ O
 }
 
 // Tiled wavefront: one workgroup sweeps a 4 x 4 block of rows (sweep-ordered
-// j' in [4 tj, 4 tj + 4), k' in [4 tk, 4 tk + 4)) of one ordinate, one
-// wavefront per row.  Blocks on the block anti-diagonal tj + tk = D are
-// independent, so Ny/4 + Nz/4 - 1 launches make a source iteration.  Inside
-// the block the seven row diagonals run in order; a row's y and z inflows
-// come from LDS when they lie in the block and from global memory (written by
-// an earlier launch, or the stale periodic wrap / boundary) otherwise.  All
-// global inputs are read before any row of the block is written, and the
-// rows are stored at the end, so the stale wrap of small Ny still reads the
-// previous iteration.
-template <typename T, int PER>
-__global__ void __launch_bounds__(1024)
+// j' in [4 tj, 4 tj + 4), k' in [4 tk, 4 tk + 4)) of one ordinate.  Blocks on
+// the block anti-diagonal tj + tk = D are independent, so Ny/4 + Nz/4 - 1
+// launches make a source iteration.  Wavefront w owns the column j' = 4 tj + w
+// and walks k' upward: at stage s (0..6) it computes row (w, s - w), so the
+// row diagonals of the block run in order with every wavefront busy on the
+// middle stages.  The z inflow of a row is the same wavefront's previous row
+// (registers); the y inflow is wavefront w - 1's row of the previous stage
+// (LDS, one slot per row).  Inflows from outside the block come from global
+// memory: rows written by earlier launches, the ground or top boundary, or
+// the stale periodic wrap.  All global inputs are read before any row of the
+// block is stored, so the wrap of small Ny still reads the previous
+// iteration.  One reciprocal per cell: a = ax r, b = (...) r, I = num r with
+// r = 1 / (kappa + ax + ay + az).
+template <typename T, int PER, bool FULL>
+__global__ void __launch_bounds__(256)
 k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ Om,
                  T* __restrict__ Iset, const T* __restrict__ kap,
                  const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb) {
   extern __shared__ unsigned char smem_raw[];
-  T* sh = reinterpret_cast<T*>(smem_raw);   // 16 rows of Nx
+  T* sh = reinterpret_cast<T*>(smem_raw);   // 16 rows in sweep order: slot (w, kk)
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int jj = wv & 3, kk = wv >> 2;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int n = blockIdx.y;
   const int tj = tjlo + blockIdx.x, tk = D - tj;
-  const int jp = 4 * tj + jj, kp = 4 * tk + kk;
-  const bool active = jp < Ny && kp < Nz;     // uniform per wavefront
+  const int jp = 4 * tj + w;
+  const bool col = jp < Ny;                 // uniform per wavefront
   const double xi = Om[3 * n], eta = Om[3 * n + 1], mu = Om[3 * n + 2];
   const int j = eta >= 0.0 ? jp : Ny - 1 - jp;
-  const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
-  const size_t plane = (size_t)Nx * Ny;
-  T* In = Iset + (size_t)n * plane * Nz;
+  T* In = Iset + (size_t)n * Nx * Ny * Nz;
   const bool fwd = xi >= 0.0;
-  T kv[PER], bv[PER], Iy[PER], Iz[PER], out[PER];
-  T ax = T(0), ay = T(0), az = T(0);
-  size_t row = 0;
-  if (active) {
-    ax = T(fabs(xi) / g.dx);
-    ay = T(fabs(eta) / g.dy);
-    az = T(fabs(mu) / g.dz[k]);
-    const int jy = eta >= 0.0 ? (j > 0 ? j - 1 : Ny - 1) : (j < Ny - 1 ? j + 1 : 0);
-    const bool zb = mu >= 0.0 ? (k == 0) : (k == Nz - 1);
-    const int kz = mu >= 0.0 ? k - 1 : k + 1;
-    row = ((size_t)k * Ny + j) * Nx;
-    const size_t rowy = ((size_t)k * Ny + jy) * Nx;
-    const size_t rowz = zb ? 0 : ((size_t)kz * Ny + j) * Nx;
+  const T ax = T(fabs(xi) * g.inv_dx);
+  const T ay = T(fabs(eta) * g.inv_dy);
+  const int jy = eta >= 0.0 ? (j > 0 ? j - 1 : Ny - 1) : (j < Ny - 1 ? j + 1 : 0);
+  // x index of each of the lane's cells (sweep position s = lane PER + q);
+  // byte offsets within one ordinate's field fit 32 bits (field < 4 GiB)
+  constexpr uint32_t E = sizeof(T);
+  uint32_t ix[PER];
+  bool ok[PER];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int s = lane * PER + q;
-      if (s < Nx) {
-        const int i = fwd ? s : Nx - 1 - s;
-        kv[q] = kap[row + i];
-        bv[q] = B[row + i];
-        if (jj == 0) Iy[q] = In[rowy + i];
-        if (kk == 0)
-          Iz[q] = zb ? (mu >= 0.0 ? Ig_out[(size_t)j * Nx + i] : I_amb) : In[rowz + i];
+  for (int q = 0; q < PER; ++q) {
+    const int sq = lane * PER + q;
+    ok[q] = FULL || sq < Nx;
+    ix[q] = (uint32_t)(fwd ? sq : Nx - 1 - sq);
+  }
+  // global inputs read up front, so each stage is pure arithmetic: kappa and
+  // B of the column's four rows, the y inflow of the column's rows (w == 0)
+  // and the z inflow of its first row (kk == 0)
+  T Iyg[4][PER], Iz[PER], kvs[4][PER], bvs[4][PER];
+  if (col) {
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int kp = 4 * tk + kk;
+      if (kp < Nz) {
+        const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
+        const uint32_t row = ((uint32_t)k * Ny + j) * Nx;
+        const uint32_t rowy = ((uint32_t)k * Ny + jy) * Nx;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          if (ok[q]) {
+            kvs[kk][q] = at(kap, (row + ix[q]) * E);
+            bvs[kk][q] = at(B, (row + ix[q]) * E);
+            if (w == 0) Iyg[kk][q] = at((const T*)In, (rowy + ix[q]) * E);
+          }
+        }
       }
     }
+    const int kp0 = 4 * tk;
+    const int k0 = mu >= 0.0 ? kp0 : Nz - 1 - kp0;
+    const bool zb = mu >= 0.0 ? (k0 == 0) : (k0 == Nz - 1);
+    const int kz = mu >= 0.0 ? k0 - 1 : k0 + 1;
+    const uint32_t rowz = zb ? 0u : ((uint32_t)kz * Ny + j) * Nx;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      if (ok[q])
+        Iz[q] = zb ? (mu >= 0.0 ? at(Ig_out, ((uint32_t)j * Nx + ix[q]) * E) : I_amb)
+                   : at((const T*)In, (rowz + ix[q]) * E);
+    }
   }
+  __syncthreads();   // every global inflow is read before any row is stored
   for (int st = 0; st < 7; ++st) {
-    if (active && jj + kk == st) {
+    const int kk = st - w;
+    const int kp = 4 * tk + kk;
+    if (col && kk >= 0 && kk < 4 && kp < Nz) {
+      const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
+      const T az = T(fabs(mu) * g.inv_dz[k]);
+      const uint32_t row = ((uint32_t)k * Ny + j) * Nx;
+      const T* shy = sh + ((w - 1) * 4 + kk) * Nx;   // wavefront w - 1's row
+      T* sho = sh + (w * 4 + kk) * Nx;
+      T num0[PER], rc[PER];
       T A = T(1), Bc = T(0);
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
-        const int s = lane * PER + q;
-        if (s < Nx) {
-          const int i = fwd ? s : Nx - 1 - s;
-          if (jj > 0) Iy[q] = sh[(wv - 1) * Nx + i];
-          if (kk > 0) Iz[q] = sh[(wv - 4) * Nx + i];
-          const T den = kv[q] + ax + ay + az;
-          const T a = ax / den;
-          const T b = (kv[q] * bv[q] + ay * Iy[q] + az * Iz[q]) / den;
+        if (ok[q]) {
+          const T kv = kvs[0][q];
+          const T Iy = w == 0 ? Iyg[0][q] : shy[lane * PER + q];
+          rc[q] = recip(kv + ax + ay + az);
+          num0[q] = kv * bvs[0][q] + ay * Iy + az * Iz[q];
+          const T a = ax * rc[q];
+          const T b = num0[q] * rc[q];
           Bc = a * Bc + b;
           A = a * A;
         }
       }
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const T pa = __shfl_up(A, off, 64);
-        const T pb = __shfl_up(Bc, off, 64);
-        if (lane >= off) {
-          Bc = A * pb + Bc;
-          A = A * pa;
-        }
-      }
-      const T eA = __shfl_up(A, 1, 64), eB = __shfl_up(Bc, 1, 64);
-      T Iprev = lane == 0 ? I_amb : eA * I_amb + eB;
+      T eA, eB;
+      affine_scan64(A, Bc, eA, eB);
+      T Iprev = eA * I_amb + eB;
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
-        const int s = lane * PER + q;
-        if (s < Nx) {
-          const int i = fwd ? s : Nx - 1 - s;
-          const T num = kv[q] * bv[q] + ax * Iprev + ay * Iy[q] + az * Iz[q];
-          const T v = num / (kv[q] + ax + ay + az);
-          out[q] = v;
-          sh[wv * Nx + i] = v;
+        if (ok[q]) {
+          const T v = (num0[q] + ax * Iprev) * rc[q];
+          Iz[q] = v;                      // z inflow of the next row up
+          sho[lane * PER + q] = v;        // y inflow of wavefront w + 1
+          at(In, (row + ix[q]) * E) = v;
           Iprev = v;
         }
       }
+      // the wavefront's rows run in order kk = 0..3: shift the prefetched
+      // inputs down so the next row's sit in slot 0 (static register indices)
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          kvs[r][q] = kvs[r + 1][q];
+          bvs[r][q] = bvs[r + 1][q];
+          Iyg[r][q] = Iyg[r + 1][q];
+        }
     }
     __syncthreads();
-  }
-  if (active) {
-#pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int s = lane * PER + q;
-      if (s < Nx) In[row + (fwd ? s : Nx - 1 - s)] = out[q];
-    }
   }
 }
 
@@ -404,9 +464,15 @@ static inline unsigned nb(size_t n) { return (unsigned)((n + 255) / 256); }
 template <typename T, int PER>
 static void launch_tile(DomHandle* h, int D, int tjlo, int ntiles, T I_amb, hipStream_t s) {
   const size_t shm = 16 * (size_t)h->g.nx * sizeof(T);
-  hipLaunchKernelGGL((k_dom_sweep_tile<T, PER>), dim3(ntiles, h->M), dim3(1024), shm, s,
-                     h->ga, D, tjlo, h->M, (const double*)h->Om, (T*)h->I,
-                     (const T*)h->kap_safe, (const T*)h->B, (const T*)h->Ig_out, I_amb);
+  const dim3 grid(ntiles, h->M);
+  if (h->g.nx == 64 * PER)
+    hipLaunchKernelGGL((k_dom_sweep_tile<T, PER, true>), grid, dim3(256), shm, s, h->ga, D,
+                       tjlo, h->M, (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
+                       (const T*)h->B, (const T*)h->Ig_out, I_amb);
+  else
+    hipLaunchKernelGGL((k_dom_sweep_tile<T, PER, false>), grid, dim3(256), shm, s, h->ga, D,
+                       tjlo, h->M, (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
+                       (const T*)h->B, (const T*)h->Ig_out, I_amb);
 }
 
 template <typename T>
@@ -414,7 +480,8 @@ static bool tile_fits(const DomHandle* h) {
   return 16 * (size_t)h->g.nx * sizeof(T) <= 65536;
 }
 
-// One source iteration of the tiled sweep (cells per lane as below).
+// One source iteration of the tiled sweep (cells per lane as below; a float64
+// row of more than 512 cells does not fit the LDS and never gets here).
 template <typename T>
 static int sweep_tiled(DomHandle* h, T I_amb, hipStream_t s) {
   const int Nx = h->g.nx;
@@ -427,7 +494,7 @@ static int sweep_tiled(DomHandle* h, T I_amb, hipStream_t s) {
     else if (Nx <= 128) launch_tile<T, 2>(h, D, tjlo, nt, I_amb, s);
     else if (Nx <= 256) launch_tile<T, 4>(h, D, tjlo, nt, I_amb, s);
     else if (Nx <= 512) launch_tile<T, 8>(h, D, tjlo, nt, I_amb, s);
-    else launch_tile<T, 16>(h, D, tjlo, nt, I_amb, s);
+    else if constexpr (sizeof(T) == 4) launch_tile<T, 16>(h, D, tjlo, nt, I_amb, s);
   }
   return FW_OK;
 }
@@ -552,6 +619,8 @@ extern "C" {
 int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* weights,
                   double eps_w_ground, int max_source_iter, double tol_source_iter,
                   double omega_relax, double kappa_gas_max, void** handle) {
+  FW_GRID(g);
+  FW_GRID_OFFSETS32(g);
   FW_REQUIRE(g && Omega && weights && handle && M > 0, "dom_create: bad argument");
   FW_REQUIRE(g->nx <= 64 * 32, "dom_create: Nx > 2048 not supported by the row scan");
   DomHandle* h = new DomHandle();

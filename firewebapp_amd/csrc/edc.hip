@@ -223,7 +223,8 @@ extern "C" int fw_edc(const fw_grid* g, const void* rho, void* T_g,
                       double chi_rad, double cp_g, double dt, int n_substeps,
                       int extinction_enable, double s_stoich, double hoc_J,
                       int* work, void* stream) {
-  FW_REQUIRE(g && work, "edc: null grid or workspace");
+  FW_GRID(g);
+  FW_REQUIRE(work, "edc: null workspace");
   fw::EdcParams P{chi_rad, cp_g, dt, s_stoich, hoc_J, n_substeps,
                   extinction_enable ? 1 : 0};
   return FW_DISPATCH(g, fw::edc_impl, g, rho, T_g, Y_fuel, Y_O2, k_turb,

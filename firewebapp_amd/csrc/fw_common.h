@@ -7,6 +7,7 @@
 // a template over that type and is instantiated for float and double.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 #include <stdio.h>
 #include "../../include/firewave.h"
@@ -22,6 +23,14 @@ struct Launch {
   dim3 grid, block;
 };
 
+// Events on which the next species launch records its own start and stop
+// (the dispatch's timestamps, as a kernel trace sees them), or null.  Set by
+// the step driver's optional timer around one call, cleared after it.
+struct KernelEvents {
+  hipEvent_t start = nullptr, stop = nullptr;
+};
+extern KernelEvents g_species3_events;
+
 inline Launch cell_launch(const fw_grid* g) {
   int bx = 64;
   while (bx < g->nx && bx < 256) bx *= 2;
@@ -32,21 +41,31 @@ inline Launch cell_launch(const fw_grid* g) {
 }
 
 // Plain-value view of fw_grid passed to kernels by value.
+// The reciprocals are the host's IEEE float64 quotients (1.0 / dx, ...), the
+// same values the kernels would divide out per thread, so using them keeps
+// every float64 result unchanged while saving the per-thread divisions.
 struct GridArgs {
   int nx, ny, nz, n_z_bed;
   double dx, dy;
+  double inv_dx, inv_dy, inv_dx2, inv_dy2;  // 1/dx, 1/dy, 1/(dx dx), 1/(dy dy)
   const double* dz;
   const double* d_above;
   const double* d_below;
   const double* z_face;
+  const double* inv_dz;
+  const double* inv_da;
+  const double* inv_db;
 };
 
 inline GridArgs grid_args(const fw_grid* g) {
   GridArgs a;
   a.nx = g->nx; a.ny = g->ny; a.nz = g->nz; a.n_z_bed = g->n_z_bed;
   a.dx = g->dx; a.dy = g->dy;
+  a.inv_dx = 1.0 / g->dx; a.inv_dy = 1.0 / g->dy;
+  a.inv_dx2 = 1.0 / (g->dx * g->dx); a.inv_dy2 = 1.0 / (g->dy * g->dy);
   a.dz = g->dz; a.d_above = g->d_above; a.d_below = g->d_below;
   a.z_face = g->z_face;
+  a.inv_dz = g->inv_dz; a.inv_da = g->inv_d_above; a.inv_db = g->inv_d_below;
   return a;
 }
 
@@ -79,26 +98,93 @@ __device__ __forceinline__ size_t cidx(const GridArgs& g, int k, int j, int i) {
   return ((size_t)k * g.ny + j) * g.nx + i;
 }
 
+// 32-bit byte offsets of a cell's 13-point stencil (x, z: +-1, +-2; y: +-1,
+// +-2 periodic), shared by every field read at the cell: a load is then the
+// field's SGPR base plus one VGPR offset, with no 64-bit address per field
+// and neighbour.  Valid while a field is smaller than 4 GiB (the entry points
+// that use it check this).  Offsets past a non-periodic edge wrap and are
+// only dereferenced under the same edge tests as the index form.
+struct Nbr {
+  uint32_t c, xm2, xm1, xp1, xp2, ym2, ym1, yp1, yp2, zm2, zm1, zp1, zp2;
+};
+
+template <typename T>
+__device__ __forceinline__ Nbr nbr_offsets(const GridArgs& g, int i, int j, int k) {
+  const uint32_t e = sizeof(T);
+  const uint32_t sy = (uint32_t)g.nx * e, sz = (uint32_t)g.nx * (uint32_t)g.ny * e;
+  Nbr n;
+  n.c = (((uint32_t)k * (uint32_t)g.ny + (uint32_t)j) * (uint32_t)g.nx + (uint32_t)i) * e;
+  n.xm2 = n.c - 2 * e;
+  n.xm1 = n.c - e;
+  n.xp1 = n.c + e;
+  n.xp2 = n.c + 2 * e;
+  n.ym2 = n.c + (uint32_t)(wrap(j - 2, g.ny) - j) * sy;
+  n.ym1 = n.c + (uint32_t)(wrap(j - 1, g.ny) - j) * sy;
+  n.yp1 = n.c + (uint32_t)(wrap(j + 1, g.ny) - j) * sy;
+  n.yp2 = n.c + (uint32_t)(wrap(j + 2, g.ny) - j) * sy;
+  n.zm2 = n.c - 2 * sz;
+  n.zm1 = n.c - sz;
+  n.zp1 = n.c + sz;
+  n.zp2 = n.c + 2 * sz;
+  return n;
+}
+
+// p[off / sizeof(T)] for a byte offset from nbr_offsets
+template <typename T>
+__device__ __forceinline__ T at(const T* __restrict__ p, uint32_t off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + off);
+}
+template <typename T>
+__device__ __forceinline__ T& at(T* __restrict__ p, uint32_t off) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off);
+}
+
 // minmod limiter (muscl_3d.py:36-45): smaller-magnitude argument when both
 // have the same sign, zero otherwise.
+// Written as selects (no branches): the same values as the if/else form.
 template <typename T>
 __device__ __forceinline__ T minmod(T a, T b) {
-  if (a * b <= T(0)) return T(0);
-  if (a > T(0)) return a < b ? a : b;
-  return a > b ? a : b;
+  const T lo = a < b ? a : b;
+  const T hi = a > b ? a : b;
+  const T m = a > T(0) ? lo : hi;
+  return a * b <= T(0) ? T(0) : m;
+}
+
+// float32: minmod(a, b) is the median of a, b and 0 (one v_med3_f32).  Same
+// value as above for finite arguments, except that a product a*b which
+// underflows float32 no longer zeroes the slope (the float64 reference's
+// product does not underflow there either).
+template <>
+__device__ __forceinline__ float minmod<float>(float a, float b) {
+  return __builtin_amdgcn_fmed3f(a, b, 0.0f);
+}
+
+// 1 / y for quot() below: v_rcp_f32 (1 ulp) in float32; float64 never uses
+// it (quot divides), so the division is dead code there.
+template <typename T>
+__device__ __forceinline__ T recip(T y) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_rcpf(y);
+  else return T(1) / y;
+}
+
+// x / y.  The float32 build multiplies by ry = 1 / y, formed once by the
+// caller and shared by several quotients (one rounding more, inside the
+// float32 tolerance); float64 keeps the reference's division bit for bit.
+template <typename T>
+__device__ __forceinline__ T quot(T x, T y, T ry) {
+  if constexpr (sizeof(T) == 4) return x * ry;
+  else return x / y;
 }
 
 // MUSCL face value at face i+1/2 from the upwind side (muscl_3d.py:48-64).
 // The upwind side is chosen from the sign of `uf` (the caller passes the
-// cell-centre velocity, as the reference does).
+// cell-centre velocity, as the reference does).  Both one-sided candidates
+// are formed and one is selected, which keeps the wavefront converged.
 template <typename T>
 __device__ __forceinline__ T muscl_face(T pm1, T p0, T pp1, T pp2, T uf) {
-  if (uf >= T(0)) {
-    T s = minmod<T>(p0 - pm1, pp1 - p0);
-    return p0 + T(0.5) * s;
-  }
-  T s = minmod<T>(pp1 - p0, pp2 - pp1);
-  return pp1 - T(0.5) * s;
+  const T fwd = p0 + T(0.5) * minmod<T>(p0 - pm1, pp1 - p0);
+  const T bwd = pp1 - T(0.5) * minmod<T>(pp1 - p0, pp2 - pp1);
+  return uf >= T(0) ? fwd : bwd;
 }
 
 }  // namespace fw
@@ -124,6 +210,22 @@ int fw_set_error(int code, const char* fmt, ...);
   do {                                                                        \
     if (!(cond)) return fw_set_error(FW_E_CONFIG, __VA_ARGS__);               \
   } while (0)
+
+// Every metric array of a grid must be present (the kernels read them
+// unconditionally).
+#define FW_GRID(g)                                                            \
+  FW_REQUIRE((g) && (g)->dz && (g)->d_above && (g)->d_below && (g)->inv_dz &&  \
+                 (g)->inv_d_above && (g)->inv_d_below && (g)->nx > 0 &&       \
+                 (g)->ny > 0 && (g)->nz > 0,                                  \
+             "fw_grid: missing metric arrays (dz, d_above, d_below and their "  \
+             "reciprocals are required) or empty grid")
+
+// Kernels that address fields through 32-bit byte offsets (nbr_offsets)
+// need every field to be smaller than 4 GiB.
+#define FW_GRID_OFFSETS32(g)                                                  \
+  FW_REQUIRE((size_t)(g)->nx * (g)->ny * (g)->nz *                            \
+                     ((g)->dtype == FW_F64 ? 8 : 4) <= 0xFFFFFFFFull,         \
+             "fw_grid: a field of this grid exceeds 4 GiB")
 
 // Dispatch a templated implementation on the grid's storage type.
 #define FW_DISPATCH(g, fn, ...)                                               \

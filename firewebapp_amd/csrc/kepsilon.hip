@@ -12,8 +12,8 @@ namespace fw {
 
 template <typename T>
 __device__ __forceinline__ T inv_dz_central(const GridArgs& g, int k) {
-  if (k == 0) return T(1.0 / g.d_above[0]);
-  if (k == g.nz - 1) return T(1.0 / g.d_below[g.nz - 1]);
+  if (k == 0) return T(g.inv_da[0]);
+  if (k == g.nz - 1) return T(g.inv_db[g.nz - 1]);
   return T(1.0 / (g.d_below[k] + g.d_above[k]));
 }
 
@@ -36,7 +36,7 @@ k_kepsilon(GridArgs g, const T* __restrict__ kt, const T* __restrict__ et,
   const size_t rm1 = c + (size_t)(wrap(j - 1, Ny) - j) * sy;
   const size_t rp1 = c + (size_t)(wrap(j + 1, Ny) - j) * sy;
   const size_t rp2 = c + (size_t)(wrap(j + 2, Ny) - j) * sy;
-  const T inv_dx = T(1.0 / g.dx), inv_dy = T(1.0 / g.dy);
+  const T inv_dx = T(g.inv_dx), inv_dy = T(g.inv_dy);
   const T idzc = inv_dz_central<T>(g, k);
 
   // ---- strain / vorticity (Way-B ghosts) ----
@@ -120,9 +120,9 @@ k_kepsilon(GridArgs g, const T* __restrict__ kt, const T* __restrict__ et,
   }
 
   const bool mx = (i >= 2 && i <= Nx - 3), mz = (k >= 2 && k <= Nz - 3);
-  const T inv_dx2 = T(1.0 / (g.dx * g.dx)), inv_dy2 = T(1.0 / (g.dy * g.dy));
-  const T inv_dz = T(1.0 / g.dz[k]);
-  const T inv_da = T(1.0 / g.d_above[k]), inv_db = T(1.0 / g.d_below[k]);
+  const T inv_dx2 = T(g.inv_dx2), inv_dy2 = T(g.inv_dy2);
+  const T inv_dz = T(g.inv_dz[k]);
+  const T inv_da = T(g.inv_da[k]), inv_db = T(g.inv_db[k]);
   const T dzm = T(0.5) * (T(g.d_above[k]) + T(g.d_below[k]));
 
   // advection + diffusion of one of (k, eps)
@@ -216,7 +216,8 @@ extern "C" int fw_k_epsilon(const fw_grid* g, const void* k_turb,
                             const void* eps_wall_ghost, double sigma_sav,
                             double dt, double T_amb, double beta_p,
                             double beta_d, void* stream) {
-  FW_REQUIRE(g && k_turb != k_out && eps_turb != eps_out,
+  FW_GRID(g);
+  FW_REQUIRE(k_turb != k_out && eps_turb != eps_out,
              "k_epsilon: outputs must not alias inputs");
   FW_REQUIRE(g->nz >= 2, "k_epsilon: needs Nz >= 2");
   return FW_DISPATCH(g, fw::kepsilon_impl, g, k_turb, eps_turb, k_out, eps_out,

@@ -32,6 +32,8 @@ namespace fw {
 // the Krylov state untouched once S_DONE is set, so iterations can be
 // enqueued in batches and the host reads S_DONE once per batch.
 enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_DONE, S_IT, S_TOL, S_NS };
+// what a final sum is stored as (k_finalize)
+enum { F_STORE = 0, F_ALPHA = 1, F_BETA = 2, F_RR = 3 };
 
 // ---------------------------------------------------------------------------
 // operator
@@ -77,7 +79,7 @@ __device__ __forceinline__ C apply_A(const GridArgs& g, const TR* __restrict__ r
     acc += cyp * C(x[cp]) + cym * C(x[cm]);
   }
   if (Nz > 1) {
-    const C inv_dz = C(1.0 / g.dz[k]);
+    const C inv_dz = C(g.inv_dz[k]);
     if (k < Nz - 1) {
       C czp = (C(0.5) * (irc + inv_rho_at<C>(rho, c + sz))) * inv_dz / C(g.d_above[k]);
       diag -= czp;
@@ -117,6 +119,93 @@ __device__ __forceinline__ void block_max_store(double v, double* sh, double* ou
   if (t == 0) out[blockIdx.x] = sh[0];
 }
 
+// Sum of v over the workgroup (64, 128 or 256 threads), in a fixed order: a
+// shuffle tree inside each wavefront, then the wavefront sums in order.
+// Valid on every thread; sh holds one double per wavefront.
+__device__ __forceinline__ double block_sum_wg(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  double a = sh[0];
+  for (int q = 1; q < nw; ++q) a += sh[q];
+  return a;
+}
+
+// Rows (k, j) of the grid for the PJ_RED workgroups of a row-blocked
+// reduction kernel: contiguous runs of rows, with the same XCD-aware order as
+// cell_ijk (each XCD gets a contiguous range of runs, so the y/z neighbour
+// rows of a run mostly sit in the same L2).
+__device__ __forceinline__ void row_range(int rows, int& r0, int& r1) {
+  const unsigned nb = gridDim.x;
+  const unsigned q = nb >> 3, rr = nb & 7u;
+  const unsigned xcd = blockIdx.x & 7u, pos = blockIdx.x >> 3;
+  const unsigned m = xcd * q + (xcd < rr ? xcd : rr) + pos;
+  const int R = (rows + (int)nb - 1) / (int)nb;
+  r0 = (int)m * R;
+  r1 = r0 + R < rows ? r0 + R : rows;
+}
+
+// Fixed-order sum of np (<= a few thousand) partials, computed identically
+// by every workgroup that calls it (so all of them agree bit for bit).
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int np,
+                                               double* sh) {
+  double a = 0.0;
+  for (int b = threadIdx.x; b < np; b += blockDim.x) a += part[b];
+  return block_sum_wg(a, sh);
+}
+
+// The inner (Krylov) operator: A x at cell (i, j, k) in precision T from a
+// precomputed 1 / max(rho, 0.01) field, with the metric reciprocals
+// multiplied in.  It equals apply_A up to rounding; the refinement's float64
+// residual uses apply_A itself, so the solution is unchanged.
+template <typename T>
+__device__ __forceinline__ T apply_A_irho(const GridArgs& g, const T* __restrict__ ir,
+                                          const T* __restrict__ x, int i, int k,
+                                          const Nbr& n, T eps_reg) {
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  const T irc = at(ir, n.c), xc = at(x, n.c);
+  const T idx2 = T(g.inv_dx2), idy2 = T(g.inv_dy2);
+  T diag = eps_reg;
+  T acc = T(0);
+  if (Nx > 1) {
+    if (i < Nx - 1) {
+      const T cxp = T(0.5) * (irc + at(ir, n.xp1)) * idx2;
+      diag -= cxp;
+      acc += cxp * at(x, n.xp1);
+    } else {
+      diag -= T(2) * irc * idx2;
+    }
+    if (i > 0) {
+      const T cxm = T(0.5) * (at(ir, n.xm1) + irc) * idx2;
+      diag -= cxm;
+      acc += cxm * at(x, n.xm1);
+    }
+  }
+  if (Ny > 1) {
+    const T cyp = T(0.5) * (irc + at(ir, n.yp1)) * idy2;
+    const T cym = T(0.5) * (at(ir, n.ym1) + irc) * idy2;
+    diag -= cyp + cym;
+    acc += cyp * at(x, n.yp1) + cym * at(x, n.ym1);
+  }
+  if (Nz > 1) {
+    const T idz = T(g.inv_dz[k]);
+    if (k < Nz - 1) {
+      const T czp = T(0.5) * (irc + at(ir, n.zp1)) * (idz * T(g.inv_da[k]));
+      diag -= czp;
+      acc += czp * at(x, n.zp1);
+    } else {
+      diag -= T(2) * irc * idz * idz;
+    }
+    if (k > 0) {
+      const T czm = T(0.5) * (at(ir, n.zm1) + irc) * (idz * T(g.inv_db[k]));
+      diag -= czm;
+      acc += czm * at(x, n.zm1);
+    }
+  }
+  return diag * xc + acc;
+}
+
 struct Cell {
   int i, j, k;
 };
@@ -147,6 +236,137 @@ k_matvec_pq(GridArgs g, size_t n, const TR* __restrict__ rho,
   block_sum_store(acc, sh, part);
 }
 
+// ir = 1 / max(rho, 0.01) in the Krylov precision (once per solve)
+template <typename T, typename TR>
+__global__ void __launch_bounds__(256)
+k_irho(size_t n, const TR* __restrict__ rho, T* __restrict__ ir) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c < n) ir[c] = inv_rho_at<T>(rho, c);
+}
+
+// The inner PCG iteration.  Each reduction leaves PJ_RED partials; the next
+// kernel that needs the total sums them itself (sum_partials, the same fixed
+// order in every workgroup), so no final-sum launch and no cross-workgroup
+// synchronisation is needed:
+//   k_matvec_rows   q = A p                          -> Ppq
+//   k_update_xr2    alpha = sum Prz / sum Ppq; d += alpha p; r -= alpha q
+//                                                    -> Prr
+//   (preconditioner z = M^-1 r)
+//   k_dot_rows      convergence test on sum Prr (workgroup 0); <r, z>_W
+//                                                    -> Prz
+//   k_p_update2     beta = sum Prz / rz; p = z + beta p
+// S_DONE is written only by workgroup 0 of k_dot_rows, which is also the
+// only workgroup of that launch to read it.
+
+// q = A p over the workgroup's rows; partial <p, q>_W
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_matvec_rows(GridArgs g, const T* __restrict__ ir, const T* __restrict__ p,
+              T* __restrict__ q, T eps_reg, double* __restrict__ Ppq) {
+  __shared__ double sh[4];
+  int r0, r1;
+  row_range(g.ny * g.nz, r0, r1);
+  double acc = 0.0;
+  for (int row = r0; row < r1; ++row) {
+    const int k = row / g.ny, j = row - k * g.ny;
+    const double dzk = g.dz[k];
+    for (int i = threadIdx.x; i < g.nx; i += 256) {
+      const Nbr nb = nbr_offsets<T>(g, i, j, k);
+      const T y = apply_A_irho<T>(g, ir, p, i, k, nb, eps_reg);
+      at(q, nb.c) = y;
+      acc += dzk * (double)at(p, nb.c) * (double)y;
+    }
+  }
+  const double t = block_sum_wg(acc, sh);
+  if (threadIdx.x == 0) Ppq[blockIdx.x] = t;
+}
+
+// alpha = rz / <p, q>_W; d += alpha p ; r -= alpha q ; partial ||r||^2.
+// Nothing moves once S_DONE is set.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_update_xr2(size_t n, T* __restrict__ d, T* __restrict__ r, const T* __restrict__ p,
+             const T* __restrict__ q, const double* __restrict__ Prz,
+             const double* __restrict__ Ppq, double* __restrict__ S,
+             double* __restrict__ Prr) {
+  __shared__ double sh[4];
+  if (S[S_DONE] != 0.0) return;
+  const double rz = sum_partials(Prz, PJ_RED, sh);
+  __syncthreads();
+  const double pq = sum_partials(Ppq, PJ_RED, sh);
+  const double alpha_d = rz / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S[S_RZ] = rz;
+    S[S_PQ] = pq;
+    S[S_ALPHA] = alpha_d;
+  }
+  const T alpha = (T)alpha_d;
+  double acc = 0.0;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256) {
+    d[c] += alpha * p[c];
+    const T rv = r[c] - alpha * q[c];
+    r[c] = rv;
+    acc += (double)rv * (double)rv;
+  }
+  __syncthreads();
+  const double t = block_sum_wg(acc, sh);
+  if (threadIdx.x == 0) Prr[blockIdx.x] = t;
+}
+
+// <a, b>_W over the workgroup's rows -> Prz.  check != 0: workgroup 0 first
+// applies the convergence test of the iteration to sum Prr (stores ||r||^2,
+// counts the iteration, raises S_DONE; 2 on a NaN).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dot_rows(GridArgs g, const T* __restrict__ a, const T* __restrict__ b,
+           double* __restrict__ Prz, const double* __restrict__ Prr,
+           double* __restrict__ S, int check) {
+  __shared__ double sh[4];
+  if (check && blockIdx.x == 0) {
+    const double rr = sum_partials(Prr, PJ_RED, sh);
+    if (threadIdx.x == 0 && S[S_DONE] == 0.0) {
+      S[S_RR] = rr;
+      S[S_IT] += 1.0;
+      if (!(rr == rr)) S[S_DONE] = 2.0;
+      else if (sqrt(rr) <= S[S_TOL]) S[S_DONE] = 1.0;
+    }
+    __syncthreads();
+  }
+  int r0, r1;
+  row_range(g.ny * g.nz, r0, r1);
+  double acc = 0.0;
+  for (int row = r0; row < r1; ++row) {
+    const int k = row / g.ny;
+    const double dzk = g.dz[k];
+    const size_t base = (size_t)row * g.nx;
+    for (int i = threadIdx.x; i < g.nx; i += 256)
+      acc += dzk * (double)a[base + i] * (double)b[base + i];
+  }
+  const double t = block_sum_wg(acc, sh);
+  if (threadIdx.x == 0) Prz[blockIdx.x] = t;
+}
+
+// p = z + beta p with beta = sum Prz / rz (first: p = z).  Nothing moves
+// once S_DONE is set.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_p_update2(size_t n, T* __restrict__ p, const T* __restrict__ z,
+            const double* __restrict__ Prz, double* __restrict__ S, int first) {
+  __shared__ double sh[4];
+  if (S[S_DONE] != 0.0) return;
+  T beta = T(0);
+  if (!first) {
+    const double rz_new = sum_partials(Prz, PJ_RED, sh);
+    const double bd = rz_new / S[S_RZ];
+    if (blockIdx.x == 0 && threadIdx.x == 0) S[S_BETA] = bd;
+    beta = (T)bd;
+  }
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
+       c += (size_t)gridDim.x * 256)
+    p[c] = first ? z[c] : z[c] + beta * p[c];
+}
+
 // r64 = b64 - A64 x64 (float64 operator from rho), r_t = (T) r64,
 // partial sum of r64^2
 template <typename T, typename TR>
@@ -169,40 +389,6 @@ k_residual64(GridArgs g, size_t n, const TR* __restrict__ rho,
   block_sum_store(acc, sh, part);
 }
 
-// d += alpha p ; r -= alpha q ; partial sum r^2
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_update_xr(size_t n, T* __restrict__ d, T* __restrict__ r,
-            const T* __restrict__ p, const T* __restrict__ q,
-            const double* __restrict__ S, double* __restrict__ part) {
-  __shared__ double sh[256];
-  if (S[S_DONE] != 0.0) return;   // converged: leave d, r and the partials
-  const T alpha = (T)S[S_ALPHA];
-  double acc = 0.0;
-  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * 256) {
-    d[c] += alpha * p[c];
-    T rv = r[c] - alpha * q[c];
-    r[c] = rv;
-    acc += (double)rv * (double)rv;
-  }
-  block_sum_store(acc, sh, part);
-}
-
-// partial sum of dz_k a b
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_dot_w(GridArgs g, size_t n, const T* __restrict__ a, const T* __restrict__ b,
-        double* __restrict__ part) {
-  __shared__ double sh[256];
-  double acc = 0.0;
-  const size_t plane = (size_t)g.nx * g.ny;
-  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * 256)
-    acc += g.dz[c / plane] * (double)a[c] * (double)b[c];
-  block_sum_store(acc, sh, part);
-}
-
 // partial sum of a^2 (unweighted)
 __global__ void __launch_bounds__(256)
 k_sumsq(size_t n, const double* __restrict__ a, double* __restrict__ part) {
@@ -212,16 +398,6 @@ k_sumsq(size_t n, const double* __restrict__ a, double* __restrict__ part) {
        c += (size_t)gridDim.x * 256)
     acc += a[c] * a[c];
   block_sum_store(acc, sh, part);
-}
-
-// p = z + beta p   (beta = 0 on the first iteration)
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_p_update(size_t n, T* __restrict__ p, const T* __restrict__ z,
-           const double* __restrict__ S, int first) {
-  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= n || S[S_DONE] != 0.0) return;
-  p[c] = first ? z[c] : z[c] + (T)S[S_BETA] * p[c];
 }
 
 template <typename T>
@@ -246,7 +422,6 @@ __global__ void __launch_bounds__(256) k_zero(size_t n, T* __restrict__ a) {
 }
 
 // Fixed-order final sum of PJ_RED partials; mode selects what is stored.
-enum { F_STORE = 0, F_ALPHA = 1, F_BETA = 2, F_RR = 3 };
 __global__ void __launch_bounds__(256)
 k_finalize(const double* __restrict__ part, int np, double* __restrict__ S,
            int slot, int mode) {
@@ -583,6 +758,8 @@ struct ProjHandle {
   double *x64, *b64, *r64;
   void *r, *z, *p, *q, *d, *w1, *w2;
   double *part, *S, *S_host;
+  double *Ppq, *Prr, *Prz;  // PJ_RED partials of the PCG iteration's reductions
+  void* irho;          // 1 / max(rho, 0.01) of the running solve (type T)
   bool have_prev;
   int last_iters, last_outer;
   double last_rel, last_div_max;
@@ -688,9 +865,11 @@ static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
   } else {
     FW_CHECK_HIP(hipMemcpyAsync(z, r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
   }
-  hipLaunchKernelGGL(k_dot_w<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n, r, z, h->part);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_RZ, F_STORE);
-  hipLaunchKernelGGL(k_p_update<T>, dim3(nblk(n)), dim3(256), 0, s, n, p, z, h->S, 1);
+  (void)rho;   // the iterations use h->irho, formed from rho by solve_refined
+  const T* ir = (const T*)h->irho;
+  hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z, h->Prz,
+                     h->Prr, h->S, 0);
+  hipLaunchKernelGGL(k_p_update2<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Prz, h->S, 1);
   FW_CHECK_LAUNCH("pcg setup");
   // iterations in batches with one host read per batch; the device flag stops
   // the Krylov updates at the first converged iteration, so the result is
@@ -704,23 +883,19 @@ static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
     const int nb_it = maxit - queued < batch ? maxit - queued : batch;
     batch = 1;
     for (int b = 0; b < nb_it; ++b) {
-      hipLaunchKernelGGL((k_matvec_pq<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga,
-                         n, rho, p, q, h->eps_reg, h->part);
-      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0,
-                         F_ALPHA);
-      hipLaunchKernelGGL(k_update_xr<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
-                         h->S, h->part);
-      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0,
-                         F_RR);
+      hipLaunchKernelGGL(k_matvec_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, ir, p, q,
+                         (T)h->eps_reg, h->Ppq);
+      hipLaunchKernelGGL(k_update_xr2<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
+                         h->Prz, h->Ppq, h->S, h->Prr);
       if (h->precond_enabled) {
         if ((rc = precond_apply<T>(h, r, z, s))) return rc;
       } else {
         FW_CHECK_HIP(hipMemcpyAsync(z, r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
       }
-      hipLaunchKernelGGL(k_dot_w<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n, r, z, h->part);
-      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, 0,
-                         F_BETA);
-      hipLaunchKernelGGL(k_p_update<T>, dim3(nblk(n)), dim3(256), 0, s, n, p, z, h->S, 0);
+      hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z, h->Prz,
+                         h->Prr, h->S, 1);
+      hipLaunchKernelGGL(k_p_update2<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Prz,
+                         h->S, 0);
     }
     queued += nb_it;
     FW_CHECK_LAUNCH("pcg iteration");
@@ -745,6 +920,7 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm
   const double inner_floor = (sizeof(T) == 4) ? 2e-5 : 0.0;
   int total = 0, outer = 0, rc;
   double rnorm = 0.0;
+  hipLaunchKernelGGL((k_irho<T, TR>), dim3(nblk(n)), dim3(256), 0, s, n, rho, (T*)h->irho);
   for (;;) {
     if (outer == 0 && r0 >= 0.0) {
       rnorm = r0;
@@ -848,6 +1024,8 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
                    const double* Uz, const double* lam_x, const double* lam_y,
                    const double* lam_z, double eps_reg, double cg_rtol,
                    int maxiter, void** handle) {
+  FW_GRID(g);
+  FW_GRID_OFFSETS32(g);
   FW_REQUIRE(g && handle && Ux && Uy && Uz && lam_x && lam_y && lam_z,
              "proj_create: null argument");
   FW_REQUIRE(g->nx >= 2 && g->nz >= 2 && g->ny >= 1, "proj_create: needs Nx, Nz >= 2");
@@ -945,6 +1123,10 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   void** vecs[] = {&h->r, &h->z, &h->p, &h->q, &h->d, &h->w1, &h->w2};
   for (void** v : vecs) FW_CHECK_HIP(hipMalloc(v, n * h->esz));
   FW_CHECK_HIP(hipMalloc((void**)&h->part, PJ_RED * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->Ppq, 3 * PJ_RED * sizeof(double)));
+  h->Prr = h->Ppq + PJ_RED;
+  h->Prz = h->Ppq + 2 * PJ_RED;
+  FW_CHECK_HIP(hipMalloc(&h->irho, n * h->esz));
   FW_CHECK_HIP(hipMalloc((void**)&h->S, S_NS * sizeof(double)));
   FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, S_NS * sizeof(double)));
   FW_CHECK_HIP(hipMemset(h->x64, 0, n * 8));
@@ -957,7 +1139,7 @@ int fw_proj_destroy(void* handle) {
   if (!h) return FW_OK;
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
                   h->x64, h->b64, h->r64, h->r, h->z, h->p, h->q, h->d, h->w1,
-                  h->w2, h->part, h->S, h->twx};
+                  h->w2, h->part, h->S, h->twx, h->Ppq, h->irho};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (h->S_host) (void)hipHostFree(h->S_host);

@@ -64,9 +64,10 @@ static const double Q_RAD_MAX = 1.0e5, Q_PULSE = 240000.0, T_CAP = 10000.0;
 static const double T_BOOTSTRAP_S = 2.0;
 static const int N_SUB = 10, REINIT_INTERVAL = 10, REINIT_SUBSTEPS = 5;
 
-// Optional HIP-event timing of the species launches (three back-to-back
-// launches of k_species per sub-step share one event pair), for the
-// roofline figure of the benchmark.
+// Optional HIP-event timing of the sub-steps' species launch (k_species3,
+// one per sub-step), for the roofline figure of the benchmark.  The launch
+// itself records the events (hipExtLaunchKernel), so a pair spans the
+// kernel's execution and not the queue gap before it.
 struct SpeciesTimer {
   bool on = false;
   int used = 0, launches = 0;
@@ -122,6 +123,7 @@ int fw_spread_timer_read(double* total_ms, int* launches) {
 
 int fw_spread_front(const fw_grid* g, const fw_spread_fields* f, fw_spread_state* st,
                     void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && f && st && f->phi, "spread_front: null argument");
   const int Nx = g->nx, Ny = g->ny, Nz = g->nz;
   const int k_exit = Nz > 1 ? 1 : 0, k_mid = g->n_z_bed / 2;
@@ -149,6 +151,7 @@ int fw_spread_front(const fw_grid* g, const fw_spread_fields* f, fw_spread_state
 int fw_spread_reduce(const fw_grid* g, const fw_spread_fields* f,
                      const fw_spread_params* p, int with_sources, fw_spread_state* st,
                      void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && f && p && st, "spread_reduce: null argument");
   hipStream_t s = (hipStream_t)stream;
   TRY(fw_step_reduce(g, f->u, f->v, f->w, f->nu_t, f->T_g, f->T_s,
@@ -166,6 +169,7 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                    const fw_spread_params* p, fw_spread_state* st, void* proj, void* dom,
                    const fw_bed_particles* particles, const fw_bed_index* index,
                    fw_bed_params* bed, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && g64 && f && p && st && proj && dom && particles && index && bed,
              "spread_step: null argument");
   FW_REQUIRE(g->n_z_bed > 0 && p->i_src1 > p->i_src0, "spread_step: needs a fuel bed");
@@ -276,33 +280,47 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
     // species with the dilution sources; the Y_O2 cap of :3059 rides in
     // y_max, the [0, 1] clip is the kernel's own
     const bool timed = g_timer.on && g_timer.used < SpeciesTimer::CAP;
-    if (timed) FW_CHECK_HIP(hipEventRecord(g_timer.ev[2 * g_timer.used], s));
-    for (int q = 0; q < 3; ++q) {
-      TRY(fw_species_transport_dilution(g, *Y[q], f->tmp1, f->rho, f->u, f->v, f->w,
-                                        f->src[0], f->src[1], q + 1, dt_sub, 1.0e-5, y_in[q],
-                                        y_max[q], stream));
-      SWAP(*Y[q], f->tmp1);
+    if (timed) {
+      g_species3_events.start = g_timer.ev[2 * g_timer.used];
+      g_species3_events.stop = g_timer.ev[2 * g_timer.used + 1];
+    }
+    {
+      const void* yin[3] = {*Y[0], *Y[1], *Y[2]};
+      void* yout[3] = {f->tmp1, f->tmp2, f->tmp3};
+      const int rc = fw_species_transport3(g, yin, yout, f->rho, f->u, f->v, f->w,
+                                           f->src[0], f->src[1], dt_sub, 1.0e-5, y_in,
+                                           y_max, stream);
+      g_species3_events = KernelEvents();
+      if (rc) return rc;
+      SWAP(*Y[0], f->tmp1);
+      SWAP(*Y[1], f->tmp2);
+      SWAP(*Y[2], f->tmp3);
     }
     if (timed) {
-      FW_CHECK_HIP(hipEventRecord(g_timer.ev[2 * g_timer.used + 1], s));
       g_timer.used += 1;
-      g_timer.launches += 3;
+      g_timer.launches += 1;
     }
     // turbulent diffusion; the clips of :3082-3084 fold into the last pass
     // (with no pass they are the identity: each field is already in range)
-    for (int q = 0; q < 3; ++q)
-      for (int m = 0; m < nsc; ++m) {
-        const bool last = m == nsc - 1;
-        TRY(fw_turbulent_diffusion_pass(g, *Y[q], f->tmp1, f->nu_t, SC_T, dsc,
-                                        last ? 0.0 : 1.0, last ? y_max[q] : 0.0, stream));
-        SWAP(*Y[q], f->tmp1);
-      }
-    TRY(fw_gas_energy_update(g, f->T_g, f->T_s, f->rho, f->Y_H2O, f->src[0], f->src[1],
-                             f->src[7], f->q_rad_gas, sub == N_SUB - 1 ? f->Q_comb : nullptr,
-                             nzb, s0, s1, drip_gas, dt_sub, p->T_amb, stream));
-    // vertical solid conduction with the T_s clamps of :3195-3196
-    TRY(fw_solid_conduction_vertical(g, f->T_s, f->tmp1, f->alpha_s, K_SOLID_GRASS, RHO_SOLID,
-                                     CP_SOLID, dt_sub, p->T_amb, T_CAP, stream));
+    for (int m = 0; m < nsc; ++m) {
+      const bool last = m == nsc - 1;
+      const double lo[3] = {last ? 0.0 : 1.0, last ? 0.0 : 1.0, last ? 0.0 : 1.0};
+      const double hi[3] = {last ? y_max[0] : 0.0, last ? y_max[1] : 0.0,
+                            last ? y_max[2] : 0.0};
+      const void* fin[3] = {*Y[0], *Y[1], *Y[2]};
+      void* fout[3] = {f->tmp1, f->tmp2, f->tmp3};
+      TRY(fw_turbulent_diffusion_pass3(g, fin, fout, f->nu_t, SC_T, dsc, lo, hi, stream));
+      SWAP(*Y[0], f->tmp1);
+      SWAP(*Y[1], f->tmp2);
+      SWAP(*Y[2], f->tmp3);
+    }
+    // gas-energy update and vertical solid conduction with the T_s clamps
+    // of :3195-3196, in one pass
+    TRY(fw_gas_update_solid_conduction(g, f->T_g, f->T_s, f->tmp1, f->alpha_s, f->rho,
+                                       f->Y_H2O, f->src[0], f->src[1], f->src[7],
+                                       f->q_rad_gas, sub == N_SUB - 1 ? f->Q_comb : nullptr,
+                                       nzb, s0, s1, drip_gas, dt_sub, p->T_amb, K_SOLID_GRASS,
+                                       RHO_SOLID, CP_SOLID, p->T_amb, T_CAP, stream));
     SWAP(f->T_s, f->tmp1);
     // T_g turbulent diffusion, the clamps of :3193-3194 in the last pass
     for (int m = 0; m < npr; ++m) {

@@ -92,6 +92,72 @@ k_gas_update(GridArgs g, T* __restrict__ Tg, const T* __restrict__ Ts,
   Tg[c] = nv;
 }
 
+// The sub-step's gas-energy update (as k_gas_update, in place on T_g) fused
+// with the vertical solid conduction that follows it
+// (solid_conduction_3d.py:56-127, out of place Ts -> Ts_out, with the T_s
+// clamps of spread_3d.py:3195-3196).  Both read only the T_s of before the
+// conduction, as in the reference order.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gas_solid(GridArgs g, T* __restrict__ Tg, const T* __restrict__ Ts,
+            T* __restrict__ Ts_out, const T* __restrict__ alpha,
+            const T* __restrict__ rho, const T* __restrict__ Yw,
+            const T* __restrict__ Sp, const T* __restrict__ Sd,
+            const T* __restrict__ Qgc, const T* __restrict__ qrg,
+            T* __restrict__ Qcomb_out, int nzb, int i0, int i1, T drip, T dt_sub,
+            T T_amb, T k_solid, T inv_rho_cp, T ts_lo, T ts_hi) {
+  int i, j, k;
+  cell_ijk(i, j, k);
+  if (i >= g.nx) return;
+  const size_t c = cidx(g, k, j, i);
+  const size_t sz = (size_t)g.nx * g.ny;
+  const T tsc = Ts[c];
+  // gas energy (spread_3d.py:2916-2957, :3104-3105, :3129-3146)
+  T Q = T(0);
+  if (k < nzb && i >= i0 && i < i1) Q += drip;
+  Q += qrg[c] / T(g.dz[k]);
+  if (Qcomb_out) Qcomb_out[c] = Q;
+  const T y = Yw[c];
+  const T cp = (T(1) - y) * T(1100.0) + y * T(2000.0);
+  const T tg = Tg[c];
+  T da = tg - tsc;
+  da = da > T(0) ? da : T(0);
+  T db = tg - T(373.15);
+  db = db > T(0) ? db : T(0);
+  const T debit = (Sp[c] * da + Sd[c] * db) * T(2000.0);
+  const T r = rho[c];
+  const T inv = dt_sub / ((r > T(1.0e-3) ? r : T(1.0e-3)) * cp);
+  T nv = tg + (Q - Qgc[c] - debit) * inv;
+  nv = (nv > T_amb || nv != nv) ? nv : T_amb;
+  Tg[c] = nv;
+  // solid conduction
+  const T a = alpha[c];
+  T ns = tsc;
+  if (a > T(0)) {
+    T flux_up = T(0), flux_dn = T(0);
+    if (k < g.nz - 1) {
+      const T aa = alpha[c + sz];
+      if (aa > T(0)) {
+        const T a_eff = T(2) * a * aa / (a + aa);
+        flux_up = k_solid * a_eff * (Ts[c + sz] - tsc) / T(g.d_above[k]);
+      }
+    }
+    if (k > 0) {
+      const T ab = alpha[c - sz];
+      if (ab > T(0)) {
+        const T a_eff = T(2) * a * ab / (a + ab);
+        flux_dn = k_solid * a_eff * (Ts[c - sz] - tsc) / T(g.d_below[k]);
+      }
+    }
+    const T src = (flux_up + flux_dn) / T(g.dz[k]);
+    const T dT = dt_sub * src * inv_rho_cp / a;
+    ns = tsc + dT;
+  }
+  ns = ns > ts_hi ? ts_hi : ns;
+  ns = ns < ts_lo ? ts_lo : ns;
+  Ts_out[c] = ns;
+}
+
 // q_rad[k, :, i0:i1] += add   (spread_3d.py:2528-2532)
 template <typename T>
 __global__ void k_add_patch(GridArgs g, T* __restrict__ q, int k, int i0, int i1,
@@ -213,10 +279,10 @@ template <typename T>
 __device__ __forceinline__ void dz_inv_pm(const GridArgs& g, int k, T& im, T& ip) {
   const int Nz = g.nz;
   if (k == 0) {
-    im = T(1.0 / g.dz[0]);
-    ip = Nz > 1 ? T(2.0 / (g.dz[0] + g.dz[1])) : T(1.0 / g.dz[0]);
+    im = T(g.inv_dz[0]);
+    ip = Nz > 1 ? T(2.0 / (g.dz[0] + g.dz[1])) : T(g.inv_dz[0]);
   } else if (k == Nz - 1) {
-    ip = T(1.0 / g.dz[Nz - 1]);
+    ip = T(g.inv_dz[Nz - 1]);
     im = T(2.0 / (g.dz[Nz - 1] + g.dz[Nz - 2]));
   } else {
     im = T(2.0 / (g.dz[k] + g.dz[k - 1]));
@@ -236,7 +302,7 @@ k_lset(GridArgs g, const T* __restrict__ phi, const T* __restrict__ phi0,
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t c = cidx(g, k, j, i);
   const size_t sz = (size_t)Nx * Ny;
-  const T inv_dx = T(1.0 / g.dx), inv_dy = T(1.0 / g.dy);
+  const T inv_dx = T(g.inv_dx), inv_dy = T(g.inv_dy);
   T im, ip;
   dz_inv_pm<T>(g, k, im, ip);
   const T pc = phi[c];
@@ -278,6 +344,7 @@ extern "C" {
 
 int fw_step_prologue(const fw_grid* g, const void* T_g, void* rho, void* rho_prev,
                      void* Y_O2, double T_amb, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && T_g && rho && rho_prev && Y_O2, "step_prologue: null argument");
   Launch L = cell_launch(g);
   hipStream_t s = (hipStream_t)stream;
@@ -296,6 +363,7 @@ int fw_step_prologue(const fw_grid* g, const void* T_g, void* rho, void* rho_pre
 int fw_bed_external_heat(const fw_grid* g, const void* q_rad, void* Q_ext, int n_z_bed,
                          int i_src0, int i_src1, double drip_W_m3, double q_rad_max,
                          void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && q_rad && Q_ext, "bed_external_heat: null argument");
   const int nzb = n_z_bed < g->nz ? n_z_bed : g->nz;
   if (nzb <= 0) return FW_OK;
@@ -317,6 +385,7 @@ int fw_bed_external_heat(const fw_grid* g, const void* q_rad, void* Q_ext, int n
 int fw_div_target(const fw_grid* g, const void* S_pyro_part, const void* S_dry_part,
                   const void* rho, const void* rho_prev, double dt, void* S_pyro,
                   void* div_target, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && S_pyro_part && S_dry_part && rho && rho_prev && S_pyro && div_target,
              "div_target: null argument");
   const size_t n = (size_t)g->nx * g->ny * g->nz;
@@ -341,6 +410,7 @@ int fw_gas_energy_update(const fw_grid* g, void* T_g, const void* T_s, const voi
                          const void* q_rad_gas, void* Q_comb_out, int n_z_bed,
                          int i_src0, int i_src1, double drip_W_m3, double dt_sub,
                          double T_amb, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && T_g && T_s && rho && Y_H2O && S_pyro_part && S_dry_part && Q_g_conv &&
                  q_rad_gas,
              "gas_energy_update: null argument");
@@ -364,8 +434,46 @@ int fw_gas_energy_update(const fw_grid* g, void* T_g, const void* T_s, const voi
   return FW_OK;
 }
 
+int fw_gas_update_solid_conduction(const fw_grid* g, void* T_g, const void* T_s,
+                                   void* T_s_out, const void* alpha_s, const void* rho,
+                                   const void* Y_H2O, const void* S_pyro_part,
+                                   const void* S_dry_part, const void* Q_g_conv,
+                                   const void* q_rad_gas, void* Q_comb_out, int n_z_bed,
+                                   int i_src0, int i_src1, double drip_W_m3, double dt_sub,
+                                   double T_amb, double k_solid, double rho_solid,
+                                   double cp_solid, double T_s_lo, double T_s_hi,
+                                   void* stream) {
+  FW_GRID(g);
+  FW_REQUIRE(g && T_g && T_s && T_s_out && T_s != T_s_out && alpha_s && rho && Y_H2O &&
+                 S_pyro_part && S_dry_part && Q_g_conv && q_rad_gas,
+             "gas_update_solid_conduction: bad argument");
+  Launch L = cell_launch(g);
+  hipStream_t s = (hipStream_t)stream;
+  const double irc = 1.0 / (rho_solid * cp_solid);
+  if (g->dtype == FW_F64)
+    hipLaunchKernelGGL(k_gas_solid<double>, L.grid, L.block, 0, s, grid_args(g),
+                       (double*)T_g, (const double*)T_s, (double*)T_s_out,
+                       (const double*)alpha_s, (const double*)rho, (const double*)Y_H2O,
+                       (const double*)S_pyro_part, (const double*)S_dry_part,
+                       (const double*)Q_g_conv, (const double*)q_rad_gas, (double*)Q_comb_out,
+                       n_z_bed, i_src0, i_src1, drip_W_m3, dt_sub, T_amb, k_solid, irc,
+                       T_s_lo, T_s_hi);
+  else
+    hipLaunchKernelGGL(k_gas_solid<float>, L.grid, L.block, 0, s, grid_args(g),
+                       (float*)T_g, (const float*)T_s, (float*)T_s_out,
+                       (const float*)alpha_s, (const float*)rho, (const float*)Y_H2O,
+                       (const float*)S_pyro_part, (const float*)S_dry_part,
+                       (const float*)Q_g_conv, (const float*)q_rad_gas, (float*)Q_comb_out,
+                       n_z_bed, i_src0, i_src1, (float)drip_W_m3, (float)dt_sub,
+                       (float)T_amb, (float)k_solid, (float)irc, (float)T_s_lo,
+                       (float)T_s_hi);
+  FW_CHECK_LAUNCH("gas_update_solid_conduction");
+  return FW_OK;
+}
+
 int fw_add_patch(const fw_grid* g, void* field, int k, int i0, int i1, double add,
                  void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && field && k >= 0 && k < g->nz && i0 >= 0 && i1 <= g->nx,
              "add_patch: bad argument");
   if (i1 <= i0) return FW_OK;
@@ -384,6 +492,7 @@ int fw_add_patch(const fw_grid* g, void* field, int k, int i0, int i1, double ad
 int fw_flame_mask_age(const fw_grid* g, const void* omega, const void* T_g,
                       const void* Y_fuel, void* cell_age, uint8_t* mask, double dt,
                       void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && omega && T_g && Y_fuel, "flame_mask_age: null argument");
   const size_t n = (size_t)g->nx * g->ny * g->nz;
   hipStream_t s = (hipStream_t)stream;
@@ -405,6 +514,7 @@ int fw_step_reduce(const fw_grid* g, const void* u, const void* v, const void* w
                    const void* Y_fuel, const void* alpha_s, double T_ign,
                    double T_front, int n_z_bed, double* work,
                    double* out, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && u && v && w && nu_t && T_g && T_s && omega && Y_fuel && alpha_s &&
                  work && out,
              "step_reduce: null argument");
@@ -434,6 +544,7 @@ int fw_step_reduce(const fw_grid* g, const void* u, const void* v, const void* w
 int fw_levelset_update(const fw_grid* g, const void* phi, const void* phi0,
                        const void* v_n, void* phi_out, double step, int mode,
                        void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && phi && phi_out && phi != phi_out && (mode == 0 || mode == 1),
              "levelset_update: bad argument");
   FW_REQUIRE(mode == 1 ? phi0 != nullptr : v_n != nullptr,

@@ -29,8 +29,8 @@ k_advect_muscl(GridArgs g, const T* __restrict__ phi, const T* __restrict__ u,
   const size_t sy = Nx, sz = (size_t)Nx * Ny;
   const T dx = T(g.dx), dy = T(g.dy);
   const T da = T(g.d_above[k]), db = T(g.d_below[k]);
-  const T inv_db = T(1.0 / g.d_below[k]);
-  const T inv_da = T(1.0 / g.d_above[k]);
+  const T inv_db = T(g.inv_db[k]);
+  const T inv_da = T(g.inv_da[k]);
   const T uc = u[c], vc = v[c], wc = w[c], pc = phi[c];
   const T xL = (i == 0) ? phi_inlet : phi[c - 1];
   const T xR = (i == Nx - 1) ? pc : phi[c + 1];
@@ -65,6 +65,76 @@ k_advect_muscl(GridArgs g, const T* __restrict__ phi, const T* __restrict__ u,
 // ---------------------------------------------------------------------------
 // species transport (species_3d.py:40-175)
 // ---------------------------------------------------------------------------
+// New value of species Y at cell (i, j, k) (species_3d.py:40-175): MUSCL
+// advection (x/z MUSCL on 2..N-3, upwind with ghosts elsewhere, y MUSCL
+// periodic), D central diffusion, source / rho, clip [0, 1], then y_max.
+// kind 0: the source is S[c]; kind 1/2/3: the dilution source of the time
+// step (spread_3d.py:2990-3004) from S_pyro part sp and S_dry part sd.
+// irc = 1/rho and rhz = 1/(0.5 (d_above + d_below)) are used by the float32
+// build only (quot); float64 divides as the reference does.
+template <typename T>
+__device__ __forceinline__ T species_cell(const GridArgs& g, int i, int k, const Nbr& n,
+                                          const T* __restrict__ Y, T rc, T irc, T rhz,
+                                          T ui, T vi, T wi, T Sgiven, T sp, T sd, int kind,
+                                          T dt, T D, T Y_inlet, T y_max) {
+  const int Nx = g.nx, Nz = g.nz;
+  const T inv_dx = T(g.inv_dx), inv_dy = T(g.inv_dy);
+  const T inv_dx2 = inv_dx * inv_dx, inv_dy2 = inv_dy * inv_dy;
+  const T Yi = at(Y, n.c);
+  const T YL_x = (i == 0) ? Y_inlet : at(Y, n.xm1);
+  const T YR_x = (i == Nx - 1) ? Yi : at(Y, n.xp1);
+  const T YL_z = (k == 0) ? Yi : at(Y, n.zm1);
+  const T YR_z = (k == Nz - 1) ? Yi : at(Y, n.zp1);
+  T flux_x;
+  if (i >= 2 && i <= Nx - 3) {
+    T fp = muscl_face<T>(YL_x, Yi, YR_x, at(Y, n.xp2), ui);
+    T fm = muscl_face<T>(at(Y, n.xm2), YL_x, Yi, YR_x, ui);
+    flux_x = ui * (fp - fm) * inv_dx;
+  } else {
+    flux_x = (ui >= T(0)) ? ui * (Yi - YL_x) * inv_dx : ui * (YR_x - Yi) * inv_dx;
+  }
+  const T Yjm1 = at(Y, n.ym1), Yjp1 = at(Y, n.yp1);
+  T fyp = muscl_face<T>(Yjm1, Yi, Yjp1, at(Y, n.yp2), vi);
+  T fym = muscl_face<T>(at(Y, n.ym2), Yjm1, Yi, Yjp1, vi);
+  T flux_y = vi * (fyp - fym) * inv_dy;
+  const T inv_da = T(g.inv_da[k]);
+  const T inv_db = T(g.inv_db[k]);
+  const T inv_dz = T(g.inv_dz[k]);
+  T flux_z;
+  if (k >= 2 && k <= Nz - 3) {
+    T fp = muscl_face<T>(YL_z, Yi, YR_z, at(Y, n.zp2), wi);
+    T fm = muscl_face<T>(at(Y, n.zm2), YL_z, Yi, YR_z, wi);
+    flux_z = quot(wi * (fp - fm), T(0.5) * (T(g.d_above[k]) + T(g.d_below[k])), rhz);
+  } else {
+    flux_z = (wi >= T(0)) ? wi * (Yi - YL_z) * inv_db : wi * (YR_z - Yi) * inv_da;
+  }
+  T adv = -(flux_x + flux_y + flux_z);
+  T d2x = (YR_x - T(2) * Yi + YL_x) * inv_dx2;
+  T d2y = (Yjp1 - T(2) * Yi + Yjm1) * inv_dy2;
+  T d2z = ((YR_z - Yi) * inv_da - (Yi - YL_z) * inv_db) * inv_dz;
+  T diff = D * (d2x + d2y + d2z);
+  T Se;
+  if (kind == 0) {
+    Se = Sgiven;
+  } else {
+    const T st = sp + sd;
+    Se = kind == 1 ? sp - Yi * st : (kind == 2 ? -Yi * st : sd - Yi * st);
+  }
+  T src = quot(Se, rc, irc);
+  T dY = (adv + diff + src) * dt;
+  T nv = Yi + dY;
+  if (nv < T(0)) nv = T(0);
+  else if (nv > T(1)) nv = T(1);
+  if (nv > y_max) nv = y_max;
+  return nv;
+}
+
+// 1 / (0.5 (d_above + d_below)) of level k, for quot() in float32
+template <typename T>
+__device__ __forceinline__ T rhz_of(const GridArgs& g, int k) {
+  return recip(T(0.5) * (T(g.d_above[k]) + T(g.d_below[k])));
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_species(GridArgs g, const T* __restrict__ Y, T* __restrict__ Yo,
@@ -75,65 +145,37 @@ k_species(GridArgs g, const T* __restrict__ Y, T* __restrict__ Yo,
   int i, j, k;
   cell_ijk(i, j, k);
   if (i >= g.nx) return;
-  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
-  const size_t c = cidx(g, k, j, i);
-  const size_t sy = Nx, sz = (size_t)Nx * Ny;
-  const T inv_dx = T(1.0 / g.dx), inv_dy = T(1.0 / g.dy);
-  const T inv_dx2 = inv_dx * inv_dx, inv_dy2 = inv_dy * inv_dy;
-  const T ui = u[c], vi = v[c], wi = w[c], Yi = Y[c];
-  const T YL_x = (i == 0) ? Y_inlet : Y[c - 1];
-  const T YR_x = (i == Nx - 1) ? Yi : Y[c + 1];
-  const T YL_z = (k == 0) ? Yi : Y[c - sz];
-  const T YR_z = (k == Nz - 1) ? Yi : Y[c + sz];
-  T flux_x;
-  if (i >= 2 && i <= Nx - 3) {
-    T fp = muscl_face<T>(Y[c - 1], Yi, Y[c + 1], Y[c + 2], ui);
-    T fm = muscl_face<T>(Y[c - 2], Y[c - 1], Yi, Y[c + 1], ui);
-    flux_x = ui * (fp - fm) * inv_dx;
-  } else {
-    flux_x = (ui >= T(0)) ? ui * (Yi - YL_x) * inv_dx : ui * (YR_x - Yi) * inv_dx;
-  }
-  const size_t rm2 = c + (size_t)(wrap(j - 2, Ny) - j) * sy;
-  const size_t rm1 = c + (size_t)(wrap(j - 1, Ny) - j) * sy;
-  const size_t rp1 = c + (size_t)(wrap(j + 1, Ny) - j) * sy;
-  const size_t rp2 = c + (size_t)(wrap(j + 2, Ny) - j) * sy;
-  const T Yjm1 = Y[rm1], Yjp1 = Y[rp1];
-  T fyp = muscl_face<T>(Yjm1, Yi, Yjp1, Y[rp2], vi);
-  T fym = muscl_face<T>(Y[rm2], Yjm1, Yi, Yjp1, vi);
-  T flux_y = vi * (fyp - fym) * inv_dy;
-  const T inv_da = T(1.0 / g.d_above[k]);
-  const T inv_db = T(1.0 / g.d_below[k]);
-  const T inv_dz = T(1.0 / g.dz[k]);
-  T flux_z;
-  if (k >= 2 && k <= Nz - 3) {
-    T fp = muscl_face<T>(Y[c - sz], Yi, Y[c + sz], Y[c + 2 * sz], wi);
-    T fm = muscl_face<T>(Y[c - 2 * sz], Y[c - sz], Yi, Y[c + sz], wi);
-    flux_z = wi * (fp - fm) / (T(0.5) * (T(g.d_above[k]) + T(g.d_below[k])));
-  } else {
-    flux_z = (wi >= T(0)) ? wi * (Yi - YL_z) * inv_db : wi * (YR_z - Yi) * inv_da;
-  }
-  T adv = -(flux_x + flux_y + flux_z);
-  T d2x = (YR_x - T(2) * Yi + YL_x) * inv_dx2;
-  T d2y = (Yjp1 - T(2) * Yi + Yjm1) * inv_dy2;
-  T d2z = ((YR_z - Yi) * inv_da - (Yi - YL_z) * inv_db) * inv_dz;
-  T diff = D * (d2x + d2y + d2z);
-  // effective source: given, or the dilution form of spread_3d.py:2990-3004
-  // (S_tot = Sp + Sd; fuel Sp - Y S_tot, O2 -Y S_tot, H2O Sd - Y S_tot)
-  T Se;
-  if (kind == 0) {
-    Se = S[c];
-  } else {
-    const T sp = Sp[c], sd = Sd[c];
-    const T st = sp + sd;
-    Se = kind == 1 ? sp - Yi * st : (kind == 2 ? -Yi * st : sd - Yi * st);
-  }
-  T src = Se / rho[c];
-  T dY = (adv + diff + src) * dt;
-  T nv = Yi + dY;
-  if (nv < T(0)) nv = T(0);
-  else if (nv > T(1)) nv = T(1);
-  if (nv > y_max) nv = y_max;
-  Yo[c] = nv;
+  const Nbr n = nbr_offsets<T>(g, i, j, k);
+  const T rc = at(rho, n.c);
+  at(Yo, n.c) = species_cell<T>(g, i, k, n, Y, rc, recip(rc), rhz_of<T>(g, k), at(u, n.c),
+                                at(v, n.c), at(w, n.c), kind == 0 ? at(S, n.c) : T(0),
+                                kind == 0 ? T(0) : at(Sp, n.c), kind == 0 ? T(0) : at(Sd, n.c),
+                                kind, dt, D, Y_inlet, y_max);
+}
+
+// The three species of a sub-step (fuel, O2, H2O) in one pass: rho, u, v, w,
+// the two particle sources and the stencil offsets serve all three.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_species3(GridArgs g, const T* __restrict__ Y1, const T* __restrict__ Y2,
+           const T* __restrict__ Y3, T* __restrict__ O1, T* __restrict__ O2,
+           T* __restrict__ O3, const T* __restrict__ rho, const T* __restrict__ u,
+           const T* __restrict__ v, const T* __restrict__ w, const T* __restrict__ Sp,
+           const T* __restrict__ Sd, T dt, T D, T in1, T in2, T in3, T mx1, T mx2,
+           T mx3) {
+  int i, j, k;
+  cell_ijk(i, j, k);
+  if (i >= g.nx) return;
+  const Nbr n = nbr_offsets<T>(g, i, j, k);
+  const T rc = at(rho, n.c), ui = at(u, n.c), vi = at(v, n.c), wi = at(w, n.c);
+  const T sp = at(Sp, n.c), sd = at(Sd, n.c);
+  const T irc = recip(rc), rhz = rhz_of<T>(g, k);
+  at(O1, n.c) = species_cell<T>(g, i, k, n, Y1, rc, irc, rhz, ui, vi, wi, T(0), sp, sd, 1,
+                                dt, D, in1, mx1);
+  at(O2, n.c) = species_cell<T>(g, i, k, n, Y2, rc, irc, rhz, ui, vi, wi, T(0), sp, sd, 2,
+                                dt, D, in2, mx2);
+  at(O3, n.c) = species_cell<T>(g, i, k, n, Y3, rc, irc, rhz, ui, vi, wi, T(0), sp, sd, 3,
+                                dt, D, in3, mx3);
 }
 
 // ---------------------------------------------------------------------------
@@ -153,8 +195,8 @@ k_gas_energy(GridArgs g, const T* __restrict__ Tg, T* __restrict__ To,
   const size_t sy = Nx, sz = (size_t)Nx * Ny;
   const T dx = T(g.dx), dy = T(g.dy);
   const T da = T(g.d_above[k]), db = T(g.d_below[k]);
-  const T inv_db = T(1.0 / g.d_below[k]);
-  const T inv_da = T(1.0 / g.d_above[k]);
+  const T inv_db = T(g.inv_db[k]);
+  const T inv_da = T(g.inv_da[k]);
   const T uc = u[c], vc = v[c], wc = w[c], pc = Tg[c];
   const T xL = (i == 0) ? T_amb : Tg[c - 1];
   const T xR = (i == Nx - 1) ? pc : Tg[c + 1];
@@ -186,7 +228,7 @@ k_gas_energy(GridArgs g, const T* __restrict__ Tg, T* __restrict__ To,
   }
   T dT = T(0) - (flux_x + flux_y + flux_z);
   // x diffusion: interior / inlet ghost T_amb / outlet zero-flux
-  const T inv_dx2 = T(1.0 / (g.dx * g.dx));
+  const T inv_dx2 = T(g.inv_dx2);
   if (i >= 1 && i <= Nx - 2)
     dT += alpha_th * (Tg[c + 1] - T(2) * pc + Tg[c - 1]) * inv_dx2;
   else if (i == 0)
@@ -197,7 +239,7 @@ k_gas_energy(GridArgs g, const T* __restrict__ Tg, T* __restrict__ To,
   // z diffusion
   if (k >= 1 && k <= Nz - 2) {
     dT += alpha_th * ((Tg[c + sz] - pc) * inv_da - (pc - Tg[c - sz]) * inv_db) *
-          T(1.0 / g.dz[k]);
+          T(g.inv_dz[k]);
   }
   if (k == 0) dT += alpha_th * ((Tg[c + sz] - pc) / da) / T(g.dz[0]);
   if (k == Nz - 1) dT += alpha_th * (-(pc - Tg[c - sz]) / db) / T(g.dz[Nz - 1]);
@@ -210,19 +252,17 @@ k_gas_energy(GridArgs g, const T* __restrict__ Tg, T* __restrict__ To,
 // ---------------------------------------------------------------------------
 // turbulent diffusion pass (turbulence_3d.py:920-949, one sub-step)
 // ---------------------------------------------------------------------------
+// One explicit turbulent-diffusion sub-step of f at cell c with zero-flux
+// ghosts (turbulence_3d.py:920-949); optional clip folded from the caller
+// (spread_3d.py:3079-3084), lo > hi disables it.
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_turb_diff(GridArgs g, const T* __restrict__ f, T* __restrict__ fo,
-            const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo, T hi) {
-  int i, j, k;
-  cell_ijk(i, j, k);
-  if (i >= g.nx) return;
+__device__ __forceinline__ T turb_cell(const GridArgs& g, int i, int j, int k, size_t c,
+                                       const T* __restrict__ f, T D_t, T dt_sub, T lo, T hi) {
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
-  const size_t c = cidx(g, k, j, i);
   const size_t sy = Nx, sz = (size_t)Nx * Ny;
-  const T inv_dx2 = T(1.0 / (g.dx * g.dx)), inv_dy2 = T(1.0 / (g.dy * g.dy));
-  const T inv_dz = T(1.0 / g.dz[k]);
-  const T inv_da = T(1.0 / g.d_above[k]), inv_db = T(1.0 / g.d_below[k]);
+  const T inv_dx2 = T(g.inv_dx2), inv_dy2 = T(g.inv_dy2);
+  const T inv_dz = T(g.inv_dz[k]);
+  const T inv_da = T(g.inv_da[k]), inv_db = T(g.inv_db[k]);
   const T fc = f[c];
   const T fxL = (i == 0) ? fc : f[c - 1];
   const T fxR = (i == Nx - 1) ? fc : f[c + 1];
@@ -230,19 +270,44 @@ k_turb_diff(GridArgs g, const T* __restrict__ f, T* __restrict__ fo,
   const T fzR = (k == Nz - 1) ? fc : f[c + sz];
   const T fyp = f[c + (size_t)(wrap(j + 1, Ny) - j) * sy];
   const T fym = f[c + (size_t)(wrap(j - 1, Ny) - j) * sy];
-  const T D_t = nu_t[c] / sc_t;
   T d2x = (fxR - T(2) * fc + fxL) * inv_dx2;
   T d2y = (fyp - T(2) * fc + fym) * inv_dy2;
   T d2z = ((fzR - fc) * inv_da - (fc - fzL) * inv_db) * inv_dz;
   T df = D_t * (d2x + d2y + d2z) * dt_sub;
   T nv = fc + df;
-  // optional clip folded from the caller (spread_3d.py:3079-3084);
-  // lo > hi disables it.
   if (lo <= hi) {
     nv = nv < lo ? lo : nv;
     nv = nv > hi ? hi : nv;
   }
-  fo[c] = nv;
+  return nv;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_turb_diff(GridArgs g, const T* __restrict__ f, T* __restrict__ fo,
+            const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo, T hi) {
+  int i, j, k;
+  cell_ijk(i, j, k);
+  if (i >= g.nx) return;
+  const size_t c = cidx(g, k, j, i);
+  fo[c] = turb_cell<T>(g, i, j, k, c, f, nu_t[c] / sc_t, dt_sub, lo, hi);
+}
+
+// The same pass for three fields sharing nu_t (the species of a sub-step).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_turb_diff3(GridArgs g, const T* __restrict__ f1, const T* __restrict__ f2,
+             const T* __restrict__ f3, T* __restrict__ o1, T* __restrict__ o2,
+             T* __restrict__ o3, const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo1,
+             T hi1, T lo2, T hi2, T lo3, T hi3) {
+  int i, j, k;
+  cell_ijk(i, j, k);
+  if (i >= g.nx) return;
+  const size_t c = cidx(g, k, j, i);
+  const T D_t = nu_t[c] / sc_t;
+  o1[c] = turb_cell<T>(g, i, j, k, c, f1, D_t, dt_sub, lo1, hi1);
+  o2[c] = turb_cell<T>(g, i, j, k, c, f2, D_t, dt_sub, lo2, hi2);
+  o3[c] = turb_cell<T>(g, i, j, k, c, f3, D_t, dt_sub, lo3, hi3);
 }
 
 // Max of nu_t over interior cells 1<=k<=Nz-2, 1<=i<=Nx-2, all j, starting
@@ -366,7 +431,7 @@ k_momentum(GridArgs g, const T* __restrict__ u, const T* __restrict__ v,
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t c = cidx(g, k, j, i);
   const size_t sy = Nx, sz = (size_t)Nx * Ny;
-  const T inv_dx = T(1.0 / g.dx), inv_dy = T(1.0 / g.dy);
+  const T inv_dx = T(g.inv_dx), inv_dy = T(g.inv_dy);
   const T inv_dx2 = inv_dx * inv_dx, inv_dy2 = inv_dy * inv_dy;
   const T ui = u[c], vi = v[c], wi = w[c];
   const T rho_i = rho[c];
@@ -376,8 +441,8 @@ k_momentum(GridArgs g, const T* __restrict__ u, const T* __restrict__ v,
   const size_t rp1 = c + (size_t)(wrap(j + 1, Ny) - j) * sy;
   const size_t rp2 = c + (size_t)(wrap(j + 2, Ny) - j) * sy;
   const bool mx = (i >= 2 && i <= Nx - 3), mz = (k >= 2 && k <= Nz - 3);
-  const T inv_da = T(1.0 / g.d_above[k]), inv_db = T(1.0 / g.d_below[k]);
-  const T inv_dz = T(1.0 / g.dz[k]);
+  const T inv_da = T(g.inv_da[k]), inv_db = T(g.inv_db[k]);
+  const T inv_dz = T(g.inv_dz[k]);
   const T inv_dz_eff = T(1.0 / (0.5 * (g.d_above[k] + g.d_below[k])));
   const T* F[3] = {u, v, w};
   const T* Gin[3] = {u_in, v_in, w_in};
@@ -509,6 +574,37 @@ static int species_impl(const fw_grid* g, const void* Y, void* Yo,
   return FW_OK;
 }
 
+KernelEvents g_species3_events;
+
+template <typename T>
+static int species3_impl(const fw_grid* g, const void* const* Y, void* const* Yo,
+                         const void* rho, const void* u, const void* v, const void* w,
+                         const void* Sp, const void* Sd, double dt, double D,
+                         const double* y_in, const double* y_max, hipStream_t s) {
+  Launch L = cell_launch(g);
+  hipExtLaunchKernelGGL(k_species3<T>, L.grid, L.block, 0, s, g_species3_events.start,
+                     g_species3_events.stop, 0, grid_args(g), (const T*)Y[0],
+                     (const T*)Y[1], (const T*)Y[2], (T*)Yo[0], (T*)Yo[1], (T*)Yo[2],
+                     (const T*)rho, (const T*)u, (const T*)v, (const T*)w, (const T*)Sp,
+                     (const T*)Sd, T(dt), T(D), T(y_in[0]), T(y_in[1]), T(y_in[2]),
+                     T(y_max[0]), T(y_max[1]), T(y_max[2]));
+  FW_CHECK_LAUNCH("species_transport3");
+  return FW_OK;
+}
+
+template <typename T>
+static int turb3_impl(const fw_grid* g, const void* const* f, void* const* fo,
+                      const void* nu_t, double sc_t, double dt_sub, const double* lo,
+                      const double* hi, hipStream_t s) {
+  Launch L = cell_launch(g);
+  hipLaunchKernelGGL(k_turb_diff3<T>, L.grid, L.block, 0, s, grid_args(g), (const T*)f[0],
+                     (const T*)f[1], (const T*)f[2], (T*)fo[0], (T*)fo[1], (T*)fo[2],
+                     (const T*)nu_t, T(sc_t), T(dt_sub), T(lo[0]), T(hi[0]), T(lo[1]),
+                     T(hi[1]), T(lo[2]), T(hi[2]));
+  FW_CHECK_LAUNCH("turbulent_diffusion3");
+  return FW_OK;
+}
+
 template <typename T>
 static int gas_energy_impl(const fw_grid* g, const void* Tg, void* To,
                            const void* u, const void* v, const void* w,
@@ -631,6 +727,7 @@ extern "C" {
 int fw_advect_scalar_muscl(const fw_grid* g, const void* phi, const void* u,
                            const void* v, const void* w, void* rhs,
                            double phi_inlet, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && g->nx >= 1 && g->ny >= 1 && g->nz >= 1, "bad grid");
   return FW_DISPATCH(g, advect_impl, g, phi, u, v, w, rhs, phi_inlet,
                      (hipStream_t)stream);
@@ -640,6 +737,8 @@ int fw_species_transport(const fw_grid* g, const void* Y, void* Y_out,
                          const void* rho, const void* u, const void* v,
                          const void* w, const void* S, double dt, double D,
                          double Y_inlet, double y_max, void* stream) {
+  FW_GRID(g);
+  FW_GRID_OFFSETS32(g);
   FW_REQUIRE(g && Y != Y_out && S, "species_transport: Y_out must not alias Y");
   return FW_DISPATCH(g, species_impl, g, Y, Y_out, rho, u, v, w, S, nullptr,
                      nullptr, 0, dt, D, Y_inlet, y_max, (hipStream_t)stream);
@@ -651,6 +750,8 @@ int fw_species_transport_dilution(const fw_grid* g, const void* Y, void* Y_out,
                                   const void* S_dry_part, int kind, double dt,
                                   double D, double Y_inlet, double y_max,
                                   void* stream) {
+  FW_GRID(g);
+  FW_GRID_OFFSETS32(g);
   FW_REQUIRE(g && Y != Y_out && S_pyro_part && S_dry_part && kind >= 1 && kind <= 3,
              "species_transport_dilution: bad argument");
   return FW_DISPATCH(g, species_impl, g, Y, Y_out, rho, u, v, w, nullptr,
@@ -658,9 +759,39 @@ int fw_species_transport_dilution(const fw_grid* g, const void* Y, void* Y_out,
                      (hipStream_t)stream);
 }
 
+int fw_species_transport3(const fw_grid* g, const void* const* Y, void* const* Y_out,
+                          const void* rho, const void* u, const void* v, const void* w,
+                          const void* S_pyro_part, const void* S_dry_part, double dt,
+                          double D, const double* Y_inlet, const double* y_max,
+                          void* stream) {
+  FW_GRID(g);
+  FW_GRID_OFFSETS32(g);
+  FW_REQUIRE(g && Y && Y_out && Y_inlet && y_max && S_pyro_part && S_dry_part,
+             "species_transport3: null argument");
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      FW_REQUIRE(Y[a] != Y_out[b], "species_transport3: outputs must not alias inputs");
+  return FW_DISPATCH(g, species3_impl, g, Y, Y_out, rho, u, v, w, S_pyro_part,
+                     S_dry_part, dt, D, Y_inlet, y_max, (hipStream_t)stream);
+}
+
+int fw_turbulent_diffusion_pass3(const fw_grid* g, const void* const* f, void* const* f_out,
+                                 const void* nu_t, double sc_t, double dt_sub,
+                                 const double* clip_lo, const double* clip_hi,
+                                 void* stream) {
+  FW_GRID(g);
+  FW_REQUIRE(g && f && f_out && clip_lo && clip_hi, "turbulent_diffusion3: null argument");
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      FW_REQUIRE(f[a] != f_out[b], "turbulent_diffusion3: outputs must not alias inputs");
+  return FW_DISPATCH(g, turb3_impl, g, f, f_out, nu_t, sc_t, dt_sub, clip_lo, clip_hi,
+                     (hipStream_t)stream);
+}
+
 int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
                   const void* u, const void* v, const void* w, double dt,
                   double alpha_th, double T_amb, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && T_g != T_out, "gas_energy: T_out must not alias T_g");
   FW_REQUIRE(g->nx >= 2 && g->nz >= 2, "gas_energy: needs Nx, Nz >= 2");
   return FW_DISPATCH(g, gas_energy_impl, g, T_g, T_out, u, v, w, dt, alpha_th,
@@ -670,6 +801,7 @@ int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
 int fw_turbulent_diffusion_pass(const fw_grid* g, const void* f, void* f_out,
                                 const void* nu_t, double sc_t, double dt_sub,
                                 double clip_lo, double clip_hi, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && f != f_out, "turbulent_diffusion: f_out must not alias f");
   return FW_DISPATCH(g, turb_diff_pass_impl, g, f, f_out, nu_t, sc_t, dt_sub,
                      clip_lo, clip_hi, (hipStream_t)stream);
@@ -677,6 +809,7 @@ int fw_turbulent_diffusion_pass(const fw_grid* g, const void* f, void* f_out,
 
 int fw_nu_t_max_interior(const fw_grid* g, const void* nu_t, double* work,
                          double* out_host, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && work && out_host, "nu_t_max_interior: null pointer");
   hipStream_t s = (hipStream_t)stream;
   int rc = FW_DISPATCH(g, nu_t_max_impl, g, nu_t, work, s);
@@ -696,6 +829,7 @@ int fw_solid_conduction_vertical(const fw_grid* g, const void* T_s,
                                  double k_solid, double rho_solid,
                                  double cp_solid, double dt, double clip_lo,
                                  double clip_hi, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && T_s != T_out, "solid_conduction: T_out must not alias T_s");
   return FW_DISPATCH(g, solid_cond_impl, g, T_s, T_out, alpha_s, k_solid,
                      rho_solid, cp_solid, dt, clip_lo, clip_hi,
@@ -706,6 +840,7 @@ int fw_drag_force(const fw_grid* g, const void* u, const void* v,
                   const void* w, const void* rho, const void* alpha_s,
                   double sigma_sav, double C_D, void* Fx, void* Fy, void* Fz,
                   void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g, "bad grid");
   return FW_DISPATCH(g, drag_impl, g, u, v, w, rho, alpha_s, sigma_sav, C_D,
                      Fx, Fy, Fz, (hipStream_t)stream);
@@ -718,6 +853,7 @@ int fw_tentative_velocity(const fw_grid* g, const void* u, const void* v,
                           const void* u_inlet, const void* v_inlet,
                           const void* w_inlet, double dt, double T_amb,
                           void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && u != u_out && v != v_out && w != w_out,
              "tentative_velocity: outputs must not alias inputs");
   return FW_DISPATCH(g, momentum_impl, g, u, v, w, u_out, v_out, w_out, rho,
@@ -728,6 +864,7 @@ int fw_tentative_velocity(const fw_grid* g, const void* u, const void* v,
 int fw_outflow_sponge(const fw_grid* g, void* u, const void* u_target,
                       const void* sigma_x, const void* Y_F, double Y_F_skip,
                       double dt, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g, "bad grid");
   return FW_DISPATCH(g, sponge_impl, g, u, u_target, sigma_x, Y_F, Y_F_skip,
                      dt, (hipStream_t)stream);
@@ -735,6 +872,7 @@ int fw_outflow_sponge(const fw_grid* g, void* u, const void* u_target,
 
 int fw_eos(const fw_grid* g, const void* T_g, void* rho, double T_amb,
            void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g, "bad grid");
   return FW_DISPATCH(g, eos_impl, g, T_g, rho, T_amb, (hipStream_t)stream);
 }
@@ -744,6 +882,7 @@ int fw_soil_conduction(const fw_grid* g, int n_soil, void* T_soil,
                        const double* soil_d_above, const double* soil_d_below,
                        double dt, double k_s, double rho_s, double cp_s,
                        double eps_s, double T_amb, void* stream) {
+  FW_GRID(g);
   FW_REQUIRE(g && n_soil >= 1, "soil_conduction: n_soil >= 1");
   return FW_DISPATCH(g, soil_impl, g, n_soil, T_soil, q_in_surface, soil_dz,
                      soil_d_above, soil_d_below, dt, k_s, rho_s, cp_s, eps_s,

@@ -63,10 +63,18 @@ class DeviceGrid:
             zf = np.asarray(z_face, np.float64)
         self.z_face = torch.as_tensor(zf, **f64).contiguous()
         self.n_z_bed = int(n_z_bed)
+        # float64 reciprocals, IEEE quotients from the host (the kernels multiply
+        # by them instead of dividing)
+        inv = lambda a: torch.as_tensor(1.0 / np.asarray(a, np.float64), **f64).contiguous()
+        self.inv_dz = inv(dz_arr)
+        self.inv_d_above = inv(d_face_above)
+        self.inv_d_below = inv(d_face_below)
         self.c = _lib.FwGrid(self.nx, self.ny, self.nz, fw_dtype(dtype),
                              self.dx, self.dy, self.dz.data_ptr(),
                              self.d_above.data_ptr(), self.d_below.data_ptr(),
-                             self.z_face.data_ptr(), self.n_z_bed, 0)
+                             self.z_face.data_ptr(), self.n_z_bed, 0,
+                             self.inv_dz.data_ptr(), self.inv_d_above.data_ptr(),
+                             self.inv_d_below.data_ptr())
 
     @property
     def ref(self):
@@ -97,7 +105,8 @@ class DeviceGrid:
         g.c = _lib.FwGrid(self.nx, self.ny, self.nz, fw_dtype(dtype), self.dx,
                           self.dy, self.dz.data_ptr(), self.d_above.data_ptr(),
                           self.d_below.data_ptr(), self.z_face.data_ptr(),
-                          self.n_z_bed, 0)
+                          self.n_z_bed, 0, self.inv_dz.data_ptr(),
+                          self.inv_d_above.data_ptr(), self.inv_d_below.data_ptr())
         return g
 
 

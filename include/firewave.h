@@ -35,7 +35,9 @@ enum { FW_F32 = 0, FW_F64 = 1 };
 
 /* Grid metrics (Grid3D, spread_3d.py:160-424).  The 1-D arrays are device
  * pointers to float64 arrays: dz, d_above, d_below of length nz, z_face of
- * length nz+1.  y is periodic (the only y boundary of the hot path). */
+ * length nz+1, and inv_dz, inv_d_above, inv_d_below = the elementwise
+ * float64 reciprocals 1.0 / dz etc. (all required).  y is periodic (the only
+ * y boundary of the hot path). */
 typedef struct fw_grid {
   int32_t nx, ny, nz;
   int32_t dtype;   /* FW_F32 or FW_F64: storage and arithmetic type of fields */
@@ -46,6 +48,9 @@ typedef struct fw_grid {
   const double* z_face;
   int32_t n_z_bed;
   int32_t reserved;
+  const double* inv_dz;
+  const double* inv_d_above;
+  const double* inv_d_below;
 } fw_grid;
 
 const char* fw_last_error(void);
@@ -79,6 +84,16 @@ int fw_species_transport_dilution(const fw_grid* g, const void* Y, void* Y_out,
                                   double D, double Y_inlet, double y_max,
                                   void* stream);
 
+/* The three species of a sub-step in one pass (fw_species_transport_dilution
+ * with kind 1, 2, 3 for Y[0], Y[1], Y[2]): rho, u, v, w and the particle
+ * sources are read once.  Y, Y_out: 3 field pointers each (no aliasing);
+ * Y_inlet, y_max: 3 host doubles each. */
+int fw_species_transport3(const fw_grid* g, const void* const* Y, void* const* Y_out,
+                          const void* rho, const void* u, const void* v, const void* w,
+                          const void* S_pyro_part, const void* S_dry_part, double dt,
+                          double D, const double* Y_inlet, const double* y_max,
+                          void* stream);
+
 /* _adv_gas_energy (spread_3d.py:3827-3897): MUSCL advection with inlet ghost
  * T_amb + alpha_th FV diffusion, then clamp to [T_amb, 1e4]; out of place. */
 int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
@@ -93,6 +108,13 @@ int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
 int fw_turbulent_diffusion_pass(const fw_grid* g, const void* f, void* f_out,
                                 const void* nu_t, double sc_t, double dt_sub,
                                 double clip_lo, double clip_hi, void* stream);
+
+/* fw_turbulent_diffusion_pass for three fields sharing nu_t; f, f_out: 3
+ * field pointers each (no aliasing); clip_lo, clip_hi: 3 host doubles each. */
+int fw_turbulent_diffusion_pass3(const fw_grid* g, const void* const* f,
+                                 void* const* f_out, const void* nu_t, double sc_t,
+                                 double dt_sub, const double* clip_lo,
+                                 const double* clip_hi, void* stream);
 
 /* max(nu_t) over interior cells (turbulence_3d.py:890-897); synchronises the
  * stream and writes the host scalar.  work: FW_REDUCE_BLOCKS (256) doubles. */
@@ -346,6 +368,19 @@ int fw_gas_energy_update(const fw_grid* g, void* T_g, const void* T_s,
                          double drip_W_m3, double dt_sub, double T_amb,
                          void* stream);
 
+/* fw_gas_energy_update followed by fw_solid_conduction_vertical in one pass
+ * (spread_3d.py:3104-3146 then :3168-3177): T_g in place, T_s -> T_s_out
+ * clamped to [T_s_lo, T_s_hi]; both read the T_s of before the conduction. */
+int fw_gas_update_solid_conduction(const fw_grid* g, void* T_g, const void* T_s,
+                                   void* T_s_out, const void* alpha_s, const void* rho,
+                                   const void* Y_H2O, const void* S_pyro_part,
+                                   const void* S_dry_part, const void* Q_g_conv,
+                                   const void* q_rad_gas, void* Q_comb_out, int n_z_bed,
+                                   int i_src0, int i_src1, double drip_W_m3, double dt_sub,
+                                   double T_amb, double k_solid, double rho_solid,
+                                   double cp_solid, double T_s_lo, double T_s_hi,
+                                   void* stream);
+
 /* field[k, :, i0:i1] += add  (ignition pulse, spread_3d.py:2528-2532). */
 int fw_add_patch(const fw_grid* g, void* field, int k, int i0, int i1,
                  double add, void* stream);
@@ -468,7 +503,7 @@ int fw_spread_reduce(const fw_grid* g, const fw_spread_fields* f,
                      fw_spread_state* st, void* stream);
 
 /* Benchmark timing: while enabled, fw_spread_step records a HIP event pair
- * on its stream around each sub-step's three k_species launches (at most
+ * on its stream around each sub-step's k_species3 launch (at most
  * 4096 pairs).  fw_spread_timer(1) resets and starts, fw_spread_timer(0)
  * stops; fw_spread_timer_read waits for the events and returns the summed
  * milliseconds and the number of launches covered. */

@@ -264,3 +264,75 @@ def test_glue_kernels_fp64():
            Ts2.max(), Sp.max(), Qe.max(), om.max(), YF.max(), float((Ts2 >= 600.0).sum()),
            float(LS.flame_active(om, Tg, YF).sum()), float(cols[-1])]
     assert list(r) == exp
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_fused_substep_kernels_match_single(dtype):
+    """k_species3, k_turb_diff3 and k_gas_solid reuse the per-cell code of the
+    single-field kernels: results are bit-identical in both precisions."""
+    import torch
+    from firewebapp_amd import _lib
+    from firewebapp_amd.device import DeviceGrid, device, ptr, stream_ptr
+    import ctypes as C
+    td = torch.float64 if dtype == "float64" else torch.float32
+    rng = np.random.default_rng(21)
+    nz, ny, nx, nzb = 9, 5, 23, 3
+    dz = 0.02 + 0.05 * rng.random(nz)
+    da, db = dz * 1.1, dz * 0.9
+    g = DeviceGrid(nx, ny, nz, 0.1, 0.12, dz, da, db, n_z_bed=nzb, dtype=td)
+    sh = (nz, ny, nx)
+    keep = []
+
+    def d(a):
+        t = torch.as_tensor(np.ascontiguousarray(a), dtype=td, device=device())
+        keep.append(t)
+        return t
+
+    Ys = [d(0.3 * rng.random(sh)), d(0.232 * rng.random(sh)), d(0.2 * rng.random(sh))]
+    rho, u, v, w = d(0.4 + rng.random(sh)), d(rng.normal(size=sh)), \
+        d(rng.normal(size=sh)), d(rng.normal(size=sh))
+    Sp, Sd = d(0.05 * rng.random(sh)), d(0.02 * rng.random(sh))
+    y_in, y_max = (0.0, 0.232, 0.0), (1.0, 0.232, 1.0)
+    single = [torch.empty_like(Ys[0]) for _ in range(3)]
+    for q in range(3):
+        _lib.call("fw_species_transport_dilution", g.ref, ptr(Ys[q]), ptr(single[q]),
+                  ptr(rho), ptr(u), ptr(v), ptr(w), ptr(Sp), ptr(Sd), q + 1, 1e-3, 1e-5,
+                  y_in[q], y_max[q], stream_ptr())
+    fused = [torch.empty_like(Ys[0]) for _ in range(3)]
+    VP3 = C.c_void_p * 3
+    D3 = C.c_double * 3
+    _lib.call("fw_species_transport3", g.ref, VP3(*[ptr(t) for t in Ys]),
+              VP3(*[ptr(t) for t in fused]), ptr(rho), ptr(u), ptr(v), ptr(w), ptr(Sp),
+              ptr(Sd), 1e-3, 1e-5, D3(*y_in), D3(*y_max), stream_ptr())
+    for a, b in zip(single, fused):
+        assert torch.equal(a, b)
+    nu = d(0.05 * rng.random(sh))
+    lo, hi = (0.0, 0.0, 1.0), (1.0, 0.232, 0.0)
+    single = [torch.empty_like(Ys[0]) for _ in range(3)]
+    for q in range(3):
+        _lib.call("fw_turbulent_diffusion_pass", g.ref, ptr(Ys[q]), ptr(single[q]), ptr(nu),
+                  0.7, 2e-4, lo[q], hi[q], stream_ptr())
+    fused = [torch.empty_like(Ys[0]) for _ in range(3)]
+    _lib.call("fw_turbulent_diffusion_pass3", g.ref, VP3(*[ptr(t) for t in Ys]),
+              VP3(*[ptr(t) for t in fused]), ptr(nu), 0.7, 2e-4, D3(*lo), D3(*hi),
+              stream_ptr())
+    for a, b in zip(single, fused):
+        assert torch.equal(a, b)
+    Tg = d(250.0 + 1200.0 * rng.random(sh))
+    alpha = np.zeros(sh)
+    alpha[:nzb] = 0.003 * rng.random((nzb, ny, nx))
+    alpha = d(alpha)
+    Ts = d(300.0 + 900.0 * rng.random(sh))
+    Yw, Qgc, qrg = d(0.3 * rng.random(sh)), d(1e4 * rng.normal(size=sh)), \
+        d(1e3 * rng.normal(size=sh))
+    Tg1, Qc1, Ts1 = Tg.clone(), torch.empty_like(Tg), torch.empty_like(Tg)
+    _lib.call("fw_gas_energy_update", g.ref, ptr(Tg1), ptr(Ts), ptr(rho), ptr(Yw), ptr(Sp),
+              ptr(Sd), ptr(Qgc), ptr(qrg), ptr(Qc1), nzb, 2, 7, 6.0e4, 1e-4, 300.0,
+              stream_ptr())
+    _lib.call("fw_solid_conduction_vertical", g.ref, ptr(Ts), ptr(Ts1), ptr(alpha), 0.2,
+              500.0, 1300.0, 1e-4, 300.0, 1.0e4, stream_ptr())
+    Tg2, Qc2, Ts2 = Tg.clone(), torch.empty_like(Tg), torch.empty_like(Tg)
+    _lib.call("fw_gas_update_solid_conduction", g.ref, ptr(Tg2), ptr(Ts), ptr(Ts2),
+              ptr(alpha), ptr(rho), ptr(Yw), ptr(Sp), ptr(Sd), ptr(Qgc), ptr(qrg), ptr(Qc2),
+              nzb, 2, 7, 6.0e4, 1e-4, 300.0, 0.2, 500.0, 1300.0, 300.0, 1.0e4, stream_ptr())
+    assert torch.equal(Tg1, Tg2) and torch.equal(Qc1, Qc2) and torch.equal(Ts1, Ts2)
