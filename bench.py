@@ -17,6 +17,16 @@ HBM.  The JSON line also carries:
   itself records (hipExtLaunchKernel, on the stream the kernel is launched
   on: torch's current stream), against 8 TB/s; traffic is the HBM bytes per
   launch from the FETCH_SIZE and WRITE_SIZE counters;
+* step_roofline: the whole step against the HBM roofline, with the
+  phase-fused compulsory traffic per cell-step (each phase's distinct
+  N-sized fields read once and written once, particles 72 B each);
+* pcg_time_to_1e8: the second half of the metric.  The projection systems
+  of the next few steps (b = (div - target)/dt and rho, as the step hands
+  them to its Krylov solver) are solved again from x0 = 0 to
+  ||b - A x||_2 / ||b||_2 <= 1e-8 (float64 residual, float32 inner PCG with
+  iterative refinement): plain PCG (the separable preconditioner) and
+  Jacobi-PCG, iterations and wall time per system.  No learned
+  preconditioner exists in this build, so that row is null;
 * cpu_baseline: the oracle's float64 NumPy step on one host core, timed on a
   bounded sample of the same workload (rank 0, N = 1 only).
 """
@@ -70,6 +80,71 @@ ROOFLINE_KERNEL = ("k_species3", 12)
 # FETCH_SIZE + WRITE_SIZE counters scaled by a 256 MiB copy:
 # profiles/step_256x64x48_s8_hbm_traffic.md
 ROOFLINE_TRAFFIC = {("spread3d_256x64x48_s8", "float32"): 43337925}
+
+
+# Phase-fused compulsory traffic of one outer step in field-passes (each
+# phase's distinct N-sized fields read once plus each field written once),
+# production configuration: 10 chemistry sub-steps of EDC 11, species x3 12,
+# turbulent diffusion x3 7, gas update + solid conduction 11, gas
+# turbulent diffusion 3; projection 8 + 26 per Krylov iteration + 12; DOM
+# (9 + 5 M per source iteration + 6) once every dom_every steps.
+STEP_PHASES = {"dt reduction": 4, "EoS": 2, "bed grid I/O": 8, "horizontal conduction": 3,
+               "drag + momentum": 9, "k-epsilon": 11, "level set": 6,
+               "gas-energy advection": 5, "diagnostics + EoS": 9}
+SUBSTEP_PASSES = 11 + 12 + 7 + 11 + 3
+PARTICLE_BYTES = 72
+I_CG, I_DOM = 2, 2     # Krylov and DOM iterations the table is frozen at
+
+
+def step_bytes_per_cell(esz, n_sub, M, dom_every, particles, cells):
+    """Algorithmic HBM bytes per cell-step of the phase-fused definition."""
+    passes = (sum(STEP_PHASES.values()) + (8 + 26 * I_CG + 12)
+              + (9 + 5 * M * I_DOM + 6) / dom_every + SUBSTEP_PASSES * n_sub)
+    return passes * esz + PARTICLE_BYTES * particles / cells
+
+
+PCG_RTOL = 1.0e-8
+
+
+def pcg_time_to_rtol(sv, n_rhs, rtol):
+    """Plain PCG vs Jacobi-PCG to rtol on the projection systems of the next
+    n_rhs steps, solved from x0 = 0 with a handle of their own."""
+    import ctypes as C
+    import torch
+    from firewebapp_amd import _lib
+    from firewebapp_amd.physics_3d.fft_poisson_3d import create_handle
+    stream = torch.cuda.current_stream().cuda_stream
+    seq = []
+    for _ in range(n_rhs):
+        sv.step()
+        b = torch.empty(sv.shape, dtype=torch.float64, device="cuda")
+        _lib.call("fw_proj_rhs", sv.proj, C.c_void_p(b.data_ptr()), C.c_void_p(stream))
+        seq.append((sv.f["rho"].clone(), b))
+    h = create_handle(sv.dg, 1.0e-6, rtol, maxiter=50000)
+    out = {"rhs": n_rhs, "x0": "zero", "rtol": rtol, "learned": None}
+    try:
+        for name, mode in (("plain", 1), ("jacobi", 2)):
+            _lib.call("fw_proj_set_preconditioner", h, mode)
+            its, ms, rel = [], [], []
+            for rho, b in seq:
+                x = torch.zeros_like(b)
+                it, rr = C.c_int(), C.c_double()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                _lib.call("fw_proj_solve", h, C.c_void_p(rho.data_ptr()),
+                          C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), rtol,
+                          C.byref(it), C.byref(rr), C.c_void_p(stream))
+                torch.cuda.synchronize()
+                ms.append(1e3 * (time.perf_counter() - t0))
+                its.append(it.value)
+                rel.append(rr.value)
+            out[name] = {"iters_mean": float(np.mean(its)), "ms_mean": float(np.mean(ms)),
+                         "iters": its, "max_rel_resid": float(max(rel))}
+            print(f"# {name} PCG to {rtol:g}: {np.mean(its):.1f} iterations, "
+                  f"{np.mean(ms):.2f} ms per system", file=sys.stderr, flush=True)
+    finally:
+        _lib.lib().fw_proj_destroy(h)
+    return out
 
 
 def deck(name, dtype):
@@ -131,6 +206,8 @@ def main(argv=None):
     ap.add_argument("--dtype", default="float32", choices=("float32", "float64"))
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of oracle CPU work for cpu_baseline (0 = skip)")
+    ap.add_argument("--pcg-rhs", type=int, default=4,
+                    help="projection systems for pcg_time_to_1e8 (0 = skip)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -191,6 +268,15 @@ def main(argv=None):
                      "mean_launch_ms": ms, "launches": launches},
         "cpu_baseline": None,
     }
+    bpc = step_bytes_per_cell(esz, 10, sv.dom.M,
+                              cfg.dom_subcycle_every, sv.n_particles, cells)
+    ach = bpc * line["value"] / 1e9
+    line["step_roofline"] = {
+        "bound": "hbm", "bytes_per_cell_step": bpc, "achieved": ach, "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+        "definition": f"phase-fused compulsory traffic at I_cg={I_CG}, I_dom={I_DOM}"}
+    if a.pcg_rhs > 0:
+        line["pcg_time_to_1e8"] = pcg_time_to_rtol(sv, a.pcg_rhs, PCG_RTOL)
     del sv
     torch.cuda.empty_cache()
     if a.cpu_budget > 0:

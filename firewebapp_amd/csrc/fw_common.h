@@ -167,6 +167,14 @@ __device__ __forceinline__ T recip(T y) {
   else return T(1) / y;
 }
 
+// x / y: float32 as x * v_rcp_f32(y) (within 2 ulp), float64 the IEEE
+// quotient (the reference's rounding)
+template <typename T>
+__device__ __forceinline__ T fdiv(T x, T y) {
+  if constexpr (sizeof(T) == 4) return x * __builtin_amdgcn_rcpf(y);
+  else return x / y;
+}
+
 // x / y.  The float32 build multiplies by ry = 1 / y, formed once by the
 // caller and shared by several quotients (one rounding more, inside the
 // float32 tolerance); float64 keeps the reference's division bit for bit.

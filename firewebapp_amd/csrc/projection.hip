@@ -244,6 +244,34 @@ k_irho(size_t n, const TR* __restrict__ rho, T* __restrict__ ir) {
   if (c < n) ir[c] = inv_rho_at<T>(rho, c);
 }
 
+// z = r / diag(A) (Jacobi), diag from the same 1/rho field as the operator
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_jacobi(GridArgs g, const T* __restrict__ ir, const T* __restrict__ r, T* __restrict__ z,
+         T eps_reg) {
+  int i, j, k;
+  cell_ijk(i, j, k);
+  if (i >= g.nx) return;
+  const Nbr n = nbr_offsets<T>(g, i, j, k);
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  const T irc = at(ir, n.c);
+  const T idx2 = T(g.inv_dx2), idy2 = T(g.inv_dy2);
+  T diag = eps_reg;
+  if (Nx > 1) {
+    diag -= i < Nx - 1 ? T(0.5) * (irc + at(ir, n.xp1)) * idx2 : T(2) * irc * idx2;
+    if (i > 0) diag -= T(0.5) * (at(ir, n.xm1) + irc) * idx2;
+  }
+  if (Ny > 1)
+    diag -= T(0.5) * (irc + at(ir, n.yp1)) * idy2 + T(0.5) * (at(ir, n.ym1) + irc) * idy2;
+  if (Nz > 1) {
+    const T idz = T(g.inv_dz[k]);
+    diag -= k < Nz - 1 ? T(0.5) * (irc + at(ir, n.zp1)) * (idz * T(g.inv_da[k]))
+                       : T(2) * irc * idz * idz;
+    if (k > 0) diag -= T(0.5) * (at(ir, n.zm1) + irc) * (idz * T(g.inv_db[k]));
+  }
+  at(z, n.c) = at(r, n.c) / diag;
+}
+
 // The inner PCG iteration.  Each reduction leaves PJ_RED partials; the next
 // kernel that needs the total sums them itself (sum_partials, the same fixed
 // order in every workgroup), so no final-sum launch and no cross-workgroup
@@ -286,12 +314,12 @@ k_matvec_rows(GridArgs g, const T* __restrict__ ir, const T* __restrict__ p,
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_update_xr2(size_t n, T* __restrict__ d, T* __restrict__ r, const T* __restrict__ p,
-             const T* __restrict__ q, const double* __restrict__ Prz,
+             const T* __restrict__ q, const double* __restrict__ Prz, int npz,
              const double* __restrict__ Ppq, double* __restrict__ S,
              double* __restrict__ Prr) {
   __shared__ double sh[4];
   if (S[S_DONE] != 0.0) return;
-  const double rz = sum_partials(Prz, PJ_RED, sh);
+  const double rz = sum_partials(Prz, npz, sh);
   __syncthreads();
   const double pq = sum_partials(Ppq, PJ_RED, sh);
   const double alpha_d = rz / pq;
@@ -352,12 +380,12 @@ k_dot_rows(GridArgs g, const T* __restrict__ a, const T* __restrict__ b,
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_p_update2(size_t n, T* __restrict__ p, const T* __restrict__ z,
-            const double* __restrict__ Prz, double* __restrict__ S, int first) {
+            const double* __restrict__ Prz, int npz, double* __restrict__ S, int first) {
   __shared__ double sh[4];
   if (S[S_DONE] != 0.0) return;
   T beta = T(0);
   if (!first) {
-    const double rz_new = sum_partials(Prz, PJ_RED, sh);
+    const double rz_new = sum_partials(Prz, npz, sh);
     const double bd = rz_new / S[S_RZ];
     if (blockIdx.x == 0 && threadIdx.x == 0) S[S_BETA] = bd;
     beta = (T)bd;
@@ -666,6 +694,70 @@ k_gemm(GemmArgs a, const T* __restrict__ A, const T* __restrict__ B,
   }
 }
 
+// Column transform of the separable preconditioner along y or z on the
+// matrix cores (float32): Out (K x ncol per slab) = Mat (K x KP, zero-padded
+// past K) * X (K rows of ncol values at row stride sq).  One wavefront per
+// 16 x 16 output tile with v_mfma_f32_16x16x4f32 over KP/4 steps (small
+// tiles: many wavefronts in flight to cover the load latency):
+//   A (16 x 4): lane l holds Mat[m0 + l%16][k0 + l/16],
+//   B (4 x 16): lane l holds X[k0 + l/16][n0 + l%16],
+//   D (16 x 16): register r of lane l holds row 4(l/16) + r, column l%16.
+// Every operand is loaded before the first MFMA.  spectral: out[m] /=
+// lz[m] + ly[j] + lx[i] - eps for the column c = j Nx + i.
+struct ColArgs {
+  int K, ncol, nx;
+  long sq, souter;
+  const double *lz, *ly, *lx;
+  double eps;
+};
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <int KP>
+__global__ void __launch_bounds__(256)
+k_coltrans_mfma(ColArgs a, const float* __restrict__ Mat, const float* __restrict__ in,
+                float* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n0 = (blockIdx.x * 4 + wv) * 16;
+  if (n0 >= a.ncol) return;   // uniform per wavefront; no barriers below
+  const int m0 = blockIdx.y * 16;
+  const float* inb = in + (size_t)blockIdx.z * a.souter;
+  float* outb = out + (size_t)blockIdx.z * a.souter;
+  const int col = n0 + (lane & 15);
+  const int colc = col < a.ncol ? col : a.ncol - 1;
+  const uint32_t lo = (uint32_t)colc * 4u;
+  const int kq = lane >> 4;
+  const int mrow = m0 + (lane & 15);
+  float av[KP / 4], bv[KP / 4];
+#pragma unroll
+  for (int st = 0; st < KP / 4; ++st) {
+    const int k = 4 * st + kq;
+    // rows k >= K re-read row K-1; Mat's zero padding cancels them
+    bv[st] = at(inb + (size_t)(k < a.K ? k : a.K - 1) * a.sq, lo);
+    av[st] = mrow < a.K ? Mat[(size_t)mrow * KP + k] : 0.0f;
+  }
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int st = 0; st < KP / 4; ++st)
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[st], bv[st], acc, 0, 0, 0);
+  double sd = 0.0;
+  if (a.lz) {
+    const int j = colc / a.nx, i = colc - j * a.nx;
+    sd = a.ly[j] + a.lx[i] - a.eps;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = m0 + 4 * kq + r;
+    if (m < a.K && col < a.ncol) {
+      float v = acc[r];
+      if (a.lz) v = v * recip((float)(a.lz[m] + sd));
+      at(outb + (size_t)m * a.sq, lo) = v;
+    }
+  }
+}
+
+static int col_pad(int K) { return K <= 16 ? 16 : K <= 32 ? 32 : K <= 64 ? 64 : 128; }
+
 // Orthonormal DCT-IV of each row (the x eigenbasis of the Neumann-left /
 // Dirichlet-right operator) through an M-point complex FFT, N = 2M, M a
 // power of two:
@@ -676,11 +768,30 @@ k_gemm(GemmArgs a, const T* __restrict__ A, const T* __restrict__ B,
 // in LDS.  tw: pre-twiddles (M complex), post-twiddles (M complex), FFT
 // twiddles e^{-2 pi i k/M} (M/2 complex), re/im interleaved.  O(N log N)
 // instead of the dense N x N product.
+// Work fused into the row transform inside the PCG iteration.
+//   DCT_UPDATE (forward): alpha = sum Prz / sum Ppq (workgroup 0 stores it);
+//     d += alpha p, r -= alpha q on the rows, then the transform of the new r;
+//     partial ||r||^2 -> Pout.  Returns at once when S_DONE is set.
+//   DCT_DOT (inverse): with check, workgroup 0 first applies the convergence
+//     test to sum Prr (stores ||r||^2, counts the iteration, raises S_DONE);
+//     then partial <r, out>_W of the rows -> Pout.
+enum { DCT_PLAIN = 0, DCT_UPDATE = 1, DCT_DOT = 2 };
 template <typename T>
+struct DctFuse {
+  T *d, *r;
+  const T *p, *q;
+  const double *Prz, *Ppq, *Prr, *dz;
+  int npz, npq, nrr, check, ny;
+  double* Pout;
+  double* S;
+};
+
+template <typename T, int MODE>
 __global__ void __launch_bounds__(256)
 k_dct4_rows(int nrows, int N, int logM, const T* __restrict__ tw,
-            const T* __restrict__ in, T* __restrict__ out) {
+            const T* __restrict__ in, T* __restrict__ out, DctFuse<T> f) {
   extern __shared__ unsigned char smem_raw[];
+  __shared__ double shr[4];
   const int M = N >> 1;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int row = blockIdx.x * 4 + wv;
@@ -689,10 +800,53 @@ k_dct4_rows(int nrows, int N, int logM, const T* __restrict__ tw,
   const T* pre = tw;
   const T* post = tw + 2 * M;
   const T* W = tw + 4 * M;
+  double acc = 0.0;
+  T alpha = T(0);
+  if constexpr (MODE == DCT_UPDATE) {
+    if (f.S[S_DONE] != 0.0) return;   // uniform: nothing moves once converged
+    const double rz = sum_partials(f.Prz, f.npz, shr);
+    __syncthreads();
+    const double pq = sum_partials(f.Ppq, f.npq, shr);
+    __syncthreads();
+    const double ad = rz / pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      f.S[S_RZ] = rz;
+      f.S[S_PQ] = pq;
+      f.S[S_ALPHA] = ad;
+    }
+    alpha = (T)ad;
+  }
+  if constexpr (MODE == DCT_DOT) {
+    if (f.check && blockIdx.x == 0) {
+      const double rr = sum_partials(f.Prr, f.nrr, shr);
+      if (threadIdx.x == 0 && f.S[S_DONE] == 0.0) {
+        f.S[S_RR] = rr;
+        f.S[S_IT] += 1.0;
+        if (!(rr == rr)) f.S[S_DONE] = 2.0;
+        else if (sqrt(rr) <= f.S[S_TOL]) f.S[S_DONE] = 1.0;
+      }
+      __syncthreads();
+    }
+  }
   if (act) {
-    const T* x = in + (size_t)row * N;
+    const size_t rb = (size_t)row * N;
+    const T* x = in + rb;
     for (int n = lane; n < M; n += 64) {
-      const T re = x[2 * n], im = x[N - 1 - 2 * n];
+      T re, im;
+      if constexpr (MODE == DCT_UPDATE) {
+        // each lane updates exactly the two entries it transforms
+        const size_t e0 = rb + 2 * n, e1 = rb + N - 1 - 2 * n;
+        f.d[e0] += alpha * f.p[e0];
+        f.d[e1] += alpha * f.p[e1];
+        re = f.r[e0] - alpha * f.q[e0];
+        im = f.r[e1] - alpha * f.q[e1];
+        f.r[e0] = re;
+        f.r[e1] = im;
+        acc += (double)re * (double)re + (double)im * (double)im;
+      } else {
+        re = x[2 * n];
+        im = x[N - 1 - 2 * n];
+      }
       const T pr = pre[2 * n], pi = pre[2 * n + 1];
       const int r = (int)(__brev((unsigned)n) >> (32 - logM));
       a[2 * r] = re * pr - im * pi;
@@ -721,14 +875,26 @@ k_dct4_rows(int nrows, int N, int logM, const T* __restrict__ tw,
     __syncthreads();
   }
   if (act) {
-    T* y = out + (size_t)row * N;
+    const size_t rb = (size_t)row * N;
+    T* y = out + rb;
     const T sc = T(sqrt(2.0 / N));
+    const double dzk = MODE == DCT_DOT ? f.dz[row / f.ny] : 0.0;
     for (int kk = lane; kk < M; kk += 64) {
       const T vr = a[2 * kk], vi = a[2 * kk + 1];
       const T pr = post[2 * kk], pi = post[2 * kk + 1];
-      y[2 * kk] = (vr * pr - vi * pi) * sc;
-      y[N - 1 - 2 * kk] = -((vr * pi + vi * pr) * sc);
+      const T y0 = (vr * pr - vi * pi) * sc;
+      const T y1 = -((vr * pi + vi * pr) * sc);
+      y[2 * kk] = y0;
+      y[N - 1 - 2 * kk] = y1;
+      if constexpr (MODE == DCT_DOT)
+        acc += dzk * ((double)f.r[rb + 2 * kk] * (double)y0 +
+                      (double)f.r[rb + N - 1 - 2 * kk] * (double)y1);
     }
+  }
+  if constexpr (MODE != DCT_PLAIN) {
+    __syncthreads();
+    const double t = block_sum_wg(acc, shr);
+    if (threadIdx.x == 0) f.Pout[blockIdx.x] = t;
   }
 }
 
@@ -746,6 +912,9 @@ struct ProjHandle {
   // preconditioner (type T): Ux (Nx*Nx), Yf (Ny*Ny forward = basis^T),
   // Yi (Ny*Ny inverse = basis), Zf (Nz*Nz, D folded), Zi (Nz*Nz, D^-1 folded)
   void *Ux, *UxT, *Yf, *Yi, *Zf, *Zi;
+  // float32: the y and z matrices zero-padded to K x col_pad(K) for
+  // k_coltrans_mfma (null for float64 or K > 128: k_gemm then)
+  void *Yf_p, *Yi_p, *Zf_p, *Zi_p;
   // x transform through k_dct4_rows when Ux is the orthonormal DCT-IV basis
   // and Nx is a power of two (twx: its twiddle table, type T)
   bool fft_x;
@@ -758,7 +927,7 @@ struct ProjHandle {
   double *x64, *b64, *r64;
   void *r, *z, *p, *q, *d, *w1, *w2;
   double *part, *S, *S_host;
-  double *Ppq, *Prr, *Prz;  // PJ_RED partials of the PCG iteration's reductions
+  double *Ppq, *Prr, *Prz;  // partials of the PCG iteration's reductions
   void* irho;          // 1 / max(rho, 0.01) of the running solve (type T)
   bool have_prev;
   int last_iters, last_outer;
@@ -789,8 +958,8 @@ static void x_transform(ProjHandle* h, const T* in, T* out, bool inverse, hipStr
   const int Nx = h->g.nx, rows = h->g.nz * h->g.ny;
   if (h->fft_x) {
     const size_t shm = 4 * (size_t)Nx * sizeof(T);
-    hipLaunchKernelGGL(k_dct4_rows<T>, dim3((rows + 3) / 4), dim3(256), shm, s, rows, Nx,
-                       h->logMx, (const T*)h->twx, in, out);
+    hipLaunchKernelGGL((k_dct4_rows<T, DCT_PLAIN>), dim3((rows + 3) / 4), dim3(256), shm, s,
+                       rows, Nx, h->logMx, (const T*)h->twx, in, out, DctFuse<T>{});
   } else if (!inverse) {
     launch_gemm<T>(h, in, (const T*)h->Ux, out, rows, Nx, Nx, 1, 0, Nx, 1, 0, Nx, 1, 0, Nx,
                    1, false, s);
@@ -800,44 +969,97 @@ static void x_transform(ProjHandle* h, const T* in, T* out, bool inverse, hipStr
   }
 }
 
+// The float32 column transform (k_coltrans_mfma); KP = col_pad(K).
+static void launch_col(ProjHandle* h, const void* Mat, int K, long sq, int ncol, int nouter,
+                       long souter, bool spectral, const float* in, float* out,
+                       hipStream_t s) {
+  ColArgs a;
+  a.K = K; a.ncol = ncol; a.nx = h->g.nx; a.sq = sq; a.souter = souter;
+  a.lz = spectral ? h->lz : nullptr; a.ly = h->ly; a.lx = h->lx; a.eps = h->eps_reg;
+  const dim3 grid((ncol + 63) / 64, (K + 15) / 16, nouter);
+  const float* M = (const float*)Mat;
+  switch (col_pad(K)) {
+    case 16: hipLaunchKernelGGL(k_coltrans_mfma<16>, grid, dim3(256), 0, s, a, M, in, out); break;
+    case 32: hipLaunchKernelGGL(k_coltrans_mfma<32>, grid, dim3(256), 0, s, a, M, in, out); break;
+    case 64: hipLaunchKernelGGL(k_coltrans_mfma<64>, grid, dim3(256), 0, s, a, M, in, out); break;
+    default: hipLaunchKernelGGL(k_coltrans_mfma<128>, grid, dim3(256), 0, s, a, M, in, out); break;
+  }
+}
+
+// The y and z parts of the separable solve on the x-transformed field in w1
+// (w2 is scratch): y forward, z forward with the spectral divide, z
+// inverse, y inverse.  Returns the buffer (w1 or w2) holding the result.
+template <typename T>
+static const T* yz_transforms(ProjHandle* h, T* w1, T* w2, hipStream_t s) {
+  const int Nx = h->g.nx, Ny = h->g.ny, Nz = h->g.nz;
+  const long NP = (long)Nx * Ny;
+  const bool mf = sizeof(T) == 4;
+  T* cur = w1;
+  T* other = w2;
+  if (Ny > 1) {  // y forward per k-plane: w2_k = Yf (Ny x Ny) * w1_k (Ny x Nx)
+    if (mf && h->Yf_p)
+      launch_col(h, h->Yf_p, Ny, Nx, Nx, Nz, NP, false, (const float*)cur, (float*)other, s);
+    else
+      launch_gemm<T>(h, (const T*)h->Yf, cur, other, Ny, Nx, Ny, Nz, 0, Ny, 1, NP, Nx,
+                     1, NP, Nx, 1, false, s);
+    T* t = cur; cur = other; other = t;
+  }
+  // z forward with sqrt(dz) folded in, spectral divide in the epilogue
+  if (mf && h->Zf_p)
+    launch_col(h, h->Zf_p, Nz, NP, (int)NP, 1, 0, true, (const float*)cur, (float*)other, s);
+  else
+    launch_gemm<T>(h, (const T*)h->Zf, cur, other, Nz, (int)NP, Nz, 1, 0, Nz, 1, 0,
+                   NP, 1, 0, NP, 1, true, s);
+  { T* t = cur; cur = other; other = t; }
+  // z inverse with 1/sqrt(dz) folded in
+  if (mf && h->Zi_p)
+    launch_col(h, h->Zi_p, Nz, NP, (int)NP, 1, 0, false, (const float*)cur, (float*)other, s);
+  else
+    launch_gemm<T>(h, (const T*)h->Zi, cur, other, Nz, (int)NP, Nz, 1, 0, Nz, 1, 0,
+                   NP, 1, 0, NP, 1, false, s);
+  { T* t = cur; cur = other; other = t; }
+  if (Ny > 1) {
+    if (mf && h->Yi_p)
+      launch_col(h, h->Yi_p, Ny, Nx, Nx, Nz, NP, false, (const float*)cur, (float*)other, s);
+    else
+      launch_gemm<T>(h, (const T*)h->Yi, cur, other, Ny, Nx, Ny, Nz, 0, Ny, 1, NP, Nx,
+                     1, NP, Nx, 1, false, s);
+    T* t = cur; cur = other; other = t;
+  }
+  return cur;
+}
+
 // z = M^{-1} r  (separable solve of fft_poisson_3d.py:184-228)
 template <typename T>
 static int precond_apply(ProjHandle* h, const T* r, T* z, hipStream_t s) {
-  const int Nx = h->g.nx, Ny = h->g.ny, Nz = h->g.nz;
-  const long NP = (long)Nx * Ny;
   T* w1 = (T*)h->w1;
   T* w2 = (T*)h->w2;
   // x forward: w1[r][i'] = sum_i r[r][i] Ux[i][i']
   x_transform<T>(h, r, w1, false, s);
-  const T* cur = w1;
-  T* other = w2;
-  if (Ny > 1) {  // y forward per k-plane: w2_k = Yf (Ny x Ny) * w1_k (Ny x Nx)
-    launch_gemm<T>(h, (const T*)h->Yf, w1, w2, Ny, Nx, Ny, Nz, 0, Ny, 1, NP, Nx,
-                   1, NP, Nx, 1, false, s);
-    cur = w2;
-    other = w1;
-  }
-  // z forward with sqrt(dz) folded in, spectral divide in the epilogue
-  launch_gemm<T>(h, (const T*)h->Zf, cur, other, Nz, (int)NP, Nz, 1, 0, Nz, 1, 0,
-                 NP, 1, 0, NP, 1, true, s);
-  // z inverse with 1/sqrt(dz) folded in
-  T* t2 = (other == w1) ? w2 : w1;
-  launch_gemm<T>(h, (const T*)h->Zi, other, t2, Nz, (int)NP, Nz, 1, 0, Nz, 1, 0,
-                 NP, 1, 0, NP, 1, false, s);
-  const T* c2 = t2;
-  if (Ny > 1) {
-    T* t3 = (t2 == w1) ? w2 : w1;
-    launch_gemm<T>(h, (const T*)h->Yi, t2, t3, Ny, Nx, Ny, Nz, 0, Ny, 1, NP, Nx,
-                   1, NP, Nx, 1, false, s);
-    c2 = t3;
-  }
+  const T* c2 = yz_transforms<T>(h, w1, w2, s);
   // x inverse: z[r][i] = sum_i' c2[r][i'] Ux[i][i']  (B = Ux^T)
   x_transform<T>(h, c2, z, true, s);
-  FW_CHECK_LAUNCH("precond gemm");
+  FW_CHECK_LAUNCH("precond");
   return FW_OK;
 }
 
 static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+// z = M^-1 r for the handle's preconditioner: the separable inverse, Jacobi
+// (from the solve's 1/rho field) or none.
+template <typename T>
+static int apply_M(ProjHandle* h, const T* r, T* z, hipStream_t s) {
+  if (h->precond_enabled == FW_PRECOND_SEPARABLE) return precond_apply<T>(h, r, z, s);
+  if (h->precond_enabled == FW_PRECOND_JACOBI) {
+    const Launch L = cell_launch(&h->g);
+    hipLaunchKernelGGL(k_jacobi<T>, L.grid, L.block, 0, s, h->ga, (const T*)h->irho, r, z,
+                       (T)h->eps_reg);
+    FW_CHECK_LAUNCH("jacobi");
+    return FW_OK;
+  }
+  FW_CHECK_HIP(hipMemcpyAsync(z, r, h->n * sizeof(T), hipMemcpyDeviceToDevice, s));
+  return FW_OK;
+}
 
 static int read_scalars(ProjHandle* h, hipStream_t s) {
   FW_CHECK_HIP(hipMemcpyAsync(h->S_host, h->S, S_NS * sizeof(double),
@@ -860,16 +1082,43 @@ static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n)), dim3(256), 0, s, n, d);
   hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, h->S, tol_abs);
   int rc;
-  if (h->precond_enabled) {
-    if ((rc = precond_apply<T>(h, r, z, s))) return rc;
-  } else {
-    FW_CHECK_HIP(hipMemcpyAsync(z, r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-  }
   (void)rho;   // the iterations use h->irho, formed from rho by solve_refined
   const T* ir = (const T*)h->irho;
-  hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z, h->Prz,
-                     h->Prr, h->S, 0);
-  hipLaunchKernelGGL(k_p_update2<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Prz, h->S, 1);
+  // separable preconditioner with the DCT-IV x transform: the update of d, r
+  // rides in the forward row transform and <r, z> in the inverse one
+  const bool fused = h->fft_x && h->precond_enabled == FW_PRECOND_SEPARABLE;
+  const int rows = h->g.nz * h->g.ny, ndct = (rows + 3) / 4;
+  const size_t shm = 4 * (size_t)h->g.nx * sizeof(T);
+  const int npz = fused ? ndct : PJ_RED;
+  DctFuse<T> fu{};
+  fu.d = d; fu.r = r; fu.p = p; fu.q = q;
+  fu.Prz = h->Prz; fu.Ppq = h->Ppq; fu.Prr = h->Prr; fu.dz = h->g.dz;
+  fu.npz = ndct; fu.npq = PJ_RED; fu.nrr = ndct; fu.ny = h->g.ny; fu.S = h->S;
+  // z = M^-1 r; with fused, the inverse row transform also leaves <r, z>_W
+  auto precond_dot = [&](int check) -> int {
+    T* w1 = (T*)h->w1;
+    T* w2 = (T*)h->w2;
+    const T* c2 = yz_transforms<T>(h, w1, w2, s);
+    DctFuse<T> fd = fu;
+    fd.check = check;
+    fd.Pout = h->Prz;
+    hipLaunchKernelGGL((k_dct4_rows<T, DCT_DOT>), dim3(ndct), dim3(256), shm, s, rows,
+                       h->g.nx, h->logMx, (const T*)h->twx, c2, z, fd);
+    FW_CHECK_LAUNCH("precond row transform");
+    return FW_OK;
+  };
+  if (fused) {
+    DctFuse<T> f0{};
+    hipLaunchKernelGGL((k_dct4_rows<T, DCT_PLAIN>), dim3(ndct), dim3(256), shm, s, rows,
+                       h->g.nx, h->logMx, (const T*)h->twx, (const T*)r, (T*)h->w1, f0);
+    if ((rc = precond_dot(0))) return rc;
+  } else {
+    if ((rc = apply_M<T>(h, r, z, s))) return rc;
+    hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z, h->Prz,
+                       h->Prr, h->S, 0);
+  }
+  hipLaunchKernelGGL(k_p_update2<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Prz, npz,
+                     h->S, 1);
   FW_CHECK_LAUNCH("pcg setup");
   // iterations in batches with one host read per batch; the device flag stops
   // the Krylov updates at the first converged iteration, so the result is
@@ -885,17 +1134,22 @@ static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
     for (int b = 0; b < nb_it; ++b) {
       hipLaunchKernelGGL(k_matvec_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, ir, p, q,
                          (T)h->eps_reg, h->Ppq);
-      hipLaunchKernelGGL(k_update_xr2<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
-                         h->Prz, h->Ppq, h->S, h->Prr);
-      if (h->precond_enabled) {
-        if ((rc = precond_apply<T>(h, r, z, s))) return rc;
+      if (fused) {
+        DctFuse<T> fw = fu;
+        fw.Pout = h->Prr;
+        hipLaunchKernelGGL((k_dct4_rows<T, DCT_UPDATE>), dim3(ndct), dim3(256), shm, s,
+                           rows, h->g.nx, h->logMx, (const T*)h->twx, (const T*)r,
+                           (T*)h->w1, fw);
+        if ((rc = precond_dot(1))) return rc;
       } else {
-        FW_CHECK_HIP(hipMemcpyAsync(z, r, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_update_xr2<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
+                           h->Prz, npz, h->Ppq, h->S, h->Prr);
+        if ((rc = apply_M<T>(h, r, z, s))) return rc;
+        hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z,
+                           h->Prz, h->Prr, h->S, 1);
       }
-      hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z, h->Prz,
-                         h->Prr, h->S, 1);
       hipLaunchKernelGGL(k_p_update2<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Prz,
-                         h->S, 0);
+                         npz, h->S, 0);
     }
     queued += nb_it;
     FW_CHECK_LAUNCH("pcg iteration");
@@ -1039,7 +1293,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   h->rtol = cg_rtol;
   h->maxiter = maxiter > 0 ? maxiter : 400;
   h->esz = g->dtype == FW_F64 ? 8 : 4;
-  h->precond_enabled = 1;
+  h->precond_enabled = FW_PRECOND_SEPARABLE;
   const int Nx = g->nx, Ny = g->ny, Nz = g->nz;
   // host copies of the dz metric for folding sqrt(dz) into the z matrices
   std::vector<double> dz(Nz);
@@ -1110,6 +1364,22 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
       if (rc) return rc;
     }
   }
+  // zero-padded y and z matrices for the float32 column transforms
+  {
+    auto pad = [&](void** dst, const double* src, int K) -> int {
+      if (K > 128) return FW_OK;
+      const int KP = col_pad(K);
+      std::vector<double> m((size_t)K * KP, 0.0);
+      for (int a = 0; a < K; ++a)
+        for (int b = 0; b < K; ++b) m[(size_t)a * KP + b] = src[(size_t)a * K + b];
+      return g->dtype == FW_F64 ? upload_cast<double>(dst, m.data(), m.size())
+                                : upload_cast<float>(dst, m.data(), m.size());
+    };
+    if (g->dtype == FW_F32 &&
+        ((rc = pad(&h->Yf_p, Yf.data(), Ny)) || (rc = pad(&h->Yi_p, Uy, Ny)) ||
+         (rc = pad(&h->Zf_p, Zf.data(), Nz)) || (rc = pad(&h->Zi_p, Zi.data(), Nz))))
+      return rc;
+  }
   FW_CHECK_HIP(hipMalloc((void**)&h->lx, Nx * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->ly, Ny * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->lz, Nz * sizeof(double)));
@@ -1123,9 +1393,13 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   void** vecs[] = {&h->r, &h->z, &h->p, &h->q, &h->d, &h->w1, &h->w2};
   for (void** v : vecs) FW_CHECK_HIP(hipMalloc(v, n * h->esz));
   FW_CHECK_HIP(hipMalloc((void**)&h->part, PJ_RED * sizeof(double)));
-  FW_CHECK_HIP(hipMalloc((void**)&h->Ppq, 3 * PJ_RED * sizeof(double)));
-  h->Prr = h->Ppq + PJ_RED;
-  h->Prz = h->Ppq + 2 * PJ_RED;
+  {
+    const int ndct = (g->nz * g->ny + 3) / 4;   // row-transform workgroups
+    const size_t np = ndct > PJ_RED ? ndct : PJ_RED;
+    FW_CHECK_HIP(hipMalloc((void**)&h->Ppq, 3 * np * sizeof(double)));
+    h->Prr = h->Ppq + np;
+    h->Prz = h->Ppq + 2 * np;
+  }
   FW_CHECK_HIP(hipMalloc(&h->irho, n * h->esz));
   FW_CHECK_HIP(hipMalloc((void**)&h->S, S_NS * sizeof(double)));
   FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, S_NS * sizeof(double)));
@@ -1139,7 +1413,8 @@ int fw_proj_destroy(void* handle) {
   if (!h) return FW_OK;
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
                   h->x64, h->b64, h->r64, h->r, h->z, h->p, h->q, h->d, h->w1,
-                  h->w2, h->part, h->S, h->twx, h->Ppq, h->irho};
+                  h->w2, h->part, h->S, h->twx, h->Ppq, h->irho, h->Yf_p, h->Yi_p,
+                  h->Zf_p, h->Zi_p};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (h->S_host) (void)hipHostFree(h->S_host);
@@ -1182,6 +1457,14 @@ int fw_proj_set_pressure(void* handle, const double* p_in, void* stream) {
   FW_CHECK_HIP(hipMemcpyAsync(h->x64, p_in, h->n * 8, hipMemcpyDeviceToDevice,
                               (hipStream_t)stream));
   h->have_prev = true;
+  return FW_OK;
+}
+
+int fw_proj_rhs(void* handle, double* b_out, void* stream) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && b_out, "proj_rhs: null argument");
+  FW_CHECK_HIP(hipMemcpyAsync(b_out, h->b64, h->n * 8, hipMemcpyDeviceToDevice,
+                              (hipStream_t)stream));
   return FW_OK;
 }
 
@@ -1231,7 +1514,10 @@ int fw_proj_precondition(void* handle, const void* r, void* z, void* stream) {
 int fw_proj_set_preconditioner(void* handle, int enabled) {
   ProjHandle* h = (ProjHandle*)handle;
   FW_REQUIRE(h, "proj_set_preconditioner: null handle");
-  h->precond_enabled = enabled ? 1 : 0;
+  FW_REQUIRE(enabled == FW_PRECOND_NONE || enabled == FW_PRECOND_SEPARABLE ||
+                 enabled == FW_PRECOND_JACOBI,
+             "proj_set_preconditioner: mode must be 0 (none), 1 (separable) or 2 (Jacobi)");
+  h->precond_enabled = enabled;
   return FW_OK;
 }
 

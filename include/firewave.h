@@ -225,8 +225,17 @@ int fw_proj_matvec(void* handle, const void* rho, const void* x, void* y,
 /* z = SeparableLaplacian3D.solve(r) (fft_poisson_3d.py:184-228). */
 int fw_proj_precondition(void* handle, const void* r, void* z, void* stream);
 
-/* 1 = separable preconditioner (default), 0 = none (identity). */
-int fw_proj_set_preconditioner(void* handle, int enabled);
+/* Preconditioner of the PCG: FW_PRECOND_SEPARABLE (default; the reference's
+ * SeparableLaplacian3D, fft_poisson_3d.py:184-228), FW_PRECOND_NONE, or
+ * FW_PRECOND_JACOBI (diag(A)^-1; not in the reference, a comparison row for
+ * the time-to-residual measurement of the plain PCG). */
+enum { FW_PRECOND_NONE = 0, FW_PRECOND_SEPARABLE = 1, FW_PRECOND_JACOBI = 2 };
+int fw_proj_set_preconditioner(void* handle, int mode);
+
+/* Copy the right-hand side b = (div - target)/dt of the last fw_proj_project
+ * (float64, N values) out: the system the reference hands to its Krylov
+ * solver (projection_3d.py:742-791). */
+int fw_proj_rhs(void* handle, double* b_out, void* stream);
 
 /* Solve A x = b (float64 b, x; x is the warm start) to ||b-Ax||_2 <=
  * rtol ||b||_2 with the handle's precision policy; rho in the grid's type. */

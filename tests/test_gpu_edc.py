@@ -71,3 +71,29 @@ def test_edc_flag_carry_across_segments():
     got = _gpu_edc([rho, T.copy(), Yf.copy(), YO.copy(), k, e, Yw], c)
     for x, y in zip(got, want):
         assert relerr(x, y) <= 1e-12
+
+
+@pytest.mark.parametrize("fuel_cells", [5, 0])
+def test_edc_flag_carry_long_lookback(fuel_cells):
+    """Resetting cells only at the very start (or none): segments more than
+    256 segments later inherit their flag through several look-back rounds
+    (or keep the initial false flag)."""
+    rng = np.random.default_rng(77)
+    sh = (20, 16, 256)          # 320 segments of 256 cells
+    rho = 0.4 + 0.8 * rng.random(sh)
+    T = 350.0 + 1500.0 * rng.random(sh)
+    Yf = np.zeros(sh)
+    Yf.reshape(-1)[:fuel_cells] = 0.05
+    YO = 0.232 * rng.random(sh)
+    Yw = 0.2 * rng.random(sh)
+    Yw.reshape(-1)[fuel_cells - 1 if fuel_cells else 0] = 0.17   # quenching flag set
+    k = 0.8 * rng.random(sh)
+    e = 0.8 * rng.random(sh)
+    c = dict(chi_rad=0.34, cp_g=1100.0, dt=0.002, n_substeps=1,
+             extinction_enable=False)
+    want = [a.copy() for a in (T, Yf, YO)] + [np.zeros(sh)]
+    E.step_chemistry_ode_edc_sequential(rho, want[0], want[1], want[2], k, e,
+                                        0.34, 1100.0, 0.002, 1, want[3], Yw)
+    got = _gpu_edc([rho, T.copy(), Yf.copy(), YO.copy(), k, e, Yw], c)
+    for x, y in zip(got, want):
+        assert relerr(x, y) <= 1e-12

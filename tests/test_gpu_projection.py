@@ -166,10 +166,11 @@ def test_mixed_precision_solve_to_1e8(dtype):
 
 @pytest.mark.parametrize("shape", [(13, 6, 37), (48, 64, 256)])
 def test_separable_fp32_matrix_cores(shape):
-    """The float32 preconditioner runs its transforms on the matrix cores
-    (MFMA GEMM); on ragged and production shapes it must agree with the
-    float64 separable inverse to float32 accuracy, and invert the constant-
-    density operator."""
+    """The float32 preconditioner (x by DCT-IV FFT where the basis allows,
+    y and z on the matrix cores, v_mfma_f32_16x16x4f32) must agree with the
+    float64 separable inverse to float32 accuracy on ragged and production
+    shapes (K = 6/13 and 64/48 exercise the zero padding to 16 and 64), and
+    invert the constant-density operator."""
     fp, _ = _mods()
     Nz, Ny, Nx = shape
     dz, da, db = _grid_z(Nz)
@@ -181,3 +182,56 @@ def test_separable_fp32_matrix_cores(shape):
     e2 = relerr(s.solve(OP.matvec(coef, p64).astype(np.float32)), p64)
     print("fp32 separable: vs fp64", e1, "inverse", e2, flush=True)
     assert e1 <= 1e-4 and e2 <= 1e-4
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_rhs_accessor_and_preconditioner_modes(dtype):
+    """fw_proj_rhs returns the system of the last projection, (div - target)/dt
+    in float64; the same system solved to 1e-8 from zero with each
+    preconditioner mode reaches the residual, and the separable one needs
+    the fewest iterations (plain PCG vs Jacobi vs none, the comparison rows
+    of the time-to-1e-8 measurement)."""
+    import ctypes as C
+    import torch
+    from firewebapp_amd import _lib
+    from firewebapp_amd.device import ptr, stream_ptr
+    _, pj = _mods()
+    dz, da, db, rho, u, v, w, uin, tgt = _fire_case(seed=5)
+    sh = rho.shape
+    dt = 0.01
+    sol = pj.ProjectionSolver3D(*sh, 0.1, 0.1, dz, da, db, cg_rtol=1e-6)
+    sol.set_inlet_BC(uin)
+    to = (lambda a: torch.as_tensor(a.astype(dtype), device="cuda"))
+    U, V, W, R, TG = (to(a) for a in (u, v, w, rho, tgt))
+    sol.rebuild_for_rho(R)
+    sol.project(U, V, W, R, dt, div_target=TG)
+    h = sol._handle(torch.float64 if dtype == np.float64 else torch.float32)
+    B = torch.empty(sh, dtype=torch.float64, device="cuda")
+    _lib.call("fw_proj_rhs", h, ptr(B), stream_ptr())
+    c = lambda a: a.astype(dtype).astype(np.float64)
+    div = OP.divergence_compatible(c(u), c(v), c(w), 0.1, 0.1, dz, c(uin))
+    b_ref = (div - c(tgt)) / dt
+    assert relerr(B.cpu().numpy(), b_ref) <= 1e-12
+    coef = OP.coefficients(c(rho), 0.1, 0.1, dz, da, db)
+    b = B.cpu().numpy()
+    from firewebapp_amd.physics_3d.fft_poisson_3d import create_handle
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    h = create_handle(sol._grids[tdt], 1.0e-6, 1.0e-8, maxiter=20000)
+    iters = {}
+    try:
+        for mode in (1, 2, 0):
+            _lib.call("fw_proj_set_preconditioner", h, mode)
+            X = torch.zeros_like(B)
+            it, rel = C.c_int(), C.c_double()
+            _lib.call("fw_proj_solve", h, ptr(R), ptr(B), ptr(X), 1e-8, C.byref(it),
+                      C.byref(rel), stream_ptr())
+            true_rel = np.linalg.norm(b - OP.matvec(coef, X.cpu().numpy())) / np.linalg.norm(b)
+            assert true_rel <= 1.05e-8, (mode, true_rel, it.value)
+            iters[mode] = it.value
+        with pytest.raises(_lib.FirewaveError):
+            _lib.call("fw_proj_set_preconditioner", h, 3)
+    finally:
+        _lib.lib().fw_proj_destroy(h)
+    print("iterations to 1e-8 (separable, Jacobi, none):", iters[1], iters[2], iters[0],
+          flush=True)
+    assert iters[1] < iters[2] and iters[1] < iters[0]
