@@ -24,9 +24,10 @@ HBM.  The JSON line also carries:
   of the next few steps (b = (div - target)/dt and rho, as the step hands
   them to its Krylov solver) are solved again from x0 = 0 to
   ||b - A x||_2 / ||b||_2 <= 1e-8 (float64 residual, float32 inner PCG with
-  iterative refinement): plain PCG (the separable preconditioner) and
-  Jacobi-PCG, iterations and wall time per system.  No learned
-  preconditioner exists in this build, so that row is null;
+  iterative refinement): plain PCG (the separable preconditioner),
+  Jacobi-PCG and the learned preconditioner (flexible PCG with the trained
+  network's correction, handing a solve to plain PCG when a round stalls),
+  iterations and wall time per system;
 * cpu_baseline: the oracle's float64 NumPy step on one host core, timed on a
   bounded sample of the same workload (rank 0, N = 1 only).
 """
@@ -104,6 +105,9 @@ def step_bytes_per_cell(esz, n_sub, M, dom_every, particles, cells):
 
 
 PCG_RTOL = 1.0e-8
+# parameters of the learned preconditioner, trained on 128 x 32 x 32 runs by
+# python -m firewebapp_amd.learned --out firewebapp_amd/data/precond_net.npz
+NET_PARAMS = os.path.join(ROOT, "firewebapp_amd", "data", "precond_net.npz")
 
 
 def pcg_time_to_rtol(sv, n_rhs, rtol):
@@ -121,10 +125,22 @@ def pcg_time_to_rtol(sv, n_rhs, rtol):
         _lib.call("fw_proj_rhs", sv.proj, C.c_void_p(b.data_ptr()), C.c_void_p(stream))
         seq.append((sv.f["rho"].clone(), b))
     h = create_handle(sv.dg, 1.0e-6, rtol, maxiter=50000)
-    out = {"rhs": n_rhs, "x0": "zero", "rtol": rtol, "learned": None}
+    out = {"rhs": n_rhs, "x0": "zero", "rtol": rtol}
+    net = None
+    modes = [("plain", 1), ("jacobi", 2)]
+    if os.path.exists(NET_PARAMS):
+        from firewebapp_amd.learned import ConvNet, attach
+        nz, ny, nx = sv.shape
+        net = ConvNet(nx, ny, nz, theta=np.load(NET_PARAMS)["theta"])
+        modes.append(("learned", 3))
+    else:
+        out["learned"] = None
     try:
-        for name, mode in (("plain", 1), ("jacobi", 2)):
-            _lib.call("fw_proj_set_preconditioner", h, mode)
+        for name, mode in modes:
+            if mode == 3:
+                attach(h, net)
+            else:
+                _lib.call("fw_proj_set_preconditioner", h, mode)
             its, ms, rel = [], [], []
             for rho, b in seq:
                 x = torch.zeros_like(b)
@@ -140,10 +156,16 @@ def pcg_time_to_rtol(sv, n_rhs, rtol):
                 rel.append(rr.value)
             out[name] = {"iters_mean": float(np.mean(its)), "ms_mean": float(np.mean(ms)),
                          "iters": its, "max_rel_resid": float(max(rel))}
+            if mode == 3:
+                fb = C.c_int()
+                _lib.call("fw_proj_learned_stats", h, C.byref(fb))
+                out[name]["fallbacks_to_plain"] = fb.value
             print(f"# {name} PCG to {rtol:g}: {np.mean(its):.1f} iterations, "
                   f"{np.mean(ms):.2f} ms per system", file=sys.stderr, flush=True)
     finally:
         _lib.lib().fw_proj_destroy(h)
+        if net is not None:
+            net.close()
     return out
 
 

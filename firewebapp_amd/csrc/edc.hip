@@ -100,13 +100,27 @@ k_edc(size_t n, const T* __restrict__ rho, T* __restrict__ Tg,
   // last earlier segment with a resetting cell, looked up 256 at a time
   const bool need_carry = __syncthreads_or(prev < 0 && valid);
   if (need_carry) {
-    int found = -1;
-    for (int top = (int)blockIdx.x - 1; top >= 0 && found < 0; top -= 256) {
-      const int b = top - 255 + t;          // thread t, segment b: ascending
-      const bool has = b >= 0 && summary[b] >= 0;
-      const int last = block_last_true(has, wmax);   // the largest such b
+    // round 1: the 256 segments just before this one; round 2 (rare): all
+    // earlier segments at once, every thread keeping the largest index it
+    // saw, so the look-back costs at most two dependent load rounds
+    const int b0 = (int)blockIdx.x - 256 + t;   // thread t, segment b0: ascending
+    const bool has = b0 >= 0 && summary[b0] >= 0;
+    int last = block_last_true(has, wmax);
+    __syncthreads();
+    int found = last >= 0 ? (int)blockIdx.x - 256 + last : -1;
+    if (found < 0 && (int)blockIdx.x > 256) {
+      int best = -1;
+#pragma unroll 4
+      for (int b = t; b < (int)blockIdx.x - 256; b += 256)
+        if (summary[b] >= 0) best = b;
+      // block max of best: per-wavefront shuffle, then the four wavefronts
+      for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_down(best, off, 64);
+        best = o > best ? o : best;
+      }
+      if ((t & 63) == 0) wmax[t >> 6] = best;
       __syncthreads();
-      if (last >= 0) found = top - 255 + last;
+      for (int q = 0; q < 4; ++q) found = wmax[q] > found ? wmax[q] : found;
     }
     if (t == 0) s_carry = found >= 0 ? summary[found] : 0;
     __syncthreads();

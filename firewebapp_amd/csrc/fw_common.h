@@ -23,6 +23,12 @@ struct Launch {
   dim3 grid, block;
 };
 
+// The learned preconditioner's network (learned.hip), for projection.hip.
+int net_apply(void* net, const float* r, const float* q, const float** c, int* n,
+              hipStream_t s);
+int net_dims(void* net, int* nx, int* ny, int* nz);
+int density_feature(const float* rho, float* q, size_t n, hipStream_t s);
+
 // Events on which the next species launch records its own start and stop
 // (the dispatch's timestamps, as a kernel trace sees them), or null.  Set by
 // the step driver's optional timer around one call, cleared after it.

@@ -31,7 +31,7 @@ namespace fw {
 // residual; S_IT counts the iterations taken.  The iteration kernels leave
 // the Krylov state untouched once S_DONE is set, so iterations can be
 // enqueued in batches and the host reads S_DONE once per batch.
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_DONE, S_IT, S_TOL, S_NS };
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_DONE, S_IT, S_TOL, S_SIG, S_NS };
 // what a final sum is stored as (k_finalize)
 enum { F_STORE = 0, F_ALPHA = 1, F_BETA = 2, F_RR = 3 };
 
@@ -395,6 +395,101 @@ k_p_update2(size_t n, T* __restrict__ p, const T* __restrict__ z,
     p[c] = first ? z[c] : z[c] + beta * p[c];
 }
 
+// Learned preconditioner (flexible PCG).  The first direction of each inner
+// round is z = M_sep r + s N(r / s, q), s the RMS of r (the network is
+// trained on the rounds' initial residuals, the projection right-hand
+// sides); later directions use M_sep.  The preconditioner therefore varies
+// (and the network is nonlinear), hence the flexible (Polak-Ribiere) beta:
+//   beta = <z+, r+ - r>_W / <z, r>_W = -alpha <z+, q>_W / rz,
+// with q = A p of the iteration (r+ = r - alpha q).
+
+// partial sums of r^2 (unweighted) over PJ_RED workgroups
+__global__ void __launch_bounds__(256)
+k_rr_part(size_t n, const float* __restrict__ r, double* __restrict__ P) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256)
+    acc += (double)r[c] * (double)r[c];
+  const double t = block_sum_wg(acc, sh);
+  if (threadIdx.x == 0) P[blockIdx.x] = t;
+}
+
+// rs = r / s with s = sqrt(sum P / n) (workgroup 0 stores s in S[S_SIG])
+__global__ void __launch_bounds__(256)
+k_scale_rms(size_t n, const float* __restrict__ r, const double* __restrict__ P, int np,
+            double* __restrict__ S, float* __restrict__ rs) {
+  __shared__ double sh[4];
+  const double rr = sum_partials(P, np, sh);
+  const double sig = rr > 0.0 ? sqrt(rr / (double)n) : 1.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) S[S_SIG] = sig;
+  const float is = (float)(1.0 / sig);
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256)
+    rs[c] = r[c] * is;
+}
+
+// z += s c
+__global__ void __launch_bounds__(256)
+k_add_scaled(size_t n, const float* __restrict__ c, const double* __restrict__ S,
+             float* __restrict__ z) {
+  const float sg = (float)S[S_SIG];
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (size_t)gridDim.x * 256)
+    z[e] += sg * c[e];
+}
+
+// <r, z>_W -> Prz and <q, z>_W -> Pqz over the workgroup's rows; check as
+// in k_dot_rows
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_dot2_rows(GridArgs g, const T* __restrict__ r, const T* __restrict__ q,
+            const T* __restrict__ z, double* __restrict__ Prz, double* __restrict__ Pqz,
+            const double* __restrict__ Prr, double* __restrict__ S, int check) {
+  __shared__ double sh[4];
+  if (check && blockIdx.x == 0) {
+    const double rr = sum_partials(Prr, PJ_RED, sh);
+    if (threadIdx.x == 0 && S[S_DONE] == 0.0) {
+      S[S_RR] = rr;
+      S[S_IT] += 1.0;
+      if (!(rr == rr)) S[S_DONE] = 2.0;
+      else if (sqrt(rr) <= S[S_TOL]) S[S_DONE] = 1.0;
+    }
+    __syncthreads();
+  }
+  int r0, r1;
+  row_range(g.ny * g.nz, r0, r1);
+  double a1 = 0.0, a2 = 0.0;
+  for (int row = r0; row < r1; ++row) {
+    const double dzk = g.dz[row / g.ny];
+    const size_t base = (size_t)row * g.nx;
+    for (int i = threadIdx.x; i < g.nx; i += 256) {
+      const double zv = (double)z[base + i];
+      a1 += dzk * (double)r[base + i] * zv;
+      a2 += dzk * (double)q[base + i] * zv;
+    }
+  }
+  const double t1 = block_sum_wg(a1, sh);
+  __syncthreads();
+  const double t2 = block_sum_wg(a2, sh);
+  if (threadIdx.x == 0) {
+    Prz[blockIdx.x] = t1;
+    Pqz[blockIdx.x] = t2;
+  }
+}
+
+// p = z + beta p with the flexible beta = -alpha sum Pqz / rz
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_p_update_flex(size_t n, T* __restrict__ p, const T* __restrict__ z,
+                const double* __restrict__ Pqz, double* __restrict__ S) {
+  __shared__ double sh[4];
+  if (S[S_DONE] != 0.0) return;
+  const double qz = sum_partials(Pqz, PJ_RED, sh);
+  const double bd = -S[S_ALPHA] * qz / S[S_RZ];
+  if (blockIdx.x == 0 && threadIdx.x == 0) S[S_BETA] = bd;
+  const T beta = (T)bd;
+  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n; c += (size_t)gridDim.x * 256)
+    p[c] = z[c] + beta * p[c];
+}
+
 // r64 = b64 - A64 x64 (float64 operator from rho), r_t = (T) r64,
 // partial sum of r64^2
 template <typename T, typename TR>
@@ -699,7 +794,8 @@ k_gemm(GemmArgs a, const T* __restrict__ A, const T* __restrict__ B,
 // past K) * X (K rows of ncol values at row stride sq).  One wavefront per
 // 16 x 16 output tile with v_mfma_f32_16x16x4f32 over KP/4 steps (small
 // tiles: many wavefronts in flight to cover the load latency):
-//   A (16 x 4): lane l holds Mat[m0 + l%16][k0 + l/16],
+//   A (16 x 4): lane l holds Mat[m0 + l%16][k0 + l/16] (stored on the host
+//     in that fragment order, so a lane's KP/4 values are contiguous),
 //   B (4 x 16): lane l holds X[k0 + l/16][n0 + l%16],
 //   D (16 x 16): register r of lane l holds row 4(l/16) + r, column l%16.
 // Every operand is loaded before the first MFMA.  spectral: out[m] /=
@@ -725,21 +821,25 @@ k_coltrans_mfma(ColArgs a, const float* __restrict__ Mat, const float* __restric
   float* outb = out + (size_t)blockIdx.z * a.souter;
   const int col = n0 + (lane & 15);
   const int colc = col < a.ncol ? col : a.ncol - 1;
-  const uint32_t lo = (uint32_t)colc * 4u;
   const int kq = lane >> 4;
-  const int mrow = m0 + (lane & 15);
+  const int nst = (a.K + 3) >> 2;   // steps holding rows < K (uniform)
+  const float* af = Mat + ((size_t)blockIdx.y * 64 + lane) * (KP / 4);
   float av[KP / 4], bv[KP / 4];
 #pragma unroll
   for (int st = 0; st < KP / 4; ++st) {
-    const int k = 4 * st + kq;
-    // rows k >= K re-read row K-1; Mat's zero padding cancels them
-    bv[st] = at(inb + (size_t)(k < a.K ? k : a.K - 1) * a.sq, lo);
-    av[st] = mrow < a.K ? Mat[(size_t)mrow * KP + k] : 0.0f;
+    if (st < nst) {
+      // 32-bit element offsets (a field is smaller than 4 GiB); a row k >= K
+      // of the last step re-reads row K-1, cancelled by Mat's zero padding
+      const int k = 4 * st + kq;
+      const uint32_t e = (uint32_t)(k < a.K ? k : a.K - 1) * (uint32_t)a.sq + (uint32_t)colc;
+      bv[st] = at(inb, e * 4u);
+      av[st] = af[st];
+    }
   }
   f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int st = 0; st < KP / 4; ++st)
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[st], bv[st], acc, 0, 0, 0);
+    if (st < nst) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[st], bv[st], acc, 0, 0, 0);
   double sd = 0.0;
   if (a.lz) {
     const int j = colc / a.nx, i = colc - j * a.nx;
@@ -751,7 +851,7 @@ k_coltrans_mfma(ColArgs a, const float* __restrict__ Mat, const float* __restric
     if (m < a.K && col < a.ncol) {
       float v = acc[r];
       if (a.lz) v = v * recip((float)(a.lz[m] + sd));
-      at(outb + (size_t)m * a.sq, lo) = v;
+      at(outb, ((uint32_t)m * (uint32_t)a.sq + (uint32_t)col) * 4u) = v;
     }
   }
 }
@@ -912,8 +1012,8 @@ struct ProjHandle {
   // preconditioner (type T): Ux (Nx*Nx), Yf (Ny*Ny forward = basis^T),
   // Yi (Ny*Ny inverse = basis), Zf (Nz*Nz, D folded), Zi (Nz*Nz, D^-1 folded)
   void *Ux, *UxT, *Yf, *Yi, *Zf, *Zi;
-  // float32: the y and z matrices zero-padded to K x col_pad(K) for
-  // k_coltrans_mfma (null for float64 or K > 128: k_gemm then)
+  // float32: the y and z matrices in k_coltrans_mfma's A-fragment order,
+  // zero-padded to col_pad(K) (null for float64 or K > 128: k_gemm then)
   void *Yf_p, *Yi_p, *Zf_p, *Zi_p;
   // x transform through k_dct4_rows when Ux is the orthonormal DCT-IV basis
   // and Nx is a power of two (twx: its twiddle table, type T)
@@ -929,6 +1029,13 @@ struct ProjHandle {
   double *part, *S, *S_host;
   double *Ppq, *Prr, *Prz;  // partials of the PCG iteration's reductions
   void* irho;          // 1 / max(rho, 0.01) of the running solve (type T)
+  void* net;           // learned correction (learned.hip), or null
+  double* Pqz;         // partials of <q, z>_W (flexible PCG)
+  int learned_budget;     // inner iterations a learned round may take
+  int learned_fallback;   // this solve fell back to plain PCG
+  int learned_fallbacks;  // solves that fell back, since creation
+  float* rs;           // r / RMS(r), the network's input
+  float* qf;           // the network's density feature of the running solve
   bool have_prev;
   int last_iters, last_outer;
   double last_rel, last_div_max;
@@ -1050,6 +1157,25 @@ static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 template <typename T>
 static int apply_M(ProjHandle* h, const T* r, T* z, hipStream_t s) {
   if (h->precond_enabled == FW_PRECOND_SEPARABLE) return precond_apply<T>(h, r, z, s);
+  if (h->precond_enabled == FW_PRECOND_LEARNED) {
+    if constexpr (sizeof(T) != 4) {
+      return fw_set_error(FW_E_CONFIG, "learned preconditioner: float32 grids only");
+    } else {
+      int rc = precond_apply<T>(h, r, z, s);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_rr_part, dim3(PJ_RED), dim3(256), 0, s, h->n, (const float*)r,
+                         h->Pqz);
+      hipLaunchKernelGGL(k_scale_rms, dim3(PJ_RED), dim3(256), 0, s, h->n, (const float*)r,
+                         (const double*)h->Pqz, PJ_RED, h->S, h->rs);
+      const float* c = nullptr;
+      int nn = 0;
+      if ((rc = net_apply(h->net, h->rs, h->qf, &c, &nn, s))) return rc;
+      hipLaunchKernelGGL(k_add_scaled, dim3(PJ_RED), dim3(256), 0, s, h->n, c,
+                         (const double*)h->S, (float*)z);
+      FW_CHECK_LAUNCH("learned preconditioner");
+      return FW_OK;
+    }
+  }
   if (h->precond_enabled == FW_PRECOND_JACOBI) {
     const Launch L = cell_launch(&h->g);
     hipLaunchKernelGGL(k_jacobi<T>, L.grid, L.block, 0, s, h->ga, (const T*)h->irho, r, z,
@@ -1087,6 +1213,7 @@ static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
   // separable preconditioner with the DCT-IV x transform: the update of d, r
   // rides in the forward row transform and <r, z> in the inverse one
   const bool fused = h->fft_x && h->precond_enabled == FW_PRECOND_SEPARABLE;
+  const bool flex = h->precond_enabled == FW_PRECOND_LEARNED;
   const int rows = h->g.nz * h->g.ny, ndct = (rows + 3) / 4;
   const size_t shm = 4 * (size_t)h->g.nx * sizeof(T);
   const int npz = fused ? ndct : PJ_RED;
@@ -1144,12 +1271,23 @@ static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
       } else {
         hipLaunchKernelGGL(k_update_xr2<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
                            h->Prz, npz, h->Ppq, h->S, h->Prr);
-        if ((rc = apply_M<T>(h, r, z, s))) return rc;
-        hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z,
-                           h->Prz, h->Prr, h->S, 1);
+        // learned mode: the network corrects the first direction of the
+        // round only (its training inputs are the rounds' initial residuals);
+        // the later ones use the separable inverse, with the flexible beta
+        if ((rc = flex ? precond_apply<T>(h, r, z, s) : apply_M<T>(h, r, z, s))) return rc;
+        if (flex)
+          hipLaunchKernelGGL(k_dot2_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, q, z,
+                             h->Prz, h->Pqz, h->Prr, h->S, 1);
+        else
+          hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z,
+                             h->Prz, h->Prr, h->S, 1);
       }
-      hipLaunchKernelGGL(k_p_update2<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Prz,
-                         npz, h->S, 0);
+      if (flex)
+        hipLaunchKernelGGL(k_p_update_flex<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Pqz,
+                           h->S);
+      else
+        hipLaunchKernelGGL(k_p_update2<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Prz,
+                           npz, h->S, 0);
     }
     queued += nb_it;
     FW_CHECK_LAUNCH("pcg iteration");
@@ -1175,6 +1313,13 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm
   int total = 0, outer = 0, rc;
   double rnorm = 0.0;
   hipLaunchKernelGGL((k_irho<T, TR>), dim3(nblk(n)), dim3(256), 0, s, n, rho, (T*)h->irho);
+  if (h->precond_enabled == FW_PRECOND_LEARNED) {
+    if constexpr (sizeof(TR) == 4) {
+      if ((rc = density_feature((const float*)rho, h->qf, n, s))) return rc;
+    } else {
+      return fw_set_error(FW_E_CONFIG, "learned preconditioner: float32 grids only");
+    }
+  }
   for (;;) {
     if (outer == 0 && r0 >= 0.0) {
       rnorm = r0;
@@ -1191,6 +1336,10 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm
     if (rnorm <= tol_abs || bnorm == 0.0) break;
     if (total >= h->maxiter || outer >= 20) {
       h->last_iters = total;
+      if (h->learned_fallback) {
+        h->precond_enabled = FW_PRECOND_LEARNED;
+        h->learned_fallback = 0;
+      }
       return fw_set_error(FW_E_NOCONV,
                           "projection PCG did not converge: ||r||/||b|| = %.3e after "
                           "%d iterations (rtol %.1e)", rnorm / bnorm, total, rtol);
@@ -1199,8 +1348,17 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm
     if (inner_rel < inner_floor) inner_rel = inner_floor;
     int it = 0;
     h->round = outer;
-    if ((rc = pcg_inner<T, TR>(h, rho, inner_rel * rnorm, h->maxiter - total, &it, s)))
-      return rc;
+    // the learned mode gets an iteration budget per inner solve; a round
+    // that does not converge within it hands the rest of this solve to plain
+    // PCG (the separable preconditioner stays the fallback and the judge)
+    const bool learned = h->precond_enabled == FW_PRECOND_LEARNED;
+    int budget = h->maxiter - total;
+    if (learned && budget > h->learned_budget) budget = h->learned_budget;
+    if ((rc = pcg_inner<T, TR>(h, rho, inner_rel * rnorm, budget, &it, s))) return rc;
+    if (learned && h->S_host[S_DONE] != 1.0) {
+      h->precond_enabled = FW_PRECOND_SEPARABLE;
+      h->learned_fallback = 1;
+    }
     total += it;
     ++outer;
     hipLaunchKernelGGL(k_x64_add<T>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64,
@@ -1210,6 +1368,11 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm
   h->last_iters = total;
   h->last_outer = outer;
   h->last_rel = bnorm > 0.0 ? rnorm / bnorm : 0.0;
+  if (h->learned_fallback) {
+    h->precond_enabled = FW_PRECOND_LEARNED;
+    h->learned_fallbacks += 1;
+    h->learned_fallback = 0;
+  }
   return FW_OK;
 }
 
@@ -1294,6 +1457,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   h->maxiter = maxiter > 0 ? maxiter : 400;
   h->esz = g->dtype == FW_F64 ? 8 : 4;
   h->precond_enabled = FW_PRECOND_SEPARABLE;
+  h->learned_budget = 60;
   const int Nx = g->nx, Ny = g->ny, Nz = g->nz;
   // host copies of the dz metric for folding sqrt(dz) into the z matrices
   std::vector<double> dz(Nz);
@@ -1366,12 +1530,19 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   }
   // zero-padded y and z matrices for the float32 column transforms
   {
+    // fragment order of k_coltrans_mfma's A operand: for the 16-row tile
+    // mt, lane l and step st, Mat[16 mt + l%16][4 st + l/16] (zero past K)
     auto pad = [&](void** dst, const double* src, int K) -> int {
       if (K > 128) return FW_OK;
-      const int KP = col_pad(K);
-      std::vector<double> m((size_t)K * KP, 0.0);
-      for (int a = 0; a < K; ++a)
-        for (int b = 0; b < K; ++b) m[(size_t)a * KP + b] = src[(size_t)a * K + b];
+      const int KP = col_pad(K), MT = (K + 15) / 16;
+      std::vector<double> m((size_t)MT * 64 * (KP / 4), 0.0);
+      for (int mt = 0; mt < MT; ++mt)
+        for (int l = 0; l < 64; ++l)
+          for (int st = 0; st < KP / 4; ++st) {
+            const int row = 16 * mt + (l & 15), col = 4 * st + (l >> 4);
+            if (row < K && col < K)
+              m[((size_t)mt * 64 + l) * (KP / 4) + st] = src[(size_t)row * K + col];
+          }
       return g->dtype == FW_F64 ? upload_cast<double>(dst, m.data(), m.size())
                                 : upload_cast<float>(dst, m.data(), m.size());
     };
@@ -1401,6 +1572,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
     h->Prz = h->Ppq + 2 * np;
   }
   FW_CHECK_HIP(hipMalloc(&h->irho, n * h->esz));
+  FW_CHECK_HIP(hipMalloc((void**)&h->Pqz, PJ_RED * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->S, S_NS * sizeof(double)));
   FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, S_NS * sizeof(double)));
   FW_CHECK_HIP(hipMemset(h->x64, 0, n * 8));
@@ -1414,7 +1586,7 @@ int fw_proj_destroy(void* handle) {
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
                   h->x64, h->b64, h->r64, h->r, h->z, h->p, h->q, h->d, h->w1,
                   h->w2, h->part, h->S, h->twx, h->Ppq, h->irho, h->Yf_p, h->Yi_p,
-                  h->Zf_p, h->Zi_p};
+                  h->Zf_p, h->Zi_p, h->Pqz, h->rs, h->qf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (h->S_host) (void)hipHostFree(h->S_host);
@@ -1457,6 +1629,33 @@ int fw_proj_set_pressure(void* handle, const double* p_in, void* stream) {
   FW_CHECK_HIP(hipMemcpyAsync(h->x64, p_in, h->n * 8, hipMemcpyDeviceToDevice,
                               (hipStream_t)stream));
   h->have_prev = true;
+  return FW_OK;
+}
+
+int fw_proj_set_learned(void* handle, void* net) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h, "proj_set_learned: null handle");
+  if (!net) {
+    h->net = nullptr;
+    if (h->precond_enabled == FW_PRECOND_LEARNED) h->precond_enabled = FW_PRECOND_SEPARABLE;
+    return FW_OK;
+  }
+  FW_REQUIRE(h->dtype == FW_F32, "proj_set_learned: float32 grids only");
+  int nx, ny, nz;
+  net_dims(net, &nx, &ny, &nz);
+  FW_REQUIRE(nx == h->g.nx && ny == h->g.ny && nz == h->g.nz,
+             "proj_set_learned: network grid %d x %d x %d != solver grid %d x %d x %d", nx, ny,
+             nz, h->g.nx, h->g.ny, h->g.nz);
+  if (!h->rs) FW_CHECK_HIP(hipMalloc((void**)&h->rs, h->n * sizeof(float)));
+  if (!h->qf) FW_CHECK_HIP(hipMalloc((void**)&h->qf, h->n * sizeof(float)));
+  h->net = net;
+  return FW_OK;
+}
+
+int fw_proj_learned_stats(void* handle, int* fallbacks) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && fallbacks, "proj_learned_stats: null argument");
+  *fallbacks = h->learned_fallbacks;
   return FW_OK;
 }
 
@@ -1515,8 +1714,11 @@ int fw_proj_set_preconditioner(void* handle, int enabled) {
   ProjHandle* h = (ProjHandle*)handle;
   FW_REQUIRE(h, "proj_set_preconditioner: null handle");
   FW_REQUIRE(enabled == FW_PRECOND_NONE || enabled == FW_PRECOND_SEPARABLE ||
-                 enabled == FW_PRECOND_JACOBI,
-             "proj_set_preconditioner: mode must be 0 (none), 1 (separable) or 2 (Jacobi)");
+                 enabled == FW_PRECOND_JACOBI || enabled == FW_PRECOND_LEARNED,
+             "proj_set_preconditioner: mode must be 0 (none), 1 (separable), 2 (Jacobi) "
+             "or 3 (learned)");
+  FW_REQUIRE(enabled != FW_PRECOND_LEARNED || h->net,
+             "proj_set_preconditioner: attach a network with fw_proj_set_learned first");
   h->precond_enabled = enabled;
   return FW_OK;
 }

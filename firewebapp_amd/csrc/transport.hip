@@ -154,18 +154,25 @@ k_species(GridArgs g, const T* __restrict__ Y, T* __restrict__ Yo,
 }
 
 // The three species of a sub-step (fuel, O2, H2O) in one pass: rho, u, v, w,
-// the two particle sources and the stencil offsets serve all three.
+// the two particle sources and the stencil offsets serve all three.  A
+// thread updates CZ vertically adjacent cells (the grid's z dimension is
+// ceil(Nz / CZ)): their z stencils overlap, and the compiler shares the
+// common loads, differences and limited slopes between them (the same
+// expressions on the same read-only data), and the launch has fewer
+// wavefronts than cells, so they are all resident at once.
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_species3(GridArgs g, const T* __restrict__ Y1, const T* __restrict__ Y2,
-           const T* __restrict__ Y3, T* __restrict__ O1, T* __restrict__ O2,
-           T* __restrict__ O3, const T* __restrict__ rho, const T* __restrict__ u,
-           const T* __restrict__ v, const T* __restrict__ w, const T* __restrict__ Sp,
-           const T* __restrict__ Sd, T dt, T D, T in1, T in2, T in3, T mx1, T mx2,
-           T mx3) {
-  int i, j, k;
-  cell_ijk(i, j, k);
-  if (i >= g.nx) return;
+__device__ __forceinline__ void species3_at(const GridArgs& g, int i, int j, int k,
+                                            const T* __restrict__ Y1,
+                                            const T* __restrict__ Y2,
+                                            const T* __restrict__ Y3, T* __restrict__ O1,
+                                            T* __restrict__ O2, T* __restrict__ O3,
+                                            const T* __restrict__ rho,
+                                            const T* __restrict__ u,
+                                            const T* __restrict__ v,
+                                            const T* __restrict__ w,
+                                            const T* __restrict__ Sp,
+                                            const T* __restrict__ Sd, T dt, T D, T in1,
+                                            T in2, T in3, T mx1, T mx2, T mx3) {
   const Nbr n = nbr_offsets<T>(g, i, j, k);
   const T rc = at(rho, n.c), ui = at(u, n.c), vi = at(v, n.c), wi = at(w, n.c);
   const T sp = at(Sp, n.c), sd = at(Sd, n.c);
@@ -176,6 +183,26 @@ k_species3(GridArgs g, const T* __restrict__ Y1, const T* __restrict__ Y2,
                                 dt, D, in2, mx2);
   at(O3, n.c) = species_cell<T>(g, i, k, n, Y3, rc, irc, rhz, ui, vi, wi, T(0), sp, sd, 3,
                                 dt, D, in3, mx3);
+}
+
+template <typename T, int CZ>
+__global__ void __launch_bounds__(256)
+k_species3(GridArgs g, const T* __restrict__ Y1, const T* __restrict__ Y2,
+           const T* __restrict__ Y3, T* __restrict__ O1, T* __restrict__ O2,
+           T* __restrict__ O3, const T* __restrict__ rho, const T* __restrict__ u,
+           const T* __restrict__ v, const T* __restrict__ w, const T* __restrict__ Sp,
+           const T* __restrict__ Sd, T dt, T D, T in1, T in2, T in3, T mx1, T mx2,
+           T mx3) {
+  int i, j, kb;
+  cell_ijk(i, j, kb);
+  if (i >= g.nx) return;
+#pragma unroll
+  for (int q = 0; q < CZ; ++q) {
+    const int k = kb * CZ + q;
+    if (k < g.nz)
+      species3_at<T>(g, i, j, k, Y1, Y2, Y3, O1, O2, O3, rho, u, v, w, Sp, Sd, dt, D, in1,
+                     in2, in3, mx1, mx2, mx3);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -581,8 +608,10 @@ static int species3_impl(const fw_grid* g, const void* const* Y, void* const* Yo
                          const void* rho, const void* u, const void* v, const void* w,
                          const void* Sp, const void* Sd, double dt, double D,
                          const double* y_in, const double* y_max, hipStream_t s) {
+  constexpr int CZ = 2;
   Launch L = cell_launch(g);
-  hipExtLaunchKernelGGL(k_species3<T>, L.grid, L.block, 0, s, g_species3_events.start,
+  L.grid.z = (g->nz + CZ - 1) / CZ;
+  hipExtLaunchKernelGGL((k_species3<T, CZ>), L.grid, L.block, 0, s, g_species3_events.start,
                      g_species3_events.stop, 0, grid_args(g), (const T*)Y[0],
                      (const T*)Y[1], (const T*)Y[2], (T*)Yo[0], (T*)Yo[1], (T*)Yo[2],
                      (const T*)rho, (const T*)u, (const T*)v, (const T*)w, (const T*)Sp,

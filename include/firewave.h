@@ -229,8 +229,24 @@ int fw_proj_precondition(void* handle, const void* r, void* z, void* stream);
  * SeparableLaplacian3D, fft_poisson_3d.py:184-228), FW_PRECOND_NONE, or
  * FW_PRECOND_JACOBI (diag(A)^-1; not in the reference, a comparison row for
  * the time-to-residual measurement of the plain PCG). */
-enum { FW_PRECOND_NONE = 0, FW_PRECOND_SEPARABLE = 1, FW_PRECOND_JACOBI = 2 };
+enum { FW_PRECOND_NONE = 0, FW_PRECOND_SEPARABLE = 1, FW_PRECOND_JACOBI = 2,
+       FW_PRECOND_LEARNED = 3 };
 int fw_proj_set_preconditioner(void* handle, int mode);
+
+/* FW_PRECOND_LEARNED (no counterpart in the reference): the first direction
+ * of each inner PCG round is z = M_sep r + s N(r / s, RHO_REF / rho - 1),
+ * s the RMS of r, the later ones M_sep r; N is the attached network
+ * (fw_net_create on the same grid, float32); flexible PCG
+ * (beta = <z+, r+ - r>_W / <z, r>_W).
+ * net = null detaches it (the mode falls back to separable).  The network
+ * stays owned by the caller and must outlive its use. */
+int fw_proj_set_learned(void* handle, void* net);
+
+/* In FW_PRECOND_LEARNED an inner PCG round has a budget of 60 iterations;
+ * a round that does not converge within it hands the rest of that solve to
+ * the separable preconditioner.  fallbacks = the solves that did, since
+ * the handle was created. */
+int fw_proj_learned_stats(void* handle, int* fallbacks);
 
 /* Copy the right-hand side b = (div - target)/dt of the last fw_proj_project
  * (float64, N values) out: the system the reference hands to its Krylov
@@ -523,6 +539,37 @@ int fw_spread_timer_read(double* total_ms, int* launches);
  * (k=n_z_bed/2, j=Ny/2) into st->front_exit_x / st->front_mid_x. */
 int fw_spread_front(const fw_grid* g64, const fw_spread_fields* f,
                     fw_spread_state* st, void* stream);
+
+/* ---- learned preconditioner (no counterpart in the reference) ----------- */
+/* The reference preconditions its Krylov solve with the separable inverse
+ * only (projection_3d.py:756-791).  These entry points add a learned
+ * correction: a two-level 3-D convolutional network (float32, y periodic,
+ * zero padding in x and z, grid dims even) trained on the solver's own
+ * projection systems, applied inside flexible PCG (FW_PRECOND_LEARNED).
+ * Parameters are one flat float32 vector of fw_net_nparams() values (per
+ * layer: weights (co, ci, 27) then biases (co); layers 2->8, 8->16 coarse,
+ * 16->16 coarse, 8+16->8, 8->1). */
+int fw_net_nparams(void);
+/* q = RHO_REF / max(rho, 0.01) - 1 for n float32 values (the network's
+ * density input, as the learned preconditioner forms it). */
+int fw_net_density_feature(const void* rho, void* q, int64_t n, void* stream);
+int fw_net_create(int nx, int ny, int nz, const float* theta_host, void** handle);
+int fw_net_destroy(void* handle);
+int fw_net_get_params(void* handle, float* theta_host);
+/* Replaces the parameters and zeroes the Adam moments. */
+int fw_net_set_params(void* handle, const float* theta_host);
+/* c = N(r, q) (device pointers, N float32 values each; q = RHO_REF / rho - 1,
+ * RHO_REF = 101325 / (8.314 / 0.029 * 300)). */
+int fw_net_forward(void* handle, const void* r, const void* q, void* c_out, void* stream);
+/* One training step on the sample (r, q, t): forward, the loss
+ * L = sum dz_k (N(r, q) - t)^2 / sum dz_k t^2 (dz: device float64, Nz values),
+ * backward, then an Adam step (lr, step number it >= 1; lr = 0 leaves the
+ * parameters unchanged).  Returns L of the parameters before the step and
+ * synchronises the stream. */
+int fw_net_train_step(void* handle, const void* r, const void* q, const void* t,
+                      const double* dz, double lr, int it, double* loss, void* stream);
+/* Copies the gradient of the last fw_net_train_step to the host. */
+int fw_net_grad_copy(void* handle, float* grad_host);
 
 #ifdef __cplusplus
 }
