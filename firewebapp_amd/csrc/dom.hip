@@ -210,11 +210,12 @@ k_dom_sweep_diag(GridArgs g, int d, int jlo, int M, const double* __restrict__ O
   }
 }
 
-// Tiled wavefront: one workgroup sweeps a 4 x 4 block of rows (sweep-ordered
-// j' in [4 tj, 4 tj + 4), k' in [4 tk, 4 tk + 4)) of one ordinate.  Blocks on
-// the block anti-diagonal tj + tk = D are independent, so Ny/4 + Nz/4 - 1
-// launches make a source iteration.  Wavefront w owns the column j' = 4 tj + w
-// and walks k' upward: at stage s (0..6) it computes row (w, s - w), so the
+// Tiled wavefront: one workgroup (TS wavefronts) sweeps a TS x TS block of
+// rows (sweep-ordered j' in [TS tj, TS tj + TS), k' in [TS tk, TS tk + TS))
+// of one ordinate.  Blocks on the block anti-diagonal tj + tk = D are
+// independent, so Ny/TS + Nz/TS - 1 launches make a source iteration.
+// Wavefront w owns the column j' = TS tj + w and walks k' upward: at stage
+// s (0 .. 2 TS - 2) it computes row (w, s - w), so the
 // row diagonals of the block run in order with every wavefront busy on the
 // middle stages.  The z inflow of a row is the same wavefront's previous row
 // (registers); the y inflow is wavefront w - 1's row of the previous stage
@@ -224,18 +225,18 @@ k_dom_sweep_diag(GridArgs g, int d, int jlo, int M, const double* __restrict__ O
 // block is stored, so the wrap of small Ny still reads the previous
 // iteration.  One reciprocal per cell: a = ax r, b = (...) r, I = num r with
 // r = 1 / (kappa + ax + ay + az).
-template <typename T, int PER, bool FULL>
-__global__ void __launch_bounds__(256)
+template <typename T, int PER, bool FULL, int TS>
+__global__ void __launch_bounds__(64 * TS)
 k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ Om,
                  T* __restrict__ Iset, const T* __restrict__ kap,
                  const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb) {
   extern __shared__ unsigned char smem_raw[];
-  T* sh = reinterpret_cast<T*>(smem_raw);   // 16 rows in sweep order: slot (w, kk)
+  T* sh = reinterpret_cast<T*>(smem_raw);   // TS*TS rows in sweep order: slot (w, kk)
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int n = blockIdx.y;
   const int tj = tjlo + blockIdx.x, tk = D - tj;
-  const int jp = 4 * tj + w;
+  const int jp = TS * tj + w;
   const bool col = jp < Ny;                 // uniform per wavefront
   const double xi = Om[3 * n], eta = Om[3 * n + 1], mu = Om[3 * n + 2];
   const int j = eta >= 0.0 ? jp : Ny - 1 - jp;
@@ -256,13 +257,13 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
     ix[q] = (uint32_t)(fwd ? sq : Nx - 1 - sq);
   }
   // global inputs read up front, so each stage is pure arithmetic: kappa and
-  // B of the column's four rows, the y inflow of the column's rows (w == 0)
+  // B of the column's TS rows, the y inflow of the column's rows (w == 0)
   // and the z inflow of its first row (kk == 0)
-  T Iyg[4][PER], Iz[PER], kvs[4][PER], bvs[4][PER];
+  T Iyg[TS][PER], Iz[PER], kvs[TS][PER], bvs[TS][PER];
   if (col) {
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      const int kp = 4 * tk + kk;
+    for (int kk = 0; kk < TS; ++kk) {
+      const int kp = TS * tk + kk;
       if (kp < Nz) {
         const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
         const uint32_t row = ((uint32_t)k * Ny + j) * Nx;
@@ -277,7 +278,7 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
         }
       }
     }
-    const int kp0 = 4 * tk;
+    const int kp0 = TS * tk;
     const int k0 = mu >= 0.0 ? kp0 : Nz - 1 - kp0;
     const bool zb = mu >= 0.0 ? (k0 == 0) : (k0 == Nz - 1);
     const int kz = mu >= 0.0 ? k0 - 1 : k0 + 1;
@@ -290,15 +291,15 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
     }
   }
   __syncthreads();   // every global inflow is read before any row is stored
-  for (int st = 0; st < 7; ++st) {
+  for (int st = 0; st < 2 * TS - 1; ++st) {
     const int kk = st - w;
-    const int kp = 4 * tk + kk;
-    if (col && kk >= 0 && kk < 4 && kp < Nz) {
+    const int kp = TS * tk + kk;
+    if (col && kk >= 0 && kk < TS && kp < Nz) {
       const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
       const T az = T(fabs(mu) * g.inv_dz[k]);
       const uint32_t row = ((uint32_t)k * Ny + j) * Nx;
-      const T* shy = sh + ((w - 1) * 4 + kk) * Nx;   // wavefront w - 1's row
-      T* sho = sh + (w * 4 + kk) * Nx;
+      const T* shy = sh + ((w - 1) * TS + kk) * Nx;   // wavefront w - 1's row
+      T* sho = sh + (w * TS + kk) * Nx;
       T num0[PER], rc[PER];
       T A = T(1), Bc = T(0);
 #pragma unroll
@@ -327,10 +328,10 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
           Iprev = v;
         }
       }
-      // the wavefront's rows run in order kk = 0..3: shift the prefetched
+      // the wavefront's rows run in order kk = 0..TS-1: shift the prefetched
       // inputs down so the next row's sit in slot 0 (static register indices)
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+      for (int r = 0; r < TS - 1; ++r)
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
           kvs[r][q] = kvs[r + 1][q];
@@ -460,43 +461,53 @@ static inline unsigned nb(size_t n) { return (unsigned)((n + 255) / 256); }
     if (_rc) return _rc;     \
   } while (0)
 
-// Tiled sweep launch for block diagonal D (16 rows of LDS per workgroup).
-template <typename T, int PER>
+// Tiled sweep launch for block diagonal D (TS*TS rows of LDS per workgroup).
+template <typename T, int PER, int TS>
 static void launch_tile(DomHandle* h, int D, int tjlo, int ntiles, T I_amb, hipStream_t s) {
-  const size_t shm = 16 * (size_t)h->g.nx * sizeof(T);
+  const size_t shm = TS * TS * (size_t)h->g.nx * sizeof(T);
   const dim3 grid(ntiles, h->M);
   if (h->g.nx == 64 * PER)
-    hipLaunchKernelGGL((k_dom_sweep_tile<T, PER, true>), grid, dim3(256), shm, s, h->ga, D,
-                       tjlo, h->M, (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
-                       (const T*)h->B, (const T*)h->Ig_out, I_amb);
+    hipLaunchKernelGGL((k_dom_sweep_tile<T, PER, true, TS>), grid, dim3(64 * TS), shm, s,
+                       h->ga, D, tjlo, h->M, (const double*)h->Om, (T*)h->I,
+                       (const T*)h->kap_safe, (const T*)h->B, (const T*)h->Ig_out, I_amb);
   else
-    hipLaunchKernelGGL((k_dom_sweep_tile<T, PER, false>), grid, dim3(256), shm, s, h->ga, D,
-                       tjlo, h->M, (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
-                       (const T*)h->B, (const T*)h->Ig_out, I_amb);
+    hipLaunchKernelGGL((k_dom_sweep_tile<T, PER, false, TS>), grid, dim3(64 * TS), shm, s,
+                       h->ga, D, tjlo, h->M, (const double*)h->Om, (T*)h->I,
+                       (const T*)h->kap_safe, (const T*)h->B, (const T*)h->Ig_out, I_amb);
 }
 
+// Tile side: 4 x 4 rows when their LDS rows fit 64 KB, else 0 (the per-row
+// kernel).  8 x 8 tiles halve the launches but not the time: at 256 x 64 x
+// 48, S8, a source iteration is VALU-throughput-bound across the tiles of a
+// diagonal (13 launches of 29.2 us against 27 of 13.6 us).
 template <typename T>
-static bool tile_fits(const DomHandle* h) {
-  return 16 * (size_t)h->g.nx * sizeof(T) <= 65536;
+static int tile_side(const DomHandle* h) {
+  const size_t row = (size_t)h->g.nx * sizeof(T);
+  return 16 * row <= 65536 ? 4 : 0;
 }
 
 // One source iteration of the tiled sweep (cells per lane as below; a float64
 // row of more than 512 cells does not fit the LDS and never gets here).
-template <typename T>
-static int sweep_tiled(DomHandle* h, T I_amb, hipStream_t s) {
+template <typename T, int TS>
+static int sweep_tiled_ts(DomHandle* h, T I_amb, hipStream_t s) {
   const int Nx = h->g.nx;
-  const int TJ = (h->g.ny + 3) / 4, TK = (h->g.nz + 3) / 4;
+  const int TJ = (h->g.ny + TS - 1) / TS, TK = (h->g.nz + TS - 1) / TS;
   for (int D = 0; D <= TJ + TK - 2; ++D) {
     const int tjlo = D - (TK - 1) > 0 ? D - (TK - 1) : 0;
     const int tjhi = D < TJ - 1 ? D : TJ - 1;
     const int nt = tjhi - tjlo + 1;
-    if (Nx <= 64) launch_tile<T, 1>(h, D, tjlo, nt, I_amb, s);
-    else if (Nx <= 128) launch_tile<T, 2>(h, D, tjlo, nt, I_amb, s);
-    else if (Nx <= 256) launch_tile<T, 4>(h, D, tjlo, nt, I_amb, s);
-    else if (Nx <= 512) launch_tile<T, 8>(h, D, tjlo, nt, I_amb, s);
-    else if constexpr (sizeof(T) == 4) launch_tile<T, 16>(h, D, tjlo, nt, I_amb, s);
+    if (Nx <= 64) launch_tile<T, 1, TS>(h, D, tjlo, nt, I_amb, s);
+    else if (Nx <= 128) launch_tile<T, 2, TS>(h, D, tjlo, nt, I_amb, s);
+    else if (Nx <= 256) launch_tile<T, 4, TS>(h, D, tjlo, nt, I_amb, s);
+    else if (Nx <= 512) launch_tile<T, 8, TS>(h, D, tjlo, nt, I_amb, s);
+    else if constexpr (sizeof(T) == 4 && TS == 4) launch_tile<T, 16, TS>(h, D, tjlo, nt, I_amb, s);
   }
   return FW_OK;
+}
+
+template <typename T>
+static int sweep_tiled(DomHandle* h, T I_amb, hipStream_t s) {
+  return sweep_tiled_ts<T, 4>(h, I_amb, s);
 }
 
 // Cells per lane: ceil(Nx / 64) rounded up to a power of two (Nx <= 2048).
@@ -549,7 +560,7 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
     void* tmp = h->G;
     h->G = h->Gp;
     h->Gp = tmp;
-    if (tile_fits<T>(h)) {
+    if (tile_side<T>(h) > 0) {
       TRY_DOM(sweep_tiled<T>(h, I_amb, s));
     } else {
       for (int d = 0; d <= Ny + Nz - 2; ++d) {

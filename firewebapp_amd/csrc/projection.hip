@@ -858,6 +858,77 @@ k_coltrans_mfma(ColArgs a, const float* __restrict__ Mat, const float* __restric
 
 static int col_pad(int K) { return K <= 16 ? 16 : K <= 32 ? 32 : K <= 64 ? 64 : 128; }
 
+// The z pair of the separable solve in one pass (float32, matrix cores):
+// out = Zi * D^-1 (Zf * X) for every column, D^-1 the spectral divide.
+// One wavefront per 16 columns: the forward product (all ceil(K/16) row
+// tiles) leaves the scaled intermediate in the wavefront's own LDS slab,
+// re-read as the B operand of the inverse product, so the intermediate
+// never goes to HBM and the pair is one launch.  Operand layouts as in
+// k_coltrans_mfma; Zf, Zi in its A-fragment order.
+template <int KP>
+__global__ void __launch_bounds__(256)
+k_zpair_mfma(ColArgs a, const float* __restrict__ Zf, const float* __restrict__ Zi,
+             const float* __restrict__ in, float* __restrict__ out) {
+  __shared__ float tl[4][KP * 16];   // per wavefront: rows 0..KP-1 x 16 columns
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int n0 = (blockIdx.x * 4 + wv) * 16;
+  const bool live = n0 < a.ncol;           // uniform per wavefront
+  const int col = n0 + (lane & 15);
+  const int colc = col < a.ncol ? col : a.ncol - 1;
+  const int kq = lane >> 4;
+  const int nst = (a.K + 3) >> 2;
+  const int MT = (a.K + 15) >> 4;
+  float* T = tl[wv];
+  if (live) {
+    float bv[KP / 4];
+#pragma unroll
+    for (int st = 0; st < KP / 4; ++st) {
+      if (st < nst) {
+        const int k = 4 * st + kq;
+        const uint32_t e = (uint32_t)(k < a.K ? k : a.K - 1) * (uint32_t)a.sq + (uint32_t)colc;
+        bv[st] = at(in, e * 4u);
+      }
+    }
+    const int j = colc / a.nx, i = colc - j * a.nx;
+    const double sd = a.ly[j] + a.lx[i] - a.eps;
+    for (int mt = 0; mt < MT; ++mt) {
+      const float* af = Zf + ((size_t)mt * 64 + lane) * (KP / 4);
+      f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int st = 0; st < KP / 4; ++st)
+        if (st < nst) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[st], bv[st], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = 16 * mt + 4 * kq + r;
+        // rows past K stay finite (zero): the inverse's zero-padded A
+        // columns multiply them
+        T[m * 16 + (lane & 15)] = m < a.K ? acc[r] * recip((float)(a.lz[m] + sd)) : 0.0f;
+      }
+    }
+  }
+  __syncthreads();   // every wavefront reaches it (no early return above)
+  if (!live) return;
+  float tv[KP / 4];
+#pragma unroll
+  for (int st = 0; st < KP / 4; ++st)
+    if (st < nst) tv[st] = T[(4 * st + kq) * 16 + (lane & 15)];
+  for (int mt = 0; mt < MT; ++mt) {
+    const float* af = Zi + ((size_t)mt * 64 + lane) * (KP / 4);
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int st = 0; st < KP / 4; ++st)
+      if (st < nst) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(af[st], tv[st], acc, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = 16 * mt + 4 * kq + r;
+      if (m < a.K && col < a.ncol)
+        at(out, ((uint32_t)m * (uint32_t)a.sq + (uint32_t)col) * 4u) = acc[r];
+    }
+  }
+}
+
+
+
 // Orthonormal DCT-IV of each row (the x eigenbasis of the Neumann-left /
 // Dirichlet-right operator) through an M-point complex FFT, N = 2M, M a
 // power of two:
@@ -1093,6 +1164,26 @@ static void launch_col(ProjHandle* h, const void* Mat, int K, long sq, int ncol,
   }
 }
 
+// The fused z pair (k_zpair_mfma) for K <= 64 (its LDS slab is KP x 16).
+static bool zpair_ok(const ProjHandle* h) {
+  return h->Zf_p && h->Zi_p && h->g.nz <= 64;
+}
+static void launch_zpair(ProjHandle* h, const float* in, float* out, hipStream_t s) {
+  ColArgs a;
+  const int Nz = h->g.nz;
+  const long NP = (long)h->g.nx * h->g.ny;
+  a.K = Nz; a.ncol = (int)NP; a.nx = h->g.nx; a.sq = NP; a.souter = 0;
+  a.lz = h->lz; a.ly = h->ly; a.lx = h->lx; a.eps = h->eps_reg;
+  const dim3 grid((unsigned)((NP + 63) / 64));
+  const float* Zf = (const float*)h->Zf_p;
+  const float* Zi = (const float*)h->Zi_p;
+  switch (col_pad(Nz)) {
+    case 16: hipLaunchKernelGGL(k_zpair_mfma<16>, grid, dim3(256), 0, s, a, Zf, Zi, in, out); break;
+    case 32: hipLaunchKernelGGL(k_zpair_mfma<32>, grid, dim3(256), 0, s, a, Zf, Zi, in, out); break;
+    default: hipLaunchKernelGGL(k_zpair_mfma<64>, grid, dim3(256), 0, s, a, Zf, Zi, in, out); break;
+  }
+}
+
 // The y and z parts of the separable solve on the x-transformed field in w1
 // (w2 is scratch): y forward, z forward with the spectral divide, z
 // inverse, y inverse.  Returns the buffer (w1 or w2) holding the result.
@@ -1111,6 +1202,10 @@ static const T* yz_transforms(ProjHandle* h, T* w1, T* w2, hipStream_t s) {
                      1, NP, Nx, 1, false, s);
     T* t = cur; cur = other; other = t;
   }
+  if (mf && zpair_ok(h)) {   // z forward, spectral divide, z inverse: one pass
+    launch_zpair(h, (const float*)cur, (float*)other, s);
+    T* t = cur; cur = other; other = t;
+  } else {
   // z forward with sqrt(dz) folded in, spectral divide in the epilogue
   if (mf && h->Zf_p)
     launch_col(h, h->Zf_p, Nz, NP, (int)NP, 1, 0, true, (const float*)cur, (float*)other, s);
@@ -1125,6 +1220,7 @@ static const T* yz_transforms(ProjHandle* h, T* w1, T* w2, hipStream_t s) {
     launch_gemm<T>(h, (const T*)h->Zi, cur, other, Nz, (int)NP, Nz, 1, 0, Nz, 1, 0,
                    NP, 1, 0, NP, 1, false, s);
   { T* t = cur; cur = other; other = t; }
+  }
   if (Ny > 1) {
     if (mf && h->Yi_p)
       launch_col(h, h->Yi_p, Ny, Nx, Nx, Nz, NP, false, (const float*)cur, (float*)other, s);
