@@ -346,14 +346,51 @@ def particle_mass_scale(cfg: Spread3DConfig, n: int):
 
 
 @dataclass
+class CellState3D:
+    """The state arrays of the result (spread_3d.py:88-158), float64
+    (Nz, Ny, Nx).  On this configuration p_dyn and Y_CO stay zero (no step
+    writes them) and the Eulerian solid pools m_hemi, m_cell, m_lign, m_char
+    are zero (the particles own the bed, spread_3d.py:1318-1327)."""
+    rho: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    p_dyn: np.ndarray
+    T_g: np.ndarray
+    Y_fuel: np.ndarray
+    Y_O2: np.ndarray
+    Y_H2O: np.ndarray
+    Y_CO: np.ndarray
+    alpha_s: np.ndarray
+    T_s: np.ndarray
+    m_hemi: np.ndarray
+    m_cell: np.ndarray
+    m_lign: np.ndarray
+    m_char: np.ndarray
+
+
+_STATE_KEYS = ("rho", "u", "v", "w", "p_dyn", "T_g", "Y_fuel", "Y_O2", "Y_H2O", "Y_CO",
+               "alpha_s", "T_s", "m_hemi", "m_cell", "m_lign", "m_char")
+_GRID_KEYS = ("x_mid", "y_mid", "z_mid", "dz_arr", "inv_dz_arr", "z_face", "d_face_above",
+              "d_face_below")
+_DIAG_KEYS = ("diag_t", "diag_Tg_max", "diag_Ts_max", "diag_Sp_max", "diag_Qc_max",
+              "diag_omega_max", "diag_Y_max", "diag_n_ign")
+
+
+def _npz_path(path):
+    path = Path(path)
+    return path if str(path).endswith(".npz") else path.with_suffix(".npz")
+
+
+@dataclass
 class Spread3DResult:
-    """The Spread3DResult fields the step produces (spread_3d.py:428-559)."""
+    """The Spread3DResult of run_3d_spread (spread_3d.py:428-559)."""
     ros_m_s: float
     n_cells_ignited: int
     front_t: list
     front_x: list
     grid: Grid3D
-    state_final: dict
+    state_final: CellState3D
     diag_t: list = field(default_factory=list)
     diag_Tg_max: list = field(default_factory=list)
     diag_Ts_max: list = field(default_factory=list)
@@ -364,8 +401,59 @@ class Spread3DResult:
     diag_n_ign: list = field(default_factory=list)
     diag_proj_div_max: list = field(default_factory=list)
     diag_proj_n_iter: list = field(default_factory=list)
+    k_turb_final: Optional[np.ndarray] = None
+    eps_turb_final: Optional[np.ndarray] = None
+    nu_t_final: Optional[np.ndarray] = None
     n_steps: int = 0
     stop_reason: Optional[str] = None
+
+    def save(self, path, metadata: Optional[dict] = None) -> None:
+        """One compressed .npz (``.npz`` appended when missing) with the keys
+        of Spread3DResult.save (spread_3d.py:460-511): ``_meta_json`` (the
+        scalars and ``user`` = metadata), the grid axes, the 16 state arrays,
+        the front history and the eight per-step diagnostics."""
+        import json
+        path = _npz_path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        g, s = self.grid, self.state_final
+        meta = {"ros_m_s": float(self.ros_m_s), "n_cells_ignited": int(self.n_cells_ignited),
+                "Nx": int(g.Nx), "Ny": int(g.Ny), "Nz": int(g.Nz), "dx": float(g.dx),
+                "dy": float(g.dy), "dz": float(g.dz), "Lx": float(g.Lx), "Ly": float(g.Ly),
+                "Lz": float(g.Lz), "n_z_bed": int(g.n_z_bed), "user": metadata or {}}
+        arrays = {"_meta_json": np.array(json.dumps(meta))}
+        arrays.update({k: getattr(g, k) for k in _GRID_KEYS})
+        arrays.update({k: getattr(s, k) for k in _STATE_KEYS})
+        arrays["front_t"] = np.array(self.front_t)
+        arrays["front_x"] = np.array(self.front_x)
+        arrays.update({k: np.array(getattr(self, k)) for k in _DIAG_KEYS})
+        np.savez_compressed(path, **arrays)
+
+    @classmethod
+    def load(cls, path) -> "Spread3DResult":
+        """Read a file of ``save`` (spread_3d.py:513-559), with no pickles;
+        Y_H2O, Y_CO and m_char default to zeros when absent."""
+        import json
+        z = np.load(_npz_path(path), allow_pickle=False)
+        meta = json.loads(str(z["_meta_json"]))
+        grid = Grid3D(Nx=int(meta["Nx"]), Ny=int(meta["Ny"]), Nz=int(meta["Nz"]),
+                      dx=float(meta["dx"]), dy=float(meta["dy"]), dz=float(meta["dz"]),
+                      Lx=float(meta["Lx"]), Ly=float(meta["Ly"]), Lz=float(meta["Lz"]),
+                      n_z_bed=int(meta["n_z_bed"]), **{k: z[k] for k in _GRID_KEYS})
+        st = {}
+        for k in _STATE_KEYS:
+            if k in z.files:
+                st[k] = z[k]
+            elif k in ("Y_H2O", "Y_CO"):
+                st[k] = np.zeros_like(z["Y_O2"])
+            elif k == "m_char":
+                st[k] = np.zeros_like(z["m_hemi"])
+            else:
+                raise KeyError(f"{path}: missing state array {k!r}")
+        return cls(ros_m_s=float(meta["ros_m_s"]),
+                   n_cells_ignited=int(meta["n_cells_ignited"]),
+                   front_t=z["front_t"].tolist(), front_x=z["front_x"].tolist(),
+                   grid=grid, state_final=CellState3D(**st),
+                   **{k: z[k].tolist() for k in _DIAG_KEYS})
 
 
 # ---------------------------------------------------------------------------
@@ -624,17 +712,39 @@ class Spread3DSolver:
         return None
 
     # -- whole run -------------------------------------------------------------
-    def run(self, n_steps=None):
+    def run(self, n_steps=None, snapshot_dir=None, snapshot_interval_s=1.0):
+        """Step until n_steps (or the deck's n_steps, else max_wall_time_s)
+        or an exit.  With snapshot_dir, write snap_%04d.npz after each step
+        that ends >= snapshot_interval_s after the last one
+        (spread_3d.py:3537-3670); the extinction and front exits leave before
+        the snapshot, the domain-end exit after it, as in the reference."""
         cfg = self.cfg
         n_target = n_steps if n_steps is not None else cfg.n_steps
-        while True:
-            if n_target is not None:
-                if self.step_no >= n_target:
+        writer = None
+        if snapshot_dir is not None:
+            from .snapshot import SnapshotWriter, snapshot_path
+            Path(snapshot_dir).mkdir(parents=True, exist_ok=True)
+            writer = SnapshotWriter(self, cfg.wind_speed_m_s)
+            last_t, idx = -1.0e30, 0
+        try:
+            while True:
+                if n_target is not None:
+                    if self.step_no >= n_target:
+                        break
+                elif self.t >= cfg.max_wall_time_s:
                     break
-            elif self.t >= cfg.max_wall_time_s:
-                break
-            if self.step() is not None:
-                break
+                r = self.step()
+                if r in ("extinct", "front_exit"):
+                    break
+                if writer is not None and self.t - last_t >= snapshot_interval_s:
+                    last_t = self.t
+                    writer.submit(snapshot_path(snapshot_dir, idx))
+                    idx += 1
+                if r is not None:
+                    break
+        finally:
+            if writer is not None:
+                writer.close()
         return self
 
     def state_numpy(self):
@@ -662,16 +772,28 @@ class Spread3DSolver:
         return (fx[-1] - fx[0]) / (self.t - ft[0]) if self.t > ft[0] else 0.0
 
     def result(self):
-        st = self.state_numpy()
+        """Spread3DResult of the current state (spread_3d.py:3708-3729)."""
+        torch.cuda.current_stream().synchronize()
+        f = self.f
+
+        def h64(t):
+            return t.double().cpu().numpy()
+
+        st = {k: h64(f[k]) for k in ("rho", "u", "v", "w", "T_g", "Y_fuel", "Y_O2", "Y_H2O",
+                                     "alpha_s", "T_s")}
+        zero = np.zeros(self.shape)
+        for k in ("p_dyn", "Y_CO", "m_hemi", "m_cell", "m_lign", "m_char"):
+            st[k] = zero.copy()
         d = self.diag
         return Spread3DResult(
             ros_m_s=self.ros(), n_cells_ignited=int(np.sum(st["T_s"] >= self.T_ign)),
             front_t=list(self.front_t), front_x=list(self.front_x), grid=self.grid,
-            state_final=st, diag_t=d["t"], diag_Tg_max=d["Tg"], diag_Ts_max=d["Ts"],
-            diag_Sp_max=d["Sp"], diag_Qc_max=d["Qc"], diag_omega_max=d["om"],
-            diag_Y_max=d["Y"], diag_n_ign=d["n_ign"], diag_proj_div_max=d["div"],
-            diag_proj_n_iter=d["iters"], n_steps=self.step_no,
-            stop_reason=self.stop_reason)
+            state_final=CellState3D(**st), diag_t=d["t"], diag_Tg_max=d["Tg"],
+            diag_Ts_max=d["Ts"], diag_Sp_max=d["Sp"], diag_Qc_max=d["Qc"],
+            diag_omega_max=d["om"], diag_Y_max=d["Y"], diag_n_ign=d["n_ign"],
+            diag_proj_div_max=d["div"], diag_proj_n_iter=d["iters"],
+            k_turb_final=h64(f["k"]), eps_turb_final=h64(f["eps"]),
+            nu_t_final=h64(f["nu_t"]), n_steps=self.step_no, stop_reason=self.stop_reason)
 
     def close(self):
         if self.proj is not None:
@@ -685,11 +807,16 @@ class Spread3DSolver:
             pass
 
 
-def run_3d_spread(ri_or_path, *, wind_speed_m_s=0.0, **kwargs) -> Spread3DResult:
+def run_3d_spread(ri_or_path, *, wind_speed_m_s=0.0, snapshot_dir=None,
+                  snapshot_interval_s=1.0, **kwargs) -> Spread3DResult:
     """Drop-in for run_3d_spread (spread_3d.py:563-3729): deck + keyword
-    overrides, run until max_wall_time_s (or the new deck key n_steps), and
+    overrides, run until max_wall_time_s (or the new deck key n_steps),
+    optional periodic snapshots (snapshot_dir, snapshot_interval_s), and
     return the result summary."""
     cfg = Spread3DConfig.resolve(ri_or_path, wind_speed_m_s, **kwargs)
     solver = Spread3DSolver(cfg)
-    solver.run()
-    return solver.result()
+    try:
+        solver.run(snapshot_dir=snapshot_dir, snapshot_interval_s=snapshot_interval_s)
+        return solver.result()
+    finally:
+        solver.close()

@@ -540,6 +540,44 @@ int fw_spread_timer_read(double* total_ms, int* launches);
 int fw_spread_front(const fw_grid* g64, const fw_spread_fields* f,
                     fw_spread_state* st, void* stream);
 
+/* ---- snapshot fields (snap_%04d.npz, spread_3d.py:3537-3670) ------------ */
+
+/* compute_phi_flame_from_state (flame_front_3d.py:756-800): signed exact
+ * Euclidean distance between cell centres to the active-flame set
+ * (omega > 1e-3 or (T_g > 1000 and Y_fuel > 1e-3)), negative inside, with
+ * sampling (dz_eff, dy, dx); +1e6 everywhere when no cell is active, -1e6
+ * when every cell is.  dz_eff is the caller's mean of dz_arr (:788).
+ * phi and work: float64 fields of the grid's size.  Axes up to 2048 cells. */
+int fw_phi_flame(const fw_grid* g, const void* omega, const void* T_g,
+                 const void* Y_fuel, double dz_eff, double* phi, double* work,
+                 void* stream);
+
+/* LevelSetFront3D.ahead_band_mask (flame_front_3d.py:387-389) cleared above
+ * the bed top (spread_3d.py:2555-2565): mask = (0 < phi <= band_m) for
+ * k < n_z_top, 0 elsewhere.  phi: the float64 level set. */
+int fw_ahead_band_mask(const fw_grid* g, const double* phi, double band_m,
+                       int n_z_top, uint8_t* mask, void* stream);
+
+/* q_in_at_front_2d (spread_3d.py:3227-3233): the z sum of
+ * compute_q_in_at_front_3d (flame_front_3d.py:440-471) with no Frankman flux
+ * and no burst term, i.e. of q_dom_fwd where mask != 0 (mask may be NULL).
+ * out: float64 (Ny, Nx). */
+int fw_q_in_at_front_2d(const fw_grid* g, const void* q_dom_fwd,
+                        const uint8_t* mask, double* out, void* stream);
+
+/* tau_mix = max(k / max(eps, 1e-12), tau_buoy) (spread_3d.py:2454-2455),
+ * tau_buoy = sqrt(2 max(dx, dy, dz) / g) (:1560-1561). */
+int fw_tau_mix(const fw_grid* g, const void* k, const void* eps,
+               double tau_buoy, void* out, void* stream);
+
+/* Per-cell aggregates of the live bed particles (spread_3d.py:3569-3610):
+ * float32 count, m_solid, m_water, m_char sums and the (m_solid + m_water +
+ * m_char)-weighted T_s (0 where that mass is 0), over each cell's particles
+ * of the index in slot order.  Outputs: float32 grid-sized fields. */
+int fw_bed_snapshot(const fw_grid* g, const fw_bed_particles* particles,
+                    const fw_bed_index* index, float* count, float* m_solid,
+                    float* m_water, float* m_char, float* T_s_avg, void* stream);
+
 /* ---- learned preconditioner (no counterpart in the reference) ----------- */
 /* The reference preconditions its Krylov solve with the separable inverse
  * only (projection_3d.py:756-791).  These entry points add a learned
