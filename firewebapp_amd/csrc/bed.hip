@@ -476,6 +476,18 @@ static int bed_step_impl(const fw_grid* g, const fw_bed_particles* bp,
   return FW_OK;
 }
 
+// Dynamic bed seeding behind the empirical-ROS front (spread_3d.py:3271-3300)
+__global__ void __launch_bounds__(256)
+k_bed_seed(int n, const double* __restrict__ x, const signed char* __restrict__ alive,
+           const double* __restrict__ m_solid, double* __restrict__ T_s, double x_lo,
+           double x_front, double T_ign, double T_seed) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  const double xp = x[p];
+  if (alive[p] == 1 && xp >= x_lo && xp <= x_front && T_s[p] < T_ign && m_solid[p] > 1.0e-9)
+    T_s[p] = T_seed;
+}
+
 }  // namespace fw
 
 using namespace fw;
@@ -561,6 +573,20 @@ int fw_bed_horizontal_conduction(const fw_grid* g, const fw_bed_particles* parti
                        particles->alive, particles->T_s, (const double*)dT_work);
     FW_CHECK_LAUNCH("bed_hcond_particles");
   }
+  return FW_OK;
+}
+
+int fw_bed_seed(const fw_bed_particles* particles, double x_lo, double x_front,
+                double T_ign, double T_seed, void* stream) {
+  FW_REQUIRE(particles && particles->n >= 0, "bed_seed: bad argument");
+  if (particles->n == 0) return FW_OK;
+  FW_REQUIRE(particles->x && particles->alive && particles->m_solid && particles->T_s,
+             "bed_seed: particle arrays missing");
+  hipLaunchKernelGGL(k_bed_seed, dim3((particles->n + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, particles->n, particles->x,
+                     (const signed char*)particles->alive, particles->m_solid,
+                     particles->T_s, x_lo, x_front, T_ign, T_seed);
+  FW_CHECK_LAUNCH("bed_seed");
   return FW_OK;
 }
 

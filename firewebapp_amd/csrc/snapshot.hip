@@ -1,8 +1,9 @@
-// Fields the periodic snapshot writes besides the state itself
-// (spread_3d.py:3537-3670): the flame signed distance, the ahead-band mask
-// and the forward heat flux into it, the mixing time scale and the per-cell
-// aggregates of the bed particles.  They are computed on the device only
-// when a snapshot is due; nothing in the step reads them.
+// Level-set forcing and the fields the periodic snapshot writes besides the
+// state itself (spread_3d.py:3227-3256, :3537-3670): the flame signed
+// distance, the ahead-band mask and the forward heat flux into it, the front
+// speed of the active level set, the mixing time scale and the per-cell
+// aggregates of the bed particles.  With a passive level set nothing in the
+// step reads them and they are computed only when a snapshot is due.
 #include "fw_common.h"
 
 namespace fw {
@@ -194,6 +195,33 @@ k_bed_snapshot(size_t n, int n_seg, const int* __restrict__ seg_ptr,
   tavg[c] = mt_sum > 0.0 ? (float)(tw / fmax(mt_sum, 1.0e-30)) : 0.0f;
 }
 
+// ---- front speed of the active level set ------------------------------------
+// compute_v_n_3d (flame_front_3d.py:474-514) with q_in = q_dom_fwd, then the
+// empirical-ROS blend (spread_3d.py:3252-3255), in the reference's order:
+// E = max(E_sens + E_lat_coef M, 1); v = max(q / E, 0); v *= 1 - w; v += w ros.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_lset_speed(size_t n, const T* __restrict__ q, const T* __restrict__ M, double E_sens,
+             double E_lat_coef, double one_minus_w, double w_ros, int blend,
+             double* __restrict__ v_n) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const double E = fmax(E_sens + E_lat_coef * (double)M[c], 1.0);
+  double v = fmax((double)q[c] / E, 0.0);
+  if (blend) v = v * one_minus_w + w_ros;
+  v_n[c] = v;
+}
+
+// numpy's mean over axis 0: the z sum in k order, divided by Nz
+__global__ void __launch_bounds__(256)
+k_v_n_2d(size_t nxy, int nz, const double* __restrict__ v, double* __restrict__ out) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= nxy) return;
+  double s = 0.0;
+  for (int k = 0; k < nz; ++k) s += v[(size_t)k * nxy + c];
+  out[c] = s / (double)nz;
+}
+
 static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace fw
@@ -295,6 +323,36 @@ int fw_bed_snapshot(const fw_grid* g, const fw_bed_particles* particles,
                      (const signed char*)particles->alive, count, m_solid, m_water, m_char,
                      T_s_avg);
   FW_CHECK_LAUNCH("bed_snapshot");
+  return FW_OK;
+}
+
+int fw_lset_speed(const fw_grid* g, const void* q_dom_fwd, const void* M_local,
+                  double E_sens, double E_lat_coef, double emp_w, double emp_ros,
+                  double* v_n, void* stream) {
+  FW_REQUIRE(g && q_dom_fwd && M_local && v_n, "lset_speed: null argument");
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  const int blend = emp_w > 0.0 ? 1 : 0;
+  // the reference forms (1 - w) and w * ros as Python floats first
+  const double omw = 1.0 - emp_w, wr = emp_w * emp_ros;
+  hipStream_t s = (hipStream_t)stream;
+  if (g->dtype == FW_F64)
+    hipLaunchKernelGGL(k_lset_speed<double>, dim3(nb256(n)), dim3(256), 0, s, n,
+                       (const double*)q_dom_fwd, (const double*)M_local, E_sens, E_lat_coef,
+                       omw, wr, blend, v_n);
+  else
+    hipLaunchKernelGGL(k_lset_speed<float>, dim3(nb256(n)), dim3(256), 0, s, n,
+                       (const float*)q_dom_fwd, (const float*)M_local, E_sens, E_lat_coef,
+                       omw, wr, blend, v_n);
+  FW_CHECK_LAUNCH("lset_speed");
+  return FW_OK;
+}
+
+int fw_v_n_2d(const fw_grid* g, const double* v_n, double* out, void* stream) {
+  FW_REQUIRE(g && v_n && out, "v_n_2d: null argument");
+  const size_t nxy = (size_t)g->nx * g->ny;
+  hipLaunchKernelGGL(k_v_n_2d, dim3(nb256(nxy)), dim3(256), 0, (hipStream_t)stream, nxy,
+                     g->nz, v_n, out);
+  FW_CHECK_LAUNCH("v_n_2d");
   return FW_OK;
 }
 

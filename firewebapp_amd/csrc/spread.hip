@@ -1,13 +1,13 @@
 // Native driver of one outer time step (spread_3d.py:1897-3683 of the
-// reference), for the Lagrangian-bed / k-epsilon / EDC / DOM / passive
-// level-set configuration.
+// reference), for the Lagrangian-bed / k-epsilon / EDC / DOM configuration
+// with a passive or an active level set.
 //
 // The whole step is issued from here, so the host cost per kernel is one
 // hipLaunchKernelGGL instead of an interpreter round trip.  Host-side
 // decisions are the reference's scalar ones: dt from the previous step's
 // fused reduction, the projection's divergence loop, the turbulent
 // diffusion sub-step count from max nu_t, the level-set front rows after
-// each reinitialisation, and the exits.  Ping-pong fields are swapped by
+// each evolve or reinitialisation, and the exits.  Ping-pong fields are swapped by
 // exchanging pointers inside fw_spread_fields; the caller re-reads them.
 #include <math.h>
 #include <string.h>
@@ -339,8 +339,26 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
       FW_CHECK_LAUNCH("clamp");
     }
   }
-  // passive level set: reinitialise every 10th step, float64 (:3204-3262)
+  // level set (:3204-3300).  Active: the ahead band of the step's level set
+  // and the DOM forward flux into it (:2555-2565, :2591; the intensities and
+  // phi do not change between there and here), the front speed with the
+  // empirical blend (:3227-3256) and the evolve (:3260).  Passive: v_n = 0,
+  // the evolve is the identity.  Both: reinitialise every 10th step, float64.
   st->lset_step += 1;
+  bool moved = false;
+  if (p->lset_active) {
+    FW_REQUIRE(f->lset_mask && f->q_dom_fwd && f->v_n, "spread_step: active level set "
+               "needs lset_mask, q_dom_fwd and v_n");
+    TRY(fw_ahead_band_mask(g64, f->phi, p->band_m, nzb, f->lset_mask, stream));
+    TRY(fw_dom_forward_flux(dom, f->lset_mask, f->q_dom_fwd, stream));
+    TRY(fw_lset_speed(g, f->q_dom_fwd, f->M_local, p->E_sens, p->E_lat_coef, p->emp_w,
+                      p->emp_ros, f->v_n, stream));
+    TRY(fw_levelset_update(g64, f->phi, nullptr, f->v_n, f->phi_tmp, dt, 0, stream));
+    double* tp = f->phi;
+    f->phi = f->phi_tmp;
+    f->phi_tmp = tp;
+    moved = true;
+  }
   if (st->lset_step % REINIT_INTERVAL == 0) {
     const size_t nb = n * sizeof(double);
     FW_CHECK_HIP(hipMemcpyAsync(f->phi0, f->phi, nb, hipMemcpyDeviceToDevice, s));
@@ -351,7 +369,18 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
       f->phi = f->phi_tmp;
       f->phi_tmp = tp;
     }
-    TRY(fw_spread_front(g64, f, st, stream));
+    moved = true;
+  }
+  if (moved) TRY(fw_spread_front(g64, f, st, stream));
+  // dynamic seeding behind the empirical-ROS front, every 50th step
+  // (:3271-3300); front_exit_x is lset.front_x(k=1, j=Ny/2) after the evolve
+  if (p->lset_active && p->emp_w > 0.0) {
+    st->seed_count += 1;
+    if (st->seed_count >= 50) {
+      st->seed_count = 0;
+      TRY(fw_bed_seed(particles, p->x_lo_bed, st->front_exit_x, p->T_ign, p->seed_T_s,
+                      stream));
+    }
   }
   // exits (:3414-3436), before the EoS as in the reference
   if (flame_n == 0 && t > p->ignition_duration + T_BOOTSTRAP_S) {
@@ -388,7 +417,14 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
     st->last_advance_t = t;
   }
   st->t = t + dt;
-  if (st->front_x_last > p->Lx * 0.85) st->stop = 3;
+  // exits after the snapshot (:3673-3683): domain end, then the stall of an
+  // active level set
+  if (st->front_x_last > p->Lx * 0.85) {
+    st->stop = 3;
+  } else if (p->lset_active && st->t - st->last_advance_t > p->stall_window &&
+             st->t > p->ignition_duration) {
+    st->stop = 4;
+  }
   return FW_OK;
 }
 

@@ -15,8 +15,9 @@ and writes the file while the next steps run.
 
 Fields that are identically zero on this configuration are written as zeros,
 as the reference writes them: q_frankman_3d (the Frankman flux is disabled,
-spread_3d.py:2586), v_n_2d (passive level set, :3234-3240) and the six ft_*
-tendril inventories (finney_tendril_enable is false).
+spread_3d.py:2586), v_n_2d with a passive level set (:3234-3240) and the six
+ft_* tendril inventories (finney_tendril_enable is false).  With an active
+level set the band, the forward flux and v_n are the step's own.
 """
 from __future__ import annotations
 
@@ -77,6 +78,8 @@ class SnapshotWriter:
         # spread_3d.py:1559-1561 and :1601
         self.tau_buoy = math.sqrt(2.0 * max(g.dx, g.dy, g.dz) / _G)
         self.band = max(g.dx, DX_VN_BAND_M)
+        self.active = not solver.cfg.level_set_passive
+        self.v_n_2d = torch.zeros((g.Ny, g.Nx), **f64)
         self._pool = None
         self._workers = max(int(workers), 1)
         nslot = self._workers + 1
@@ -86,9 +89,10 @@ class SnapshotWriter:
         self._const_cache = None
 
     def _band_phi(self):
-        """The level set the step's band mask was taken from: the one before
-        that step's reinitialisation (spread_3d.py:2555 precedes :3260-3261);
-        phi0 holds it on a reinitialising step, phi is unchanged otherwise."""
+        """The passive level set the step's band mask was taken from: the one
+        before that step's reinitialisation (spread_3d.py:2555 precedes
+        :3260-3261); phi0 holds it on a reinitialising step, phi is unchanged
+        otherwise."""
         sv = self.sv
         if sv.state.lset_step % LEVELSET_REINIT_INTERVAL == 0:
             return sv.phi0
@@ -105,10 +109,16 @@ class SnapshotWriter:
                   ptr(self.tau_mix), s)
         _lib.call("fw_phi_flame", sv.dg.ref, ptr(f["omega"]), ptr(f["T_g"]),
                   ptr(f["Y_fuel"]), self.dz_eff, ptr(self.phi_flame), ptr(self.edt_work), s)
-        _lib.call("fw_ahead_band_mask", sv.dg.ref, ptr(self._band_phi()), self.band,
-                  int(g.n_z_bed), ptr(self.mask), s)
-        _lib.call("fw_dom_forward_flux", sv.dom_h, ptr(self.mask), ptr(self.q_fwd), s)
-        _lib.call("fw_q_in_at_front_2d", sv.dg.ref, ptr(self.q_fwd), ptr(self.mask),
+        if self.active:
+            # the step's own band, forward flux and front speed
+            mask, q_fwd = sv.lset_mask, sv.q_dom_fwd
+            _lib.call("fw_v_n_2d", sv.dg.ref, ptr(sv.v_n), ptr(self.v_n_2d), s)
+        else:
+            mask, q_fwd = self.mask, self.q_fwd
+            _lib.call("fw_ahead_band_mask", sv.dg.ref, ptr(self._band_phi()), self.band,
+                      int(g.n_z_bed), ptr(mask), s)
+            _lib.call("fw_dom_forward_flux", sv.dom_h, ptr(mask), ptr(q_fwd), s)
+        _lib.call("fw_q_in_at_front_2d", sv.dg.ref, ptr(q_fwd), ptr(mask),
                   ptr(self.q_in_2d), s)
         bp = self.bp
         _lib.call("fw_bed_snapshot", sv.dg.ref, C.byref(sv.P.c), C.byref(sv.ix.c),
@@ -116,10 +126,12 @@ class SnapshotWriter:
                   ptr(bp["m_char"]), ptr(bp["T_s_avg"]), s)
         dev = {k: f[src].float() for k, src in _STATE_SRC}
         dev.update(tau_mix=self.tau_mix.float(), phi_bed=sv.phi.float(),
-                   phi_flame=self.phi_flame.float(), q_dom_fwd_3d=self.q_fwd.float(),
+                   phi_flame=self.phi_flame.float(), q_dom_fwd_3d=q_fwd.float(),
                    q_in_at_front_2d=self.q_in_2d.float(), bp_count=bp["count"],
                    bp_m_solid=bp["m_solid"], bp_m_water=bp["m_water"],
                    bp_m_char=bp["m_char"], bp_T_s_avg=bp["T_s_avg"])
+        if self.active:
+            dev["v_n_2d"] = self.v_n_2d.float()
         host = dict(t=np.float64(sv.t), step=np.int64(sv.step_no),
                     front_x=np.float64(sv.front_x[-1] if sv.front_x else 0.0))
         return dev, host

@@ -1,7 +1,8 @@
 """GPU driver of the Tier-3 spread time step: ``run_3d_spread`` of the
 reference (spread_3d.py:563-3729) for the Lagrangian-bed / k-epsilon / EDC /
-DOM / passive-level-set configuration, with every field resident in device
-memory.
+DOM configuration with a passive or an active level set
+(``level_set_passive``, default false as in the reference), every field
+resident in device memory.
 
 Precision: the grid fields are float32 by default (deck key
 ``outdoor.dtype = float64`` selects the float64 parity build); bed particles,
@@ -15,13 +16,15 @@ step plus the projection and nu_t scalars.
 Work the reference computes but never consumes on this path is skipped; the
 skip is exact because nothing downstream reads it: the O2-supply limiter and
 omega_max_T (ignored by the EDC closure), tau_mix, burning_mask, Q_pyro,
-q_dom_fwd / q_in_at_front (passive level set), update_cell_age (read only by
-the ebu_bootstrap closure), the level-set evolve with v_n = 0 (the
-identity), and the distance transforms of compute_phi_flame_from_state: the
-flame-body mask with band 0 is exactly the `active` predicate
-(flame_front_3d.py:756-813), and its count for step n is the count of the
-state the previous step ended with (omega, T_g and Y_fuel are not written
-between the end of a step and that point of the next one).
+q_dom_fwd / q_in_at_front with a passive level set, update_cell_age (read
+only by the ebu_bootstrap closure), the passive level set's evolve with
+v_n = 0 (the identity), and the distance transforms of
+compute_phi_flame_from_state: the flame-body mask with band 0 is exactly the
+`active` predicate (flame_front_3d.py:756-813), and its count for step n is
+the count of the state the previous step ended with (omega, T_g and Y_fuel
+are not written between the end of a step and that point of the next one).
+The snapshot writer computes the skipped fields it writes when a snapshot is
+due.
 """
 from __future__ import annotations
 
@@ -50,6 +53,10 @@ _R_AIR = 8.314 / 0.029
 _G = 9.81
 _RHO_PARTICLE = 500.0
 L_BURNOUT_M = 0.50          # flame_front_3d.py:62
+DX_VN_BAND_M = 0.20         # flame_front_3d.py:64
+L_VAP_WATER = 2.26e6        # flame_front_3d.py:437
+_CP_SOLID_VN = 1300.0       # spread_3d.py:80, the cp_s of compute_v_n_3d
+_STALL_WINDOW_S = 3.0       # spread_3d.py:1800
 
 
 class FwSpreadFields(C.Structure):
@@ -62,7 +69,7 @@ class FwSpreadFields(C.Structure):
             "u_inlet", "v_inlet", "w_inlet", "sigma_x", "k_wall", "eps_wall", "T_soil",
             "q_in_soil", "soil_dz", "soil_d_above", "soil_d_below", "phi", "phi_tmp",
             "phi0", "red_work", "red_out", "red_host", "nu_work", "contrib", "dT_work",
-            "counters", "edc_work")])
+            "counters", "edc_work", "lset_mask", "q_dom_fwd", "v_n")])
 
 
 class FwSpreadParams(C.Structure):
@@ -73,7 +80,10 @@ class FwSpreadParams(C.Structure):
         "ignition_q_mult", "bed_x_end", "Lx", "x_step", "proj_div_tol", "rho_solid_true",
         "cp_solid", "soil_k", "soil_rho", "soil_cp", "soil_eps")] + [
         (n, C.c_int32) for n in ("n_soil", "i_src0", "i_src1", "proj_max_iter",
-                                 "dom_every", "reserved")]
+                                 "dom_every", "reserved")] + [
+        (n, C.c_double) for n in ("band_m", "E_sens", "E_lat_coef", "emp_w", "emp_ros",
+                                  "seed_T_s", "x_lo_bed", "stall_window")] + [
+        (n, C.c_int32) for n in ("lset_active", "reserved2")]
 
 
 class FwSpreadState(C.Structure):
@@ -83,7 +93,7 @@ class FwSpreadState(C.Structure):
         "front_x_last", "front_t_new", "front_x_new", "proj_div")] + [
         ("red", C.c_double * 14)] + [(n, C.c_int32) for n in (
             "step_no", "lset_step", "extinct_count", "stop", "front_appended", "proj_n",
-            "pcg_iters", "reserved")]
+            "pcg_iters", "reserved", "seed_count", "reserved2")]
 
 
 # grid-type fields the native step reads, writes and swaps
@@ -92,7 +102,7 @@ _GRID_SLOTS = ("rho", "rho_prev", "u", "v", "w", "T_g", "T_s", "Y_fuel", "Y_O2",
                "div_target", "Q_comb", "q_rad", "q_rad_gas", "Q_ext", "M_local", "T_s_old",
                "tmp1", "tmp2", "tmp3")
 _SRC_NAMES = ("Sp", "Sd", "Qp", "Qd", "YF", "Qch", "Qsm", "Qgc")
-_STOPS = {1: "extinct", 2: "front_exit", 3: "domain_end"}
+_STOPS = {1: "extinct", 2: "front_exit", 3: "domain_end", 4: "stalled"}
 
 
 def _adv_gas_energy(T_g, u, v, w, dt, dx, dy, dz_arr, d_face_above,
@@ -126,11 +136,9 @@ _REQUIRED = (
     ("boundary_condition_kind", "outdoor_wind", "outdoor_wind"),
     ("chemistry_family", "biomass", "biomass"),
     ("lagrangian_bed_enable", True, False),
-    ("level_set_passive", True, False),
     ("ignition_T_pin_enable", False, False),
     ("edc_extinction_enable", False, False),
     ("finney_tendril_enable", False, False),
-    ("empirical_ros_enable", False, False),
     ("moisture_jump_enable", False, False),
 )
 _OUTDOOR_OFF = ("sem_enable", "fuel_pert_enable", "volume_weighted_projection",
@@ -189,6 +197,10 @@ class Spread3DConfig:
     particle_jitter: float
     particle_jitter_seed: int
     spin_up_s: float
+    level_set_passive: bool = True
+    empirical_ros_enable: bool = False
+    empirical_ros_m_s: float = 0.0
+    empirical_blend_w: float = 0.0
 
     @classmethod
     def resolve(cls, ri_or_path, wind_speed_m_s=0.0, **kw):
@@ -297,6 +309,21 @@ class Spread3DConfig:
         y_bc = kw.pop("y_bc", "periodic")
         if y_bc != "periodic":
             raise ValueError("only y_bc='periodic' is implemented on the GPU step")
+        # level-set mode and the empirical-ROS blend (spread_3d.py:1075-1090,
+        # :1246-1272); the ROS and its weight are fixed for the run
+        passive = bool(deck_first("level_set_passive", False))
+        emp_on = bool(deck_first("empirical_ros_enable", False))
+        emp_ros, emp_w = 0.0, 0.0
+        if emp_on:
+            from . import empirical_ros as _er
+            model = str(deck_first("empirical_ros_model", "cheney_eq6"))
+            a_ch = float(deck_first("empirical_ros_a_ch", 0.406))
+            u_thr = float(deck_first("empirical_ros_u_threshold_m_s", 1.4))
+            width = float(deck_first("empirical_ros_blend_width_m_s", 0.5))
+            age = float(deck_first("empirical_ros_age_yr", 10.0))
+            emp_ros = _er.evaluate_empirical_ros(model, float(wind_speed_m_s), moisture, a_ch,
+                                                 age_yr=age)
+            emp_w = _er.blend_resolved_empirical(float(wind_speed_m_s), u_thr, width)
         out = cls(
             wind_speed_m_s=float(wind_speed_m_s), Lx=Lx, Ly=Ly, Lz=Lz, dx=dx, dy=dy,
             n_z_bed=n_z_bed, mesh_kwargs=mesh, bed_x_start=bed_x_start,
@@ -331,6 +358,8 @@ class Spread3DConfig:
             particle_jitter=float(deck_first("particle_jitter", 0.0)),
             particle_jitter_seed=int(deck_first("particle_jitter_seed", 0)),
             spin_up_s=float(ocfg.spin_up_s),
+            level_set_passive=passive, empirical_ros_enable=emp_on,
+            empirical_ros_m_s=float(emp_ros), empirical_blend_w=float(emp_w),
         )
         if kw:
             raise TypeError(f"unsupported keyword arguments: {sorted(kw)}")
@@ -626,6 +655,26 @@ class Spread3DSolver:
                         ("contrib", self.contrib), ("dT_work", self.dT_work),
                         ("counters", self.counters), ("edc_work", self.edc_work)):
             setattr(fs, name, t.data_ptr())
+        # active level set: the band, its forward flux and the front speed,
+        # kept across the step for the snapshot writer
+        cfg_active = not cfg.level_set_passive
+        if cfg_active:
+            self.lset_mask = torch.zeros(self.shape, dtype=torch.uint8, device=dev)
+            self.q_dom_fwd = torch.zeros(self.shape, dtype=self.dtype, device=dev)
+            self.v_n = torch.zeros(self.shape, **f64)
+            for name, t in (("lset_mask", self.lset_mask), ("q_dom_fwd", self.q_dom_fwd),
+                            ("v_n", self.v_n)):
+                setattr(fs, name, t.data_ptr())
+        pr = self.params
+        pr.band_m = max(g.dx, DX_VN_BAND_M)
+        pr.E_sens = cfg.rho_b * _CP_SOLID_VN * cfg.h_bed * (self.T_ign - T_amb)
+        pr.E_lat_coef = cfg.rho_b * cfg.h_bed * L_VAP_WATER * 1.0
+        pr.emp_w = cfg.empirical_blend_w if cfg.empirical_ros_enable else 0.0
+        pr.emp_ros = cfg.empirical_ros_m_s if cfg.empirical_ros_enable else 0.0
+        pr.seed_T_s = cfg.solid_phase_T_s_K
+        pr.x_lo_bed = float(g.x_mid[i0] - 0.5 * g.dx)
+        pr.stall_window = _STALL_WINDOW_S
+        pr.lset_active = 1 if cfg_active else 0
         self._by_ptr = {t.data_ptr(): t for t in list(f.values()) +
                         [self.phi, self.phi_tmp, self.phi0]}
         self.state = FwSpreadState()
@@ -689,7 +738,7 @@ class Spread3DSolver:
     # -- one outer step (spread_3d.py:1897-3683) --------------------------------
     def step(self):
         """Advance one outer step in the native driver; returns the stop reason
-        (``extinct``, ``front_exit``, ``domain_end``) or None."""
+        (``extinct``, ``front_exit``, ``domain_end``, ``stalled``) or None."""
         st = self.state
         rc = _lib.lib().fw_spread_step(
             self.dg.ref, self.dg64.ref, C.byref(self.fields), C.byref(self.params),
@@ -706,8 +755,8 @@ class Spread3DSolver:
             self.front_t.append(st.front_t_new)
             self.front_x.append(st.front_x_new)
         self._append_diag(st.t, st.red, st.proj_div, st.proj_n)
-        if st.stop == 3:
-            self.stop_reason = _STOPS[3]
+        if st.stop in (3, 4):
+            self.stop_reason = _STOPS[st.stop]
             return self.stop_reason
         return None
 

@@ -485,6 +485,10 @@ typedef struct fw_spread_fields {
   double* dT_work;                   /* N */
   int32_t* counters;                 /* 2 */
   int32_t* edc_work;                 /* fw_edc_workspace_ints(g) */
+  /* active level set only (may be NULL when lset_active == 0): */
+  uint8_t* lset_mask;                /* N, the ahead band of the step */
+  void* q_dom_fwd;                   /* N, grid type: DOM forward flux in the band */
+  double* v_n;                       /* N, float64 front speed */
 } fw_spread_fields;
 
 /* Run constants resolved from the deck (spread_3d.py:959-1896). */
@@ -495,6 +499,16 @@ typedef struct fw_spread_params {
   double rho_solid_true, cp_solid;   /* bed particles (horizontal conduction) */
   double soil_k, soil_rho, soil_cp, soil_eps;
   int32_t n_soil, i_src0, i_src1, proj_max_iter, dom_every, reserved;
+  /* active level set (level_set_passive = false, spread_3d.py:3227-3300) */
+  double band_m;        /* ahead band width max(dx, DX_VN_BAND_M) (:1601) */
+  double E_sens;        /* rho_b cp_s h_bed (T_ign - T_amb) (flame_front_3d.py:503) */
+  double E_lat_coef;    /* rho_b h_bed L_vap f_dry (:507), times M_local */
+  double emp_w;         /* empirical-ROS blend weight (0 = off) */
+  double emp_ros;       /* empirical ROS [m/s] */
+  double seed_T_s;      /* T_s of the dynamic seeding (solid-phase ignition T) */
+  double x_lo_bed;      /* x of the bed's leading face */
+  double stall_window;  /* s without front advance before the stall exit */
+  int32_t lset_active, reserved2;
 } fw_spread_params;
 
 /* Host-side scalars carried from step to step. */
@@ -506,16 +520,19 @@ typedef struct fw_spread_state {
   double proj_div;                   /* max|div(u) - target| after the projection loop */
   double red[14];                    /* end-of-step fw_step_reduce slots */
   int32_t step_no, lset_step, extinct_count;
-  int32_t stop;                      /* 0 running, 1 extinct, 2 front exit, 3 domain end */
+  int32_t stop;                      /* 0 running, 1 extinct, 2 front exit, 3 domain end,
+                                      4 stalled (active level set) */
   int32_t front_appended, proj_n, pcg_iters, reserved;
+  int32_t seed_count, reserved2;     /* steps since the last dynamic seeding */
 } fw_spread_state;
 
 /* One outer step of run_3d_spread for the Lagrangian-bed / k-epsilon / EDC /
- * DOM / passive-level-set configuration.  g: the fields' grid; g64: the same
- * grid with dtype FW_F64 (level set).  proj / dom: handles from
- * fw_proj_create / fw_dom_create.  The dt, NaN halt, projection loop, exits,
- * front tracking and diagnostics of the reference live in `st`.  Returns
- * FW_E_NONFINITE where the reference raises on NaN/Inf. */
+ * DOM configuration, passive or active level set (p->lset_active).  g: the
+ * fields' grid; g64: the same grid with dtype FW_F64 (level set).  proj /
+ * dom: handles from fw_proj_create / fw_dom_create.  The dt, NaN halt,
+ * projection loop, exits, front tracking and diagnostics of the reference
+ * live in `st`.  Returns FW_E_NONFINITE where the reference raises on
+ * NaN/Inf. */
 int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                    const fw_spread_params* p, fw_spread_state* st, void* proj,
                    void* dom, const fw_bed_particles* particles,
@@ -577,6 +594,28 @@ int fw_tau_mix(const fw_grid* g, const void* k, const void* eps,
 int fw_bed_snapshot(const fw_grid* g, const fw_bed_particles* particles,
                     const fw_bed_index* index, float* count, float* m_solid,
                     float* m_water, float* m_char, float* T_s_avg, void* stream);
+
+/* ---- active level set (spread_3d.py:3227-3300) --------------------------- */
+
+/* compute_v_n_3d (flame_front_3d.py:474-514) on q_in = q_dom_fwd (the
+ * Frankman flux is zero and there is no burst term), then the empirical
+ * blend (spread_3d.py:3252-3255):
+ *   E = max(E_sens + E_lat_coef M_local, 1); v = max(q / E, 0);
+ *   v = v (1 - emp_w) + emp_w emp_ros   when emp_w > 0.
+ * q_dom_fwd, M_local: grid type; v_n: float64. */
+int fw_lset_speed(const fw_grid* g, const void* q_dom_fwd, const void* M_local,
+                  double E_sens, double E_lat_coef, double emp_w, double emp_ros,
+                  double* v_n, void* stream);
+
+/* v_n_2d of the snapshot (spread_3d.py:3645): the z mean of v_n, float64
+ * (Ny, Nx). */
+int fw_v_n_2d(const fw_grid* g, const double* v_n, double* out, void* stream);
+
+/* Dynamic bed seeding behind the empirical-ROS front (spread_3d.py:3271-3300):
+ * T_s = T_seed for particles with alive == 1, x_lo <= x <= x_front,
+ * T_s < T_ign and m_solid > 1e-9. */
+int fw_bed_seed(const fw_bed_particles* particles, double x_lo, double x_front,
+                double T_ign, double T_seed, void* stream);
 
 /* ---- learned preconditioner (no counterpart in the reference) ----------- */
 /* The reference preconditions its Krylov solve with the separable inverse

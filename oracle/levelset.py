@@ -7,6 +7,8 @@ Restates physics_3d/flame_front_3d.py of the reference:
 * ``reinitialize``             flame_front_3d.py:339-355 (5 sub-steps,
                                dtau = 0.5 min(dx, dy, dz_min))
 * ``front_x``                  flame_front_3d.py:391-402
+* ``compute_v_n_3d``           flame_front_3d.py:474-514 (front speed from the
+                               heat flux into the band, moisture-aware)
 * ``flame_active``             the `active` predicate of
                                compute_phi_flame_from_state (:756-800); the
                                flame-body mask with band 0 (:803-813) equals it
@@ -23,6 +25,7 @@ PHI_INIT_UNBURNED = 100.0
 LEVELSET_REINIT_INTERVAL = 10
 REINIT_SUBSTEPS = 5
 DT_REINIT_FRAC = 0.5
+L_VAP_WATER = 2.26e6
 OMEGA_MIN_FLAME = 1.0e-3
 T_PLUME_MIN = 1000.0
 Y_F_MIN_PLUME = 1.0e-3
@@ -119,3 +122,14 @@ def front_x(phi, k, j, dx):
 
 def flame_active(omega, T_g, Y_fuel):
     return (omega > OMEGA_MIN_FLAME) | ((T_g > T_PLUME_MIN) & (Y_fuel > Y_F_MIN_PLUME))
+
+
+def compute_v_n_3d(q_in, rho_b, cp_s, h_bed, T_ign, T_amb, M_local=None, L_vap=L_VAP_WATER,
+                   f_dry_to_ignite=1.0):
+    E_sens = rho_b * cp_s * h_bed * (T_ign - T_amb)
+    if M_local is None:
+        E = np.full_like(q_in, E_sens)
+    else:
+        E = E_sens + rho_b * h_bed * L_vap * f_dry_to_ignite * M_local
+    E = np.maximum(E, 1.0)
+    return np.maximum(q_in / E, 0.0)

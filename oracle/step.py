@@ -5,7 +5,7 @@ Follows run_3d_spread of the reference (spread_3d.py:1126-1896 setup,
 :1897-3683 loop body, :3743-3824 front tracking and ROS) for the
 configuration the GPU driver implements: k-epsilon (any turbulence_model
 other than smagorinsky), EDC, DOM every ``dom_subcycle_every`` steps,
-Lagrangian bed particles, passive level set, outdoor log-law inflow,
+Lagrangian bed particles, passive or active level set, outdoor log-law inflow,
 periodic y.  The kernels are the oracle's own restatements (``oracle.bed``,
 ``oracle.dom``, ``oracle.edc``, ``oracle.levelset``, ``oracle.projection``,
 ``oracle.transport``).
@@ -30,6 +30,7 @@ from . import bed as Bd
 from . import dom as Dm
 from . import edc as Ed
 from . import levelset as LS
+from . import snapshot as Sn
 from . import projection as Pj
 from . import transport as Tr
 
@@ -48,6 +49,9 @@ EPS_MIN = 1.0e-8
 T_BOOTSTRAP_S = 2.0        # flame_front_3d.py
 N_SUB = 10
 T_CAP = 10_000.0
+CP_SOLID_VN = 1300.0       # spread_3d.py:80, cp_s of compute_v_n_3d
+Sn_DX_VN_BAND_M = 0.20     # flame_front_3d.py:64
+STALL_WINDOW_S = 3.0       # spread_3d.py:1800
 
 
 def midflame_wind_speed(U_10m, terrain="open"):
@@ -138,6 +142,8 @@ class OracleSpread3D:
         if g.n_z_bed > 0 and s1 > s0:
             self.phi = LS.initialize_source_patch(sh, s0, s1, g.n_z_bed - 1, g.x_mid, g.dx)
         self.lset_step = 0
+        self.seed_count = 0
+        self.v_n = np.zeros(sh)
         # solvers (:1719-1780)
         self.proj = Pj.ProjectionSolver3D(g.Nz, g.Ny, g.Nx, g.dy, g.dx, g.dz_arr,
                                           g.d_face_above, g.d_face_below,
@@ -300,12 +306,39 @@ class OracleSpread3D:
             np.maximum(self.T_g, T_amb, out=self.T_g)
             np.minimum(self.T_s, T_CAP, out=self.T_s)
             np.maximum(self.T_s, T_amb, out=self.T_s)
-        # level set (passive: v_n = 0 evolve is the identity), exits (:3204-3440)
+        # level set (:3204-3300; passive: the v_n = 0 evolve is the identity),
+        # exits (:3413-3440)
         if nzb > 0:
             self.lset_step += 1
+            if not cfg.level_set_passive:
+                # band of the step's level set and its DOM forward flux
+                # (:2555-2565, :2591), front speed (:3227-3256), evolve (:3260)
+                mask = Sn.ahead_band_mask(self.phi, max(dx, Sn_DX_VN_BAND_M), nzb)
+                q = Sn.q_dom_fwd_at_band(self.dom.Omega, self.dom.weights, self.dom.I_set,
+                                         mask)
+                q_in = np.where(mask, 0.0 + q, 0.0)
+                v = LS.compute_v_n_3d(q_in, cfg.rho_b, CP_SOLID_VN, cfg.h_bed, self.T_ign,
+                                      T_amb, M_local=self.M_local)
+                w = cfg.empirical_blend_w if cfg.empirical_ros_enable else 0.0
+                if w > 0.0:
+                    v *= (1.0 - w)
+                    v += w * cfg.empirical_ros_m_s
+                self.v_n = v
+                self.phi = LS.evolve(self.phi, dt, v, dx, dy, dz)
             if self.lset_step % LS.LEVELSET_REINIT_INTERVAL == 0:
                 self.phi = LS.reinitialize(self.phi, dx, dy, dz)
             fx_exit = LS.front_x(self.phi, 1, g.Ny // 2, dx)
+            # dynamic seeding behind the empirical-ROS front (:3271-3300)
+            if (not cfg.level_set_passive and cfg.empirical_ros_enable
+                    and cfg.empirical_blend_w > 0.0):
+                self.seed_count += 1
+                if self.seed_count >= 50:
+                    self.seed_count = 0
+                    b = self.buf
+                    x_lo = g.x_mid[self.i_bed[0]] - 0.5 * dx
+                    sel = ((b["alive"] == 1) & (b["x"] >= x_lo) & (b["x"] <= fx_exit)
+                           & (b["T_s"] < self.T_ign) & (b["m_solid"] > 1.0e-9))
+                    b["T_s"][sel] = cfg.solid_phase_T_s_K
             if flame_n == 0 and t > cfg.ignition_duration_s + T_BOOTSTRAP_S:
                 self.extinct += 1
                 if self.extinct > 50:
@@ -347,6 +380,11 @@ class OracleSpread3D:
         d["iters"].append(proj_n)
         if self.front_x[-1] > g.Lx * 0.85:
             self.stop_reason = "domain_end"
+            return self.stop_reason
+        # stall of an active level set (:3675-3683)
+        if (not cfg.level_set_passive and self.t - self.last_advance_t > STALL_WINDOW_S
+                and self.t > cfg.ignition_duration_s):
+            self.stop_reason = "stalled"
             return self.stop_reason
         return None
 

@@ -62,3 +62,13 @@ def test_front_x_linear_crossing():
     # zero crossing half-way between cells 1 and 2 -> x = (1 + 0.5 + 0.5) dx
     assert LS.front_x(phi, 1, 0, 0.1) == pytest.approx(0.2, abs=1e-15)
     assert LS.front_x(np.ones((1, 1, 4)), 0, 0, 0.1) == math.inf
+
+
+@pytest.mark.parametrize("c", cases(SEC), ids=case_ids(SEC))
+def test_v_n_golden(c):
+    """compute_v_n_3d, dry and moisture-aware, on the golden q_in."""
+    q = field(c, "q_in")
+    dry = LS.compute_v_n_3d(q, 1.07, 1850.0, 0.25, 600.0, 300.0)
+    wet = LS.compute_v_n_3d(q, 1.07, 1850.0, 0.25, 600.0, 300.0, M_local=field(c, "M_local"))
+    assert np.array_equal(dry, field(c, "v_n_dry"))
+    assert np.array_equal(wet, field(c, "v_n_wet"))
