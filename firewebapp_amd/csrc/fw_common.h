@@ -31,11 +31,12 @@ int density_feature(const float* rho, float* q, size_t n, hipStream_t s);
 
 // Events on which the next species launch records its own start and stop
 // (the dispatch's timestamps, as a kernel trace sees them), or null.  Set by
-// the step driver's optional timer around one call, cleared after it.
+// the step driver's optional timer around one call, cleared after it; one
+// per host thread, so steps driven from several threads do not share them.
 struct KernelEvents {
   hipEvent_t start = nullptr, stop = nullptr;
 };
-extern KernelEvents g_species3_events;
+extern thread_local KernelEvents g_species3_events;
 
 inline Launch cell_launch(const fw_grid* g) {
   int bx = 64;

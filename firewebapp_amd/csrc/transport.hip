@@ -601,7 +601,7 @@ static int species_impl(const fw_grid* g, const void* Y, void* Yo,
   return FW_OK;
 }
 
-KernelEvents g_species3_events;
+thread_local KernelEvents g_species3_events;
 
 template <typename T>
 static int species3_impl(const fw_grid* g, const void* const* Y, void* const* Yo,

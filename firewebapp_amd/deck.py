@@ -13,6 +13,11 @@ Keys this build adds (all additive; absent keys reproduce the reference):
 ``moisture_min`` / ``moisture_max`` / ``moisture_seed`` (per-seed moisture
 draw), ``particle_jitter`` / ``particle_jitter_seed`` (per-particle +-load
 jitter), ``dtype`` (float32 / float64).
+
+A new ``batch.*`` section (raw value strings, keys lower-cased) describes a
+sweep of cases over the deck; ``firewebapp_amd.batch`` expands it into the
+batch table.  The reference parser ignores unknown sections, so a batch deck
+is still a valid single-case deck for it.
 """
 from __future__ import annotations
 
@@ -48,6 +53,7 @@ def _parse_optional_float(text: str) -> Optional[float]:
 class RomInputs:
     """The part of the reference's RomInputs the Tier-3 step reads."""
     outdoor_overrides: Dict[str, Any] = field(default_factory=dict)
+    batch_overrides: Dict[str, str] = field(default_factory=dict)
     other: Dict[str, str] = field(default_factory=dict)
 
 
@@ -76,6 +82,9 @@ def parse_deck_text(text: str) -> RomInputs:
                 inputs.outdoor_overrides[fld] = val_opt
                 continue
             inputs.outdoor_overrides[fld] = value.strip()
+            continue
+        if key_l.startswith("batch."):
+            inputs.batch_overrides[key_l.split(".", 1)[1]] = value.strip()
             continue
         inputs.other[key_l] = value
     return inputs
