@@ -23,6 +23,17 @@ struct Launch {
   dim3 grid, block;
 };
 
+// The gas-solid coupling of the Eulerian bed (eulerian.hip), for the step
+// driver: Q_comb from the array, or (Q_comb == nullptr) formed from the
+// drip-torch gas share and q_rad_gas / dz with the solid share subtracted
+// from Q_pyro in the source bed cells first.
+int coupling_launch(const fw_grid* g, void* T_g, void* T_s, const void* rho, const void* u,
+                    const void* v, const void* w, const void* alpha_s, double sigma_sav,
+                    const void* q_rad_in, void* Q_pyro, const void* Q_comb,
+                    const void* q_rad_gas, void* Q_comb_out, double* m_water, double L_v,
+                    double dt, double T_amb, int q_loss, double h_conv_mult, int nzb, int i0,
+                    int i1, double drip_gas, double drip_solid, hipStream_t s);
+
 // The learned preconditioner's network (learned.hip), for projection.hip.
 int net_apply(void* net, const float* r, const float* q, const float** c, int* n,
               hipStream_t s);

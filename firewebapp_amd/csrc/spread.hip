@@ -1,6 +1,6 @@
 // Native driver of one outer time step (spread_3d.py:1897-3683 of the
-// reference), for the Lagrangian-bed / k-epsilon / EDC / DOM configuration
-// with a passive or an active level set.
+// reference), for the k-epsilon / EDC / DOM configuration with a Lagrangian
+// (particle) or Eulerian fuel bed and a passive or an active level set.
 //
 // The whole step is issued from here, so the host cost per kernel is one
 // hipLaunchKernelGGL instead of an interpreter round trip.  Host-side
@@ -17,11 +17,13 @@
 namespace fw {
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_clamp_max(size_t n, T* __restrict__ f, T hi) {
+__global__ void __launch_bounds__(256) k_clamp_max(size_t n, T* __restrict__ f, T hi, T lo) {
   const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (c >= n) return;
-  const T v = f[c];
-  if (v > hi) f[c] = hi;   // np.minimum(T_g, 1e4) (NaN passes through)
+  T v = f[c];
+  if (v > hi) v = hi;   // np.minimum(T_g, 1e4), then np.maximum(T_g, T_amb)
+  if (v < lo) v = lo;   // (NaN passes through both)
+  f[c] = v;
 }
 
 // apply_turbulent_diffusion's sub-step count (turbulence_3d.py:879-918);
@@ -170,8 +172,10 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                    const fw_bed_particles* particles, const fw_bed_index* index,
                    fw_bed_params* bed, void* stream) {
   FW_GRID(g);
-  FW_REQUIRE(g && g64 && f && p && st && proj && dom && particles && index && bed,
-             "spread_step: null argument");
+  FW_REQUIRE(g && g64 && f && p && st && proj && dom, "spread_step: null argument");
+  FW_REQUIRE(p->eulerian ? (f->eu_m_solid && f->eu_m_char && f->eu_m_water && f->Q_pyro)
+                         : (particles && index && bed),
+             "spread_step: the bed's buffers are missing");
   FW_REQUIRE(g->n_z_bed > 0 && p->i_src1 > p->i_src0, "spread_step: needs a fuel bed");
   hipStream_t s = (hipStream_t)stream;
   const int nzb = g->n_z_bed;
@@ -202,8 +206,19 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
 
   // rho_prev, EoS, inlet O2 (:1984-2050)
   TRY(fw_step_prologue(g, f->T_g, f->rho, f->rho_prev, f->Y_O2, p->T_amb, stream));
-  // bed particles (:2064-2182); the sources stay zero until ignition
-  if (combustion) {
+  // bed (:2064-2215): particles, or the Eulerian pools; the sources stay
+  // zero until ignition
+  if (p->eulerian) {
+    if (combustion) {
+      TRY(fw_pyro_step(g, f->T_s, f->alpha_s, f->Y_O2, f->eu_m_solid, f->eu_m_char,
+                       f->eu_m_water, p->eu_m_water_init > 0.0 ? 1 : 0, p->eu_m_water_init,
+                       p->eu_A_p, p->eu_E_p, dt, 15, f->src[0], f->Q_pyro, nullptr, nullptr,
+                       nullptr, stream));
+    } else {
+      FW_CHECK_HIP(hipMemsetAsync(f->src[0], 0, n * esz, s));
+      FW_CHECK_HIP(hipMemsetAsync(f->Q_pyro, 0, n * esz, s));
+    }
+  } else if (combustion) {
     TRY(fw_bed_external_heat(g, f->q_rad, f->Q_ext, nzb, s0, s1,
                              ign_on ? F_DRIP_TO_SOLID * Q_drip : 0.0, Q_RAD_MAX, stream));
     bed->dt = dt;
@@ -250,10 +265,11 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
   SWAP(f->eps, f->tmp2);
   // DOM every dom_every steps, soil every step (:2455-2510)
   if (st->step_no % (p->dom_every > 0 ? p->dom_every : 1) == 0) {
-    TRY(fw_bed_aggregate(g, particles, index, nullptr, f->M_local, stream));
+    // the particles' moisture scales kappa_solid; the Eulerian bed has none
+    if (!p->eulerian) TRY(fw_bed_aggregate(g, particles, index, nullptr, f->M_local, stream));
     TRY(fw_dom_solve(dom, f->T_s, f->T_g, f->alpha_s, f->omega, p->sigma_sav, p->T_amb,
                      f->q_rad, f->q_rad_gas, f->T_soil, f->q_in_soil, f->Y_H2O, f->rho,
-                     f->M_local, stream));
+                     p->eulerian ? nullptr : f->M_local, stream));
   }
   TRY(fw_soil_conduction(g, p->n_soil, f->T_soil, f->q_in_soil, f->soil_dz,
                          f->soil_d_above, f->soil_d_below, dt, p->soil_k, p->soil_rho,
@@ -314,13 +330,28 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
       SWAP(*Y[1], f->tmp2);
       SWAP(*Y[2], f->tmp3);
     }
-    // gas-energy update and vertical solid conduction with the T_s clamps
-    // of :3195-3196, in one pass
-    TRY(fw_gas_update_solid_conduction(g, f->T_g, f->T_s, f->tmp1, f->alpha_s, f->rho,
-                                       f->Y_H2O, f->src[0], f->src[1], f->src[7],
-                                       f->q_rad_gas, sub == N_SUB - 1 ? f->Q_comb : nullptr,
-                                       nzb, s0, s1, drip_gas, dt_sub, p->T_amb, K_SOLID_GRASS,
-                                       RHO_SOLID, CP_SOLID, p->T_amb, T_CAP, stream));
+    if (p->eulerian) {
+      // drip-torch split, q_rad_gas into Q_comb and the gas-solid coupling
+      // (:2916-2956, :3094-3099, :3148-3156); the solid share accumulates in
+      // Q_pyro once per sub-step, as the reference subtracts it in the loop
+      const double drip_solid = (combustion && ign_on) ? F_DRIP_TO_SOLID * Q_drip : 0.0;
+      TRY(coupling_launch(g, f->T_g, f->T_s, f->rho, f->u, f->v, f->w, f->alpha_s,
+                          p->sigma_sav, f->q_rad, f->Q_pyro, nullptr, f->q_rad_gas,
+                          sub == N_SUB - 1 ? f->Q_comb : nullptr, f->eu_m_water, p->eu_L_v,
+                          dt_sub, p->T_amb, 0, p->h_conv_mult, nzb, s0, s1, drip_gas,
+                          drip_solid, s));
+      TRY(fw_solid_conduction_vertical(g, f->T_s, f->tmp1, f->alpha_s, K_SOLID_GRASS,
+                                       RHO_SOLID, CP_SOLID, dt_sub, p->T_amb, T_CAP, stream));
+    } else {
+      // gas-energy update and vertical solid conduction with the T_s clamps
+      // of :3195-3196, in one pass
+      TRY(fw_gas_update_solid_conduction(g, f->T_g, f->T_s, f->tmp1, f->alpha_s, f->rho,
+                                         f->Y_H2O, f->src[0], f->src[1], f->src[7],
+                                         f->q_rad_gas, sub == N_SUB - 1 ? f->Q_comb : nullptr,
+                                         nzb, s0, s1, drip_gas, dt_sub, p->T_amb,
+                                         K_SOLID_GRASS, RHO_SOLID, CP_SOLID, p->T_amb, T_CAP,
+                                         stream));
+    }
     SWAP(f->T_s, f->tmp1);
     // T_g turbulent diffusion, the clamps of :3193-3194 in the last pass
     for (int m = 0; m < npr; ++m) {
@@ -330,12 +361,15 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
       SWAP(f->T_g, f->tmp1);
     }
     if (npr == 0) {
+      // the particle path's gas update already floors T_g at T_amb; the
+      // coupling does not
+      const double lo = p->eulerian ? p->T_amb : -INFINITY;
       if (g->dtype == FW_F64)
         hipLaunchKernelGGL(k_clamp_max<double>, dim3((n + 255) / 256), dim3(256), 0, s, n,
-                           (double*)f->T_g, T_CAP);
+                           (double*)f->T_g, T_CAP, lo);
       else
         hipLaunchKernelGGL(k_clamp_max<float>, dim3((n + 255) / 256), dim3(256), 0, s, n,
-                           (float*)f->T_g, (float)T_CAP);
+                           (float*)f->T_g, (float)T_CAP, (float)lo);
       FW_CHECK_LAUNCH("clamp");
     }
   }
@@ -374,7 +408,7 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
   if (moved) TRY(fw_spread_front(g64, f, st, stream));
   // dynamic seeding behind the empirical-ROS front, every 50th step
   // (:3271-3300); front_exit_x is lset.front_x(k=1, j=Ny/2) after the evolve
-  if (p->lset_active && p->emp_w > 0.0) {
+  if (p->lset_active && p->emp_w > 0.0 && !p->eulerian) {
     st->seed_count += 1;
     if (st->seed_count >= 50) {
       st->seed_count = 0;

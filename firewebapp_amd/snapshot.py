@@ -121,9 +121,14 @@ class SnapshotWriter:
         _lib.call("fw_q_in_at_front_2d", sv.dg.ref, ptr(q_fwd), ptr(mask),
                   ptr(self.q_in_2d), s)
         bp = self.bp
-        _lib.call("fw_bed_snapshot", sv.dg.ref, C.byref(sv.P.c), C.byref(sv.ix.c),
-                  ptr(bp["count"]), ptr(bp["m_solid"]), ptr(bp["m_water"]),
-                  ptr(bp["m_char"]), ptr(bp["T_s_avg"]), s)
+        if sv.ix is not None:
+            _lib.call("fw_bed_snapshot", sv.dg.ref, C.byref(sv.P.c), C.byref(sv.ix.c),
+                      ptr(bp["count"]), ptr(bp["m_solid"]), ptr(bp["m_water"]),
+                      ptr(bp["m_char"]), ptr(bp["T_s_avg"]), s)
+        else:
+            # no particles (Eulerian bed): the reference writes zeros
+            for t in bp.values():
+                t.zero_()
         dev = {k: f[src].float() for k, src in _STATE_SRC}
         dev.update(tau_mix=self.tau_mix.float(), phi_bed=sv.phi.float(),
                    phi_flame=self.phi_flame.float(), q_dom_fwd_3d=q_fwd.float(),

@@ -1,8 +1,9 @@
 """GPU driver of the Tier-3 spread time step: ``run_3d_spread`` of the
-reference (spread_3d.py:563-3729) for the Lagrangian-bed / k-epsilon / EDC /
-DOM configuration with a passive or an active level set
-(``level_set_passive``, default false as in the reference), every field
-resident in device memory.
+reference (spread_3d.py:563-3729) for the k-epsilon / EDC / DOM
+configuration with a Lagrangian (particle) or Eulerian fuel bed
+(``lagrangian_bed_enable``) and a passive or an active level set
+(``level_set_passive``; both default false as in the reference), every
+field resident in device memory.
 
 Precision: the grid fields are float32 by default (deck key
 ``outdoor.dtype = float64`` selects the float64 parity build); bed particles,
@@ -69,7 +70,8 @@ class FwSpreadFields(C.Structure):
             "u_inlet", "v_inlet", "w_inlet", "sigma_x", "k_wall", "eps_wall", "T_soil",
             "q_in_soil", "soil_dz", "soil_d_above", "soil_d_below", "phi", "phi_tmp",
             "phi0", "red_work", "red_out", "red_host", "nu_work", "contrib", "dT_work",
-            "counters", "edc_work", "lset_mask", "q_dom_fwd", "v_n")])
+            "counters", "edc_work", "lset_mask", "q_dom_fwd", "v_n", "eu_m_solid",
+            "eu_m_char", "eu_m_water", "Q_pyro")])
 
 
 class FwSpreadParams(C.Structure):
@@ -83,7 +85,10 @@ class FwSpreadParams(C.Structure):
                                  "dom_every", "reserved")] + [
         (n, C.c_double) for n in ("band_m", "E_sens", "E_lat_coef", "emp_w", "emp_ros",
                                   "seed_T_s", "x_lo_bed", "stall_window")] + [
-        (n, C.c_int32) for n in ("lset_active", "reserved2")]
+        (n, C.c_int32) for n in ("lset_active", "reserved2")] + [
+        (n, C.c_double) for n in ("eu_m_water_init", "eu_A_p", "eu_E_p", "eu_L_v",
+                                  "h_conv_mult")] + [
+        (n, C.c_int32) for n in ("eulerian", "reserved3")]
 
 
 class FwSpreadState(C.Structure):
@@ -135,7 +140,6 @@ _REQUIRED = (
     ("radiation_solver", "dom", "dom"),
     ("boundary_condition_kind", "outdoor_wind", "outdoor_wind"),
     ("chemistry_family", "biomass", "biomass"),
-    ("lagrangian_bed_enable", True, False),
     ("ignition_T_pin_enable", False, False),
     ("edc_extinction_enable", False, False),
     ("finney_tendril_enable", False, False),
@@ -198,6 +202,10 @@ class Spread3DConfig:
     particle_jitter_seed: int
     spin_up_s: float
     level_set_passive: bool = True
+    lagrangian_bed: bool = True
+    h_conv_mult: float = 1.0
+    pyro_A_p: float = 36280.0
+    pyro_E_p: float = 58600.0
     empirical_ros_enable: bool = False
     empirical_ros_m_s: float = 0.0
     empirical_blend_w: float = 0.0
@@ -312,6 +320,17 @@ class Spread3DConfig:
         # level-set mode and the empirical-ROS blend (spread_3d.py:1075-1090,
         # :1246-1272); the ROS and its weight are fixed for the run
         passive = bool(deck_first("level_set_passive", False))
+        lagrangian = bool(deck_first("lagrangian_bed_enable", False))
+        # run_3d_spread keywords of the Eulerian bed (spread_3d.py:703, :712-713,
+        # :1845-1858)
+        h_conv_mult = float(kw.pop("h_conv_mult", 1.0))
+        a_ov = kw.pop("pyrolysis_A_p_override", None)
+        e_ov = kw.pop("pyrolysis_E_p_override", None)
+        if (a_ov is None) != (e_ov is None):
+            raise ValueError("pyrolysis_A_p_override and pyrolysis_E_p_override must be "
+                             "set together")
+        pyro_A_p = float(a_ov) if a_ov is not None else 36280.0
+        pyro_E_p = float(e_ov) if e_ov is not None else 58600.0
         emp_on = bool(deck_first("empirical_ros_enable", False))
         emp_ros, emp_w = 0.0, 0.0
         if emp_on:
@@ -358,7 +377,8 @@ class Spread3DConfig:
             particle_jitter=float(deck_first("particle_jitter", 0.0)),
             particle_jitter_seed=int(deck_first("particle_jitter_seed", 0)),
             spin_up_s=float(ocfg.spin_up_s),
-            level_set_passive=passive, empirical_ros_enable=emp_on,
+            level_set_passive=passive, lagrangian_bed=lagrangian, h_conv_mult=h_conv_mult,
+            pyro_A_p=pyro_A_p, pyro_E_p=pyro_E_p, empirical_ros_enable=emp_on,
             empirical_ros_m_s=float(emp_ros), empirical_blend_w=float(emp_w),
         )
         if kw:
@@ -537,43 +557,17 @@ class Spread3DSolver:
         alpha = np.zeros(self.shape)
         alpha[:g.n_z_bed, :, i0:i1] = cfg.rho_b / _RHO_PARTICLE
         f["alpha_s"].copy_(torch.as_tensor(alpha))
-        # bed particles (spread_3d.py:1296-1330)
-        n_cells = int((alpha[:g.n_z_bed, :, i0:i1] > 0).sum())
-        buf = _lb.allocate_bed_particle_buffers(n_cells * cfg.n_per_cell)
-        n_alloc = _lb.initialize_bed_particles_from_alpha_s(
-            buf, alpha, cfg.rho_b, cfg.moisture, T_amb, g.dx, g.dy, g.dz_arr,
-            g.n_z_bed, cfg.n_per_cell, sav=cfg.sigma_sav, i_lo=i0, i_hi=i1,
-            mass_scale=particle_mass_scale(cfg, n_cells * cfg.n_per_cell))
-        if n_alloc <= 0:
-            raise ValueError("no bed particles were allocated")
-        self.n_particles = n_alloc
-        # ignition patch and solid-phase seeding (:1486-1526)
         n_src = max(1, int(round(0.5 * cfg.ignition_width_mult / g.dx)))
         s0, s1 = i0, min(i1, i0 + n_src)
         self.i_src = (s0, s1)
-        if cfg.solid_phase_ignition:
-            x_lo = g.x_mid[s0] - 0.5 * g.dx
-            x_hi = g.x_mid[s1 - 1] + 0.5 * g.dx
-            x = buf["x"][:n_alloc]
-            sel = (buf["alive"][:n_alloc] != 0) & (x_lo <= x) & (x < x_hi)
-            buf["T_s"][:n_alloc][sel] = cfg.solid_phase_T_s_K
-        self.P = _lb.DeviceParticles(buf, n_alloc)
-        cs, ca = _lb._locate(self.dg, self.P)
-        if not torch.equal(cs, ca):
-            raise RuntimeError("particle cell maps of the step and the aggregates differ")
-        if int(cs.min()) < 0:
-            raise RuntimeError("bed particle outside the grid")
-        self.ix = _lb.DeviceIndex(cs)
-        self.contrib = torch.empty(10 * n_alloc, **f64)
         self.counters = torch.zeros(2, dtype=torch.int32, device=dev)
         self.dT_work = torch.zeros(self.N, **f64)
         self.src_arr = (C.c_void_p * 8)(*[f[k].data_ptr() for k in
                                           ("Sp", "Sd", "Qp", "Qd", "YF", "Qch", "Qsm", "Qgc")])
-        self.bed_params = _lb.FwBedParams(
-            0.0, cfg.h_conv, cfg.rho_solid_true, cfg.cp_solid, cfg.eps_solid, T_amb,
-            cfg.view_factor, cfg.h_bed, cfg.sigma_sav * cfg.rho_b / cfg.rho_solid_true,
-            cfg.char_ox_flux_cap, cfg.char_ox_ash_exp, cfg.n_per_cell,
-            int(cfg.view_factor_geometric), 1, 1, 1, 1, cfg.drying_mode, 0)
+        if cfg.lagrangian_bed:
+            self._setup_particles(alpha, i0, i1)
+        else:
+            self._setup_eulerian_bed(i0, i1)
         # inflow, sponge, turbulence (:1391-1580)
         self.U_mf = midflame_wind_speed(cfg.wind_speed_m_s, cfg.terrain)
         u_in = build_u_inlet(g, cfg.wind_speed_m_s)
@@ -675,6 +669,17 @@ class Spread3DSolver:
         pr.x_lo_bed = float(g.x_mid[i0] - 0.5 * g.dx)
         pr.stall_window = _STALL_WINDOW_S
         pr.lset_active = 1 if cfg_active else 0
+        # Eulerian bed: pools, the coupling's latent heat (spread_3d.py:3152:
+        # L_v only with moisture) and the MD2004 pair
+        pr.h_conv_mult = cfg.h_conv_mult
+        if self.eu is not None:
+            for name in ("m_solid", "m_char", "m_water"):
+                setattr(fs, "eu_" + name, self.eu[name].data_ptr())
+            fs.Q_pyro = self.eu["Q_pyro"].data_ptr()
+            pr.eulerian = 1
+            pr.eu_m_water_init = self.eu_m_water_init
+            pr.eu_A_p, pr.eu_E_p = cfg.pyro_A_p, cfg.pyro_E_p
+            pr.eu_L_v = 2_257_000.0 if self.eu_m_water_init > 0.0 else 0.0
         self._by_ptr = {t.data_ptr(): t for t in list(f.values()) +
                         [self.phi, self.phi_tmp, self.phi0]}
         self.state = FwSpreadState()
@@ -692,6 +697,64 @@ class Spread3DSolver:
                                      "div", "iters")}
         self._append_diag(0.0, st.red, 0.0, 0, sources=False)
         self.stop_reason = None
+
+    def _setup_particles(self, alpha, i0, i1):
+        """Bed particles and solid-phase seeding (spread_3d.py:1296-1330,
+        :1486-1526)."""
+        cfg, g, dev = self.cfg, self.grid, self.dev
+        f64 = dict(dtype=torch.float64, device=dev)
+        T_amb = cfg.T_amb
+        n_cells = int((alpha[:g.n_z_bed, :, i0:i1] > 0).sum())
+        buf = _lb.allocate_bed_particle_buffers(n_cells * cfg.n_per_cell)
+        n_alloc = _lb.initialize_bed_particles_from_alpha_s(
+            buf, alpha, cfg.rho_b, cfg.moisture, T_amb, g.dx, g.dy, g.dz_arr,
+            g.n_z_bed, cfg.n_per_cell, sav=cfg.sigma_sav, i_lo=i0, i_hi=i1,
+            mass_scale=particle_mass_scale(cfg, n_cells * cfg.n_per_cell))
+        if n_alloc <= 0:
+            raise ValueError("no bed particles were allocated")
+        self.n_particles = n_alloc
+        s0, s1 = self.i_src
+        if cfg.solid_phase_ignition:
+            x_lo = g.x_mid[s0] - 0.5 * g.dx
+            x_hi = g.x_mid[s1 - 1] + 0.5 * g.dx
+            x = buf["x"][:n_alloc]
+            sel = (buf["alive"][:n_alloc] != 0) & (x_lo <= x) & (x < x_hi)
+            buf["T_s"][:n_alloc][sel] = cfg.solid_phase_T_s_K
+        self.P = _lb.DeviceParticles(buf, n_alloc)
+        cs, ca = _lb._locate(self.dg, self.P)
+        if not torch.equal(cs, ca):
+            raise RuntimeError("particle cell maps of the step and the aggregates differ")
+        if int(cs.min()) < 0:
+            raise RuntimeError("bed particle outside the grid")
+        self.ix = _lb.DeviceIndex(cs)
+        self.contrib = torch.empty(10 * n_alloc, **f64)
+        self.bed_params = _lb.FwBedParams(
+            0.0, cfg.h_conv, cfg.rho_solid_true, cfg.cp_solid, cfg.eps_solid, T_amb,
+            cfg.view_factor, cfg.h_bed, cfg.sigma_sav * cfg.rho_b / cfg.rho_solid_true,
+            cfg.char_ox_flux_cap, cfg.char_ox_ash_exp, cfg.n_per_cell,
+            int(cfg.view_factor_geometric), 1, 1, 1, 1, cfg.drying_mode, 0)
+        self.eu = None
+
+    def _setup_eulerian_bed(self, i0, i1):
+        """Eulerian pools (spread_3d.py:1186-1195, :1236, :1274-1278): the
+        reactive mass rho_b (uniform profile) and the water rho_b M in the bed
+        cells, no char; no particles and no solid-phase seeding (:1508)."""
+        cfg, g, dev = self.cfg, self.grid, self.dev
+        f64 = dict(dtype=torch.float64, device=dev)
+        m = np.zeros(self.shape)
+        m[:g.n_z_bed, :, i0:i1] = cfg.rho_b * 1.0
+        w = np.zeros(self.shape)
+        w[:g.n_z_bed, :, i0:i1] = cfg.rho_b * cfg.moisture
+        self.eu = {"m_solid": torch.as_tensor(m, **f64), "m_char": torch.zeros(self.shape, **f64),
+                   "m_water": torch.as_tensor(w, **f64),
+                   "Q_pyro": torch.zeros(self.shape, dtype=self.dtype, device=dev)}
+        self.eu_m_water_init = cfg.rho_b * cfg.moisture
+        self.n_particles = 0
+        empty = _lb.allocate_bed_particle_buffers(0)
+        self.P = _lb.DeviceParticles(empty, 0)
+        self.ix = None
+        self.contrib = torch.zeros(1, **f64)
+        self.bed_params = None
 
     # -- helpers ---------------------------------------------------------------
     @staticmethod
@@ -742,8 +805,9 @@ class Spread3DSolver:
         st = self.state
         rc = _lib.lib().fw_spread_step(
             self.dg.ref, self.dg64.ref, C.byref(self.fields), C.byref(self.params),
-            C.byref(st), self.proj, self.dom_h, C.byref(self.P.c), C.byref(self.ix.c),
-            C.byref(self.bed_params), stream_ptr())
+            C.byref(st), self.proj, self.dom_h, C.byref(self.P.c),
+            C.byref(self.ix.c) if self.ix is not None else None,
+            C.byref(self.bed_params) if self.bed_params is not None else None, stream_ptr())
         self._remap()
         if rc == _lib.FW_E_NONFINITE:
             raise RuntimeError(_lib.lib().fw_last_error().decode())
@@ -805,6 +869,9 @@ class Spread3DSolver:
         out["T_soil"] = self.T_soil.double().cpu().numpy()
         for k in ("x", "y", "z", "T_s", "m_solid", "m_water", "m_char", "alive"):
             out["p_" + k] = self.P.t[k].cpu().numpy().copy()
+        if self.eu is not None:
+            for k in ("m_solid", "m_char", "m_water", "Q_pyro"):
+                out["eu_" + k] = self.eu[k].double().cpu().numpy()
         return out
 
     def ros(self):
@@ -833,6 +900,9 @@ class Spread3DSolver:
         zero = np.zeros(self.shape)
         for k in ("p_dyn", "Y_CO", "m_hemi", "m_cell", "m_lign", "m_char"):
             st[k] = zero.copy()
+        if self.eu is not None:
+            st["m_hemi"] = self.eu["m_solid"].cpu().numpy()
+            st["m_char"] = self.eu["m_char"].cpu().numpy()
         d = self.diag
         return Spread3DResult(
             ros_m_s=self.ros(), n_cells_ignited=int(np.sum(st["T_s"] >= self.T_ign)),

@@ -364,6 +364,37 @@ int fw_bed_horizontal_conduction(const fw_grid* g,
                                  int n_z_bed, double dt, double* dT_work,
                                  void* stream);
 
+/* ---- Eulerian fuel bed (lagrangian_bed_enable = false) ------------------ */
+
+/* pyrolysis_3d.py:270-614 on the bed pools (float64 fields m_solid = the
+ * reference's m_hemi, m_char, m_water), float64 arithmetic.  `processes` is
+ * a mask of the kernels run, in this order, cell by cell:
+ *   1 step_drying (:270-309; skipped when has_moisture == 0, Q_dry = 0),
+ *   2 step_pyrolysis_md2004 (:311-462; m_water_init > 0 enables the
+ *     moisture gate; A_p, E_p the thermal Arrhenius pair),
+ *   4 step_char_oxidation (:465-564) of m_char,
+ *   8 step_smoldering_oxidation (:567-614) of m_char.
+ * With bit 2 set, Q_pyro_out = Q_md + Q_dry - Q_char - Q_smold over the
+ * processes run (spread_3d.py:2183-2215) and S_pyro_out the volatile
+ * source; Q_dry_out, Q_char_out, Q_smold_out (may be NULL) receive the
+ * single heats.  Grid-type fields T_s, alpha_s, Y_O2 and outputs. */
+int fw_pyro_step(const fw_grid* g, const void* T_s, const void* alpha_s, const void* Y_O2,
+                 double* m_solid, double* m_char, double* m_water, int has_moisture,
+                 double m_water_init, double A_p, double E_p, double dt, int processes,
+                 void* S_pyro_out, void* Q_pyro_out, void* Q_dry_out, void* Q_char_out,
+                 void* Q_smold_out, void* stream);
+
+/* coupling_3d.step_gas_solid_coupling (coupling_3d.py:76-192), in place on
+ * T_g, T_s (grid type) and m_water (float64): gas heating by Q_comb,
+ * Ranz-Marshall convection to the solid, absorbed radiation, ground loss,
+ * moisture evaporation from the solid's heat budget and the Q_pyro sink.
+ * sigma_sav <= 0 leaves everything unchanged, as the reference does. */
+int fw_gas_solid_coupling(const fw_grid* g, void* T_g, void* T_s, const void* rho,
+                          const void* u, const void* v, const void* w, const void* alpha_s,
+                          double sigma_sav, const void* q_rad_in, const void* Q_pyro,
+                          const void* Q_comb, double* m_water, double L_v, double dt,
+                          double T_amb, int q_loss_enable, double h_conv_mult, void* stream);
+
 /* ---- time-step glue (spread_3d.py:1897-3683) ----------------------------- */
 
 /* rho_prev <- rho; rho <- P0/(R max(T_g, T_amb)); Y_O2[:, :, 0] <- 0.232
@@ -489,6 +520,10 @@ typedef struct fw_spread_fields {
   uint8_t* lset_mask;                /* N, the ahead band of the step */
   void* q_dom_fwd;                   /* N, grid type: DOM forward flux in the band */
   double* v_n;                       /* N, float64 front speed */
+  /* Eulerian bed only (may be NULL when eulerian == 0): float64 pools and
+   * the grid-type solid heat sink */
+  double *eu_m_solid, *eu_m_char, *eu_m_water;
+  void* Q_pyro;
 } fw_spread_fields;
 
 /* Run constants resolved from the deck (spread_3d.py:959-1896). */
@@ -509,6 +544,13 @@ typedef struct fw_spread_params {
   double x_lo_bed;      /* x of the bed's leading face */
   double stall_window;  /* s without front advance before the stall exit */
   int32_t lset_active, reserved2;
+  /* Eulerian bed (lagrangian_bed_enable = false, spread_3d.py:2183-2215,
+   * :3148-3156) */
+  double eu_m_water_init; /* rho_b M (0: no moisture, no drying, no gate) */
+  double eu_A_p, eu_E_p;  /* MD2004 thermal Arrhenius pair */
+  double eu_L_v;          /* latent heat of the coupling's evaporation (0 when dry) */
+  double h_conv_mult;     /* Ranz-Marshall h_p scale */
+  int32_t eulerian, reserved3;
 } fw_spread_params;
 
 /* Host-side scalars carried from step to step. */

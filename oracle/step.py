@@ -5,7 +5,8 @@ Follows run_3d_spread of the reference (spread_3d.py:1126-1896 setup,
 :1897-3683 loop body, :3743-3824 front tracking and ROS) for the
 configuration the GPU driver implements: k-epsilon (any turbulence_model
 other than smagorinsky), EDC, DOM every ``dom_subcycle_every`` steps,
-Lagrangian bed particles, passive or active level set, outdoor log-law inflow,
+Lagrangian bed particles or the Eulerian bed, passive or active level set,
+outdoor log-law inflow,
 periodic y.  The kernels are the oracle's own restatements (``oracle.bed``,
 ``oracle.dom``, ``oracle.edc``, ``oracle.levelset``, ``oracle.projection``,
 ``oracle.transport``).
@@ -29,6 +30,7 @@ import numpy as np
 from . import bed as Bd
 from . import dom as Dm
 from . import edc as Ed
+from . import eulerian as Eu
 from . import levelset as LS
 from . import snapshot as Sn
 from . import projection as Pj
@@ -89,10 +91,18 @@ class OracleSpread3D:
             raise ValueError("empty fuel bed")
         self.i_bed = (i0, i1)
         self.alpha_s[:g.n_z_bed, :, i0:i1] = cfg.rho_b / RHO_PARTICLE
+        self.eulerian = not getattr(cfg, "lagrangian_bed", True)
+        # Eulerian pools (spread_3d.py:1186-1195, :1274-1278)
+        self.eu_m_solid, self.eu_m_char, self.eu_m_water = z(), z(), z()
+        self.eu_Q_pyro = z()
+        self.eu_m_water_init = cfg.rho_b * cfg.moisture
+        if self.eulerian:
+            self.eu_m_solid[:g.n_z_bed, :, i0:i1] = cfg.rho_b * 1.0
+            self.eu_m_water[:g.n_z_bed, :, i0:i1] = cfg.rho_b * cfg.moisture
         # bed particles (spread_3d.py:1300-1330)
-        nb = int((self.alpha_s[:g.n_z_bed, :, i0:i1] > 0).sum())
+        nb = 0 if self.eulerian else int((self.alpha_s[:g.n_z_bed, :, i0:i1] > 0).sum())
         self.buf = Bd.allocate_bed_particle_buffers(nb * cfg.n_per_cell)
-        self.n_alloc = Bd.initialize_bed_particles_from_alpha_s(
+        self.n_alloc = 0 if self.eulerian else Bd.initialize_bed_particles_from_alpha_s(
             self.buf, self.alpha_s, cfg.rho_b, cfg.moisture, T_amb, g.dx, g.dy,
             g.dz_arr, g.n_z_bed, cfg.n_per_cell, sav=cfg.sigma_sav,
             m_scale=particle_mass_scale)
@@ -100,7 +110,7 @@ class OracleSpread3D:
         n_src = max(1, int(round(0.5 * cfg.ignition_width_mult / g.dx)))
         s0, s1 = i0, min(i1, i0 + n_src)
         self.i_src = (s0, s1)
-        if cfg.solid_phase_ignition:
+        if cfg.solid_phase_ignition and not self.eulerian:
             x_lo = g.x_mid[s0] - 0.5 * g.dx
             x_hi = g.x_mid[s1 - 1] + 0.5 * g.dx
             b = self.buf
@@ -200,8 +210,29 @@ class OracleSpread3D:
         np.copyto(self.rho_prev, self.rho)
         self.rho[...] = P0 / (R_AIR * np.maximum(self.T_g, T_amb))
         self.Y_O2[:, :, 0] = 0.232
-        # bed (:2064-2182)
-        if combustion and self.n_alloc > 0:
+        # bed (:2064-2215)
+        if self.eulerian:
+            if combustion:
+                mw_init = self.eu_m_water_init
+                Q_dry = np.zeros(self.shape)
+                if mw_init > 0.0:
+                    Eu.drying(self.T_s, self.eu_m_water, self.alpha_s, dt, Q_dry)
+                Eu.pyrolysis_md2004(self.T_s, self.eu_m_solid, self.eu_m_char, self.Y_O2,
+                                    self.alpha_s, self.eu_m_water, mw_init, dt, self.S_pyro,
+                                    self.eu_Q_pyro, cfg.pyro_A_p, cfg.pyro_E_p)
+                self.eu_Q_pyro += Q_dry
+                Q_char = np.zeros(self.shape)
+                Eu.char_oxidation(self.T_s, self.eu_m_char, self.Y_O2, self.alpha_s, dt, Q_char)
+                self.eu_Q_pyro -= Q_char
+                Q_sm = np.zeros(self.shape)
+                Eu.smoldering(self.T_s, self.eu_m_char, self.Y_O2, self.alpha_s, dt, Q_sm)
+                self.eu_Q_pyro -= Q_sm
+            else:
+                self.S_pyro.fill(0.0)
+                self.eu_Q_pyro.fill(0.0)
+            self.src = {k: np.zeros(self.shape) for k in self.src}
+            self.src["S_pyro"] = self.S_pyro
+        elif combustion and self.n_alloc > 0:
             Qsx = np.zeros(self.shape)
             if ign_on:
                 Qsx[:nzb, :, s0:s1] += 0.80 * (30_000.0 / max(cfg.h_bed, 1e-3))
@@ -248,11 +279,12 @@ class OracleSpread3D:
                           cfg.canopy_beta_p, cfg.canopy_beta_d)
         # DOM every dom_subcycle_every steps, soil every step (:2455-2510)
         if self.step_no % max(cfg.dom_subcycle_every, 1) == 0:
-            Bd.aggregate_M_local_grid(self.buf, dx, dy, g.z_face, self.M_local)
+            if not self.eulerian:
+                Bd.aggregate_M_local_grid(self.buf, dx, dy, g.z_face, self.M_local)
             self.dom.solve(self.T_s, self.T_g, self.alpha_s, self.omega, cfg.sigma_sav,
                            T_amb, self.q_rad, self.q_rad_gas, T_soil_surface=self.T_soil[0],
                            q_in_soil_out=self.q_in_soil, Y_H2O=self.Y_H2O, rho=self.rho,
-                           bed_moisture_per_cell=self.M_local)
+                           bed_moisture_per_cell=None if self.eulerian else self.M_local)
         Tr.step_soil_conduction(self.T_soil, self.q_in_soil, dt, self.soil_dz,
                                 self.soil_da, self.soil_db, T_amb=T_amb)
         # ignition pulse (:2528-2532)
@@ -273,6 +305,8 @@ class OracleSpread3D:
                                       self.Y_H2O, extinction_enable=False)
             self.Q_comb.fill(0.0)
             if combustion and ign_on:
+                if self.eulerian:
+                    self.eu_Q_pyro[:nzb, :, s0:s1] -= 0.80 * (30_000.0 / max(cfg.h_bed, 1e-3))
                 self.Q_comb[:nzb, :, s0:s1] += drip_gas
             S_tot = Sp + Sd
             S_F = Sp - self.Y_fuel * S_tot
@@ -293,12 +327,22 @@ class OracleSpread3D:
             np.clip(self.Y_H2O, 0.0, 1.0, out=self.Y_H2O)
             for k in range(g.Nz):
                 self.Q_comb[k] += self.q_rad_gas[k] / dz[k]
-            cp_mix = (1.0 - self.Y_H2O) * 1100.0 + self.Y_H2O * 2000.0
-            debit = (Sp * np.maximum(self.T_g - self.T_s, 0.0)
-                     + Sd * np.maximum(self.T_g - 373.15, 0.0)) * 2000.0
-            inv = dt_sub / (np.maximum(self.rho, 1.0e-3) * cp_mix)
-            self.T_g += (self.Q_comb - Qgc - debit) * inv
-            np.maximum(self.T_g, T_amb, out=self.T_g)
+            if self.eulerian:
+                # coupling_3d with q_rad (+ a zero Frankman flux) into the
+                # solid, no emission loss (:3094, :3148-3156)
+                Eu.gas_solid_coupling(self.T_g, self.T_s, self.rho, self.u, self.v, self.w,
+                                      self.alpha_s, cfg.sigma_sav, self.q_rad + 0.0,
+                                      self.eu_Q_pyro, self.Q_comb, self.eu_m_water,
+                                      2_257_000.0 if self.eu_m_water_init > 0.0 else 0.0,
+                                      dt_sub, dz, T_amb, q_loss_enable=False,
+                                      h_conv_mult=cfg.h_conv_mult)
+            else:
+                cp_mix = (1.0 - self.Y_H2O) * 1100.0 + self.Y_H2O * 2000.0
+                debit = (Sp * np.maximum(self.T_g - self.T_s, 0.0)
+                         + Sd * np.maximum(self.T_g - 373.15, 0.0)) * 2000.0
+                inv = dt_sub / (np.maximum(self.rho, 1.0e-3) * cp_mix)
+                self.T_g += (self.Q_comb - Qgc - debit) * inv
+                np.maximum(self.T_g, T_amb, out=self.T_g)
             Tr.step_solid_conduction_vertical(self.T_s, self.alpha_s, dz, da, db,
                                               K_SOLID_GRASS, RHO_SOLID, CP_SOLID, dt_sub)
             Tr.apply_turbulent_diffusion(self.T_g, self.nu_t, PR_T, dt_sub, dx, dy, dz, da, db)
@@ -330,7 +374,7 @@ class OracleSpread3D:
             fx_exit = LS.front_x(self.phi, 1, g.Ny // 2, dx)
             # dynamic seeding behind the empirical-ROS front (:3271-3300)
             if (not cfg.level_set_passive and cfg.empirical_ros_enable
-                    and cfg.empirical_blend_w > 0.0):
+                    and cfg.empirical_blend_w > 0.0 and not self.eulerian):
                 self.seed_count += 1
                 if self.seed_count >= 50:
                     self.seed_count = 0
