@@ -13,10 +13,11 @@ with its 10 chemistry/transport sub-steps.  ``value`` = cells x steps / wall
 seconds over exactly K timed steps after W warm-up steps, inputs resident in
 HBM.  The JSON line also carries:
 * roofline: the dominant kernel's algorithmic bytes per launch / its mean
-  launch time over the timed steps, measured with HIP events that the launch
-  itself records (hipExtLaunchKernel, on the stream the kernel is launched
-  on: torch's current stream), against 8 TB/s; traffic is the HBM bytes per
-  launch from the FETCH_SIZE and WRITE_SIZE counters;
+  launch time over the timed steps (every 4th launch), measured with HIP
+  events that the launch itself records (hipExtLaunchKernel, on the stream
+  the kernel is launched on: torch's current stream), against 8 TB/s;
+  traffic is the HBM bytes per launch from the FETCH_SIZE and WRITE_SIZE
+  counters;
 * step_roofline: the whole step against the HBM roofline, with the
   phase-fused compulsory traffic per cell-step (each phase's distinct
   N-sized fields read once and written once, particles 72 B each);
@@ -177,11 +178,16 @@ def deck(name, dtype):
     return RomInputs(outdoor_overrides=ovr)
 
 
+# every SPECIES_STRIDE-th k_species3 launch of the timed steps records its
+# events (the event-recording launch costs extra host time per launch)
+SPECIES_STRIDE = 4
+
+
 def species_timer(on):
     """Start (reset) or stop the step driver's HIP-event timing of the k_species
     launches; the events are recorded on the stream the kernels run on."""
     from firewebapp_amd import _lib
-    _lib.call("fw_spread_timer", 1 if on else 0)
+    _lib.call("fw_spread_timer", SPECIES_STRIDE if on else 0)
 
 
 def species_timer_read():

@@ -28,7 +28,8 @@ namespace fw {
 #define PJ_RED 512  // partial-sum workgroups of the projection reductions
 
 // S_DONE: 0 while the inner PCG runs, 1 once ||r|| <= S_TOL, 2 on a NaN
-// residual; S_IT counts the iterations taken.  The iteration kernels leave
+// residual, 3 when k_round_init found the solve converged, 4 on a NaN
+// ||b||; S_IT counts the iterations taken.  The iteration kernels leave
 // the Krylov state untouched once S_DONE is set, so iterations can be
 // enqueued in batches and the host reads S_DONE once per batch.
 enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_DONE, S_IT, S_TOL, S_SIG, S_NS };
@@ -530,12 +531,34 @@ k_x64_add(size_t n, double* __restrict__ x, const T* __restrict__ d) {
   if (c < n) x[c] += (double)d[c];
 }
 
-__global__ void k_pcg_init(double* __restrict__ S, double tol) {
-  if (threadIdx.x == 0) {
-    S[S_DONE] = 0.0;
-    S[S_IT] = 0.0;
-    S[S_TOL] = tol;
+// Set-up of a refinement round on the device, from ||b||^2 (S_BB) and
+// ||b - A x||^2 (S_TMP) as the last finalize left them: S_DONE = 3 when the
+// solve has converged (||r|| <= rtol ||b||, or b = 0), 2 on a NaN norm,
+// else 0 with the inner tolerance S_TOL = max(rtol ||b|| / ||r||, floor) ||r||
+// (the host's former arithmetic, in its order).  The iteration kernels do
+// nothing while S_DONE != 0, so a round launched ahead of this decision is
+// harmless when the solve has already converged.
+__global__ void k_round_init(double* __restrict__ S, double rtol, double floor) {
+  if (threadIdx.x != 0) return;
+  const double bnorm = sqrt(S[S_BB]), rnorm = sqrt(S[S_TMP]);
+  const double tol_abs = rtol * bnorm;
+  S[S_IT] = 0.0;
+  if (!(bnorm == bnorm)) {
+    S[S_DONE] = 4.0;
+    return;
   }
+  if (!(rnorm == rnorm)) {
+    S[S_DONE] = 2.0;
+    return;
+  }
+  if (rnorm <= tol_abs || bnorm == 0.0) {
+    S[S_DONE] = 3.0;
+    return;
+  }
+  double inner = tol_abs / rnorm;
+  if (inner < floor) inner = floor;
+  S[S_TOL] = inner * rnorm;
+  S[S_DONE] = 0.0;
 }
 
 template <typename T>
@@ -1293,16 +1316,15 @@ static int read_scalars(ProjHandle* h, hipStream_t s) {
 // Inner PCG on A d = r (type T); r holds the right-hand side on entry and
 // the recursively updated residual on exit.  Stops when ||r||_2 <= tol_abs.
 template <typename T, typename TR>
-static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
-                     int* iters, hipStream_t s) {
+static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStream_t s) {
   const size_t n = h->n;
   T* r = (T*)h->r;
   T* z = (T*)h->z;
   T* p = (T*)h->p;
   T* q = (T*)h->q;
   T* d = (T*)h->d;
+  // S_DONE, S_IT and S_TOL come from k_round_init, launched by the caller
   hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n)), dim3(256), 0, s, n, d);
-  hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(64), 0, s, h->S, tol_abs);
   int rc;
   (void)rho;   // the iterations use h->irho, formed from rho by solve_refined
   const T* ir = (const T*)h->irho;
@@ -1392,22 +1414,32 @@ static int pcg_inner(ProjHandle* h, const TR* rho, double tol_abs, int maxit,
     if (h->S_host[S_DONE] != 0.0 || queued >= maxit) break;
   }
   *iters = (int)h->S_host[S_IT];
-  h->last_inner[rd] = *iters;
+  // a round that found the solve converged (S_DONE 3) ran no iteration and
+  // says nothing about the next round's length
+  if (h->S_host[S_DONE] != 3.0) h->last_inner[rd] = *iters;
   return FW_OK;
 }
 
-// Solve A x = b: x64 (warm start on entry), b64 given; ||b - A x||_2 <=
-// rtol ||b||_2 with the residual recomputed in float64 after each inner solve.
-// r0 >= 0: ||b - A x|| of the warm start, already computed with r64 and r
-// in place (saves one host read); r0 < 0: compute it here.
+// Solve A x = b: x64 (warm start on entry), b64 given, ||b||^2 in S_BB on
+// the device; ||b - A x||_2 <= rtol ||b||_2 with the residual recomputed in
+// float64 after each inner solve.  have_residual: ||b - A x||^2 of the warm
+// start is already in S_TMP (with r64 and r formed by residual64).
+//
+// The host reads the device scalars once per inner batch.  Round 0 starts
+// without a read: k_round_init decides on the device whether it runs.  A
+// later round is enqueued behind the previous round's residual without a
+// read when the previous solve needed more rounds than have run (the usual
+// case, so the read is saved); the last expected round's residual is read
+// before anything else is enqueued, so a converged solve queues no idle
+// round.  Iterates, iteration counts and results are those of the
+// read-every-round loop.
 template <typename T, typename TR>
-static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm,
-                         double r0, hipStream_t s) {
+static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_residual,
+                         hipStream_t s) {
   const size_t n = h->n;
-  const double tol_abs = rtol * bnorm;
   const double inner_floor = (sizeof(T) == 4) ? 2e-5 : 0.0;
+  const int expected_rounds = h->last_outer;
   int total = 0, outer = 0, rc;
-  double rnorm = 0.0;
   hipLaunchKernelGGL((k_irho<T, TR>), dim3(nblk(n)), dim3(256), 0, s, n, rho, (T*)h->irho);
   if (h->precond_enabled == FW_PRECOND_LEARNED) {
     if constexpr (sizeof(TR) == 4) {
@@ -1416,32 +1448,44 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm
       return fw_set_error(FW_E_CONFIG, "learned preconditioner: float32 grids only");
     }
   }
+  auto residual = [&]() {
+    hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n, rho,
+                       h->b64, h->x64, h->r64, (T*)h->r, h->eps_reg, h->part);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP,
+                       F_STORE);
+    hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, s, h->S, rtol, inner_floor);
+  };
+  if (have_residual)
+    hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, s, h->S, rtol, inner_floor);
+  else
+    residual();
+  FW_CHECK_LAUNCH("residual64");
+  bool known = false;   // S_host holds the outcome of the last k_round_init
   for (;;) {
-    if (outer == 0 && r0 >= 0.0) {
-      rnorm = r0;
-    } else {
-      hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga,
-                         n, rho, h->b64, h->x64, h->r64, (T*)h->r, h->eps_reg, h->part);
-      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP,
-                         F_STORE);
-      FW_CHECK_LAUNCH("residual64");
-      if ((rc = read_scalars(h, s))) return rc;
-      rnorm = sqrt(h->S_host[S_TMP]);
+    if (known) {
+      if (h->S_host[S_DONE] == 4.0)
+        return fw_set_error(FW_E_NONFINITE, "projection: NaN divergence");
+      if (h->S_host[S_DONE] == 2.0)
+        return fw_set_error(FW_E_NONFINITE, "projection: NaN residual");
+      if (h->S_host[S_DONE] == 3.0) break;
     }
-    if (!(rnorm == rnorm)) return fw_set_error(FW_E_NONFINITE, "projection: NaN residual");
-    if (rnorm <= tol_abs || bnorm == 0.0) break;
     if (total >= h->maxiter || outer >= 20) {
+      if (!known) {   // converged or not: read before deciding
+        if ((rc = read_scalars(h, s))) return rc;
+        known = true;
+        continue;
+      }
       h->last_iters = total;
       if (h->learned_fallback) {
         h->precond_enabled = FW_PRECOND_LEARNED;
         h->learned_fallback = 0;
       }
+      const double bn = sqrt(h->S_host[S_BB]);
       return fw_set_error(FW_E_NOCONV,
                           "projection PCG did not converge: ||r||/||b|| = %.3e after "
-                          "%d iterations (rtol %.1e)", rnorm / bnorm, total, rtol);
+                          "%d iterations (rtol %.1e)", sqrt(h->S_host[S_TMP]) / bn, total,
+                          rtol);
     }
-    double inner_rel = tol_abs / rnorm;
-    if (inner_rel < inner_floor) inner_rel = inner_floor;
     int it = 0;
     h->round = outer;
     // the learned mode gets an iteration budget per inner solve; a round
@@ -1450,7 +1494,11 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm
     const bool learned = h->precond_enabled == FW_PRECOND_LEARNED;
     int budget = h->maxiter - total;
     if (learned && budget > h->learned_budget) budget = h->learned_budget;
-    if ((rc = pcg_inner<T, TR>(h, rho, inner_rel * rnorm, budget, &it, s))) return rc;
+    if ((rc = pcg_inner<T, TR>(h, rho, budget, &it, s))) return rc;
+    known = true;
+    if (h->S_host[S_DONE] == 4.0)
+      return fw_set_error(FW_E_NONFINITE, "projection: NaN divergence");
+    if (h->S_host[S_DONE] == 3.0) break;   // converged before this round
     if (learned && h->S_host[S_DONE] != 1.0) {
       h->precond_enabled = FW_PRECOND_SEPARABLE;
       h->learned_fallback = 1;
@@ -1459,7 +1507,20 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, double bnorm
     ++outer;
     hipLaunchKernelGGL(k_x64_add<T>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64,
                        (const T*)h->d);
+    residual();
     FW_CHECK_LAUNCH("x64_add");
+    if (outer < expected_rounds && total < h->maxiter && outer < 20) {
+      known = false;   // the next round's first batch read will tell
+    } else {
+      if ((rc = read_scalars(h, s))) return rc;
+    }
+  }
+  const double bnorm = sqrt(h->S_host[S_BB]);
+  const double rnorm = sqrt(h->S_host[S_TMP]);
+  if (bnorm == 0.0) {
+    // scipy's bicgstab returns x = 0 for b = 0
+    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64);
+    FW_CHECK_LAUNCH("zero");
   }
   h->last_iters = total;
   h->last_outer = outer;
@@ -1486,26 +1547,14 @@ static int project_impl(ProjHandle* h, void* u, void* v, void* w,
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64);
     h->have_prev = true;
   }
-  // the warm start's residual, read together with ||b||
+  // the warm start's residual, decided on together with ||b|| on the device
   hipLaunchKernelGGL((k_residual64<T, T>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
                      (const T*)rho, h->b64, h->x64, h->r64, (T*)h->r, h->eps_reg, h->part);
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP,
                      F_STORE);
   FW_CHECK_LAUNCH("residual64");
-  int rc = read_scalars(h, s);
+  int rc = solve_refined<T, T>(h, (const T*)rho, h->rtol, true, s);
   if (rc) return rc;
-  const double bnorm = sqrt(h->S_host[S_BB]);
-  const double r0 = sqrt(h->S_host[S_TMP]);
-  if (!(bnorm == bnorm)) return fw_set_error(FW_E_NONFINITE, "projection: NaN divergence");
-  if (bnorm == 0.0) {
-    // scipy's bicgstab returns x = 0 for b = 0
-    hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64);
-    h->last_iters = 0;
-    h->last_rel = 0.0;
-  } else {
-    rc = solve_refined<T, T>(h, (const T*)rho, h->rtol, bnorm, r0, s);
-    if (rc) return rc;
-  }
   hipLaunchKernelGGL((k_correct<T, T>), dim3(nblk(n)), dim3(256), 0, s, h->ga, n,
                      (T*)u, (T*)v, (T*)w, (const double*)h->x64, (const T*)rho, dt);
   hipLaunchKernelGGL(k_div_check<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
@@ -1831,13 +1880,10 @@ int fw_proj_solve(void* handle, const void* rho, const double* b, double* x,
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
                      S_BB, F_STORE);
   FW_CHECK_LAUNCH("solve setup");
-  int rc0 = read_scalars(h, s);
-  if (rc0) return rc0;
-  const double bnorm = sqrt(h->S_host[S_BB]);
   h->have_prev = true;
   int rc = h->dtype == FW_F64
-               ? solve_refined<double, double>(h, (const double*)rho, rtol, bnorm, -1.0, s)
-               : solve_refined<float, float>(h, (const float*)rho, rtol, bnorm, -1.0, s);
+               ? solve_refined<double, double>(h, (const double*)rho, rtol, false, s)
+               : solve_refined<float, float>(h, (const float*)rho, rtol, false, s);
   if (iters) *iters = h->last_iters;
   if (rel_resid) *rel_resid = h->last_rel;
   if (rc) return rc;

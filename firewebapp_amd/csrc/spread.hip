@@ -73,6 +73,7 @@ static const int N_SUB = 10, REINIT_INTERVAL = 10, REINIT_SUBSTEPS = 5;
 struct SpeciesTimer {
   bool on = false;
   int used = 0, launches = 0;
+  int stride = 1, seen = 0;   // time every stride-th launch
   static const int CAP = 4096;
   hipEvent_t ev[2 * CAP];
   bool made = false;
@@ -96,6 +97,7 @@ using namespace fw;
 extern "C" {
 
 int fw_spread_timer(int enable) {
+  FW_REQUIRE(enable >= 0, "spread_timer: negative stride");
   if (enable && !g_timer.made) {
     for (int q = 0; q < 2 * SpeciesTimer::CAP; ++q)
       FW_CHECK_HIP(hipEventCreate(&g_timer.ev[q]));
@@ -104,6 +106,8 @@ int fw_spread_timer(int enable) {
   if (enable) {
     g_timer.used = 0;
     g_timer.launches = 0;
+    g_timer.seen = 0;
+    g_timer.stride = enable;
   }
   g_timer.on = enable != 0;
   return FW_OK;
@@ -295,7 +299,8 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                0.34, 1100.0, dt_sub, 1, 0, 1.3, 17.0e6, f->edc_work, stream));
     // species with the dilution sources; the Y_O2 cap of :3059 rides in
     // y_max, the [0, 1] clip is the kernel's own
-    const bool timed = g_timer.on && g_timer.used < SpeciesTimer::CAP;
+    const bool timed = g_timer.on && g_timer.used < SpeciesTimer::CAP &&
+                       (g_timer.seen++ % g_timer.stride) == 0;
     if (timed) {
       g_species3_events.start = g_timer.ev[2 * g_timer.used];
       g_species3_events.stop = g_timer.ev[2 * g_timer.used + 1];

@@ -587,10 +587,10 @@ int fw_spread_reduce(const fw_grid* g, const fw_spread_fields* f,
                      fw_spread_state* st, void* stream);
 
 /* Benchmark timing: while enabled, fw_spread_step records a HIP event pair
- * on its stream around each sub-step's k_species3 launch (at most
- * 4096 pairs).  fw_spread_timer(1) resets and starts, fw_spread_timer(0)
- * stops; fw_spread_timer_read waits for the events and returns the summed
- * milliseconds and the number of launches covered.  The timer is process
+ * on its stream around every k-th sub-step k_species3 launch (at most
+ * 4096 pairs).  fw_spread_timer(k), k >= 1, resets and starts with stride
+ * k, fw_spread_timer(0) stops; fw_spread_timer_read waits for the events and
+ * returns the summed milliseconds and the number of launches covered.  The timer is process
  * wide: enable it only while one thread drives steps. */
 int fw_spread_timer(int enable);
 int fw_spread_timer_read(double* total_ms, int* launches);
