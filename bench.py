@@ -4,7 +4,8 @@ Workload (default): 256 x 64 x 48 cells at dx = dy = 0.1 m (25.6 x 6.4 x
 4.51 m), float32 grid fields, grass bed 0.21 m deep in 5 layers from
 x = 1 m (bulk density 1.4762 kg/m^3 = 0.31 kg/m^2, SAV 9770 1/m), moisture
 drawn U(2 %, 10 %) from a seeded PCG64, 4 particles per bed cell with seeded
-+-10 % load jitter (314,880 particles), log-law inflow U10 = 4 m/s, k-epsilon,
++-10 % load and depth jitter (314,880 particles), log-law inflow U10 = 4 m/s,
+k-epsilon,
 EDC, DOM S8 (80 directions) every 5th step, fft-PCG projection (rtol 1e-6,
 outer divergence tolerance 1e-3), passive level set, solid-phase ignition.
 
@@ -56,6 +57,17 @@ WORKLOADS = {
         wall_bl_first_dz=0.042, wall_bl_growth=1.0, atm_uniform_dz=0.1,
         dom_quadrature=8, moisture_min=0.02, moisture_max=0.10, moisture_seed=1,
         particle_jitter=0.10, particle_jitter_seed=7),
+    # 512 x 128 x 64: a lognormal fuel depth field (mean 0.21 m, CV 0.3,
+    # correlation length 1 m) in 16 bed layers spanning 0.42 m (dz 0.02625;
+    # deeper columns clipped) + 48 air cells of 0.1 m; U10 = 6 m/s with
+    # +-15 % gusts at a 5 s period
+    "spread3d_512x128x64_s8": dict(
+        lx=51.2, ly=12.8, lz=5.22, dx=0.1, dy=0.1, n_z_bed=16, wall_bl_n=1,
+        wall_bl_first_dz=0.02625, wall_bl_growth=1.0, atm_uniform_dz=0.1,
+        dom_quadrature=8, moisture_min=0.02, moisture_max=0.10, moisture_seed=5,
+        particle_jitter=0.10, particle_jitter_seed=11, fuel_depth_cv=0.3,
+        fuel_depth_corr_length_m=1.0, fuel_depth_seed=3, fuel_depth_max_m=0.42,
+        wind_gust_amplitude=0.15, wind_gust_period_s=5.0, wind_gust_seed=9),
     # 128 x 32 x 32: n_z_bed 2 (dz 0.105) + 30 air cells of 0.1 m
     "spread3d_128x32x32_s4": dict(
         lx=12.8, ly=3.2, lz=3.2, dx=0.1, dy=0.1, n_z_bed=2, wall_bl_n=1,
@@ -72,7 +84,13 @@ COMMON = dict(
     solid_phase_ignition_enable=True, solid_phase_ignition_t_s_k=1000.0,
     ignition_duration_s=3.0, ignition_q_mult=3.0, ignition_width_mult=3.0,
     min_dt_s=1.0e-4, max_wall_time_s=1.0e9)
-WIND = 4.0
+WIND = {"spread3d_512x128x64_s8": 6.0}
+
+
+def wind(name):
+    """U10 of a workload (4 m/s unless it names another)."""
+    return WIND.get(name, 4.0)
+
 
 # the roofline kernel and its algorithmic fields per cell per launch:
 # k_species3 reads Y_fuel, Y_O2, Y_H2O, rho, u, v, w, S_pyro part, S_dry part
@@ -201,15 +219,11 @@ def species_timer_read():
 def cpu_baseline(name, budget_s):
     """Oracle float64 step on one core over a bounded sample of the workload."""
     from threadpoolctl import threadpool_limits
-    from firewebapp_amd.grid import Grid3D
-    from firewebapp_amd.spread_3d import Spread3DConfig, particle_mass_scale
+    from firewebapp_amd.spread_3d import Spread3DConfig, bed_inputs, build_grid
     from oracle.step import OracleSpread3D
-    cfg = Spread3DConfig.resolve(deck(name, "float64"), wind_speed_m_s=WIND)
-    g = Grid3D.build(Lx=cfg.Lx, Ly=cfg.Ly, Lz=cfg.Lz, dx=cfg.dx, dy=cfg.dy,
-                     h_bed=cfg.h_bed, n_z_bed=cfg.n_z_bed, **cfg.mesh_kwargs)
-    nb = int(g.n_z_bed * g.Ny * (np.sum((g.x_mid >= cfg.bed_x_start) &
-                                        (g.x_mid <= cfg.bed_x_end))))
-    o = OracleSpread3D(cfg, g, particle_mass_scale(cfg, nb * cfg.n_per_cell))
+    cfg = Spread3DConfig.resolve(deck(name, "float64"), wind_speed_m_s=wind(name))
+    g = build_grid(cfg)
+    o = OracleSpread3D(cfg, g, bed_inputs(cfg, g))
     n = 0
     with threadpool_limits(1):
         t0 = time.perf_counter()
@@ -254,7 +268,7 @@ def main(argv=None):
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    cfg = Spread3DConfig.resolve(deck(a.workload, a.dtype), wind_speed_m_s=WIND)
+    cfg = Spread3DConfig.resolve(deck(a.workload, a.dtype), wind_speed_m_s=wind(a.workload))
     sv = Spread3DSolver(cfg)
     g = sv.grid
     cells = g.Nx * g.Ny * g.Nz
@@ -271,6 +285,10 @@ def main(argv=None):
             raise RuntimeError(f"run stopped early: {sv.stop_reason}")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # device memory in use by this process (the library's own allocations,
+    # the DOM intensities among them, are outside torch's allocator)
+    free_b, total_b = torch.cuda.mem_get_info()
+    hbm_used = total_b - free_b
     species_timer(False)
     print(f"# timed {a.steps} steps in {wall:.3f} s, t = {sv.t:.4f} s", file=sys.stderr,
           flush=True)
@@ -285,11 +303,15 @@ def main(argv=None):
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32" if a.dtype == "float32" else "f64",
-        "data": "synthetic deck (seeded moisture and particle load jitter)",
+        "data": "synthetic deck (seeded moisture, particle load and depth jitter"
+                + (", lognormal fuel depth field" if cfg.fuel_depth_cv > 0.0 else "")
+                + (", inlet gusts" if cfg.gust_amplitude > 0.0 else "") + ")",
         "config": {"workload": a.workload, "cells": [g.Nx, g.Ny, g.Nz],
                    "particles": sv.n_particles, "dom": f"S{cfg.dom_quadrature}",
                    "dom_every": cfg.dom_subcycle_every, "n_sub": 10,
-                   "projection": "fft-PCG rtol 1e-6", "parallelism": "single GPU"},
+                   "projection": "fft-PCG rtol 1e-6", "parallelism": "single GPU",
+                   "u10_m_s": cfg.wind_speed_m_s,
+                   "hbm_used_gib": hbm_used / 2.0 ** 30},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": ROOFLINE_TRAFFIC.get((a.workload, a.dtype)), "bytes_per_launch": bytes_launch,

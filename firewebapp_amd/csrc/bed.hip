@@ -26,6 +26,7 @@ struct BedArgs {
   double* m_water_0; double* sav; double* m_char_max;
   signed char* alive;
   int n;
+  const double* h_top;   // per-particle fuel top, or null: the scalar h_bed
 };
 
 // kinetic constants (pyrolysis_3d.py:45, :117, :217-267, :446-562)
@@ -229,7 +230,7 @@ k_bed_particles(GridArgs g, BedArgs B, fw_bed_params P, const int* __restrict__ 
           Q_ext_p = (double)Qext_grid[c] * V_cell / P.n_per_cell_for_split;
         double fgeom = 1.0;
         if (P.view_factor_geometric) {
-          double ha = P.h_bed - B.z[p];
+          double ha = (B.h_top ? B.h_top[p] : P.h_bed) - B.z[p];
           if (ha < 0.0) ha = 0.0;
           fgeom = exp(-P.kappa_bed_eff * ha);
         }
@@ -452,7 +453,7 @@ static int bed_step_impl(const fw_grid* g, const fw_bed_particles* bp,
                          void* const* src, double* contrib, int* counters,
                          double* diag_rec, double* diag_out, hipStream_t s) {
   BedArgs B{bp->x, bp->y, bp->z, bp->m_solid, bp->m_water, bp->m_char, bp->T_s,
-            bp->m_water_0, bp->sav, bp->m_char_max, bp->alive, bp->n};
+            bp->m_water_0, bp->sav, bp->m_char_max, bp->alive, bp->n, bp->h_top};
   GridArgs ga = grid_args(g);
   FW_CHECK_HIP(hipMemsetAsync(counters, 0, 2 * sizeof(int), s));
   if (bp->n > 0) {

@@ -12,7 +12,11 @@ Keys this build adds (all additive; absent keys reproduce the reference):
 ``n_steps`` (fixed outer-step count), ``dom_quadrature`` (4 or 8),
 ``moisture_min`` / ``moisture_max`` / ``moisture_seed`` (per-seed moisture
 draw), ``particle_jitter`` / ``particle_jitter_seed`` (per-particle +-load
-jitter), ``dtype`` (float32 / float64).
+and +-depth jitter), ``fuel_depth_cv`` / ``fuel_depth_corr_length_m`` /
+``fuel_depth_seed`` / ``fuel_depth_max_m`` (a seeded lognormal fuel depth
+field with mean fuel_depth_m; the bed layers span fuel_depth_max_m),
+``wind_gust_amplitude`` / ``wind_gust_period_s`` / ``wind_gust_seed`` (seeded
+inlet gusts), ``dtype`` (float32 / float64).
 
 A new ``batch.*`` section (raw value strings, keys lower-cased) describes a
 sweep of cases over the deck; ``firewebapp_amd.batch`` expands it into the

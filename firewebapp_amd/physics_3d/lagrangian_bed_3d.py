@@ -40,7 +40,8 @@ _STATE = ("x", "y", "z", "m_solid", "m_water", "m_char", "T_s", "m_water_0",
 class FwBedParticles(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in _STATE] + [("alive", C.c_void_p),
                                                      ("n", C.c_int32),
-                                                     ("reserved", C.c_int32)]
+                                                     ("reserved", C.c_int32),
+                                                     ("h_top", C.c_void_p)]
 
 
 class FwBedIndex(C.Structure):
@@ -174,8 +175,14 @@ class DeviceParticles:
                            else torch.as_tensor(np.ascontiguousarray(np.asarray(a)[:self.n],
                                                                      dtype=np.int8),
                                                 device=dev))
+        h = buf.get("h_top")
+        if h is not None:
+            self.t["h_top"] = (h[:self.n] if isinstance(h, torch.Tensor) and h.is_cuda
+                               else torch.as_tensor(np.ascontiguousarray(
+                                   np.asarray(h)[:self.n], dtype=np.float64), device=dev))
         self.c = FwBedParticles(*[self.t[k].data_ptr() for k in _STATE],
-                                self.t["alive"].data_ptr(), self.n, 0)
+                                self.t["alive"].data_ptr(), self.n, 0,
+                                self.t["h_top"].data_ptr() if h is not None else None)
 
     def write_back(self, buf):
         torch.cuda.current_stream().synchronize()

@@ -41,6 +41,7 @@ import torch
 from . import _lib
 from .deck import load_text_input, outdoor_env_from_dict
 from .device import DeviceGrid, Marshal, device, grid_for, ptr, stream_ptr
+from .bed_variation import GustSchedule, bed_variation, fuel_depth_field
 from .grid import Grid3D, build_u_inlet, midflame_wind_speed
 from .physics_3d import dom_3d as _dom
 from .physics_3d import lagrangian_bed_3d as _lb
@@ -209,6 +210,14 @@ class Spread3DConfig:
     empirical_ros_enable: bool = False
     empirical_ros_m_s: float = 0.0
     empirical_blend_w: float = 0.0
+    # seeded bed and inflow variation (no reference counterpart)
+    fuel_depth_cv: float = 0.0
+    fuel_depth_corr_m: float = 1.0
+    fuel_depth_seed: int = 0
+    bed_zone_m: float = 0.0          # mesh height of the bed layers (0: h_bed)
+    gust_amplitude: float = 0.0
+    gust_period_s: float = 5.0
+    gust_seed: int = 0
 
     @classmethod
     def resolve(cls, ri_or_path, wind_speed_m_s=0.0, **kw):
@@ -380,18 +389,47 @@ class Spread3DConfig:
             level_set_passive=passive, lagrangian_bed=lagrangian, h_conv_mult=h_conv_mult,
             pyro_A_p=pyro_A_p, pyro_E_p=pyro_E_p, empirical_ros_enable=emp_on,
             empirical_ros_m_s=float(emp_ros), empirical_blend_w=float(emp_w),
+            fuel_depth_cv=float(deck_first("fuel_depth_cv", 0.0)),
+            fuel_depth_corr_m=float(deck_first("fuel_depth_corr_length_m", 1.0)),
+            fuel_depth_seed=int(deck_first("fuel_depth_seed", 0)),
+            bed_zone_m=float(deck_first("fuel_depth_max_m", 0.0)),
+            gust_amplitude=float(deck_first("wind_gust_amplitude", 0.0)),
+            gust_period_s=float(deck_first("wind_gust_period_s", 5.0)),
+            gust_seed=int(deck_first("wind_gust_seed", 0)),
         )
+        if out.fuel_depth_cv > 0.0 and out.bed_zone_m < out.h_bed:
+            raise ValueError("fuel_depth_cv > 0 needs fuel_depth_max_m >= fuel_depth_m "
+                             "(the mesh height of the bed layers; deeper columns are clipped)")
         if kw:
             raise TypeError(f"unsupported keyword arguments: {sorted(kw)}")
         return out
 
 
-def particle_mass_scale(cfg: Spread3DConfig, n: int):
-    """Seeded per-particle load factor 1 + jitter * U(-1, 1) (PCG64), or None."""
-    if cfg.particle_jitter <= 0.0:
-        return None
-    rng = np.random.default_rng(cfg.particle_jitter_seed)
-    return 1.0 + cfg.particle_jitter * rng.uniform(-1.0, 1.0, n)
+def build_grid(cfg: Spread3DConfig) -> Grid3D:
+    """The configuration's mesh; with a fuel depth field its bed layers span
+    fuel_depth_max_m instead of the mean depth."""
+    zone = cfg.bed_zone_m if cfg.bed_zone_m > 0.0 else cfg.h_bed
+    return Grid3D.build(Lx=cfg.Lx, Ly=cfg.Ly, Lz=cfg.Lz, dx=cfg.dx, dy=cfg.dy, h_bed=zone,
+                        n_z_bed=cfg.n_z_bed, **cfg.mesh_kwargs)
+
+
+def bed_inputs(cfg: Spread3DConfig, g: Grid3D):
+    """(fill, m_scale, h_top) of the configuration's seeded bed variation
+    (bed_variation.bed_variation; all None for the reference's uniform
+    bed): the lognormal fuel depth field clipped to the bed layers' height,
+    and the per-particle load and depth jitter."""
+    i0 = int(np.searchsorted(g.x_mid, cfg.bed_x_start, side="left"))
+    i1 = int(np.searchsorted(g.x_mid, cfg.bed_x_end, side="right"))
+    mask = np.zeros((g.Nz, g.Ny, g.Nx), bool)
+    mask[:g.n_z_bed, :, i0:i1] = True
+    depth = None
+    if cfg.fuel_depth_cv > 0.0:
+        depth = fuel_depth_field(g.Nx, g.Ny, g.dx, g.dy, cfg.h_bed, cfg.fuel_depth_cv,
+                                 cfg.fuel_depth_corr_m, cfg.fuel_depth_seed)
+        depth = np.minimum(depth, float(g.z_face[g.n_z_bed]))
+    jit = cfg.particle_jitter if cfg.lagrangian_bed else 0.0
+    return bed_variation(mask, g.z_face, g.n_z_bed, cfg.n_per_cell, cfg.h_bed, depth, jit,
+                         cfg.particle_jitter_seed)
 
 
 @dataclass
@@ -523,8 +561,7 @@ class Spread3DSolver:
         self.proj = None
         self.dev = device()
         self.dtype = cfg.dtype
-        g = grid or Grid3D.build(Lx=cfg.Lx, Ly=cfg.Ly, Lz=cfg.Lz, dx=cfg.dx, dy=cfg.dy,
-                                 h_bed=cfg.h_bed, n_z_bed=cfg.n_z_bed, **cfg.mesh_kwargs)
+        g = grid or build_grid(cfg)
         if g.n_z_bed <= 0:
             raise ValueError("the GPU step needs a fuel bed (n_z_bed > 0)")
         self.grid = g
@@ -556,6 +593,9 @@ class Spread3DSolver:
         self.i_bed = (i0, i1)
         alpha = np.zeros(self.shape)
         alpha[:g.n_z_bed, :, i0:i1] = cfg.rho_b / _RHO_PARTICLE
+        self.bed_fill, self.m_scale, self.h_top = bed_inputs(cfg, g)
+        if self.bed_fill is not None:
+            alpha *= self.bed_fill
         f["alpha_s"].copy_(torch.as_tensor(alpha))
         n_src = max(1, int(round(0.5 * cfg.ignition_width_mult / g.dx)))
         s0, s1 = i0, min(i1, i0 + n_src)
@@ -575,6 +615,8 @@ class Spread3DSolver:
         self.v_inlet = torch.zeros_like(self.u_inlet)
         self.w_inlet = torch.zeros_like(self.u_inlet)
         f["u"].copy_(self.u_inlet[:, :, None].expand(self.shape))
+        self.gust = GustSchedule(cfg.gust_amplitude, cfg.gust_period_s, cfg.gust_seed)
+        self.u_inlet_base = torch.as_tensor(u_in, dtype=torch.float64, device=dev)
         Ns = max(3, min(int(round(0.5 / g.dx)), g.Nx // 4))
         sig = np.zeros(g.Nx)
         for i in range(g.Nx - Ns, g.Nx):
@@ -709,7 +751,9 @@ class Spread3DSolver:
         n_alloc = _lb.initialize_bed_particles_from_alpha_s(
             buf, alpha, cfg.rho_b, cfg.moisture, T_amb, g.dx, g.dy, g.dz_arr,
             g.n_z_bed, cfg.n_per_cell, sav=cfg.sigma_sav, i_lo=i0, i_hi=i1,
-            mass_scale=particle_mass_scale(cfg, n_cells * cfg.n_per_cell))
+            mass_scale=self.m_scale)
+        if self.h_top is not None:
+            buf["h_top"] = np.asarray(self.h_top, np.float64)[:n_alloc].copy()
         if n_alloc <= 0:
             raise ValueError("no bed particles were allocated")
         self.n_particles = n_alloc
@@ -745,6 +789,9 @@ class Spread3DSolver:
         m[:g.n_z_bed, :, i0:i1] = cfg.rho_b * 1.0
         w = np.zeros(self.shape)
         w[:g.n_z_bed, :, i0:i1] = cfg.rho_b * cfg.moisture
+        if self.bed_fill is not None:
+            m *= self.bed_fill
+            w *= self.bed_fill
         self.eu = {"m_solid": torch.as_tensor(m, **f64), "m_char": torch.zeros(self.shape, **f64),
                    "m_water": torch.as_tensor(w, **f64),
                    "Q_pyro": torch.zeros(self.shape, dtype=self.dtype, device=dev)}
@@ -803,6 +850,9 @@ class Spread3DSolver:
         """Advance one outer step in the native driver; returns the stop reason
         (``extinct``, ``front_exit``, ``domain_end``, ``stalled``) or None."""
         st = self.state
+        if self.gust.enabled:
+            # the inlet of a step starting at t: the log law times g(t)
+            self.u_inlet.copy_(self.u_inlet_base * self.gust.factor(st.t))
         rc = _lib.lib().fw_spread_step(
             self.dg.ref, self.dg64.ref, C.byref(self.fields), C.byref(self.params),
             C.byref(st), self.proj, self.dom_h, C.byref(self.P.c),

@@ -301,6 +301,11 @@ typedef struct fw_bed_particles {
   int8_t* alive;
   int32_t n;
   int32_t reserved;
+  /* Optional per-particle top of the fuel above the particle's column (m),
+   * for the canopy attenuation exp(-kappa (h_top - z)) of the geometric view
+   * factor; NULL: the scalar h_bed of fw_bed_params for every particle (the
+   * reference's uniform bed).  Set by a seeded depth field or depth jitter. */
+  double* h_top;
 } fw_bed_particles;
 
 /* Particle -> cell map and its inverse: cell[p] (flat (k,j,i) index or -1)

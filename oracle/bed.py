@@ -179,6 +179,10 @@ def step_bed_particles(buf, T_g, Y_O2, Q_ext, n_per_cell, dx, dy, dz_arr, z_face
     sav = buf["sav"][idx]
     mcmax = buf["m_char_max"][idx].copy()
     z = buf["z"][idx]
+    # the fuel top above each particle: the buffer's optional per-particle
+    # h_top (a seeded depth field or depth jitter; no reference counterpart),
+    # else the scalar h_bed of the reference's uniform bed
+    h_top = buf["h_top"][idx] if "h_top" in buf else h_bed
     n = idx.size
     # drying (Arrhenius / combined)
     dm_evap = np.zeros(n)
@@ -280,7 +284,7 @@ def step_bed_particles(buf, T_g, Y_O2, Q_ext, n_per_cell, dx, dy, dz_arr, z_face
     Q_dry = -dm_evap * L_VAP_WATER * inv_dt
     Q_ext_p = (Q_ext.reshape(-1)[cell] * V_cell / n_per_cell) if n_per_cell > 0 else np.zeros(n)
     if view_factor_geometric:
-        h_above = h_bed - z
+        h_above = h_top - z
         h_above = np.where(h_above < 0.0, 0.0, h_above)
         fgeom = np.exp(-kappa_bed_eff * h_above)
     else:

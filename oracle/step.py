@@ -70,7 +70,12 @@ def log_law(z, U_ref, z_ref=10.0, z_0=0.01):
 
 
 class OracleSpread3D:
-    def __init__(self, cfg, grid, particle_mass_scale=None):
+    def __init__(self, cfg, grid, bed=None):
+        """bed: (fill, m_scale, h_top) of a seeded bed variation (no reference
+        counterpart; None parts keep the reference's uniform bed): fill
+        multiplies alpha_s and the Eulerian pools per cell, m_scale the
+        particles' masses, h_top is the per-particle fuel top."""
+        fill, particle_mass_scale, h_top = bed if bed is not None else (None, None, None)
         self.cfg, self.g = cfg, grid
         g = grid
         sh = (g.Nz, g.Ny, g.Nx)
@@ -91,6 +96,8 @@ class OracleSpread3D:
             raise ValueError("empty fuel bed")
         self.i_bed = (i0, i1)
         self.alpha_s[:g.n_z_bed, :, i0:i1] = cfg.rho_b / RHO_PARTICLE
+        if fill is not None:
+            self.alpha_s *= fill
         self.eulerian = not getattr(cfg, "lagrangian_bed", True)
         # Eulerian pools (spread_3d.py:1186-1195, :1274-1278)
         self.eu_m_solid, self.eu_m_char, self.eu_m_water = z(), z(), z()
@@ -99,6 +106,9 @@ class OracleSpread3D:
         if self.eulerian:
             self.eu_m_solid[:g.n_z_bed, :, i0:i1] = cfg.rho_b * 1.0
             self.eu_m_water[:g.n_z_bed, :, i0:i1] = cfg.rho_b * cfg.moisture
+            if fill is not None:
+                self.eu_m_solid *= fill
+                self.eu_m_water *= fill
         # bed particles (spread_3d.py:1300-1330)
         nb = 0 if self.eulerian else int((self.alpha_s[:g.n_z_bed, :, i0:i1] > 0).sum())
         self.buf = Bd.allocate_bed_particle_buffers(nb * cfg.n_per_cell)
@@ -106,6 +116,8 @@ class OracleSpread3D:
             self.buf, self.alpha_s, cfg.rho_b, cfg.moisture, T_amb, g.dx, g.dy,
             g.dz_arr, g.n_z_bed, cfg.n_per_cell, sav=cfg.sigma_sav,
             m_scale=particle_mass_scale)
+        if h_top is not None and self.n_alloc > 0:
+            self.buf["h_top"] = np.asarray(h_top, np.float64)[:self.n_alloc].copy()
         # ignition patch + solid-phase seeding (:1486-1526)
         n_src = max(1, int(round(0.5 * cfg.ignition_width_mult / g.dx)))
         s0, s1 = i0, min(i1, i0 + n_src)
@@ -121,6 +133,14 @@ class OracleSpread3D:
         self.u_inlet = np.zeros((g.Nz, g.Ny))
         for k in range(g.Nz):
             self.u_inlet[k, :] = log_law(g.z_mid[k], cfg.wind_speed_m_s)
+        # seeded inlet gusts (no reference counterpart): the log law times
+        # g(t), g piecewise half-cosine between nodes 1 + a_m at t = m T,
+        # a_0 = 0, a_m = A (2 U_m - 1) with U_m the m-th PCG64(seed) draw
+        self.u_inlet_base = self.u_inlet.copy()
+        self.gust_A = float(getattr(cfg, "gust_amplitude", 0.0))
+        self.gust_T = float(getattr(cfg, "gust_period_s", 5.0))
+        self._gust_rng = np.random.default_rng(int(getattr(cfg, "gust_seed", 0)))
+        self._gust_nodes = [0.0]
         self.u[:] = self.u_inlet[:, :, None]
         self.v_inlet = np.zeros((g.Nz, g.Ny))
         self.w_inlet = np.zeros((g.Nz, g.Ny))
@@ -182,9 +202,22 @@ class OracleSpread3D:
         self.stop_reason = None
 
     # -----------------------------------------------------------------------
+    def gust(self, t):
+        """g(t) of the inlet gusts (1 without them)."""
+        if not (self.gust_A > 0.0 and self.gust_T > 0.0):
+            return 1.0
+        m = int(math.floor(t / self.gust_T))
+        while len(self._gust_nodes) <= m + 1:
+            self._gust_nodes.append(self.gust_A * (2.0 * float(self._gust_rng.random()) - 1.0))
+        a0, a1 = self._gust_nodes[m], self._gust_nodes[m + 1]
+        f = t / self.gust_T - m
+        return 1.0 + a0 + (a1 - a0) * 0.5 * (1.0 - math.cos(math.pi * f))
+
     def step(self):
         """One pass of the loop body; returns a stop reason or None."""
         cfg, g = self.cfg, self.g
+        if self.gust_A > 0.0:
+            self.u_inlet[...] = self.u_inlet_base * self.gust(self.t)
         T_amb = cfg.T_amb
         dx, dy, dz, da, db = g.dx, g.dy, g.dz_arr, g.d_face_above, g.d_face_below
         self.step_no += 1
