@@ -491,14 +491,14 @@ k_p_update_flex(size_t n, T* __restrict__ p, const T* __restrict__ z,
     p[c] = z[c] + beta * p[c];
 }
 
-// r64 = b64 - A64 x64 (float64 operator from rho), r_t = (T) r64,
-// partial sum of r64^2
+// r = b64 - A64 x64 (float64 operator from rho), r_t = (T) r, partial sum
+// of r^2 (the float64 residual itself is not kept: only its norm and its
+// rounding to T are used)
 template <typename T, typename TR>
 __global__ void __launch_bounds__(256)
 k_residual64(GridArgs g, size_t n, const TR* __restrict__ rho,
              const double* __restrict__ b, const double* __restrict__ x,
-             double* __restrict__ r64, T* __restrict__ r_t, double eps_reg,
-             double* __restrict__ part) {
+             T* __restrict__ r_t, double eps_reg, double* __restrict__ part) {
   __shared__ double sh[256];
   double acc = 0.0;
   for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
@@ -506,7 +506,6 @@ k_residual64(GridArgs g, size_t n, const TR* __restrict__ rho,
     Cell e = cell_of(g, c);
     double ax = apply_A<double, double, TR>(g, rho, x, e.k, e.j, e.i, c, eps_reg);
     double r = b[c] - ax;
-    r64[c] = r;
     r_t[c] = (T)r;
     acc += r * r;
   }
@@ -1118,7 +1117,7 @@ struct ProjHandle {
   int round;          // refinement round of the running solve
   double *lx, *ly, *lz;  // eigenvalues (device, float64)
   // vectors
-  double *x64, *b64, *r64;
+  double *x64, *b64;
   void *r, *z, *p, *q, *d, *w1, *w2;
   double *part, *S, *S_host;
   double *Ppq, *Prr, *Prz;  // partials of the PCG iteration's reductions
@@ -1423,7 +1422,7 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
 // Solve A x = b: x64 (warm start on entry), b64 given, ||b||^2 in S_BB on
 // the device; ||b - A x||_2 <= rtol ||b||_2 with the residual recomputed in
 // float64 after each inner solve.  have_residual: ||b - A x||^2 of the warm
-// start is already in S_TMP (with r64 and r formed by residual64).
+// start is already in S_TMP (with r formed by residual64).
 //
 // The host reads the device scalars once per inner batch.  Round 0 starts
 // without a read: k_round_init decides on the device whether it runs.  A
@@ -1450,7 +1449,7 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
   }
   auto residual = [&]() {
     hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n, rho,
-                       h->b64, h->x64, h->r64, (T*)h->r, h->eps_reg, h->part);
+                       h->b64, h->x64, (T*)h->r, h->eps_reg, h->part);
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP,
                        F_STORE);
     hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, s, h->S, rtol, inner_floor);
@@ -1549,7 +1548,7 @@ static int project_impl(ProjHandle* h, void* u, void* v, void* w,
   }
   // the warm start's residual, decided on together with ||b|| on the device
   hipLaunchKernelGGL((k_residual64<T, T>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
-                     (const T*)rho, h->b64, h->x64, h->r64, (T*)h->r, h->eps_reg, h->part);
+                     (const T*)rho, h->b64, h->x64, (T*)h->r, h->eps_reg, h->part);
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP,
                      F_STORE);
   FW_CHECK_LAUNCH("residual64");
@@ -1705,7 +1704,6 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   const size_t n = h->n;
   FW_CHECK_HIP(hipMalloc((void**)&h->x64, n * 8));
   FW_CHECK_HIP(hipMalloc((void**)&h->b64, n * 8));
-  FW_CHECK_HIP(hipMalloc((void**)&h->r64, n * 8));
   void** vecs[] = {&h->r, &h->z, &h->p, &h->q, &h->d, &h->w1, &h->w2};
   for (void** v : vecs) FW_CHECK_HIP(hipMalloc(v, n * h->esz));
   FW_CHECK_HIP(hipMalloc((void**)&h->part, PJ_RED * sizeof(double)));
@@ -1729,7 +1727,7 @@ int fw_proj_destroy(void* handle) {
   ProjHandle* h = (ProjHandle*)handle;
   if (!h) return FW_OK;
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
-                  h->x64, h->b64, h->r64, h->r, h->z, h->p, h->q, h->d, h->w1,
+                  h->x64, h->b64, h->r, h->z, h->p, h->q, h->d, h->w1,
                   h->w2, h->part, h->S, h->twx, h->Ppq, h->irho, h->Yf_p, h->Yi_p,
                   h->Zf_p, h->Zi_p, h->Pqz, h->rs, h->qf};
   for (void* b : bufs)
