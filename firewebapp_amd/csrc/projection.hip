@@ -1313,7 +1313,8 @@ static int read_scalars(ProjHandle* h, hipStream_t s) {
 }
 
 // Inner PCG on A d = r (type T); r holds the right-hand side on entry and
-// the recursively updated residual on exit.  Stops when ||r||_2 <= tol_abs.
+// the recursively updated residual on exit.  Stops when ||r||_2 <= S_TOL,
+// the inner tolerance k_round_init left on the device.
 template <typename T, typename TR>
 static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStream_t s) {
   const size_t n = h->n;
