@@ -19,37 +19,11 @@
 
 namespace fw {
 
-#define EDC_SEG 256
-
 struct EdcParams {
   double chi_rad, cp_g, dt, s_stoich, hoc_J;
   double relax;   // 1 - exp(-(dt / n_substeps) / 0.5), formed on the host
   int n_substeps, extinction;
 };
-
-template <typename T>
-__device__ __forceinline__ bool edc_resets(T Yf, T YO2) {
-  return Yf > T(1e-9) && YO2 > T(1e-9);
-}
-
-template <typename T>
-__device__ __forceinline__ bool edc_own_flag(T Yw) {
-  T f = T(1) - fdiv(Yw, T(0.18));   // as f_h2o in k_edc
-  if (f < T(0)) f = T(0);
-  return Yw > T(0) && f < T(0.5);
-}
-
-// Index (0..255) of the last thread of the 256-thread workgroup whose
-// predicate holds, or -1; wmax[4] receives each wavefront's last index.
-__device__ __forceinline__ int block_last_true(bool pred, int* wmax) {
-  const int t = threadIdx.x, w = t >> 6;
-  const unsigned long long m = __ballot(pred);
-  if ((t & 63) == 0) wmax[w] = m ? w * 64 + 63 - __clzll(m) : -1;
-  __syncthreads();
-  int last = -1;
-  for (int q = 0; q < 4; ++q) last = wmax[q] > last ? wmax[q] : last;
-  return last;
-}
 
 // summary[b] = -1 (no resetting cell), 0 or 1 (flag of the last one)
 template <typename T>
@@ -192,21 +166,37 @@ k_edc(size_t n, const T* __restrict__ rho, T* __restrict__ Tg,
   omega_out[c] = P.dt > 0.0 ? fdiv(acc, T(P.dt)) : T(0);
 }
 
+// summary_ready: the segment summaries of the current Y_F, Y_O2, Y_H2O are
+// already in work (left by the species turbulent diffusion's last pass)
 template <typename T>
 static int edc_impl(const fw_grid* g, const void* rho, void* Tg, void* Yf,
                     void* YO2, const void* kt, const void* et, const void* Yw,
-                    void* omega, int* work, const EdcParams& P, hipStream_t s) {
+                    void* omega, int* work, const EdcParams& P, bool summary_ready,
+                    hipStream_t s) {
   const size_t n = (size_t)g->nx * g->ny * g->nz;
   const int nseg = (int)((n + EDC_SEG - 1) / EDC_SEG);
   int* summary = work;
-  hipLaunchKernelGGL(k_edc_summary<T>, dim3(nseg), dim3(256), 0, s, n,
-                     (const T*)Yf, (const T*)YO2, (const T*)Yw, summary);
-  FW_CHECK_LAUNCH("edc_summary");
+  if (!summary_ready) {
+    hipLaunchKernelGGL(k_edc_summary<T>, dim3(nseg), dim3(256), 0, s, n,
+                       (const T*)Yf, (const T*)YO2, (const T*)Yw, summary);
+    FW_CHECK_LAUNCH("edc_summary");
+  }
   hipLaunchKernelGGL(k_edc<T>, dim3(nseg), dim3(256), 0, s, n, (const T*)rho,
                      (T*)Tg, (T*)Yf, (T*)YO2, (const T*)kt, (const T*)et,
                      (const T*)Yw, (T*)omega, (const int*)summary, P);
   FW_CHECK_LAUNCH("edc");
   return FW_OK;
+}
+
+int edc_launch(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel, void* Y_O2,
+               const void* k_turb, const void* eps_turb, const void* Y_H2O, void* omega_out,
+               double chi_rad, double cp_g, double dt, int n_substeps, int extinction_enable,
+               double s_stoich, double hoc_J, int* work, bool summary_ready, hipStream_t s) {
+  const int nsub = n_substeps > 1 ? n_substeps : 1;
+  EdcParams P{chi_rad, cp_g, dt, s_stoich, hoc_J, 1.0 - exp(-(dt / nsub) / 0.5), n_substeps,
+              extinction_enable ? 1 : 0};
+  return FW_DISPATCH(g, edc_impl, g, rho, T_g, Y_fuel, Y_O2, k_turb, eps_turb, Y_H2O,
+                     omega_out, work, P, summary_ready, s);
 }
 
 }  // namespace fw
@@ -225,9 +215,7 @@ extern "C" int fw_edc(const fw_grid* g, const void* rho, void* T_g,
                       int* work, void* stream) {
   FW_GRID(g);
   FW_REQUIRE(work, "edc: null workspace");
-  const int nsub = n_substeps > 1 ? n_substeps : 1;
-  fw::EdcParams P{chi_rad, cp_g, dt, s_stoich, hoc_J, 1.0 - exp(-(dt / nsub) / 0.5),
-                  n_substeps, extinction_enable ? 1 : 0};
-  return FW_DISPATCH(g, fw::edc_impl, g, rho, T_g, Y_fuel, Y_O2, k_turb,
-                     eps_turb, Y_H2O, omega_out, work, P, (hipStream_t)stream);
+  return fw::edc_launch(g, rho, T_g, Y_fuel, Y_O2, k_turb, eps_turb, Y_H2O, omega_out,
+                        chi_rad, cp_g, dt, n_substeps, extinction_enable, s_stoich, hoc_J,
+                        work, false, (hipStream_t)stream);
 }

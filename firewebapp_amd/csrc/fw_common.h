@@ -27,6 +27,16 @@ struct Launch {
 // driver: Q_comb from the array, or (Q_comb == nullptr) formed from the
 // drip-torch gas share and q_rad_gas / dz with the solid share subtracted
 // from Q_pyro in the source bed cells first.
+// The species turbulent-diffusion pass; with edc_summary (Nx % 256 == 0) it
+// also leaves the EDC segment summaries of its output in edc_summary.
+int turb3_launch(const fw_grid* g, const void* const* f, void* const* fo, const void* nu_t,
+                 double sc_t, double dt_sub, const double* lo, const double* hi,
+                 int* edc_summary, hipStream_t s);
+// fw_edc; summary_ready: the summaries of the current species are in work.
+int edc_launch(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel, void* Y_O2,
+               const void* k_turb, const void* eps_turb, const void* Y_H2O, void* omega_out,
+               double chi_rad, double cp_g, double dt, int n_substeps, int extinction_enable,
+               double s_stoich, double hoc_J, int* work, bool summary_ready, hipStream_t s);
 int coupling_launch(const fw_grid* g, void* T_g, void* T_s, const void* rho, const void* u,
                     const void* v, const void* w, const void* alpha_s, double sigma_sav,
                     const void* q_rad_in, void* Q_pyro, const void* Q_comb,
@@ -211,6 +221,35 @@ __device__ __forceinline__ T muscl_face(T pm1, T p0, T pp1, T pp2, T uf) {
   const T fwd = p0 + T(0.5) * minmod<T>(p0 - pm1, pp1 - p0);
   const T bwd = pp1 - T(0.5) * minmod<T>(pp1 - p0, pp2 - pp1);
   return uf >= T(0) ? fwd : bwd;
+}
+
+// ---- EDC carried-flag scan helpers (edc.py:223) ----------------------------
+// The flag scan runs over fixed 256-cell segments of the flat index; a
+// segment summary is the flag of its last resetting cell, or -1.
+#define EDC_SEG 256
+
+template <typename T>
+__device__ __forceinline__ bool edc_resets(T Yf, T YO2) {
+  return Yf > T(1e-9) && YO2 > T(1e-9);
+}
+
+template <typename T>
+__device__ __forceinline__ bool edc_own_flag(T Yw) {
+  T f = T(1) - fdiv(Yw, T(0.18));   // as f_h2o in k_edc
+  if (f < T(0)) f = T(0);
+  return Yw > T(0) && f < T(0.5);
+}
+
+// Index (0..255) of the last thread of the 256-thread workgroup whose
+// predicate holds, or -1; wmax[4] receives each wavefront's last index.
+__device__ __forceinline__ int block_last_true(bool pred, int* wmax) {
+  const int t = threadIdx.x, w = t >> 6;
+  const unsigned long long m = __ballot(pred);
+  if ((t & 63) == 0) wmax[w] = m ? w * 64 + 63 - __clzll(m) : -1;
+  __syncthreads();
+  int last = -1;
+  for (int q = 0; q < 4; ++q) last = wmax[q] > last ? wmax[q] : last;
+  return last;
 }
 
 }  // namespace fw

@@ -294,9 +294,15 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
   const double drip_gas = (combustion && ign_on) ? (1.0 - F_DRIP_TO_SOLID) * Q_drip : 0.0;
   void** Y[3] = {&f->Y_fuel, &f->Y_O2, &f->Y_H2O};
   const double y_in[3] = {0.0, 0.232, 0.0}, y_max[3] = {1.0, 0.232, 1.0};
+  // the last species diffusion pass of a sub-step leaves the next EDC's
+  // segment summaries (nothing between them changes the species) when the
+  // launch rows are whole 256-cell segments
+  const bool fuse_summary = g->nx % EDC_SEG == 0 && nsc > 0;
+  bool summary_ready = false;
   for (int sub = 0; sub < N_SUB; ++sub) {
-    TRY(fw_edc(g, f->rho, f->T_g, f->Y_fuel, f->Y_O2, f->k, f->eps, f->Y_H2O, f->omega,
-               0.34, 1100.0, dt_sub, 1, 0, 1.3, 17.0e6, f->edc_work, stream));
+    TRY(edc_launch(g, f->rho, f->T_g, f->Y_fuel, f->Y_O2, f->k, f->eps, f->Y_H2O, f->omega,
+                   0.34, 1100.0, dt_sub, 1, 0, 1.3, 17.0e6, (int*)f->edc_work, summary_ready,
+                   s));
     // species with the dilution sources; the Y_O2 cap of :3059 rides in
     // y_max, the [0, 1] clip is the kernel's own
     const bool timed = g_timer.on && g_timer.used < SpeciesTimer::CAP &&
@@ -330,11 +336,13 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                             last ? y_max[2] : 0.0};
       const void* fin[3] = {*Y[0], *Y[1], *Y[2]};
       void* fout[3] = {f->tmp1, f->tmp2, f->tmp3};
-      TRY(fw_turbulent_diffusion_pass3(g, fin, fout, f->nu_t, SC_T, dsc, lo, hi, stream));
+      TRY(turb3_launch(g, fin, fout, f->nu_t, SC_T, dsc, lo, hi,
+                       fuse_summary && last ? (int*)f->edc_work : nullptr, s));
       SWAP(*Y[0], f->tmp1);
       SWAP(*Y[1], f->tmp2);
       SWAP(*Y[2], f->tmp3);
     }
+    summary_ready = fuse_summary;
     if (p->eulerian) {
       // drip-torch split, q_rad_gas into Q_comb and the gas-solid coupling
       // (:2916-2956, :3094-3099, :3148-3156); the solid share accumulates in

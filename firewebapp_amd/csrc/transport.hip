@@ -326,15 +326,31 @@ __global__ void __launch_bounds__(256)
 k_turb_diff3(GridArgs g, const T* __restrict__ f1, const T* __restrict__ f2,
              const T* __restrict__ f3, T* __restrict__ o1, T* __restrict__ o2,
              T* __restrict__ o3, const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo1,
-             T hi1, T lo2, T hi2, T lo3, T hi3) {
+             T hi1, T lo2, T hi2, T lo3, T hi3, int* __restrict__ edc_summary) {
   int i, j, k;
   cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const size_t c = cidx(g, k, j, i);
   const T D_t = nu_t[c] / sc_t;
-  o1[c] = turb_cell<T>(g, i, j, k, c, f1, D_t, dt_sub, lo1, hi1);
-  o2[c] = turb_cell<T>(g, i, j, k, c, f2, D_t, dt_sub, lo2, hi2);
-  o3[c] = turb_cell<T>(g, i, j, k, c, f3, D_t, dt_sub, lo3, hi3);
+  const T y1 = turb_cell<T>(g, i, j, k, c, f1, D_t, dt_sub, lo1, hi1);
+  const T y2 = turb_cell<T>(g, i, j, k, c, f2, D_t, dt_sub, lo2, hi2);
+  const T y3 = turb_cell<T>(g, i, j, k, c, f3, D_t, dt_sub, lo3, hi3);
+  o1[c] = y1;
+  o2[c] = y2;
+  o3[c] = y3;
+  if (edc_summary) {
+    // the EDC segment summary of the new (fuel, O2, H2O), as k_edc_summary
+    // would form it: the launch is 256 threads along x and Nx % 256 == 0
+    // (checked by the host), so the workgroup is one whole 256-cell segment
+    // and no thread returned above
+    __shared__ int wmax[4];
+    const int last = block_last_true(edc_resets<T>(y1, y2), wmax);
+    if (last < 0) {
+      if (threadIdx.x == 0) edc_summary[c / EDC_SEG] = -1;
+    } else if ((int)threadIdx.x == last) {
+      edc_summary[c / EDC_SEG] = edc_own_flag<T>(y3) ? 1 : 0;
+    }
+  }
 }
 
 // Max of nu_t over interior cells 1<=k<=Nz-2, 1<=i<=Nx-2, all j, starting
@@ -633,14 +649,22 @@ static int species3_impl(const fw_grid* g, const void* const* Y, void* const* Yo
 template <typename T>
 static int turb3_impl(const fw_grid* g, const void* const* f, void* const* fo,
                       const void* nu_t, double sc_t, double dt_sub, const double* lo,
-                      const double* hi, hipStream_t s) {
+                      const double* hi, int* edc_summary, hipStream_t s) {
   Launch L = cell_launch(g);
+  if (edc_summary && (g->nx % EDC_SEG != 0 || L.block.x != EDC_SEG))
+    return fw_set_error(FW_E_CONFIG, "turbulent_diffusion3: the EDC summary needs Nx %% 256 == 0");
   hipLaunchKernelGGL(k_turb_diff3<T>, L.grid, L.block, 0, s, grid_args(g), (const T*)f[0],
                      (const T*)f[1], (const T*)f[2], (T*)fo[0], (T*)fo[1], (T*)fo[2],
                      (const T*)nu_t, T(sc_t), T(dt_sub), T(lo[0]), T(hi[0]), T(lo[1]),
-                     T(hi[1]), T(lo[2]), T(hi[2]));
+                     T(hi[1]), T(lo[2]), T(hi[2]), edc_summary);
   FW_CHECK_LAUNCH("turbulent_diffusion3");
   return FW_OK;
+}
+
+int turb3_launch(const fw_grid* g, const void* const* f, void* const* fo, const void* nu_t,
+                 double sc_t, double dt_sub, const double* lo, const double* hi,
+                 int* edc_summary, hipStream_t s) {
+  return FW_DISPATCH(g, turb3_impl, g, f, fo, nu_t, sc_t, dt_sub, lo, hi, edc_summary, s);
 }
 
 template <typename T>
@@ -823,7 +847,7 @@ int fw_turbulent_diffusion_pass3(const fw_grid* g, const void* const* f, void* c
     for (int b = 0; b < 3; ++b)
       FW_REQUIRE(f[a] != f_out[b], "turbulent_diffusion3: outputs must not alias inputs");
   return FW_DISPATCH(g, turb3_impl, g, f, f_out, nu_t, sc_t, dt_sub, clip_lo, clip_hi,
-                     (hipStream_t)stream);
+                     (int*)nullptr, (hipStream_t)stream);
 }
 
 int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
