@@ -160,12 +160,13 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
 // precomputed 1 / max(rho, 0.01) field, with the metric reciprocals
 // multiplied in.  It equals apply_A up to rounding; the refinement's float64
 // residual uses apply_A itself, so the solution is unchanged.
-template <typename T>
-__device__ __forceinline__ T apply_A_irho(const GridArgs& g, const T* __restrict__ ir,
-                                          const T* __restrict__ x, int i, int k,
-                                          const Nbr& n, T eps_reg) {
+// The operator on values that x(offset) supplies (a stored vector, or one
+// formed on the fly).
+template <typename T, typename X>
+__device__ __forceinline__ T apply_A_irho_f(const GridArgs& g, const T* __restrict__ ir, X x,
+                                            int i, int k, const Nbr& n, T eps_reg) {
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
-  const T irc = at(ir, n.c), xc = at(x, n.c);
+  const T irc = at(ir, n.c), xc = x(n.c);
   const T idx2 = T(g.inv_dx2), idy2 = T(g.inv_dy2);
   T diag = eps_reg;
   T acc = T(0);
@@ -173,38 +174,46 @@ __device__ __forceinline__ T apply_A_irho(const GridArgs& g, const T* __restrict
     if (i < Nx - 1) {
       const T cxp = T(0.5) * (irc + at(ir, n.xp1)) * idx2;
       diag -= cxp;
-      acc += cxp * at(x, n.xp1);
+      acc += cxp * x(n.xp1);
     } else {
       diag -= T(2) * irc * idx2;
     }
     if (i > 0) {
       const T cxm = T(0.5) * (at(ir, n.xm1) + irc) * idx2;
       diag -= cxm;
-      acc += cxm * at(x, n.xm1);
+      acc += cxm * x(n.xm1);
     }
   }
   if (Ny > 1) {
     const T cyp = T(0.5) * (irc + at(ir, n.yp1)) * idy2;
     const T cym = T(0.5) * (at(ir, n.ym1) + irc) * idy2;
     diag -= cyp + cym;
-    acc += cyp * at(x, n.yp1) + cym * at(x, n.ym1);
+    acc += cyp * x(n.yp1) + cym * x(n.ym1);
   }
   if (Nz > 1) {
     const T idz = T(g.inv_dz[k]);
     if (k < Nz - 1) {
       const T czp = T(0.5) * (irc + at(ir, n.zp1)) * (idz * T(g.inv_da[k]));
       diag -= czp;
-      acc += czp * at(x, n.zp1);
+      acc += czp * x(n.zp1);
     } else {
       diag -= T(2) * irc * idz * idz;
     }
     if (k > 0) {
       const T czm = T(0.5) * (at(ir, n.zm1) + irc) * (idz * T(g.inv_db[k]));
       diag -= czm;
-      acc += czm * at(x, n.zm1);
+      acc += czm * x(n.zm1);
     }
   }
   return diag * xc + acc;
+}
+
+// The operator on a stored vector x.
+template <typename T>
+__device__ __forceinline__ T apply_A_irho(const GridArgs& g, const T* __restrict__ ir,
+                                          const T* __restrict__ x, int i, int k,
+                                          const Nbr& n, T eps_reg) {
+  return apply_A_irho_f<T>(g, ir, [x](uint32_t o) { return at(x, o); }, i, k, n, eps_reg);
 }
 
 struct Cell {
@@ -304,6 +313,45 @@ k_matvec_rows(GridArgs g, const T* __restrict__ ir, const T* __restrict__ p,
       const T y = apply_A_irho<T>(g, ir, p, i, k, nb, eps_reg);
       at(q, nb.c) = y;
       acc += dzk * (double)at(p, nb.c) * (double)y;
+    }
+  }
+  const double t = block_sum_wg(acc, sh);
+  if (threadIdx.x == 0) Ppq[blockIdx.x] = t;
+}
+
+// The direction update of the previous iteration fused into this
+// iteration's operator: p+ = z + beta p, beta = sum Prz / rz (the new
+// <r, z>_W partials over the rz the previous update stored, exactly as
+// k_p_update2 forms it), is formed at every stencil point from z and p;
+// q = A p+, p+ of the workgroup's rows -> pn (another buffer: neighbours
+// still read p), partial <p+, q>_W.  Returns at once when S_DONE is set
+// (pn is then stale; the next round's set-up rewrites the direction).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_matvec_pz(GridArgs g, const T* __restrict__ ir, const T* __restrict__ p,
+            const T* __restrict__ z, T* __restrict__ pn, T* __restrict__ q, T eps_reg,
+            const double* __restrict__ Prz, int npz, double* __restrict__ S,
+            double* __restrict__ Ppq) {
+  __shared__ double sh[4];
+  if (S[S_DONE] != 0.0) return;
+  const double bd = sum_partials(Prz, npz, sh) / S[S_RZ];
+  __syncthreads();   // sh is reused by the block sum below
+  if (blockIdx.x == 0 && threadIdx.x == 0) S[S_BETA] = bd;
+  const T beta = (T)bd;
+  const auto pz = [p, z, beta](uint32_t o) { return at(z, o) + beta * at(p, o); };
+  int r0, r1;
+  row_range(g.ny * g.nz, r0, r1);
+  double acc = 0.0;
+  for (int row = r0; row < r1; ++row) {
+    const int k = row / g.ny, j = row - k * g.ny;
+    const double dzk = g.dz[k];
+    for (int i = threadIdx.x; i < g.nx; i += 256) {
+      const Nbr nb = nbr_offsets<T>(g, i, j, k);
+      const T y = apply_A_irho_f<T>(g, ir, pz, i, k, nb, eps_reg);
+      const T pc = pz(nb.c);
+      at(q, nb.c) = y;
+      at(pn, nb.c) = pc;
+      acc += dzk * (double)pc * (double)y;
     }
   }
   const double t = block_sum_wg(acc, sh);
@@ -1119,6 +1167,7 @@ struct ProjHandle {
   // vectors
   double *x64, *b64;
   void *r, *z, *p, *q, *d, *w1, *w2;
+  void* p2;   // second direction buffer of the fused update (k_matvec_pz)
   double *part, *S, *S_host;
   double *Ppq, *Prr, *Prz;  // partials of the PCG iteration's reductions
   void* irho;          // 1 / max(rho, 0.01) of the running solve (type T)
@@ -1321,6 +1370,7 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
   T* r = (T*)h->r;
   T* z = (T*)h->z;
   T* p = (T*)h->p;
+  T* p_alt = (T*)h->p2;
   T* q = (T*)h->q;
   T* d = (T*)h->d;
   // S_DONE, S_IT and S_TOL come from k_round_init, launched by the caller
@@ -1377,10 +1427,20 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
     const int nb_it = maxit - queued < batch ? maxit - queued : batch;
     batch = 1;
     for (int b = 0; b < nb_it; ++b) {
-      hipLaunchKernelGGL(k_matvec_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, ir, p, q,
-                         (T)h->eps_reg, h->Ppq);
+      if (fused && queued + b > 0) {
+        // the previous iteration's direction update rides in this operator
+        hipLaunchKernelGGL(k_matvec_pz<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, ir, p, z,
+                           p_alt, q, (T)h->eps_reg, h->Prz, npz, h->S, h->Ppq);
+        T* t = p;
+        p = p_alt;
+        p_alt = t;
+      } else {
+        hipLaunchKernelGGL(k_matvec_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, ir, p, q,
+                           (T)h->eps_reg, h->Ppq);
+      }
       if (fused) {
         DctFuse<T> fw = fu;
+        fw.p = p;
         fw.Pout = h->Prr;
         hipLaunchKernelGGL((k_dct4_rows<T, DCT_UPDATE>), dim3(ndct), dim3(256), shm, s,
                            rows, h->g.nx, h->logMx, (const T*)h->twx, (const T*)r,
@@ -1403,7 +1463,7 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
       if (flex)
         hipLaunchKernelGGL(k_p_update_flex<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Pqz,
                            h->S);
-      else
+      else if (!fused)
         hipLaunchKernelGGL(k_p_update2<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Prz,
                            npz, h->S, 0);
     }
@@ -1705,7 +1765,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   const size_t n = h->n;
   FW_CHECK_HIP(hipMalloc((void**)&h->x64, n * 8));
   FW_CHECK_HIP(hipMalloc((void**)&h->b64, n * 8));
-  void** vecs[] = {&h->r, &h->z, &h->p, &h->q, &h->d, &h->w1, &h->w2};
+  void** vecs[] = {&h->r, &h->z, &h->p, &h->q, &h->d, &h->w1, &h->w2, &h->p2};
   for (void** v : vecs) FW_CHECK_HIP(hipMalloc(v, n * h->esz));
   FW_CHECK_HIP(hipMalloc((void**)&h->part, PJ_RED * sizeof(double)));
   {
@@ -1728,7 +1788,7 @@ int fw_proj_destroy(void* handle) {
   ProjHandle* h = (ProjHandle*)handle;
   if (!h) return FW_OK;
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
-                  h->x64, h->b64, h->r, h->z, h->p, h->q, h->d, h->w1,
+                  h->x64, h->b64, h->r, h->z, h->p, h->p2, h->q, h->d, h->w1,
                   h->w2, h->part, h->S, h->twx, h->Ppq, h->irho, h->Yf_p, h->Yi_p,
                   h->Zf_p, h->Zi_p, h->Pqz, h->rs, h->qf};
   for (void* b : bufs)
