@@ -99,7 +99,7 @@ ROOFLINE_KERNEL = ("k_species3", 12)
 # HBM bytes per k_species3 launch at the default workload (float32), from the
 # FETCH_SIZE + WRITE_SIZE counters scaled by a 256 MiB copy:
 # profiles/step_256x64x48_s8_hbm_traffic.md
-ROOFLINE_TRAFFIC = {("spread3d_256x64x48_s8", "float32"): 44072085}
+ROOFLINE_TRAFFIC = {("spread3d_256x64x48_s8", "float32"): 44060018}
 
 
 # Phase-fused compulsory traffic of one outer step in field-passes (each
