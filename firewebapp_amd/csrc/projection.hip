@@ -585,8 +585,7 @@ k_x64_add(size_t n, double* __restrict__ x, const T* __restrict__ d) {
 // (the host's former arithmetic, in its order).  The iteration kernels do
 // nothing while S_DONE != 0, so a round launched ahead of this decision is
 // harmless when the solve has already converged.
-__global__ void k_round_init(double* __restrict__ S, double rtol, double floor) {
-  if (threadIdx.x != 0) return;
+__device__ __forceinline__ void round_init(double* __restrict__ S, double rtol, double floor) {
   const double bnorm = sqrt(S[S_BB]), rnorm = sqrt(S[S_TMP]);
   const double tol_abs = rtol * bnorm;
   S[S_IT] = 0.0;
@@ -606,6 +605,31 @@ __global__ void k_round_init(double* __restrict__ S, double rtol, double floor) 
   if (inner < floor) inner = floor;
   S[S_TOL] = inner * rnorm;
   S[S_DONE] = 0.0;
+}
+
+__global__ void k_round_init(double* __restrict__ S, double rtol, double floor) {
+  if (threadIdx.x == 0) round_init(S, rtol, floor);
+}
+
+// k_finalize's fixed-order sum of the residual partials into S_TMP, then the
+// round set-up above, in one launch.
+__global__ void __launch_bounds__(256)
+k_finalize_round(const double* __restrict__ part, int np, double* __restrict__ S, double rtol,
+                 double floor) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  double a = 0.0;
+  for (int b = t; b < np; b += 256) a += part[b];
+  sh[t] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sh[t] += sh[t + w];
+    __syncthreads();
+  }
+  if (t == 0) {
+    S[S_TMP] = sh[0];
+    round_init(S, rtol, floor);
+  }
 }
 
 template <typename T>
@@ -1482,8 +1506,8 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
 
 // Solve A x = b: x64 (warm start on entry), b64 given, ||b||^2 in S_BB on
 // the device; ||b - A x||_2 <= rtol ||b||_2 with the residual recomputed in
-// float64 after each inner solve.  have_residual: ||b - A x||^2 of the warm
-// start is already in S_TMP (with r formed by residual64).
+// float64 after each inner solve.  have_residual: residual64 has already run
+// on the warm start (its partials in part, r formed).
 //
 // The host reads the device scalars once per inner batch.  Round 0 starts
 // without a read: k_round_init decides on the device whether it runs.  A
@@ -1511,12 +1535,12 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
   auto residual = [&]() {
     hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n, rho,
                        h->b64, h->x64, (T*)h->r, h->eps_reg, h->part);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP,
-                       F_STORE);
-    hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, s, h->S, rtol, inner_floor);
+    hipLaunchKernelGGL(k_finalize_round, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
+                       rtol, inner_floor);
   };
   if (have_residual)
-    hipLaunchKernelGGL(k_round_init, dim3(1), dim3(64), 0, s, h->S, rtol, inner_floor);
+    hipLaunchKernelGGL(k_finalize_round, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
+                       rtol, inner_floor);
   else
     residual();
   FW_CHECK_LAUNCH("residual64");
@@ -1610,8 +1634,6 @@ static int project_impl(ProjHandle* h, void* u, void* v, void* w,
   // the warm start's residual, decided on together with ||b|| on the device
   hipLaunchKernelGGL((k_residual64<T, T>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
                      (const T*)rho, h->b64, h->x64, (T*)h->r, h->eps_reg, h->part);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP,
-                     F_STORE);
   FW_CHECK_LAUNCH("residual64");
   int rc = solve_refined<T, T>(h, (const T*)rho, h->rtol, true, s);
   if (rc) return rc;
