@@ -15,6 +15,8 @@
 //      the earlier segments, 256 at a time, for the last one with a
 //      resetting cell (the flag is false before the first).
 // No atomics and no inter-workgroup waits: the result is deterministic.
+#include <mutex>
+
 #include "fw_common.h"
 
 namespace fw {
@@ -218,4 +220,54 @@ extern "C" int fw_edc(const fw_grid* g, const void* rho, void* T_g,
   return fw::edc_launch(g, rho, T_g, Y_fuel, Y_O2, k_turb, eps_turb, Y_H2O, omega_out,
                         chi_rad, cp_g, dt, n_substeps, extinction_enable, s_stoich, hoc_J,
                         work, false, (hipStream_t)stream);
+}
+
+// ---- fw_edc_static: the same closure with library-owned scratch ----------
+// The segment summaries live in a file-scope device array instead of a
+// caller workspace, so the entry point allocates nothing.  Each GPU holds its
+// own copy; its address on a device comes from hipGetSymbolAddress (the
+// array's name in host code is only a host-side stand-in, never a device
+// address), looked up once per device with that device current and kept in
+// g_edc_scratch_ptr.
+static constexpr int EDC_STATIC_SEGS = FW_EDC_STATIC_MAX_CELLS / EDC_SEG;
+static constexpr int EDC_STATIC_DEVICES = 64;
+static __device__ __align__(256) int scratch[EDC_STATIC_SEGS];
+static std::mutex g_edc_scratch_mu;
+static int* g_edc_scratch_ptr[EDC_STATIC_DEVICES] = {};
+
+static int edc_scratch(int** out) {
+  int dev = -1;
+  FW_CHECK_HIP(hipGetDevice(&dev));
+  FW_REQUIRE(dev >= 0 && dev < EDC_STATIC_DEVICES, "edc_static: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_edc_scratch_mu);
+  if (!g_edc_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fw_set_error(FW_E_HIP, "edc_static: hipGetSymbolAddress(scratch): %s",
+                          hipGetErrorString(e));
+    g_edc_scratch_ptr[dev] = (int*)p;
+  }
+  *out = g_edc_scratch_ptr[dev];
+  return FW_OK;
+}
+
+extern "C" int fw_edc_static(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel,
+                             void* Y_O2, const void* k_turb, const void* eps_turb,
+                             const void* Y_H2O, void* omega_out, double chi_rad, double cp_g,
+                             double dt, int n_substeps, int extinction_enable,
+                             double s_stoich, double hoc_J, void* stream) {
+  FW_GRID(g);
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  FW_REQUIRE(n <= (size_t)FW_EDC_STATIC_MAX_CELLS,
+             "edc_static: %zu cells exceed the static scratch's maximum of %d cells "
+             "(use fw_edc with a workspace)", n, FW_EDC_STATIC_MAX_CELLS);
+  int* summary = nullptr;
+  const int rc = edc_scratch(&summary);
+  if (rc) return rc;
+  // k_edc_summary writes every segment's summary before k_edc reads it, so
+  // the array needs no clearing between calls
+  return fw::edc_launch(g, rho, T_g, Y_fuel, Y_O2, k_turb, eps_turb, Y_H2O, omega_out,
+                        chi_rad, cp_g, dt, n_substeps, extinction_enable, s_stoich, hoc_J,
+                        summary, false, (hipStream_t)stream);
 }

@@ -494,6 +494,24 @@ int fw_edc(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel,
            double dt, int n_substeps, int extinction_enable, double s_stoich,
            double hoc_J, int* work, void* stream);
 
+/* fw_edc without the workspace argument.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * segment summaries live in a library-owned static device array of
+ * FW_EDC_STATIC_MAX_CELLS / 256 int32 (1 MiB), one copy per GPU, held until
+ * the process ends; the entry point calls no allocator.  Grids of more than
+ * FW_EDC_STATIC_MAX_CELLS cells are rejected with FW_E_CONFIG.  `stream`
+ * must belong to the current device.  Every call on a GPU shares that GPU's
+ * array: calls that overlap in time on one GPU (several streams or host
+ * threads) race.  The array needs no clearing: each call writes every
+ * summary it reads. */
+#define FW_EDC_STATIC_MAX_CELLS 67108864
+int fw_edc_static(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel,
+                  void* Y_O2, const void* k_turb, const void* eps_turb,
+                  const void* Y_H2O, void* omega_out, double chi_rad, double cp_g,
+                  double dt, int n_substeps, int extinction_enable, double s_stoich,
+                  double hoc_J, void* stream);
+
 /* ---- the outer time step (run_3d_spread loop, spread_3d.py:1897-3683) -- */
 
 /* Device state of one run.  Grid-type (Nz, Ny, Nx) fields unless noted.
