@@ -252,6 +252,17 @@ __device__ __forceinline__ int block_last_true(bool pred, int* wmax) {
   return last;
 }
 
+// LevelSetFront3D.front_x (flame_front_3d.py:391-402) on one host row of
+// phi: the linear zero crossing of the first sign change, +inf if none.
+inline double front_x_row(const double* row, int Nx, double dx) {
+  for (int i = 0; i < Nx - 1; ++i) {
+    const double p0 = row[i], p1 = row[i + 1];
+    if ((p0 < 0.0 && 0.0 < p1) || (p1 < 0.0 && 0.0 < p0))
+      return (i + (-p0 / (p1 - p0)) + 0.5) * dx;
+  }
+  return INFINITY;
+}
+
 }  // namespace fw
 
 // ---- error plumbing (fw_error.cpp) -----------------------------------------

@@ -78,17 +78,9 @@ struct SpeciesTimer {
   hipEvent_t ev[2 * CAP];
   bool made = false;
 };
-static SpeciesTimer g_timer;
-
-// LevelSetFront3D.front_x (flame_front_3d.py:391-402) on one host row.
-static double front_x_row(const double* row, int Nx, double dx) {
-  for (int i = 0; i < Nx - 1; ++i) {
-    const double p0 = row[i], p1 = row[i + 1];
-    if ((p0 < 0.0 && 0.0 < p1) || (p1 < 0.0 && 0.0 < p0))
-      return (i + (-p0 / (p1 - p0)) + 0.5) * dx;
-  }
-  return INFINITY;
-}
+// one timer per host thread: steps driven from several threads (a batch)
+// each time their own launches
+static thread_local SpeciesTimer g_timer;
 
 }  // namespace fw
 

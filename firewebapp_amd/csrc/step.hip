@@ -292,6 +292,8 @@ __device__ __forceinline__ void dz_inv_pm(const GridArgs& g, int k, T& im, T& ip
 
 // mode 0: godunov_grad_norm + evolve: phi_out = phi - dt vn |grad|
 // mode 1: reinit substep (sign from phi0): phi_out = phi - dtau s(|grad|-1)
+// mode 2: godunov_grad_norm alone: phi_out = |grad|
+// mode 3: reinit_godunov_grad alone (sign from phi0): phi_out = |grad|
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_lset(GridArgs g, const T* __restrict__ phi, const T* __restrict__ phi0,
@@ -315,12 +317,12 @@ k_lset(GridArgs g, const T* __restrict__ phi, const T* __restrict__ phi0,
   const T dmz = k > 0 ? (pc - phi[c - sz]) * im : T(0);
   const T dpz = k < Nz - 1 ? (phi[c + sz] - pc) * ip : T(0);
   T s = T(1);
-  if (mode == 1) {
+  if (mode == 1 || mode == 3) {
     const T p0 = phi0[c];
     s = p0 > T(0) ? T(1) : (p0 < T(0) ? T(-1) : T(0));
   }
   T gxp, gxm, gyp, gym, gzp, gzm;
-  if (mode == 0 || s > T(0)) {
+  if (mode == 0 || mode == 2 || s > T(0)) {
     gxp = fmax(dmx, T(0)); gxm = fmin(dpx, T(0));
     gyp = fmax(dmy, T(0)); gym = fmin(dpy, T(0));
     gzp = fmax(dmz, T(0)); gzm = fmin(dpz, T(0));
@@ -331,7 +333,8 @@ k_lset(GridArgs g, const T* __restrict__ phi, const T* __restrict__ phi0,
   }
   const T grad = sqrt(gxp * gxp + gxm * gxm + gyp * gyp + gym * gym + gzp * gzp + gzm * gzm);
   if (mode == 0) out[c] = pc - step * vn[c] * grad;
-  else out[c] = pc - step * s * (grad - T(1));
+  else if (mode == 1) out[c] = pc - step * s * (grad - T(1));
+  else out[c] = grad;
 }
 
 static inline unsigned nb256(size_t n) { return (unsigned)((n + 255) / 256); }
@@ -545,9 +548,9 @@ int fw_levelset_update(const fw_grid* g, const void* phi, const void* phi0,
                        const void* v_n, void* phi_out, double step, int mode,
                        void* stream) {
   FW_GRID(g);
-  FW_REQUIRE(g && phi && phi_out && phi != phi_out && (mode == 0 || mode == 1),
+  FW_REQUIRE(g && phi && phi_out && phi != phi_out && mode >= 0 && mode <= 3,
              "levelset_update: bad argument");
-  FW_REQUIRE(mode == 1 ? phi0 != nullptr : v_n != nullptr,
+  FW_REQUIRE(mode == 1 || mode == 3 ? phi0 != nullptr : (mode == 2 || v_n != nullptr),
              "levelset_update: mode needs phi0 (reinit) or v_n (evolve)");
   Launch L = cell_launch(g);
   hipStream_t s = (hipStream_t)stream;

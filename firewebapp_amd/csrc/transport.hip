@@ -850,6 +850,21 @@ int fw_turbulent_diffusion_pass3(const fw_grid* g, const void* const* f, void* c
                      (int*)nullptr, (hipStream_t)stream);
 }
 
+int fw_turbulent_diffusion_pass3_summary(const fw_grid* g, const void* const* f,
+                                         void* const* f_out, const void* nu_t, double sc_t,
+                                         double dt_sub, const double* clip_lo,
+                                         const double* clip_hi, int* edc_summary,
+                                         void* stream) {
+  FW_GRID(g);
+  FW_REQUIRE(g && f && f_out && clip_lo && clip_hi && edc_summary,
+             "turbulent_diffusion3_summary: null argument");
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      FW_REQUIRE(f[a] != f_out[b], "turbulent_diffusion3: outputs must not alias inputs");
+  return FW_DISPATCH(g, turb3_impl, g, f, f_out, nu_t, sc_t, dt_sub, clip_lo, clip_hi,
+                     edc_summary, (hipStream_t)stream);
+}
+
 int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
                   const void* u, const void* v, const void* w, double dt,
                   double alpha_th, double T_amb, void* stream) {

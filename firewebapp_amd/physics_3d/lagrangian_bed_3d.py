@@ -14,6 +14,7 @@ import torch
 
 from .. import _lib
 from ..device import Marshal, device, grid_for, ptr, stream_ptr
+from ..grid import Grid3D
 
 RHO_SOLID_TRUE_GRASS = 380.0
 CP_SOLID_GRASS = 1500.0
@@ -203,7 +204,8 @@ def _locate(g, P: DeviceParticles):
 
 def _grid(m, shape, dx, dy, dz_arr, z_face, n_z_bed=0):
     dz = np.asarray(dz_arr, np.float64)
-    return grid_for(m, shape, dx, dy, dz, dz, dz, z_face=z_face, n_z_bed=n_z_bed)
+    da, db = Grid3D._faces(dz)
+    return grid_for(m, shape, dx, dy, dz, da, db, z_face=z_face, n_z_bed=n_z_bed)
 
 
 def step_bed_particles(part_x, part_y, part_z, part_alive, part_m_solid,

@@ -116,6 +116,20 @@ int fw_turbulent_diffusion_pass3(const fw_grid* g, const void* const* f,
                                  double dt_sub, const double* clip_lo,
                                  const double* clip_hi, void* stream);
 
+/* fw_turbulent_diffusion_pass3 on (fuel, O2, H2O) that also writes the EDC
+ * segment summaries of its output, as the step driver's last species pass of
+ * a sub-step does (no counterpart in the reference; it saves fw_edc's own
+ * summary pass): for each 256-cell segment s of the flat index,
+ * edc_summary[s] = -1 if no cell of it has Y_F > 1e-9 and Y_O2 > 1e-9, else
+ * the wet-bulb flag (Y_H2O > 0 and 1 - Y_H2O/0.18 < 0.5) of the last such
+ * cell (chemistry_closures/edc.py:223).  Needs Nx % 256 == 0
+ * (FW_E_CONFIG otherwise); edc_summary: fw_edc_workspace_ints(g) int32. */
+int fw_turbulent_diffusion_pass3_summary(const fw_grid* g, const void* const* f,
+                                         void* const* f_out, const void* nu_t, double sc_t,
+                                         double dt_sub, const double* clip_lo,
+                                         const double* clip_hi, int* edc_summary,
+                                         void* stream);
+
 /* max(nu_t) over interior cells (turbulence_3d.py:890-897); synchronises the
  * stream and writes the host scalar.  work: FW_REDUCE_BLOCKS (256) doubles. */
 int fw_nu_t_max_interior(const fw_grid* g, const void* nu_t, double* work,
@@ -472,12 +486,66 @@ int fw_step_reduce(const fw_grid* g, const void* u, const void* v,
 
 /* LevelSetFront3D (flame_front_3d.py:113-360): mode 0 = evolve,
  * phi_out = phi - step v_n |grad phi|_Godunov; mode 1 = one Sussman reinit
- * substep, phi_out = phi - step sign(phi0)(|grad phi| - 1). */
+ * substep, phi_out = phi - step sign(phi0)(|grad phi| - 1); mode 2 =
+ * godunov_grad_norm (:113-188), phi_out = |grad phi|; mode 3 =
+ * reinit_godunov_grad (:191-263), phi_out = the sign(phi0)-aware |grad phi|.
+ * step is unused by modes 2 and 3, v_n by 1-3, phi0 by 0 and 2. */
 int fw_levelset_update(const fw_grid* g, const void* phi, const void* phi0,
                        const void* v_n, void* phi_out, double step, int mode,
                        void* stream);
 
+/* ---- flame front (flame_front_3d.py) ------------------------------------ */
+
+/* LevelSetFront3D.initialize_source_patch (flame_front_3d.py:304-330) on the
+ * float64 phi: -L_burnout/2 on i_start <= i < i_end, k <= k_top_bed; else
+ * d = x_mid[i] - (x_mid[i_end-1] + dx/2), kept if positive, floored at
+ * -L_burnout/2.  x_mid: device float64 (Nx).  Synchronises the stream. */
+int fw_lset_init_patch(const fw_grid* g, double* phi, const double* x_mid, int i_start,
+                       int i_end, int k_top_bed, double L_burnout, void* stream);
+
+/* mask = (lo_strict ? phi > lo : phi >= lo) && phi <= hi on k < n_z_top, 0
+ * above (n_z_top >= Nz: every layer).  phi: float64 field; mask: uint8.
+ * LevelSetFront3D.burned_mask (flame_front_3d.py:379-381: lo = -inf, hi = 0),
+ * flame_body_mask (:383-385: lo = -L_burnout, hi = 0), ahead_band_mask
+ * (:387-389: lo = 0 strict, hi = band_m), flame_body_mask_from_phi_flame
+ * (:803-813: lo = -inf, hi = band_m). */
+int fw_phi_window_mask(const fw_grid* g, const double* phi, double lo, int lo_strict,
+                       double hi, int n_z_top, uint8_t* mask, void* stream);
+
+/* LevelSetFront3D.front_x(k, j) (flame_front_3d.py:391-402): the linear zero
+ * crossing of row (k, j) of the float64 phi, +inf if none; synchronises the
+ * stream and writes the host scalar. */
+int fw_lset_front_x(const fw_grid* g, const double* phi, int k, int j, double* out_host,
+                    void* stream);
+
+/* LevelSetFront3D.enforce_z_uniformity (flame_front_3d.py:362-377): every
+ * column of the float64 phi set to its z mean, in place. */
+int fw_phi_z_mean(const fw_grid* g, double* phi, void* stream);
+
+/* update_cell_age (flame_front_3d.py:816-835), in place on cell_age (grid
+ * type): mask cells age by dt (from 0 if inf), the others become inf. */
+int fw_update_cell_age(const fw_grid* g, const uint8_t* mask, void* cell_age, double dt,
+                       void* stream);
+
+/* compute_q_in_at_front_3d (flame_front_3d.py:440-471): out = mask ?
+ * q_frankman + q_dom_fwd : 0, plus q_burst_2d (Ny, Nx) on the highest band
+ * cell of each column.  q_frankman (NULL = zero) and q_burst_2d (NULL = none)
+ * are optional; grid type throughout, mask uint8. */
+int fw_q_in_at_front_3d(const fw_grid* g, const void* q_frankman, const void* q_dom_fwd,
+                        const uint8_t* mask, const void* q_burst_2d, void* out, void* stream);
+
 /* ---- combustion ---------------------------------------------------------- */
+
+/* combustion_3d.step_o2_supply_rate (combustion_3d.py:184-260): on interior
+ * cells (1 <= k <= Nz-2, 1 <= j <= Ny-2, 1 <= i <= Nx-2) omega_O2_out = the
+ * upwind O2 mass inflow over the six faces / s_stoich; other cells are left
+ * as the caller filled them (the reference fills 1e30).  The EDC closure of
+ * the step does not read it (chemistry_closures/edc.py:133-376), so the
+ * step driver skips it. */
+int fw_o2_supply_rate(const fw_grid* g, const void* rho, const void* u, const void* v,
+                      const void* w, const void* Y_O2, void* omega_O2_out, double s_stoich,
+                      void* stream);
+
 
 /* Number of int32 workspace entries fw_edc needs for grid g. */
 int fw_edc_workspace_ints(const fw_grid* g);
@@ -613,8 +681,9 @@ int fw_spread_reduce(const fw_grid* g, const fw_spread_fields* f,
  * on its stream around every k-th sub-step k_species3 launch (at most
  * 4096 pairs).  fw_spread_timer(k), k >= 1, resets and starts with stride
  * k, fw_spread_timer(0) stops; fw_spread_timer_read waits for the events and
- * returns the summed milliseconds and the number of launches covered.  The timer is process
- * wide: enable it only while one thread drives steps. */
+ * returns the summed milliseconds and the number of launches covered.  The timer
+ * belongs to the calling host thread: it covers the steps that thread drives,
+ * and threads stepping other cases concurrently keep their own. */
 int fw_spread_timer(int enable);
 int fw_spread_timer_read(double* total_ms, int* launches);
 

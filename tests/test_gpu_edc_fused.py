@@ -42,3 +42,75 @@ def test_step_nx256_fp64_vs_oracle():
         np.testing.assert_allclose(sv.diag["t"], o.diag["t"], rtol=1e-12)
     finally:
         sv.close()
+
+
+def _summary_numpy(Yf, YO, Yw):
+    """Segment summaries by their definition: per 256-cell segment of the
+    flat index, -1 without a resetting cell (Y_F > 1e-9 and Y_O2 > 1e-9),
+    else the wet-bulb flag of the last one (edc.py:223)."""
+    rf, ro, rw = Yf.reshape(-1, 256), YO.reshape(-1, 256), Yw.reshape(-1, 256)
+    out = np.full(rf.shape[0], -1, np.int32)
+    for s in range(rf.shape[0]):
+        hit = np.flatnonzero((rf[s] > 1e-9) & (ro[s] > 1e-9))
+        if hit.size:
+            y = rw[s, hit[-1]]
+            out[s] = 1 if (y > 0.0 and max(1.0 - y / 0.18, 0.0) < 0.5) else 0
+    return out
+
+
+@pytest.mark.parametrize("dtype_name", ["float64", "float32"])
+def test_fused_diffusion_summary_carries_flag_one(dtype_name):
+    """The species diffusion pass that writes the EDC segment summaries, on a
+    wet field where they must carry flag 1.  Each x row is one 256-cell
+    segment; rows cycle through three kinds: no fuel and nu_t = 0 (so no fuel
+    diffuses in: summary -1), fuel cells away from the segment start in wet
+    gas (Y_H2O above the 0.09 wet-bulb trigger: 1) and in dry gas (0).  No
+    Y_H2O lies within 0.01 of the trigger, so neither diffusion nor the
+    float32 quotient can flip a flag."""
+    import ctypes as C
+    import torch
+    from firewebapp_amd import _lib
+    from firewebapp_amd.device import DeviceGrid, ptr, stream_ptr
+    dtype = getattr(torch, dtype_name)
+    nz, ny, nx = 12, 8, 256
+    rng = np.random.default_rng(91)
+    sh = (nz, ny, nx)
+    dz = np.full(nz, 0.1)
+    g = DeviceGrid(nx, ny, nz, 0.1, 0.1, dz, dz, dz, dtype=dtype)
+    kind = (np.arange(nz * ny) % 3).reshape(nz, ny)
+    Yf = np.zeros(sh)
+    for k in range(nz):
+        for j in range(ny):
+            if kind[k, j]:
+                Yf[k, j, rng.integers(10, 240, size=3)] = 0.05
+    YO = np.full(sh, 0.2)
+    wet = 0.10 + 0.08 * rng.random(sh)
+    dry = 0.02 + 0.06 * rng.random(sh)
+    Yw = np.where((kind == 1)[:, :, None], wet, dry)
+    nu = np.where((kind == 0)[:, :, None], 0.0, 1e-3) * np.ones(sh)
+    dev = lambda a: torch.as_tensor(a, dtype=dtype, device="cuda").contiguous()
+    nu_t = dev(nu)
+    fin = [dev(a) for a in (Yf, YO, Yw)]
+    fout = [torch.empty_like(t) for t in fin]
+    summ = torch.full((nz * ny * nx // 256,), 7, dtype=torch.int32, device="cuda")
+    pin = (C.c_void_p * 3)(*[t.data_ptr() for t in fin])
+    pout = (C.c_void_p * 3)(*[t.data_ptr() for t in fout])
+    lo = (C.c_double * 3)(0.0, 0.0, 0.0)
+    hi = (C.c_double * 3)(1.0, 0.232, 1.0)
+    _lib.call("fw_turbulent_diffusion_pass3_summary", g.ref, pin, pout, ptr(nu_t),
+              0.7, 1e-3, lo, hi, ptr(summ), stream_ptr())
+    torch.cuda.synchronize()
+    outs = [t.double().cpu().numpy() for t in fout]
+    want = _summary_numpy(*outs)
+    got = summ.cpu().numpy()
+    print("summaries -1/0/1:", [(int((want == v).sum())) for v in (-1, 0, 1)], flush=True)
+    assert [(int((want == v).sum())) for v in (-1, 0, 1)] == [32, 32, 32]
+    np.testing.assert_array_equal(got, want)
+    # the pass itself equals the plain three-field pass
+    ref = [torch.empty_like(t) for t in fin]
+    _lib.call("fw_turbulent_diffusion_pass3", g.ref, pin,
+              (C.c_void_p * 3)(*[t.data_ptr() for t in ref]), ptr(nu_t), 0.7, 1e-3,
+              lo, hi, stream_ptr())
+    torch.cuda.synchronize()
+    for a, b in zip(fout, ref):
+        assert torch.equal(a, b)
