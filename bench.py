@@ -10,13 +10,23 @@ EDC, DOM S8 (80 directions) every 5th step, fft-PCG projection (rtol 1e-6,
 outer divergence tolerance 1e-3), passive level set, solid-phase ignition.
 
 One "step" is one outer step of the reference loop (spread_3d.py:1897-3683)
-with its 10 chemistry/transport sub-steps.  ``value`` = cells x steps / wall
-seconds over exactly K timed steps after W warm-up steps, inputs resident in
-HBM.  The JSON line also carries:
-* roofline: the dominant kernel's algorithmic bytes per launch / its mean
-  launch time over the timed steps (every 4th launch), measured with HIP
-  events that the launch itself records (hipExtLaunchKernel, on the stream
-  the kernel is launched on: torch's current stream), against 8 TB/s;
+with its 10 chemistry/transport sub-steps.  The workload's configuration is
+a 500-step run (1000 at 512 x 128 x 64, 200 at 128 x 32 x 32); the timed
+window is its end, a developed fire: an untimed spin-up, W warm-up steps,
+then exactly K timed steps ending at the configuration's last step
+(--spinup overrides).  ``value`` = cells x K / wall seconds over those
+steps, inputs resident in HBM.  The JSON line also carries:
+* window: the step numbers and simulated times of the timed window;
+* full_run: the rate over every step of the run (spin-up included);
+* early_window: the rate over steps W+1 .. W+K of a fresh run (the
+  ignition transient, before the fire develops);
+* roofline: k_dom_sweep_tile, the kernel with the largest share of the
+  step's kernel time among the stencil passes: algorithmic bytes per launch
+  (5 field passes per ordinate per source iteration over the sweep's
+  launches) / its mean launch time over the timed steps, measured with HIP
+  events that each launch itself records (hipExtLaunchKernelGGL, on the
+  stream the kernel is launched on: torch's current stream), against
+  8 TB/s; roofline_species3 the same for the sub-steps' species kernel;
   traffic is the HBM bytes per launch from the FETCH_SIZE and WRITE_SIZE
   counters;
 * step_roofline: the whole step against the HBM roofline, with the
@@ -27,11 +37,11 @@ HBM.  The JSON line also carries:
   them to its Krylov solver) are solved again from x0 = 0 to
   ||b - A x||_2 / ||b||_2 <= 1e-8 (float64 residual, float32 inner PCG with
   iterative refinement): plain PCG (the separable preconditioner),
-  Jacobi-PCG and the learned preconditioner (flexible PCG with the trained
-  network's correction, handing a solve to plain PCG when a round stalls),
-  iterations and wall time per system;
-* cpu_baseline: the oracle's float64 NumPy step on one host core, timed on a
-  bounded sample of the same workload (rank 0, N = 1 only).
+  Jacobi-PCG and the learned preconditioner, iterations and wall time per
+  system;
+* cpu_baseline: the oracle's float64 NumPy step on one host core over
+  steps 1 .. dom_every of the same workload (one DOM solve among them, the
+  run's cadence), with the host's CPU model and CPU count (rank 0, N = 1).
 """
 from __future__ import annotations
 
@@ -92,14 +102,24 @@ def wind(name):
     return WIND.get(name, 4.0)
 
 
-# the roofline kernel and its algorithmic fields per cell per launch:
-# k_species3 reads Y_fuel, Y_O2, Y_H2O, rho, u, v, w, S_pyro part, S_dry part
-# and writes the three species
-ROOFLINE_KERNEL = ("k_species3", 12)
-# HBM bytes per k_species3 launch at the default workload (float32), from the
-# FETCH_SIZE + WRITE_SIZE counters scaled by a 256 MiB copy:
-# profiles/step_256x64x48_s8_hbm_traffic.md
-ROOFLINE_TRAFFIC = {("spread3d_256x64x48_s8", "float32"): 44060018}
+# Outer steps of each workload's configuration: the timed window is placed
+# at the end of that run (a developed fire), after an untimed spin-up.
+CONFIG_STEPS = {"spread3d_256x64x48_s8": 500, "spread3d_512x128x64_s8": 1000,
+                "spread3d_128x32x32_s4": 200}
+
+# The roofline kernels.  k_dom_sweep_tile (the DOM's tiled sweep, the largest
+# kernel-time share of the step) moves, by the phase-fused count, 5 field
+# passes per ordinate per source iteration (kappa, B, the upwind inflow, the
+# intensity written, its accumulation into G), spread over the sweep's
+# Ny/4 + Nz/4 - 1 launches of a source iteration.  k_species3 reads Y_fuel,
+# Y_O2, Y_H2O, rho, u, v, w, S_pyro part, S_dry part and writes the three
+# species: 12 fields per cell per launch.
+DOM_PASSES_PER_ORDINATE = 5
+SPECIES3_FIELDS = 12
+# HBM bytes per launch from the FETCH_SIZE + WRITE_SIZE counters scaled by a
+# 256 MiB copy (float32): profiles/step_256x64x48_s8_hbm_traffic.md
+ROOFLINE_TRAFFIC = {("k_species3", "spread3d_256x64x48_s8", "float32"): 44060018,
+                    ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32"): 18228147}
 
 
 # Phase-fused compulsory traffic of one outer step in field-passes (each
@@ -196,47 +216,82 @@ def deck(name, dtype):
     return RomInputs(outdoor_overrides=ovr)
 
 
-# every SPECIES_STRIDE-th k_species3 launch of the timed steps records its
-# events (the event-recording launch costs extra host time per launch)
+# kernels timed with HIP events over the timed steps: (timer id, stride);
+# every SPECIES_STRIDE-th k_species3 launch and every DOM sweep launch
+# records its events (the event-recording launch costs extra host time)
+TIMER_SPECIES3, TIMER_DOM_SWEEP = 0, 1
 SPECIES_STRIDE = 4
 
 
-def species_timer(on):
-    """Start (reset) or stop the step driver's HIP-event timing of the k_species
-    launches; the events are recorded on the stream the kernels run on."""
+def kernel_timers(on):
+    """Start (reset) or stop the HIP-event timing of the k_species3 and
+    k_dom_sweep_tile launches; the events are recorded by the launches
+    themselves, on the stream the kernels run on."""
     from firewebapp_amd import _lib
-    _lib.call("fw_spread_timer", SPECIES_STRIDE if on else 0)
+    _lib.call("fw_kernel_timer", TIMER_SPECIES3, SPECIES_STRIDE if on else 0)
+    _lib.call("fw_kernel_timer", TIMER_DOM_SWEEP, 1 if on else 0)
 
 
-def species_timer_read():
+def kernel_timer_read(which):
     import ctypes as C
     from firewebapp_amd import _lib
     ms, n = C.c_double(), C.c_int()
-    _lib.call("fw_spread_timer_read", C.byref(ms), C.byref(n))
+    _lib.call("fw_kernel_timer_read", which, C.byref(ms), C.byref(n))
     return ms.value, n.value
 
 
-def cpu_baseline(name, budget_s):
-    """Oracle float64 step on one core over a bounded sample of the workload."""
+def host_cpu():
+    """CPU model name and the logical CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+def cpu_baseline(name, dom_every):
+    """Oracle float64 step on one core over outer steps 1 .. dom_every of the
+    workload: dom_every - 1 steps without and one with the DOM solve, the
+    step cadence of the run."""
     from threadpoolctl import threadpool_limits
     from firewebapp_amd.spread_3d import Spread3DConfig, bed_inputs, build_grid
     from oracle.step import OracleSpread3D
     cfg = Spread3DConfig.resolve(deck(name, "float64"), wind_speed_m_s=wind(name))
     g = build_grid(cfg)
     o = OracleSpread3D(cfg, g, bed_inputs(cfg, g))
-    n = 0
+    times = []
     with threadpool_limits(1):
-        t0 = time.perf_counter()
-        while True:
+        for _ in range(dom_every):
+            t0 = time.perf_counter()
             o.step()
-            n += 1
-            el = time.perf_counter() - t0
-            if el >= budget_s or el * (n + 1) / n > 2.0 * budget_s:
-                break
+            times.append(time.perf_counter() - t0)
+    el = sum(times)
     cells = g.Nx * g.Ny * g.Nz
-    return {"value": cells * n / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle float64 NumPy step, outer steps 1-{n} of {name} "
-                      f"({el:.1f} s; no DOM solve in these steps, it runs every 5th)"}
+    model, ncpu, usable = host_cpu()
+    return {"value": cells * dom_every / el, "unit": "cell-steps/s", "cores": 1,
+            "kind": "port",
+            "sample": f"oracle float64 NumPy step on 1 thread, outer steps 1-{dom_every} of "
+                      f"{name} ({dom_every - 1} without and 1 with the DOM solve, the "
+                      f"run's cadence; {el:.1f} s, the DOM step {times[-1]:.1f} s)",
+            "cpu_model": model, "host_cpus": ncpu, "usable_cpus": usable,
+            "step_s": times}
+
+
+def _roofline(kernel, bytes_launch, ms, launches, workload, dtype):
+    achieved = bytes_launch / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": ROOFLINE_TRAFFIC.get((kernel, workload, dtype)),
+            "bytes_per_launch": bytes_launch, "mean_launch_ms": ms, "launches": launches}
 
 
 def main(argv=None):
@@ -244,12 +299,18 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--spinup", type=int, default=-1,
+                    help="untimed steps before the warm-up (default: so that the timed "
+                         "window ends at the configuration's last step, -1)")
     ap.add_argument("--workload", default="spread3d_256x64x48_s8", choices=sorted(WORKLOADS))
     ap.add_argument("--dtype", default="float32", choices=("float32", "float64"))
-    ap.add_argument("--cpu-budget", type=float, default=20.0,
-                    help="seconds of oracle CPU work for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-budget", type=float, default=1.0,
+                    help="0 skips cpu_baseline (the oracle on one core over the first "
+                         "dom_every steps)")
     ap.add_argument("--pcg-rhs", type=int, default=4,
                     help="projection systems for pcg_time_to_1e8 (0 = skip)")
+    ap.add_argument("--early-window", type=int, default=1,
+                    help="also time the first warmup + steps of a fresh run (0 = skip)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -269,35 +330,48 @@ def main(argv=None):
         torch.cuda.synchronize()
         del src, dst
     cfg = Spread3DConfig.resolve(deck(a.workload, a.dtype), wind_speed_m_s=wind(a.workload))
+    n_config = CONFIG_STEPS[a.workload]
+    spinup = a.spinup if a.spinup >= 0 else max(0, n_config - a.warmup - a.steps)
     sv = Spread3DSolver(cfg)
     g = sv.grid
     cells = g.Nx * g.Ny * g.Nz
     print(f"# workload {a.workload}: {g.Nx}x{g.Ny}x{g.Nz} cells, {sv.n_particles} "
-          f"particles, moisture {cfg.moisture:.4f}", file=sys.stderr, flush=True)
-    for i in range(a.warmup):
-        sv.step()
-    print(f"# warmup {a.warmup} steps done, t = {sv.t:.4f} s", file=sys.stderr, flush=True)
-    species_timer(True)
+          f"particles, moisture {cfg.moisture:.4f}; spin-up {spinup}, warm-up {a.warmup}, "
+          f"timed {a.steps} steps", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t_run0 = time.perf_counter()
+    for i in range(spinup + a.warmup):
+        if sv.step() is not None:
+            raise RuntimeError(f"run stopped early at step {sv.step_no}: {sv.stop_reason}")
+        if (i + 1) % 100 == 0:
+            print(f"# step {i + 1}, t = {sv.t:.4f} s", file=sys.stderr, flush=True)
+    t_win0 = sv.t
+    print(f"# spin-up and warm-up done: {spinup + a.warmup} steps, t = {sv.t:.4f} s",
+          file=sys.stderr, flush=True)
+    kernel_timers(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
         if sv.step() is not None:
             raise RuntimeError(f"run stopped early: {sv.stop_reason}")
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    wall = t1 - t0
     # device memory in use by this process (the library's own allocations,
     # the DOM intensities among them, are outside torch's allocator)
     free_b, total_b = torch.cuda.mem_get_info()
     hbm_used = total_b - free_b
-    species_timer(False)
-    print(f"# timed {a.steps} steps in {wall:.3f} s, t = {sv.t:.4f} s", file=sys.stderr,
-          flush=True)
-    kern, fields = ROOFLINE_KERNEL
+    kernel_timers(False)
+    print(f"# timed {a.steps} steps in {wall:.3f} s, t = {t_win0:.4f} .. {sv.t:.4f} s",
+          file=sys.stderr, flush=True)
     esz = 4 if a.dtype == "float32" else 8
-    bytes_launch = fields * esz * cells
-    tot_ms, launches = species_timer_read()
-    ms = tot_ms / max(launches, 1)
-    achieved = bytes_launch / (ms * 1e-3) / 1e9
+    M = sv.dom.M
+    TS = 4
+    sweep_launches_per_iter = -(-g.Ny // TS) + -(-g.Nz // TS) - 1
+    dom_bytes = DOM_PASSES_PER_ORDINATE * esz * M * cells / sweep_launches_per_iter
+    dom_ms, dom_n = kernel_timer_read(TIMER_DOM_SWEEP)
+    sp_ms, sp_n = kernel_timer_read(TIMER_SPECIES3)
+    total_steps = spinup + a.warmup + a.steps
     line = {
         "metric": METRIC, "value": cells * a.steps / wall, "unit": "cell-steps/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -312,14 +386,22 @@ def main(argv=None):
                    "projection": "fft-PCG rtol 1e-6", "parallelism": "single GPU",
                    "u10_m_s": cfg.wind_speed_m_s,
                    "hbm_used_gib": hbm_used / 2.0 ** 30},
-        "roofline": {"bound": "hbm", "kernel": kern, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": ROOFLINE_TRAFFIC.get((a.workload, a.dtype)), "bytes_per_launch": bytes_launch,
-                     "mean_launch_ms": ms, "launches": launches},
+        "window": {"spinup_steps": spinup, "warmup_steps": a.warmup,
+                   "timed_steps": [spinup + a.warmup + 1, total_steps],
+                   "config_steps": n_config, "t_start_s": t_win0, "t_end_s": sv.t},
+        "full_run": {"steps": total_steps,
+                     "value": cells * total_steps / (t1 - t_run0),
+                     "ms_per_step": 1e3 * (t1 - t_run0) / total_steps},
+        "roofline": _roofline("k_dom_sweep_tile", dom_bytes, dom_ms / max(dom_n, 1), dom_n,
+                              a.workload, a.dtype),
+        "roofline_species3": _roofline("k_species3", SPECIES3_FIELDS * esz * cells,
+                                       sp_ms / max(sp_n, 1), sp_n, a.workload, a.dtype),
         "cpu_baseline": None,
     }
-    bpc = step_bytes_per_cell(esz, 10, sv.dom.M,
-                              cfg.dom_subcycle_every, sv.n_particles, cells)
+    line["roofline"]["definition"] = (
+        f"{DOM_PASSES_PER_ORDINATE} field passes per ordinate per source iteration "
+        f"x {M} ordinates x {cells} cells x {esz} B / {sweep_launches_per_iter} launches")
+    bpc = step_bytes_per_cell(esz, 10, M, cfg.dom_subcycle_every, sv.n_particles, cells)
     ach = bpc * line["value"] / 1e9
     line["step_roofline"] = {
         "bound": "hbm", "bytes_per_cell_step": bpc, "achieved": ach, "peak": HBM_PEAK_GBS,
@@ -327,11 +409,32 @@ def main(argv=None):
         "definition": f"phase-fused compulsory traffic at I_cg={I_CG}, I_dom={I_DOM}"}
     if a.pcg_rhs > 0:
         line["pcg_time_to_1e8"] = pcg_time_to_rtol(sv, a.pcg_rhs, PCG_RTOL)
+    sv.close()
     del sv
     torch.cuda.empty_cache()
+    if a.early_window:
+        # the former headline window: the first warmup + steps of a fresh run
+        sv = Spread3DSolver(cfg)
+        for i in range(a.warmup):
+            sv.step()
+        torch.cuda.synchronize()
+        te0, tw0 = time.perf_counter(), sv.t
+        for i in range(a.steps):
+            sv.step()
+        torch.cuda.synchronize()
+        we = time.perf_counter() - te0
+        line["early_window"] = {"timed_steps": [a.warmup + 1, a.warmup + a.steps],
+                                "t_start_s": tw0, "t_end_s": sv.t,
+                                "value": cells * a.steps / we,
+                                "ms_per_step": 1e3 * we / a.steps}
+        print(f"# early window: {1e3 * we / a.steps:.3f} ms per step", file=sys.stderr,
+              flush=True)
+        sv.close()
+        del sv
+        torch.cuda.empty_cache()
     if a.cpu_budget > 0:
         print("# timing the oracle on the host", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(a.workload, a.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(a.workload, cfg.dom_subcycle_every)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -466,14 +466,19 @@ template <typename T, int PER, int TS>
 static void launch_tile(DomHandle* h, int D, int tjlo, int ntiles, T I_amb, hipStream_t s) {
   const size_t shm = TS * TS * (size_t)h->g.nx * sizeof(T);
   const dim3 grid(ntiles, h->M);
-  if (h->g.nx == 64 * PER)
-    hipLaunchKernelGGL((k_dom_sweep_tile<T, PER, true, TS>), grid, dim3(64 * TS), shm, s,
-                       h->ga, D, tjlo, h->M, (const double*)h->Om, (T*)h->I,
-                       (const T*)h->kap_safe, (const T*)h->B, (const T*)h->Ig_out, I_amb);
+  // the event-recording launch costs more host time than a plain one: use
+  // it only for the launches a timer asked for
+  const KernelEvents ev = timer_next(FW_TIMER_DOM_SWEEP);
+  auto kern = h->g.nx == 64 * PER ? k_dom_sweep_tile<T, PER, true, TS>
+                                  : k_dom_sweep_tile<T, PER, false, TS>;
+  if (ev.start)
+    hipExtLaunchKernelGGL(kern, grid, dim3(64 * TS), shm, s, ev.start, ev.stop, 0, h->ga, D,
+                          tjlo, h->M, (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
+                          (const T*)h->B, (const T*)h->Ig_out, I_amb);
   else
-    hipLaunchKernelGGL((k_dom_sweep_tile<T, PER, false, TS>), grid, dim3(64 * TS), shm, s,
-                       h->ga, D, tjlo, h->M, (const double*)h->Om, (T*)h->I,
-                       (const T*)h->kap_safe, (const T*)h->B, (const T*)h->Ig_out, I_amb);
+    hipLaunchKernelGGL(kern, grid, dim3(64 * TS), shm, s, h->ga, D, tjlo, h->M,
+                       (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
+                       (const T*)h->B, (const T*)h->Ig_out, I_amb);
 }
 
 // Tile side: 4 x 4 rows when their LDS rows fit 64 KB, else 0 (the per-row

@@ -50,14 +50,15 @@ int net_apply(void* net, const float* r, const float* q, const float** c, int* n
 int net_dims(void* net, int* nx, int* ny, int* nz);
 int density_feature(const float* rho, float* q, size_t n, hipStream_t s);
 
-// Events on which the next species launch records its own start and stop
-// (the dispatch's timestamps, as a kernel trace sees them), or null.  Set by
-// the step driver's optional timer around one call, cleared after it; one
-// per host thread, so steps driven from several threads do not share them.
+// Events on which a timed launch records its own start and stop (the
+// dispatch's timestamps, as a kernel trace sees them), or null.
 struct KernelEvents {
   hipEvent_t start = nullptr, stop = nullptr;
 };
-extern thread_local KernelEvents g_species3_events;
+// The pair for the next launch of kernel `which` (FW_TIMER_*) from this host
+// thread while its timer is on and the launch is one of the timed ones;
+// {null, null} otherwise (timer.hip).
+KernelEvents timer_next(int which);
 
 inline Launch cell_launch(const fw_grid* g) {
   int bx = 64;

@@ -66,58 +66,11 @@ static const double Q_RAD_MAX = 1.0e5, Q_PULSE = 240000.0, T_CAP = 10000.0;
 static const double T_BOOTSTRAP_S = 2.0;
 static const int N_SUB = 10, REINIT_INTERVAL = 10, REINIT_SUBSTEPS = 5;
 
-// Optional HIP-event timing of the sub-steps' species launch (k_species3,
-// one per sub-step), for the roofline figure of the benchmark.  The launch
-// itself records the events (hipExtLaunchKernel), so a pair spans the
-// kernel's execution and not the queue gap before it.
-struct SpeciesTimer {
-  bool on = false;
-  int used = 0, launches = 0;
-  int stride = 1, seen = 0;   // time every stride-th launch
-  static const int CAP = 4096;
-  hipEvent_t ev[2 * CAP];
-  bool made = false;
-};
-// one timer per host thread: steps driven from several threads (a batch)
-// each time their own launches
-static thread_local SpeciesTimer g_timer;
-
 }  // namespace fw
 
 using namespace fw;
 
 extern "C" {
-
-int fw_spread_timer(int enable) {
-  FW_REQUIRE(enable >= 0, "spread_timer: negative stride");
-  if (enable && !g_timer.made) {
-    for (int q = 0; q < 2 * SpeciesTimer::CAP; ++q)
-      FW_CHECK_HIP(hipEventCreate(&g_timer.ev[q]));
-    g_timer.made = true;
-  }
-  if (enable) {
-    g_timer.used = 0;
-    g_timer.launches = 0;
-    g_timer.seen = 0;
-    g_timer.stride = enable;
-  }
-  g_timer.on = enable != 0;
-  return FW_OK;
-}
-
-int fw_spread_timer_read(double* total_ms, int* launches) {
-  FW_REQUIRE(total_ms && launches, "spread_timer_read: null argument");
-  double tot = 0.0;
-  for (int q = 0; q < g_timer.used; ++q) {
-    FW_CHECK_HIP(hipEventSynchronize(g_timer.ev[2 * q + 1]));
-    float ms = 0.f;
-    FW_CHECK_HIP(hipEventElapsedTime(&ms, g_timer.ev[2 * q], g_timer.ev[2 * q + 1]));
-    tot += ms;
-  }
-  *total_ms = tot;
-  *launches = g_timer.launches;
-  return FW_OK;
-}
 
 int fw_spread_front(const fw_grid* g, const fw_spread_fields* f, fw_spread_state* st,
                     void* stream) {
@@ -297,27 +250,14 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                    s));
     // species with the dilution sources; the Y_O2 cap of :3059 rides in
     // y_max, the [0, 1] clip is the kernel's own
-    const bool timed = g_timer.on && g_timer.used < SpeciesTimer::CAP &&
-                       (g_timer.seen++ % g_timer.stride) == 0;
-    if (timed) {
-      g_species3_events.start = g_timer.ev[2 * g_timer.used];
-      g_species3_events.stop = g_timer.ev[2 * g_timer.used + 1];
-    }
     {
       const void* yin[3] = {*Y[0], *Y[1], *Y[2]};
       void* yout[3] = {f->tmp1, f->tmp2, f->tmp3};
-      const int rc = fw_species_transport3(g, yin, yout, f->rho, f->u, f->v, f->w,
-                                           f->src[0], f->src[1], dt_sub, 1.0e-5, y_in,
-                                           y_max, stream);
-      g_species3_events = KernelEvents();
-      if (rc) return rc;
+      TRY(fw_species_transport3(g, yin, yout, f->rho, f->u, f->v, f->w, f->src[0],
+                                f->src[1], dt_sub, 1.0e-5, y_in, y_max, stream));
       SWAP(*Y[0], f->tmp1);
       SWAP(*Y[1], f->tmp2);
       SWAP(*Y[2], f->tmp3);
-    }
-    if (timed) {
-      g_timer.used += 1;
-      g_timer.launches += 1;
     }
     // turbulent diffusion; the clips of :3082-3084 fold into the last pass
     // (with no pass they are the identity: each field is already in range)

@@ -617,7 +617,6 @@ static int species_impl(const fw_grid* g, const void* Y, void* Yo,
   return FW_OK;
 }
 
-thread_local KernelEvents g_species3_events;
 
 template <typename T>
 static int species3_impl(const fw_grid* g, const void* const* Y, void* const* Yo,
@@ -629,9 +628,10 @@ static int species3_impl(const fw_grid* g, const void* const* Y, void* const* Yo
   L.grid.z = (g->nz + CZ - 1) / CZ;
   // the event-recording launch costs about twice the host time of a plain
   // one: use it only while a timer asked for the events
-  if (g_species3_events.start)
-    hipExtLaunchKernelGGL((k_species3<T, CZ>), L.grid, L.block, 0, s, g_species3_events.start,
-                          g_species3_events.stop, 0, grid_args(g), (const T*)Y[0],
+  const KernelEvents ev = timer_next(FW_TIMER_SPECIES3);
+  if (ev.start)
+    hipExtLaunchKernelGGL((k_species3<T, CZ>), L.grid, L.block, 0, s, ev.start, ev.stop, 0,
+                          grid_args(g), (const T*)Y[0],
                           (const T*)Y[1], (const T*)Y[2], (T*)Yo[0], (T*)Yo[1], (T*)Yo[2],
                           (const T*)rho, (const T*)u, (const T*)v, (const T*)w,
                           (const T*)Sp, (const T*)Sd, T(dt), T(D), T(y_in[0]), T(y_in[1]),

@@ -687,6 +687,16 @@ int fw_spread_reduce(const fw_grid* g, const fw_spread_fields* f,
 int fw_spread_timer(int enable);
 int fw_spread_timer_read(double* total_ms, int* launches);
 
+/* The same timing for other kernels (no counterpart in the reference):
+ * which = FW_TIMER_SPECIES3 (the sub-steps' species kernel; what
+ * fw_spread_timer drives) or FW_TIMER_DOM_SWEEP (every launch of the DOM's
+ * tiled sweep, 27 per source iteration at 256 x 64 x 48).  stride >= 1
+ * resets and starts the timer, 0 stops it; at most 4096 launches are
+ * recorded.  Per calling host thread. */
+enum { FW_TIMER_SPECIES3 = 0, FW_TIMER_DOM_SWEEP = 1, FW_TIMER_KINDS = 2 };
+int fw_kernel_timer(int which, int stride);
+int fw_kernel_timer_read(int which, double* total_ms, int* launches);
+
 /* LevelSetFront3D.front_x (flame_front_3d.py:391-402) at (k=1, j=Ny/2) and
  * (k=n_z_bed/2, j=Ny/2) into st->front_exit_x / st->front_mid_x. */
 int fw_spread_front(const fw_grid* g64, const fw_spread_fields* f,
