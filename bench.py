@@ -149,13 +149,56 @@ PCG_RTOL = 1.0e-8
 NET_PARAMS = os.path.join(ROOT, "firewebapp_amd", "data", "precond_net.npz")
 
 
-def pcg_time_to_rtol(sv, n_rhs, rtol):
-    """Plain PCG vs Jacobi-PCG to rtol on the projection systems of the next
-    n_rhs steps, solved from x0 = 0 with a handle of their own."""
+def _time_solves(h, systems, modes, net, rtol):
+    """Iterations and wall time per system of fw_proj_solve from x0 = 0, per
+    preconditioner mode (1 plain, 2 Jacobi, 3 learned)."""
     import ctypes as C
     import torch
     from firewebapp_amd import _lib
+    from firewebapp_amd.device import ptr, stream_ptr
+    from firewebapp_amd.learned import attach
+    stream = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for name, mode in modes:
+        if mode == 3:
+            attach(h, net)
+        else:
+            _lib.call("fw_proj_set_preconditioner", h, mode)
+        its, ms, rel = [], [], []
+        for rho, b in systems:
+            x = torch.zeros_like(b)
+            it, rr = C.c_int(), C.c_double()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _lib.call("fw_proj_solve", h, C.c_void_p(rho.data_ptr()),
+                      C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), rtol,
+                      C.byref(it), C.byref(rr), C.c_void_p(stream))
+            torch.cuda.synchronize()
+            ms.append(1e3 * (time.perf_counter() - t0))
+            its.append(it.value)
+            rel.append(rr.value)
+        out[name] = {"iters_mean": float(np.mean(its)), "ms_mean": float(np.mean(ms)),
+                     "iters": its, "max_rel_resid": float(max(rel))}
+        if mode == 3:
+            fb = C.c_int()
+            _lib.call("fw_proj_learned_stats", h, C.byref(fb))
+            out[name]["fallbacks_to_plain"] = fb.value
+            attach(h, None)
+    return out
+
+
+def pcg_time_to_rtol(sv, n_rhs, rtol):
+    """Plain PCG, Jacobi-PCG and the learned preconditioner to rtol from
+    x0 = 0: on the projection systems of the bench workload's next n_rhs
+    steps, and (the learned network's own comparison) on 5 systems of a
+    held-out seeded 128 x 32 x 32 run (U10 4.5 m/s, moisture 6.5 %, a seed
+    outside the training runs)."""
+    import ctypes as C
+    import torch
+    from firewebapp_amd import _lib
+    from firewebapp_amd.learned import ConvNet, c1_deck, collect_systems
     from firewebapp_amd.physics_3d.fft_poisson_3d import create_handle
+    from firewebapp_amd.spread_3d import Spread3DConfig, Spread3DSolver
     stream = torch.cuda.current_stream().cuda_stream
     seq = []
     for _ in range(n_rhs):
@@ -163,48 +206,40 @@ def pcg_time_to_rtol(sv, n_rhs, rtol):
         b = torch.empty(sv.shape, dtype=torch.float64, device="cuda")
         _lib.call("fw_proj_rhs", sv.proj, C.c_void_p(b.data_ptr()), C.c_void_p(stream))
         seq.append((sv.f["rho"].clone(), b))
-    h = create_handle(sv.dg, 1.0e-6, rtol, maxiter=50000)
+    theta = np.load(NET_PARAMS)["theta"] if os.path.exists(NET_PARAMS) else None
     out = {"rhs": n_rhs, "x0": "zero", "rtol": rtol}
-    net = None
-    modes = [("plain", 1), ("jacobi", 2)]
-    if os.path.exists(NET_PARAMS):
-        from firewebapp_amd.learned import ConvNet, attach
-        nz, ny, nx = sv.shape
-        net = ConvNet(nx, ny, nz, theta=np.load(NET_PARAMS)["theta"])
-        modes.append(("learned", 3))
-    else:
-        out["learned"] = None
+    modes = [("plain", 1), ("jacobi", 2)] + ([("learned", 3)] if theta is not None else [])
+    h = create_handle(sv.dg, 1.0e-6, rtol, maxiter=50000)
+    nz, ny, nx = sv.shape
+    net = ConvNet(nx, ny, nz, theta=theta) if theta is not None else None
     try:
-        for name, mode in modes:
-            if mode == 3:
-                attach(h, net)
-            else:
-                _lib.call("fw_proj_set_preconditioner", h, mode)
-            its, ms, rel = [], [], []
-            for rho, b in seq:
-                x = torch.zeros_like(b)
-                it, rr = C.c_int(), C.c_double()
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                _lib.call("fw_proj_solve", h, C.c_void_p(rho.data_ptr()),
-                          C.c_void_p(b.data_ptr()), C.c_void_p(x.data_ptr()), rtol,
-                          C.byref(it), C.byref(rr), C.c_void_p(stream))
-                torch.cuda.synchronize()
-                ms.append(1e3 * (time.perf_counter() - t0))
-                its.append(it.value)
-                rel.append(rr.value)
-            out[name] = {"iters_mean": float(np.mean(its)), "ms_mean": float(np.mean(ms)),
-                         "iters": its, "max_rel_resid": float(max(rel))}
-            if mode == 3:
-                fb = C.c_int()
-                _lib.call("fw_proj_learned_stats", h, C.byref(fb))
-                out[name]["fallbacks_to_plain"] = fb.value
-            print(f"# {name} PCG to {rtol:g}: {np.mean(its):.1f} iterations, "
-                  f"{np.mean(ms):.2f} ms per system", file=sys.stderr, flush=True)
+        out.update(_time_solves(h, seq, modes, net, rtol))
     finally:
         _lib.lib().fw_proj_destroy(h)
         if net is not None:
             net.close()
+    for name in ("plain", "jacobi", "learned"):
+        if name in out:
+            print(f"# {name} PCG to {rtol:g}: {out[name]['iters_mean']:.1f} iterations, "
+                  f"{out[name]['ms_mean']:.2f} ms per system", file=sys.stderr, flush=True)
+    if theta is not None:
+        ri, w = c1_deck(4.5, 0.065, 97)
+        c1 = Spread3DSolver(Spread3DConfig.resolve(ri, wind_speed_m_s=w))
+        held = collect_systems(c1, 60, 12)
+        h = create_handle(c1.dg, 1.0e-6, rtol, maxiter=50000)
+        net = ConvNet(*c1.shape[::-1], theta=theta)
+        try:
+            r = _time_solves(h, held, [("plain", 1), ("learned", 3)], net, rtol)
+        finally:
+            _lib.lib().fw_proj_destroy(h)
+            net.close()
+            c1.close()
+        r["systems"] = "5 projection systems (steps 12-60) of a held-out 128x32x32 run"
+        out["held_out_128x32x32"] = r
+        print(f"# held-out 128x32x32 to {rtol:g}: plain {r['plain']['iters_mean']:.1f} "
+              f"iterations {r['plain']['ms_mean']:.2f} ms, learned "
+              f"{r['learned']['iters_mean']:.1f} iterations {r['learned']['ms_mean']:.2f} ms",
+              file=sys.stderr, flush=True)
     return out
 
 

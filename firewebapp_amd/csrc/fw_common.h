@@ -44,7 +44,7 @@ int coupling_launch(const fw_grid* g, void* T_g, void* T_s, const void* rho, con
                     double dt, double T_amb, int q_loss, double h_conv_mult, int nzb, int i0,
                     int i1, double drip_gas, double drip_solid, hipStream_t s);
 
-// The learned preconditioner's network (learned.hip), for projection.hip.
+// The learned preconditioner's network (unet.hip), for projection.hip.
 int net_apply(void* net, const float* r, const float* q, const float** c, int* n,
               hipStream_t s);
 int net_dims(void* net, int* nx, int* ny, int* nz);

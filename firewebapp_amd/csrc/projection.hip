@@ -1195,9 +1195,10 @@ struct ProjHandle {
   double *part, *S, *S_host;
   double *Ppq, *Prr, *Prz;  // partials of the PCG iteration's reductions
   void* irho;          // 1 / max(rho, 0.01) of the running solve (type T)
-  void* net;           // learned correction (learned.hip), or null
+  void* net;           // learned correction (unet.hip), or null
   double* Pqz;         // partials of <q, z>_W (flexible PCG)
   int learned_budget;     // inner iterations a learned round may take
+  int learned_every;      // the network corrects every k-th direction (0: the first only)
   int learned_fallback;   // this solve fell back to plain PCG
   int learned_fallbacks;  // solves that fell back, since creation
   float* rs;           // r / RMS(r), the network's input
@@ -1473,10 +1474,14 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
       } else {
         hipLaunchKernelGGL(k_update_xr2<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
                            h->Prz, npz, h->Ppq, h->S, h->Prr);
-        // learned mode: the network corrects the first direction of the
-        // round only (its training inputs are the rounds' initial residuals);
-        // the later ones use the separable inverse, with the flexible beta
-        if ((rc = flex ? precond_apply<T>(h, r, z, s) : apply_M<T>(h, r, z, s))) return rc;
+        // learned mode: the network corrects every learned_every-th
+        // direction (its training inputs are residuals of every stage of a
+        // solve), the others use the separable inverse; the flexible beta
+        // keeps the directions conjugate to the last one
+        const int itn = queued + b + 1;
+        const bool use_net = flex && h->learned_every > 0 && itn % h->learned_every == 0;
+        if ((rc = flex && !use_net ? precond_apply<T>(h, r, z, s) : apply_M<T>(h, r, z, s)))
+          return rc;
         if (flex)
           hipLaunchKernelGGL(k_dot2_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, q, z,
                              h->Prz, h->Pqz, h->Prr, h->S, 1);
@@ -1685,6 +1690,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   h->esz = g->dtype == FW_F64 ? 8 : 4;
   h->precond_enabled = FW_PRECOND_SEPARABLE;
   h->learned_budget = 60;
+  h->learned_every = 4;
   const int Nx = g->nx, Ny = g->ny, Nz = g->nz;
   // host copies of the dz metric for folding sqrt(dz) into the z matrices
   std::vector<double> dz(Nz);
@@ -1875,6 +1881,13 @@ int fw_proj_set_learned(void* handle, void* net) {
   if (!h->rs) FW_CHECK_HIP(hipMalloc((void**)&h->rs, h->n * sizeof(float)));
   if (!h->qf) FW_CHECK_HIP(hipMalloc((void**)&h->qf, h->n * sizeof(float)));
   h->net = net;
+  return FW_OK;
+}
+
+int fw_proj_set_learned_every(void* handle, int every) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h && every >= 0, "proj_set_learned_every: bad argument");
+  h->learned_every = every;
   return FW_OK;
 }
 

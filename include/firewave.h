@@ -247,9 +247,10 @@ enum { FW_PRECOND_NONE = 0, FW_PRECOND_SEPARABLE = 1, FW_PRECOND_JACOBI = 2,
        FW_PRECOND_LEARNED = 3 };
 int fw_proj_set_preconditioner(void* handle, int mode);
 
-/* FW_PRECOND_LEARNED (no counterpart in the reference): the first direction
- * of each inner PCG round is z = M_sep r + s N(r / s, RHO_REF / rho - 1),
- * s the RMS of r, the later ones M_sep r; N is the attached network
+/* FW_PRECOND_LEARNED (no counterpart in the reference): the preconditioned
+ * direction is z = M_sep r + s N(r / s, RHO_REF / rho - 1), s the RMS of r,
+ * on every k-th inner PCG direction (fw_proj_set_learned_every, default 4;
+ * the other directions use M_sep r alone); N is the attached network
  * (fw_net_create on the same grid, float32); flexible PCG
  * (beta = <z+, r+ - r>_W / <z, r>_W).
  * net = null detaches it (the mode falls back to separable).  The network
@@ -261,6 +262,10 @@ int fw_proj_set_learned(void* handle, void* net);
  * the separable preconditioner.  fallbacks = the solves that did, since
  * the handle was created. */
 int fw_proj_learned_stats(void* handle, int* fallbacks);
+
+/* The network corrects every k-th direction of the learned mode (k >= 1;
+ * k = 0: the first direction of each inner round only). */
+int fw_proj_set_learned_every(void* handle, int every);
 
 /* Copy the right-hand side b = (div - target)/dt of the last fw_proj_project
  * (float64, N values) out: the system the reference hands to its Krylov
@@ -765,12 +770,16 @@ int fw_bed_seed(const fw_bed_particles* particles, double x_lo, double x_front,
 /* ---- learned preconditioner (no counterpart in the reference) ----------- */
 /* The reference preconditions its Krylov solve with the separable inverse
  * only (projection_3d.py:756-791).  These entry points add a learned
- * correction: a two-level 3-D convolutional network (float32, y periodic,
- * zero padding in x and z, grid dims even) trained on the solver's own
- * projection systems, applied inside flexible PCG (FW_PRECOND_LEARNED).
- * Parameters are one flat float32 vector of fw_net_nparams() values (per
- * layer: weights (co, ci, 27) then biases (co); layers 2->8, 8->16 coarse,
- * 16->16 coarse, 8+16->8, 8->1). */
+ * correction: a 4-level 3-D U-Net (16/32/64/64 channels, 3x3x3 kernels, y
+ * periodic, zero padding in x and z, grid dims multiples of 8) with bfloat16
+ * weights and activations and float32 accumulation on the matrix cores,
+ * trained on the solver's own projection systems, applied inside flexible
+ * PCG (FW_PRECOND_LEARNED).  Layers (level, input channels -> output):
+ * e0 L0 [r, q] 2 -> 16; e1 L1 pool(e0) 16 -> 32; e2 L2 pool(e1) 32 -> 64;
+ * e3 L3 pool(e2) 64 -> 64; d2 L2 [e2, up(e3)] 128 -> 64; d1 L1 [e1, up(d2)]
+ * 96 -> 32; d0 L0 [e0, up(d1)] 48 -> 16; c L0 d0 16 -> 1 (linear), relu
+ * elsewhere.  Parameters are one flat float32 vector of fw_net_nparams()
+ * values (per layer: weights (27, ci, co) then biases (co)). */
 int fw_net_nparams(void);
 /* q = RHO_REF / max(rho, 0.01) - 1 for n float32 values (the network's
  * density input, as the learned preconditioner forms it). */

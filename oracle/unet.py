@@ -5,153 +5,196 @@ The reference has no learned preconditioner (its projection is BiCGSTAB with
 the separable inverse, projection_3d.py:756-791), so nothing here restates
 reference code: this is the NumPy statement of the network the GPU kernels
 implement, used to check their forward values and gradients.  It is pinned
-by central finite differences of the loss (every parameter block) and by
-closed forms (a delta kernel is a copy, a shifted delta kernel is a shift,
-pooling then upsampling a piecewise-constant field is the identity).
+by central finite differences of the loss (every layer), the adjoint
+identity of the convolution, and closed forms (a delta kernel is a copy, a
+shifted delta kernel is a shift, pooling an upsampled field is the
+identity).
 
-Network (two levels, grid dims even; y periodic, zero padding in x and z),
-on the residual r and a density feature q = RHO_REF / rho - 1 (the
-correction the separable inverse needs depends on the density field):
-    h0  = relu(conv3([r, q], w0, b0))        2  -> C0
-    d1  = avgpool2(h0)
-    h1  = relu(conv3(d1, w1, b1))            C0 -> C1   (coarse grid)
-    h1b = relu(conv3(h1, w2, b2))            C1 -> C1
-    u0  = upsample2(h1b)
-    h2  = relu(conv3([h0, u0], w3, b3))      C0+C1 -> C0
-    c   = conv3(h2, w4, b4)                  C0 -> 1
-conv3: y[co, k, j, i] = b[co] + sum_ci sum_t w[co, ci, t] x[ci, k+dk, j+dj, i+di]
-with t = (dk+1) 9 + (dj+1) 3 + (di+1).
+Network: a 4-level 3-D U-Net with 16/32/64/64 channels and 3x3x3 kernels on
+levels L0 (the grid), L1 = L0/2, L2 = L0/4, L3 = L0/8 (grid dims divisible by
+8; y periodic, zero padding in x and z).  Inputs are the residual r and the
+density feature q = RHO_REF / rho - 1.  Activations are channel-last
+(Nz, Ny, Nx, C).
+    e0 = relu(conv([r, q]))                  L0   2 -> 16
+    e1 = relu(conv(pool(e0)))                L1  16 -> 32
+    e2 = relu(conv(pool(e1)))                L2  32 -> 64
+    e3 = relu(conv(pool(e2)))                L3  64 -> 64
+    d2 = relu(conv([e2, up(e3)]))            L2 128 -> 64
+    d1 = relu(conv([e1, up(d2)]))            L1  96 -> 32
+    d0 = relu(conv([e0, up(d1)]))            L0  48 -> 16
+    c  = conv(d0)                            L0  16 -> 1
+conv: y[p, co] = b[co] + sum_t sum_ci w[t, ci, co] x[p + off_t, ci], tap
+t = (dk+1) 9 + (dj+1) 3 + (di+1); pool: mean of the 8 children; up: nearest.
+Parameters: one flat vector, per layer w (27, ci, co) then b (co).
 Loss: L = sum dz_k (c - t)^2 / sum dz_k t^2.
+
+``bf16=True`` applies the GPU kernels' rounding model: weights, the inputs r
+and q, every stored activation (after bias and relu, and after pooling) and
+every stored gradient of an activation (after the relu mask) are rounded to
+bfloat16 (round to nearest even); sums are exact here, float32 on the GPU.
+The output c and the weight gradients are not rounded.
 """
 from __future__ import annotations
 
 import numpy as np
 
-C0, C1 = 8, 16
-LAYERS = (("w0", "b0", 2, C0), ("w1", "b1", C0, C1), ("w2", "b2", C1, C1),
-          ("w3", "b3", C0 + C1, C0), ("w4", "b4", C0, 1))
+# (name, level, c1 (same-level input), c2 (coarse input, upsampled), co, relu)
+LAYERS = (("e0", 0, 2, 0, 16, True), ("e1", 1, 16, 0, 32, True),
+          ("e2", 2, 32, 0, 64, True), ("e3", 3, 64, 0, 64, True),
+          ("d2", 2, 64, 64, 64, True), ("d1", 1, 32, 64, 32, True),
+          ("d0", 0, 16, 32, 16, True), ("c", 0, 16, 0, 1, False))
 TAPS = [(dk, dj, di) for dk in (-1, 0, 1) for dj in (-1, 0, 1) for di in (-1, 0, 1)]
 
 
 def n_params():
-    return sum(co * ci * 27 + co for _, _, ci, co in LAYERS)
+    return sum(27 * (c1 + c2) * co + co for _, _, c1, c2, co, _ in LAYERS)
 
 
 def unpack(theta):
-    """Flat parameter vector (per layer: w (co, ci, 27) then b (co)) -> dict."""
+    """Flat parameter vector -> {name: (w (27, ci, co), b (co))}."""
     p, o = {}, 0
-    for wn, bn, ci, co in LAYERS:
-        p[wn] = theta[o:o + co * ci * 27].reshape(co, ci, 27)
-        o += co * ci * 27
-        p[bn] = theta[o:o + co]
-        o += co
+    for name, _, c1, c2, co, _ in LAYERS:
+        nw = 27 * (c1 + c2) * co
+        p[name] = (theta[o:o + nw].reshape(27, c1 + c2, co), theta[o + nw:o + nw + co])
+        o += nw + co
     return p
 
 
 def pack(p):
-    return np.concatenate([np.concatenate([p[wn].ravel(), p[bn].ravel()])
-                           for wn, bn, _, _ in LAYERS])
+    return np.concatenate([np.concatenate([p[n][0].ravel(), p[n][1].ravel()])
+                           for n, *_ in LAYERS])
 
 
 def init(seed=0):
-    """He-style random weights, zero biases (the GPU training starts here)."""
+    """He-style random weights, zero biases, a small output layer."""
     rng = np.random.default_rng(seed)
     p = {}
-    for wn, bn, ci, co in LAYERS:
-        p[wn] = rng.standard_normal((co, ci, 27)) * np.sqrt(2.0 / (27 * ci))
-        p[bn] = np.zeros(co)
-    p["w4"] *= 0.1
+    for name, _, c1, c2, co, _ in LAYERS:
+        ci = c1 + c2
+        w = rng.standard_normal((27, ci, co)) * np.sqrt(2.0 / (27 * ci))
+        if name == "c":
+            w *= 0.1
+        p[name] = (w, np.zeros(co))
     return pack(p)
 
 
-def _shift(x, dk, dj, di):
-    """s[..., k, j, i] = x[..., k+dk, (j+dj) mod Ny, i+di], zero outside in k, i."""
-    s = np.roll(x, -dj, axis=-2)
+def bf16(a):
+    """Round to bfloat16 (nearest, ties to even), returned as float64."""
+    u = np.ascontiguousarray(np.asarray(a, np.float32)).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def shift(x, dk, dj, di):
+    """s[k, j, i] = x[k+dk, (j+dj) mod Ny, i+di], zero outside in k and i."""
+    s = np.roll(x, -dj, axis=1)
     out = np.zeros_like(s)
-    Nz, Nx = x.shape[-3], x.shape[-1]
+    Nz, Nx = x.shape[0], x.shape[2]
     ks, ke = max(0, -dk), min(Nz, Nz - dk)
     is_, ie = max(0, -di), min(Nx, Nx - di)
-    out[..., ks:ke, :, is_:ie] = s[..., ks + dk:ke + dk, :, is_ + di:ie + di]
+    out[ks:ke, :, is_:ie] = s[ks + dk:ke + dk, :, is_ + di:ie + di]
     return out
 
 
-def conv3(x, w, b):
-    co = w.shape[0]
-    y = np.empty((co,) + x.shape[1:])
-    y[:] = b[:, None, None, None]
+def conv(x, w, b=None):
+    y = np.zeros(x.shape[:3] + (w.shape[2],))
     for t, (dk, dj, di) in enumerate(TAPS):
-        y += np.tensordot(w[:, :, t], _shift(x, dk, dj, di), axes=(1, 0))
-    return y
+        y += shift(x, dk, dj, di) @ w[t]
+    return y if b is None else y + b
 
 
-def conv3_T(g, w):
-    """Adjoint of conv3 in x: dx[ci, p] = sum_t sum_co w[co, ci, t] g[co, p - off_t]."""
-    dx = np.zeros((w.shape[1],) + g.shape[1:])
+def conv_T(g, w):
+    """Adjoint of conv in x: dx[q, ci] = sum_t sum_co w[t, ci, co] g[q - off_t, co]."""
+    dx = np.zeros(g.shape[:3] + (w.shape[1],))
     for t, (dk, dj, di) in enumerate(TAPS):
-        dx += np.tensordot(w[:, :, t].T, _shift(g, -dk, -dj, -di), axes=(1, 0))
+        dx += shift(g, -dk, -dj, -di) @ w[t].T
     return dx
 
 
-def conv3_dw(g, x):
-    """dL/dw[co, ci, t] = sum_q g[co, q] x[ci, q + off_t]."""
-    co, ci = g.shape[0], x.shape[0]
-    dw = np.empty((co, ci, 27))
+def conv_dw(g, x):
+    """dL/dw[t, ci, co] = sum_p x[p + off_t, ci] g[p, co]."""
+    dw = np.empty((27, x.shape[3], g.shape[3]))
     for t, (dk, dj, di) in enumerate(TAPS):
-        dw[:, :, t] = np.tensordot(g, _shift(x, dk, dj, di), axes=((1, 2, 3), (1, 2, 3)))
+        dw[t] = np.tensordot(shift(x, dk, dj, di), g, axes=((0, 1, 2), (0, 1, 2)))
     return dw
 
 
-def pool2(x):
-    c, Nz, Ny, Nx = x.shape
-    return x.reshape(c, Nz // 2, 2, Ny // 2, 2, Nx // 2, 2).mean(axis=(2, 4, 6))
+def pool(x):
+    Nz, Ny, Nx, c = x.shape
+    return x.reshape(Nz // 2, 2, Ny // 2, 2, Nx // 2, 2, c).mean(axis=(1, 3, 5))
 
 
-def up2(x):
-    return x.repeat(2, axis=1).repeat(2, axis=2).repeat(2, axis=3)
+def up(x):
+    return x.repeat(2, axis=0).repeat(2, axis=1).repeat(2, axis=2)
 
 
-def forward(theta, r, q):
-    """r, q: (Nz, Ny, Nx).  Returns (c, activations)."""
+def sum8(x):
+    """Adjoint of up: the sum over the 8 children."""
+    Nz, Ny, Nx, c = x.shape
+    return x.reshape(Nz // 2, 2, Ny // 2, 2, Nx // 2, 2, c).sum(axis=(1, 3, 5))
+
+
+def forward(theta, r, q, bf=False):
+    """r, q: (Nz, Ny, Nx).  Returns (c, activations, layer inputs)."""
+    R = bf16 if bf else (lambda a: np.asarray(a, np.float64))
     p = unpack(theta)
-    a = {}
-    a["r"] = np.stack([r, q])
-    a["h0"] = np.maximum(conv3(a["r"], p["w0"], p["b0"]), 0.0)
-    a["d1"] = pool2(a["h0"])
-    a["h1"] = np.maximum(conv3(a["d1"], p["w1"], p["b1"]), 0.0)
-    a["h1b"] = np.maximum(conv3(a["h1"], p["w2"], p["b2"]), 0.0)
-    a["cat"] = np.concatenate([a["h0"], up2(a["h1b"])])
-    a["h2"] = np.maximum(conv3(a["cat"], p["w3"], p["b3"]), 0.0)
-    c = conv3(a["h2"], p["w4"], p["b4"])[0]
-    return c, a
+    W = {n: (R(w), b) for n, (w, b) in p.items()}
+    a, xin = {}, {}
+    x = R(np.stack([r, q], axis=-1))
+    for name, lev, c1, c2, co, relu in LAYERS:
+        if name == "e0":
+            xi = x
+        elif name in ("e1", "e2", "e3"):
+            prev = {"e1": "e0", "e2": "e1", "e3": "e2"}[name]
+            a["p" + prev] = R(pool(a[prev]))
+            xi = a["p" + prev]
+        elif name == "c":
+            xi = a["d0"]
+        else:
+            skip, low = {"d2": ("e2", "e3"), "d1": ("e1", "d2"), "d0": ("e0", "d1")}[name]
+            xi = np.concatenate([a[skip], up(a[low])], axis=-1)
+        xin[name] = xi
+        y = conv(xi, *W[name])
+        if name == "c":
+            return y[..., 0], a, xin
+        a[name] = R(np.maximum(y, 0.0))
 
 
-def loss(theta, r, q, t, dz):
-    c, _ = forward(theta, r, q)
-    wz = dz[:, None, None]
+def loss(theta, r, q, t, dz, bf=False):
+    c, _, _ = forward(theta, r, q, bf)
+    wz = np.asarray(dz)[:, None, None]
     return float(np.sum(wz * (c - t) ** 2) / np.sum(wz * t ** 2))
 
 
-def backward(theta, r, q, t, dz):
-    """(L, dL/dtheta) for one sample."""
+def backward(theta, r, q, t, dz, bf=False):
+    """(L, dL/dtheta) for one sample, with the rounding model of ``bf``."""
+    R = bf16 if bf else (lambda a: np.asarray(a, np.float64))
     p = unpack(theta)
-    c, a = forward(theta, r, q)
-    wz = dz[:, None, None]
+    W = {n: R(w) for n, (w, _) in p.items()}
+    c, a, xin = forward(theta, r, q, bf)
+    wz = np.asarray(dz)[:, None, None]
     den = np.sum(wz * t ** 2)
     L = float(np.sum(wz * (c - t) ** 2) / den)
     g = {}
-    gc = (2.0 * wz * (c - t) / den)[None]
-    g["w4"], g["b4"] = conv3_dw(gc, a["h2"]), gc.sum(axis=(1, 2, 3))
-    dh2 = conv3_T(gc, p["w4"]) * (a["h2"] > 0)
-    g["w3"], g["b3"] = conv3_dw(dh2, a["cat"]), dh2.sum(axis=(1, 2, 3))
-    dcat = conv3_T(dh2, p["w3"])
-    du0 = dcat[C0:]
-    c_, Nz, Ny, Nx = du0.shape
-    dh1b = du0.reshape(c_, Nz // 2, 2, Ny // 2, 2, Nx // 2, 2).sum(axis=(2, 4, 6))
-    dh1b = dh1b * (a["h1b"] > 0)
-    g["w2"], g["b2"] = conv3_dw(dh1b, a["h1"]), dh1b.sum(axis=(1, 2, 3))
-    dh1 = conv3_T(dh1b, p["w2"]) * (a["h1"] > 0)
-    g["w1"], g["b1"] = conv3_dw(dh1, a["d1"]), dh1.sum(axis=(1, 2, 3))
-    dd1 = conv3_T(dh1, p["w1"])
-    dh0 = (dcat[:C0] + up2(dd1) / 8.0) * (a["h0"] > 0)
-    g["w0"], g["b0"] = conv3_dw(dh0, a["r"]), dh0.sum(axis=(1, 2, 3))
+    gc = (2.0 * wz * (c - t) / den)[..., None]
+    g["c"] = (conv_dw(gc, xin["c"]), gc.sum(axis=(0, 1, 2)))
+    m = lambda name: a[name] > 0.0
+    gd = {"d0": R(conv_T(gc, W["c"]) * m("d0"))}
+    dx = {}
+    for name in ("d0", "d1", "d2"):
+        gg = gd[name]
+        g[name] = (conv_dw(gg, xin[name]), gg.sum(axis=(0, 1, 2)))
+        dx[name] = conv_T(gg, W[name])
+        c1 = dict((l[0], l[2]) for l in LAYERS)[name]
+        low = {"d0": "d1", "d1": "d2", "d2": "e3"}[name]
+        gd[low] = R(sum8(dx[name][..., c1:]) * m(low))
+    for name, skip_of in (("e3", "d2"), ("e2", "d1"), ("e1", "d0")):
+        gg = gd[name]
+        g[name] = (conv_dw(gg, xin[name]), gg.sum(axis=(0, 1, 2)))
+        dpool = conv_T(gg, W[name])
+        prev = {"e3": "e2", "e2": "e1", "e1": "e0"}[name]
+        c1 = dict((l[0], l[2]) for l in LAYERS)[skip_of]
+        gd[prev] = R((dx[skip_of][..., :c1] + up(dpool) / 8.0) * m(prev))
+    gg = gd["e0"]
+    g["e0"] = (conv_dw(gg, xin["e0"]), gg.sum(axis=(0, 1, 2)))
     return L, pack(g)
