@@ -26,6 +26,8 @@
 namespace fw {
 
 #define PJ_RED 512  // partial-sum workgroups of the projection reductions
+#define PJ_MV_MAX FW_PROJ_MV_MAX  // most workgroups of the fused PCG operator launch
+#define PJ_MV_DEFAULT 1024        // its default (two rows or more per workgroup at 256x64x48)
 
 // S_DONE: 0 while the inner PCG runs, 1 once ||r|| <= S_TOL, 2 on a NaN
 // residual, 3 when k_round_init found the solve converged, 4 on a NaN
@@ -1021,6 +1023,174 @@ k_zpair_mfma(ColArgs a, const float* __restrict__ Zf, const float* __restrict__ 
   }
 }
 
+// The whole y-z part of the separable solve for one x frequency per
+// workgroup (float32, matrix cores): with P the (Nz x Ny) plane of the
+// x-transformed field at frequency ih (row k, column j),
+//   A = P Yf^T,  B = D^-1 (Zf A),  C = Zi B,  out = C Yi^T
+// (y forward, z forward with the spectral divide, z inverse, y inverse).
+// The planes are the contiguous blocks in[ih ldt + k Ny + j] of the
+// transposed work layout (k_dct4_tile); each is read once and written once,
+// and the four products run from LDS and registers.  One wavefront per
+// 16-column block j0 of the
+// padded plane (Nzp = ceil16(Nz) rows, Nyp = ceil16(Ny) columns, zero
+// padding); MFMA operand layouts as in k_coltrans_mfma.
+//   A: B-operand fragments of Yf from Yfp (k_coltrans_mfma's packed
+//      order, lane stride ky4 = col_pad(Ny)/4), A-operand from the plane in
+//      LDS; the result tile mt sits in acc1[mt] (row 16mt + 4(l/16) + r in
+//      register r of lane l, column j0 + l%16).
+//   B, C: the z products sum over k in the order k = 16 mt + 4 q + r for
+//      step 4 mt + r and lane quarter q, which is exactly where the previous
+//      product left its rows: acc1[mt][r] is the B operand of step 4 mt + r
+//      with no data movement.  ZfQ / ZiQ hold the matching A fragments,
+//      ZfQ[(mt' 64 + l) 16 + 4 mt + r] = Zf[16 mt' + l%16][16 mt + 4(l/16) + r]
+//      (four row tiles of 16 steps, zero past Nz).
+//   out: C goes to LDS (over P, after every wavefront has read P), then
+//      out = C Yi^T as in A.
+// MT = ceil(Nz / 16) row tiles and NYS = ceil16(Ny) / 4 y steps are template
+// constants: with runtime trip counts the guarded accumulating MFMAs make
+// the compiler copy accumulators at every join, more moves than MFMAs.
+#define YZ_MT 4   // row tiles of the largest plane (Nz <= 64)
+template <int MT, int NYS>
+__global__ void __launch_bounds__(512)
+k_yz_plane(int ldt, int Ny, int Nz, int ky4, const float* __restrict__ Yfp,
+           const float* __restrict__ Yip, const float* __restrict__ ZfQ,
+           const float* __restrict__ ZiQ, const double* __restrict__ lx,
+           const double* __restrict__ ly, const double* __restrict__ lz, double eps,
+           const float* __restrict__ in, float* __restrict__ out) {
+  // Every operand buffer is padded to compile-time bounds (YZ_MT row tiles,
+  // 16 z steps, a 64-row LDS plane with zero rows past Nz), so operand loads
+  // are unconditional and issue back to back ahead of their MFMAs instead
+  // of each waiting out its own latency inside a runtime guard.
+  extern __shared__ float pl[];          // 64 x (Nyp + 4)
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int Nyp = 4 * NYS, nys = NYS;
+  constexpr int S = Nyp + 4;             // padded row: conflict-free A reads
+  const uint32_t base = (uint32_t)blockIdx.x * (uint32_t)ldt;   // plane ih = blockIdx.x
+  // The plane: the block is 4 Nyp threads, thread (kt, jt) = (t / Nyp,
+  // t % Nyp) holds rows k = 4 i + kt, i < 16 (zero past Nz or Ny).
+  const int jt = threadIdx.x % Nyp, kt = threadIdx.x / Nyp;
+  float v[YZ_MT * 4];
+#pragma unroll
+  for (int i = 0; i < YZ_MT * 4; ++i) {
+    const int k = 4 * i + kt;
+    v[i] = k < Nz && jt < Ny ? at(in, (base + (uint32_t)k * Ny + jt) * 4u) : 0.0f;
+  }
+  const int j0 = 16 * wv, q = lane >> 4, c = lane & 15;
+  // this wavefront's B fragments of Yf (nys <= 32 steps)
+  float yb[32];
+  {
+    const float* bf = Yfp + ((size_t)wv * 64 + lane) * ky4;
+#pragma unroll
+    for (int st = 0; st < 32; ++st)
+      if (st < nys) yb[st] = bf[st];
+  }
+  // the z forward A fragments: 16 per lane and row tile, four 16-byte loads
+  f32x4 za[YZ_MT][4], zb[YZ_MT][4];
+#pragma unroll
+  for (int mo = 0; mo < YZ_MT; ++mo) {
+    const f32x4* p = reinterpret_cast<const f32x4*>(ZfQ + ((size_t)mo * 64 + lane) * 16);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) za[mo][u] = p[u];
+  }
+#pragma unroll
+  for (int i = 0; i < YZ_MT * 4; ++i) pl[(4 * i + kt) * S + jt] = v[i];
+  __syncthreads();
+  // the z inverse fragments, in flight during the first product
+#pragma unroll
+  for (int mo = 0; mo < YZ_MT; ++mo) {
+    const f32x4* p = reinterpret_cast<const f32x4*>(ZiQ + ((size_t)mo * 64 + lane) * 16);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) zb[mo][u] = p[u];
+  }
+  f32x4 acc1[YZ_MT], acc2[YZ_MT];
+#pragma unroll
+  for (int mt = 0; mt < YZ_MT; ++mt) acc1[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  // A = P Yf^T
+#pragma unroll
+  for (int st = 0; st < 32; ++st) {
+    if (st >= nys) continue;
+    const float* ar = pl + c * S + 4 * st + q;
+    float a4[YZ_MT];
+#pragma unroll
+    for (int mt = 0; mt < YZ_MT; ++mt) a4[mt] = ar[16 * mt * S];
+#pragma unroll
+    for (int mt = 0; mt < YZ_MT; ++mt)
+      if (mt < MT)
+        acc1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[mt], yb[st], acc1[mt], 0, 0, 0);
+  }
+  // the Yi fragments of the last product, in flight during the z products
+  {
+    const float* bf = Yip + ((size_t)wv * 64 + lane) * ky4;
+#pragma unroll
+    for (int st = 0; st < 32; ++st)
+      if (st < nys) yb[st] = bf[st];
+  }
+  __syncthreads();   // P is no longer read: C may overwrite it
+  // B = D^-1 (Zf A)
+  const double sd = ly[j0 + c < Ny ? j0 + c : Ny - 1] + lx[blockIdx.x] - eps;
+#pragma unroll
+  for (int mo = 0; mo < YZ_MT; ++mo) {
+    if (mo >= MT) continue;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int mt = 0; mt < YZ_MT; ++mt)
+      if (mt < MT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(za[mo][mt][r], acc1[mt][r], acc, 0, 0, 0);
+      }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = 16 * mo + 4 * q + r;
+      // rows past Nz stay zero: the inverse's zero-padded A columns meet them
+      acc[r] = m < Nz ? acc[r] * recip((float)(lz[m] + sd)) : 0.0f;
+    }
+    acc2[mo] = acc;
+  }
+  // C = Zi B, into LDS (rows past Nzp keep the zeros of the load)
+#pragma unroll
+  for (int mo = 0; mo < YZ_MT; ++mo) {
+    if (mo >= MT) continue;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int mt = 0; mt < YZ_MT; ++mt)
+      if (mt < MT) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(zb[mo][mt][r], acc2[mt][r], acc, 0, 0, 0);
+      }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pl[(16 * mo + 4 * q + r) * S + j0 + c] = acc[r];
+  }
+  __syncthreads();
+  // out = C Yi^T
+#pragma unroll
+  for (int mt = 0; mt < YZ_MT; ++mt) acc1[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int st = 0; st < 32; ++st) {
+    if (st >= nys) continue;
+    const float* ar = pl + c * S + 4 * st + q;
+    float a4[YZ_MT];
+#pragma unroll
+    for (int mt = 0; mt < YZ_MT; ++mt) a4[mt] = ar[16 * mt * S];
+#pragma unroll
+    for (int mt = 0; mt < YZ_MT; ++mt)
+      if (mt < MT)
+        acc1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[mt], yb[st], acc1[mt], 0, 0, 0);
+  }
+  const int j = j0 + c;
+  if (j >= Ny) return;
+#pragma unroll
+  for (int mt = 0; mt < YZ_MT; ++mt)
+    if (mt < MT) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int k = 16 * mt + 4 * q + r;
+        if (k < Nz) at(out, (base + (uint32_t)k * Ny + j) * 4u) = acc1[mt][r];
+      }
+    }
+}
+
 
 
 // Orthonormal DCT-IV of each row (the x eigenbasis of the Neumann-left /
@@ -1163,6 +1333,212 @@ k_dct4_rows(int nrows, int N, int logM, const T* __restrict__ tw,
   }
 }
 
+// The same DCT-IV with the preconditioner's work buffer in the transposed
+// layout Wt[i][row] (row = k Ny + j, leading dimension ldt), the layout
+// k_yz_plane reads one x frequency i at a time as a contiguous block.
+// Float32 only.  A workgroup transforms DT_ROWS consecutive rows (four per
+// wavefront) and moves them between the row layout and Wt through an LDS
+// tile, so every global access covers DT_ROWS consecutive rows of one
+// frequency (64-byte runs) instead of one element per cache line.
+//   TIN = false: rows (in, stride N) -> Wt (out); modes DCT_PLAIN, DCT_UPDATE.
+//   TIN = true:  Wt (in) -> rows (out, stride N); modes DCT_PLAIN, DCT_DOT.
+// The fused work (DCT_UPDATE / DCT_DOT) is that of k_dct4_rows, one partial
+// per workgroup.
+// 512 threads, two rows per wavefront; N <= DT_NMAX, so the tile moves in at
+// most DT_NMAX / 32 elements per thread, all issued before any is used.
+#define DT_ROWS 16
+#define DT_THREADS 512
+#define DT_RPW (DT_ROWS / (DT_THREADS / 64))
+#define DT_NMAX 512
+template <int MODE, bool TIN>
+__global__ void __launch_bounds__(DT_THREADS)
+k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
+            const float* __restrict__ in, float* __restrict__ out, int ldt,
+            DctFuse<float> f) {
+  extern __shared__ float sm[];
+  __shared__ double shr[DT_THREADS / 64];
+  const int M = N >> 1, LX = N + 1;       // LX: conflict-free tile row
+  float* A = sm;                          // DT_ROWS x N: the complex FFT rows
+  float* X = sm + DT_ROWS * N;            // DT_ROWS x LX: the transposed tile
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  constexpr int NE = DT_NMAX * DT_ROWS / DT_THREADS;   // tile elements per thread
+  const int row0 = blockIdx.x * DT_ROWS;
+  const float* pre = tw;
+  const float* post = tw + 2 * M;
+  const float* W = tw + 4 * M;
+  double acc = 0.0;
+  float alpha = 0.0f;
+  if constexpr (MODE == DCT_UPDATE) {
+    if (f.S[S_DONE] != 0.0) return;   // uniform: nothing moves once converged
+    const double rz = sum_partials(f.Prz, f.npz, shr);
+    __syncthreads();
+    const double pq = sum_partials(f.Ppq, f.npq, shr);
+    __syncthreads();
+    const double ad = rz / pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      f.S[S_RZ] = rz;
+      f.S[S_PQ] = pq;
+      f.S[S_ALPHA] = ad;
+    }
+    alpha = (float)ad;
+  }
+  if constexpr (MODE == DCT_DOT) {
+    if (f.check && blockIdx.x == 0) {
+      const double rr = sum_partials(f.Prr, f.nrr, shr);
+      if (threadIdx.x == 0 && f.S[S_DONE] == 0.0) {
+        f.S[S_RR] = rr;
+        f.S[S_IT] += 1.0;
+        if (!(rr == rr)) f.S[S_DONE] = 2.0;
+        else if (sqrt(rr) <= f.S[S_TOL]) f.S[S_DONE] = 1.0;
+      }
+      __syncthreads();
+    }
+  }
+  if constexpr (TIN) {
+    // the tile's columns of Wt: consecutive threads take consecutive rows
+    float t[NE];
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = threadIdx.x + u * DT_THREADS;
+      const int i = e / DT_ROWS, row = row0 + (e - i * DT_ROWS);
+      t[u] = i < N && row < nrows ? in[(size_t)i * ldt + row] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = threadIdx.x + u * DT_THREADS;
+      const int i = e / DT_ROWS, rl = e - i * DT_ROWS;
+      if (i < N) X[rl * LX + i] = t[u];
+    }
+    __syncthreads();
+  }
+  // The wavefront's rows rl = wv DT_RPW + rr: every operand of the row
+  // set-up first (complex point n = lane + 64 u, entries 2n and N-1-2n),
+  // then the arithmetic.
+  constexpr int NI = DT_NMAX / 128;
+  float va[DT_RPW][NI], vb[DT_RPW][NI], qa[DT_RPW][NI], qb[DT_RPW][NI];
+  float pa[DT_RPW][NI], pb[DT_RPW][NI], da[DT_RPW][NI], db[DT_RPW][NI];
+#pragma unroll
+  for (int rr = 0; rr < DT_RPW; ++rr) {
+    const int rl = wv * DT_RPW + rr, row = row0 + rl;
+    const size_t rb = (size_t)row * N;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int n = lane + 64 * u;
+      if (row < nrows && n < M) {
+        const size_t e0 = rb + 2 * n, e1 = rb + N - 1 - 2 * n;
+        if constexpr (MODE == DCT_UPDATE) {
+          va[rr][u] = f.r[e0]; vb[rr][u] = f.r[e1];
+          qa[rr][u] = f.q[e0]; qb[rr][u] = f.q[e1];
+          pa[rr][u] = f.p[e0]; pb[rr][u] = f.p[e1];
+          da[rr][u] = f.d[e0]; db[rr][u] = f.d[e1];
+        } else if constexpr (TIN) {
+          va[rr][u] = X[rl * LX + 2 * n];
+          vb[rr][u] = X[rl * LX + N - 1 - 2 * n];
+          if constexpr (MODE == DCT_DOT) {   // r for the dot of the output stage
+            qa[rr][u] = f.r[e0];
+            qb[rr][u] = f.r[e1];
+          }
+        } else {
+          va[rr][u] = in[e0];
+          vb[rr][u] = in[e1];
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < DT_RPW; ++rr) {
+    const int rl = wv * DT_RPW + rr, row = row0 + rl;
+    const size_t rb = (size_t)row * N;
+    float* a = A + rl * N;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int n = lane + 64 * u;
+      if (row < nrows && n < M) {
+        float re = va[rr][u], im = vb[rr][u];
+        if constexpr (MODE == DCT_UPDATE) {
+          const size_t e0 = rb + 2 * n, e1 = rb + N - 1 - 2 * n;
+          f.d[e0] = da[rr][u] + alpha * pa[rr][u];
+          f.d[e1] = db[rr][u] + alpha * pb[rr][u];
+          re = re - alpha * qa[rr][u];
+          im = im - alpha * qb[rr][u];
+          f.r[e0] = re;
+          f.r[e1] = im;
+          acc += (double)re * (double)re + (double)im * (double)im;
+        }
+        const float pr = pre[2 * n], pi = pre[2 * n + 1];
+        const int r = (int)(__brev((unsigned)n) >> (32 - logM));
+        a[2 * r] = re * pr - im * pi;
+        a[2 * r + 1] = re * pi + im * pr;
+      }
+    }
+  }
+  __syncthreads();
+  for (int st = 0; st < logM; ++st) {
+    const int half = 1 << st;
+    for (int rr = 0; rr < DT_RPW; ++rr) {
+      const int rl = wv * DT_RPW + rr;
+      if (row0 + rl >= nrows) break;
+      float* a = A + rl * N;
+      for (int b = lane; b < (M >> 1); b += 64) {
+        const int pos = b & (half - 1);
+        const int i1 = ((b >> st) << (st + 1)) + pos;
+        const int i2 = i1 + half;
+        const int t = pos << (logM - 1 - st);
+        const float wr = W[2 * t], wi = W[2 * t + 1];
+        const float xr = a[2 * i2], xi = a[2 * i2 + 1];
+        const float tr = wr * xr - wi * xi, ti = wr * xi + wi * xr;
+        const float ur = a[2 * i1], ui = a[2 * i1 + 1];
+        a[2 * i1] = ur + tr;
+        a[2 * i1 + 1] = ui + ti;
+        a[2 * i2] = ur - tr;
+        a[2 * i2 + 1] = ui - ti;
+      }
+    }
+    __syncthreads();
+  }
+  const float sc = (float)sqrt(2.0 / N);
+#pragma unroll
+  for (int rr = 0; rr < DT_RPW; ++rr) {
+    const int rl = wv * DT_RPW + rr, row = row0 + rl;
+    if (row >= nrows) break;
+    const size_t rb = (size_t)row * N;
+    const float* a = A + rl * N;
+    const double dzk = MODE == DCT_DOT ? f.dz[row / f.ny] : 0.0;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int kk = lane + 64 * u;
+      if (kk >= M) break;
+      const float vr = a[2 * kk], vi = a[2 * kk + 1];
+      const float pr = post[2 * kk], pi = post[2 * kk + 1];
+      const float y0 = (vr * pr - vi * pi) * sc;
+      const float y1 = -((vr * pi + vi * pr) * sc);
+      if constexpr (TIN) {
+        out[rb + 2 * kk] = y0;
+        out[rb + N - 1 - 2 * kk] = y1;
+      } else {
+        X[rl * LX + 2 * kk] = y0;
+        X[rl * LX + N - 1 - 2 * kk] = y1;
+      }
+      if constexpr (MODE == DCT_DOT)
+        acc += dzk * ((double)qa[rr][u] * (double)y0 + (double)qb[rr][u] * (double)y1);
+    }
+  }
+  if constexpr (!TIN) {
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = threadIdx.x + u * DT_THREADS;
+      const int i = e / DT_ROWS, rl = e - i * DT_ROWS, row = row0 + rl;
+      if (i < N && row < nrows) out[(size_t)i * ldt + row] = X[rl * LX + i];
+    }
+  }
+  if constexpr (MODE != DCT_PLAIN) {
+    __syncthreads();
+    const double t = block_sum_wg(acc, shr);
+    if (threadIdx.x == 0) f.Pout[blockIdx.x] = t;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // handle
 // ---------------------------------------------------------------------------
@@ -1180,6 +1556,9 @@ struct ProjHandle {
   // float32: the y and z matrices in k_coltrans_mfma's A-fragment order,
   // zero-padded to col_pad(K) (null for float64 or K > 128: k_gemm then)
   void *Yf_p, *Yi_p, *Zf_p, *Zi_p;
+  // float32, 16 <= Ny <= 128, Nz <= 64: Zf, Zi in k_yz_plane's permuted
+  // A-fragment order (null otherwise: the column transforms then)
+  void *ZfQ, *ZiQ;
   // x transform through k_dct4_rows when Ux is the orthonormal DCT-IV basis
   // and Nx is a power of two (twx: its twiddle table, type T)
   bool fft_x;
@@ -1207,6 +1586,8 @@ struct ProjHandle {
   int last_iters, last_outer;
   double last_rel, last_div_max;
   int precond_enabled;
+  int use_plane;       // k_yz_plane when available (else the column transforms)
+  int mv_blocks;       // workgroups of the fused PCG operator (k_matvec_pz)
 };
 
 template <typename T>
@@ -1225,11 +1606,39 @@ static void launch_gemm(ProjHandle* h, const T* A, const T* B, T* C, int M,
   hipLaunchKernelGGL(k_gemm<T>, grid, dim3(256), 0, s, a, A, B, C);
 }
 
-// x transform of every row: DCT-IV by FFT when available, else the GEMM
-// with B = Ux (forward; Ux is symmetric for the DCT-IV basis) or Ux^T.
+// The preconditioner's work buffers w1, w2 hold the x-transformed field in
+// the transposed layout Wt[i][row] (leading dimension ldt) when k_yz_plane
+// runs, in the row layout otherwise.
+static bool work_t(const ProjHandle* h) { return h->ZfQ && h->use_plane; }
+static int work_ldt(const ProjHandle* h) { return (h->g.nz * h->g.ny + 31) & ~31; }
+static size_t dct_tile_shm(int N) { return (size_t)DT_ROWS * (2 * N + 1) * sizeof(float); }
+
+// x transform of every row (r -> w1 forward, w1/w2 -> z inverse): DCT-IV by
+// FFT when available, else the GEMM with B = Ux (forward; Ux is symmetric
+// for the DCT-IV basis) or Ux^T.
 template <typename T>
 static void x_transform(ProjHandle* h, const T* in, T* out, bool inverse, hipStream_t s) {
   const int Nx = h->g.nx, rows = h->g.nz * h->g.ny;
+  if (work_t(h)) {
+    if constexpr (sizeof(T) == 4) {
+      const int ldt = work_ldt(h);
+      const dim3 grid((rows + DT_ROWS - 1) / DT_ROWS);
+      const size_t shm = dct_tile_shm(Nx);
+      if (h->fft_x && inverse)
+        hipLaunchKernelGGL((k_dct4_tile<DCT_PLAIN, true>), grid, dim3(DT_THREADS), shm, s, rows, Nx,
+                           h->logMx, (const float*)h->twx, in, out, ldt, DctFuse<float>{});
+      else if (h->fft_x)
+        hipLaunchKernelGGL((k_dct4_tile<DCT_PLAIN, false>), grid, dim3(DT_THREADS), shm, s, rows, Nx,
+                           h->logMx, (const float*)h->twx, in, out, ldt, DctFuse<float>{});
+      else if (!inverse)   // C[row][i] at i ldt + row
+        launch_gemm<T>(h, in, (const T*)h->Ux, out, rows, Nx, Nx, 1, 0, Nx, 1, 0, Nx, 1, 0, 1,
+                       ldt, false, s);
+      else                 // A[row][i] at i ldt + row
+        launch_gemm<T>(h, in, (const T*)h->UxT, out, rows, Nx, Nx, 1, 0, 1, ldt, 0, Nx, 1, 0,
+                       Nx, 1, false, s);
+    }
+    return;
+  }
   if (h->fft_x) {
     const size_t shm = 4 * (size_t)Nx * sizeof(T);
     hipLaunchKernelGGL((k_dct4_rows<T, DCT_PLAIN>), dim3((rows + 3) / 4), dim3(256), shm, s,
@@ -1280,6 +1689,24 @@ static void launch_zpair(ProjHandle* h, const float* in, float* out, hipStream_t
   }
 }
 
+// k_yz_plane over every x frequency (float32; ZfQ set; Wt layout).
+static void launch_yz_plane(ProjHandle* h, const float* in, float* out, hipStream_t s) {
+  const int Nx = h->g.nx, Ny = h->g.ny, Nz = h->g.nz;
+  const int Nyp = (Ny + 15) & ~15, MT = (Nz + 15) / 16;
+  const size_t shm = (size_t)16 * YZ_MT * (Nyp + 4) * sizeof(float);   // 64 rows
+  using K = void (*)(int, int, int, int, const float*, const float*, const float*,
+                     const float*, const double*, const double*, const double*, double,
+                     const float*, float*);
+#define YZ_ROW(m) {k_yz_plane<m, 4>, k_yz_plane<m, 8>, k_yz_plane<m, 12>, k_yz_plane<m, 16>, \
+                   k_yz_plane<m, 20>, k_yz_plane<m, 24>, k_yz_plane<m, 28>, k_yz_plane<m, 32>}
+  static const K table[4][8] = {YZ_ROW(1), YZ_ROW(2), YZ_ROW(3), YZ_ROW(4)};
+#undef YZ_ROW
+  hipLaunchKernelGGL(table[MT - 1][Nyp / 16 - 1], dim3(Nx), dim3(64 * (Nyp / 16)), shm, s,
+                     work_ldt(h), Ny, Nz, col_pad(Ny) / 4, (const float*)h->Yf_p,
+                     (const float*)h->Yi_p, (const float*)h->ZfQ, (const float*)h->ZiQ, h->lx,
+                     h->ly, h->lz, h->eps_reg, in, out);
+}
+
 // The y and z parts of the separable solve on the x-transformed field in w1
 // (w2 is scratch): y forward, z forward with the spectral divide, z
 // inverse, y inverse.  Returns the buffer (w1 or w2) holding the result.
@@ -1290,6 +1717,10 @@ static const T* yz_transforms(ProjHandle* h, T* w1, T* w2, hipStream_t s) {
   const bool mf = sizeof(T) == 4;
   T* cur = w1;
   T* other = w2;
+  if (mf && h->ZfQ && h->use_plane) {   // all four products in one pass
+    launch_yz_plane(h, (const float*)cur, (float*)other, s);
+    return other;
+  }
   if (Ny > 1) {  // y forward per k-plane: w2_k = Yf (Ny x Ny) * w1_k (Ny x Nx)
     if (mf && h->Yf_p)
       launch_col(h, h->Yf_p, Ny, Nx, Nx, Nz, NP, false, (const float*)cur, (float*)other, s);
@@ -1407,13 +1838,39 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
   // rides in the forward row transform and <r, z> in the inverse one
   const bool fused = h->fft_x && h->precond_enabled == FW_PRECOND_SEPARABLE;
   const bool flex = h->precond_enabled == FW_PRECOND_LEARNED;
-  const int rows = h->g.nz * h->g.ny, ndct = (rows + 3) / 4;
-  const size_t shm = 4 * (size_t)h->g.nx * sizeof(T);
+  // the row transforms in tiles (transposed work layout for k_yz_plane) or
+  // four rows per workgroup; one partial per workgroup either way
+  const bool tile = sizeof(T) == 4 && work_t(h);
+  const int rows = h->g.nz * h->g.ny;
+  const int ndct = tile ? (rows + DT_ROWS - 1) / DT_ROWS : (rows + 3) / 4;
+  const size_t shm = tile ? dct_tile_shm(h->g.nx) : 4 * (size_t)h->g.nx * sizeof(T);
+  const int ldt = work_ldt(h);
   const int npz = fused ? ndct : PJ_RED;
   DctFuse<T> fu{};
   fu.d = d; fu.r = r; fu.p = p; fu.q = q;
   fu.Prz = h->Prz; fu.Ppq = h->Ppq; fu.Prr = h->Prr; fu.dz = h->g.dz;
-  fu.npz = ndct; fu.npq = PJ_RED; fu.nrr = ndct; fu.ny = h->g.ny; fu.S = h->S;
+  // workgroups (= <p, q>_W partials) of the operator launch: the handle's
+  // tiling on the fused path, PJ_RED where k_update_xr2 sums them
+  const int mvb = fused ? h->mv_blocks : PJ_RED;
+  fu.npz = ndct; fu.npq = mvb; fu.nrr = ndct; fu.ny = h->g.ny; fu.S = h->S;
+  // the fused row transforms: MODE on rows of `in` into w1 (forward) or on
+  // the work buffer `in` into z (DCT_DOT)
+  auto row_transform = [&](auto mode, const T* in, T* out, const DctFuse<T>& f) {
+    constexpr int MODE = decltype(mode)::value;
+    if constexpr (sizeof(T) == 4) {
+      if (tile) {
+        hipLaunchKernelGGL((k_dct4_tile<MODE, MODE == DCT_DOT>), dim3(ndct), dim3(DT_THREADS), shm, s,
+                           rows, h->g.nx, h->logMx, (const float*)h->twx, (const float*)in,
+                           (float*)out, ldt, f);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_dct4_rows<T, MODE>), dim3(ndct), dim3(256), shm, s, rows, h->g.nx,
+                       h->logMx, (const T*)h->twx, in, out, f);
+  };
+  using plain_t = std::integral_constant<int, DCT_PLAIN>;
+  using update_t = std::integral_constant<int, DCT_UPDATE>;
+  using dot_t = std::integral_constant<int, DCT_DOT>;
   // z = M^-1 r; with fused, the inverse row transform also leaves <r, z>_W
   auto precond_dot = [&](int check) -> int {
     T* w1 = (T*)h->w1;
@@ -1422,15 +1879,12 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
     DctFuse<T> fd = fu;
     fd.check = check;
     fd.Pout = h->Prz;
-    hipLaunchKernelGGL((k_dct4_rows<T, DCT_DOT>), dim3(ndct), dim3(256), shm, s, rows,
-                       h->g.nx, h->logMx, (const T*)h->twx, c2, z, fd);
+    row_transform(dot_t{}, c2, z, fd);
     FW_CHECK_LAUNCH("precond row transform");
     return FW_OK;
   };
   if (fused) {
-    DctFuse<T> f0{};
-    hipLaunchKernelGGL((k_dct4_rows<T, DCT_PLAIN>), dim3(ndct), dim3(256), shm, s, rows,
-                       h->g.nx, h->logMx, (const T*)h->twx, (const T*)r, (T*)h->w1, f0);
+    row_transform(plain_t{}, (const T*)r, (T*)h->w1, DctFuse<T>{});
     if ((rc = precond_dot(0))) return rc;
   } else {
     if ((rc = apply_M<T>(h, r, z, s))) return rc;
@@ -1454,22 +1908,20 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
     for (int b = 0; b < nb_it; ++b) {
       if (fused && queued + b > 0) {
         // the previous iteration's direction update rides in this operator
-        hipLaunchKernelGGL(k_matvec_pz<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, ir, p, z,
+        hipLaunchKernelGGL(k_matvec_pz<T>, dim3(mvb), dim3(256), 0, s, h->ga, ir, p, z,
                            p_alt, q, (T)h->eps_reg, h->Prz, npz, h->S, h->Ppq);
         T* t = p;
         p = p_alt;
         p_alt = t;
       } else {
-        hipLaunchKernelGGL(k_matvec_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, ir, p, q,
+        hipLaunchKernelGGL(k_matvec_rows<T>, dim3(mvb), dim3(256), 0, s, h->ga, ir, p, q,
                            (T)h->eps_reg, h->Ppq);
       }
       if (fused) {
         DctFuse<T> fw = fu;
         fw.p = p;
         fw.Pout = h->Prr;
-        hipLaunchKernelGGL((k_dct4_rows<T, DCT_UPDATE>), dim3(ndct), dim3(256), shm, s,
-                           rows, h->g.nx, h->logMx, (const T*)h->twx, (const T*)r,
-                           (T*)h->w1, fw);
+        row_transform(update_t{}, (const T*)r, (T*)h->w1, fw);
         if ((rc = precond_dot(1))) return rc;
       } else {
         hipLaunchKernelGGL(k_update_xr2<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
@@ -1691,6 +2143,8 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   h->precond_enabled = FW_PRECOND_SEPARABLE;
   h->learned_budget = 60;
   h->learned_every = 4;
+  h->use_plane = 1;
+  h->mv_blocks = PJ_MV_DEFAULT;
   const int Nx = g->nx, Ny = g->ny, Nz = g->nz;
   // host copies of the dz metric for folding sqrt(dz) into the z matrices
   std::vector<double> dz(Nz);
@@ -1783,6 +2237,27 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
         ((rc = pad(&h->Yf_p, Yf.data(), Ny)) || (rc = pad(&h->Yi_p, Uy, Ny)) ||
          (rc = pad(&h->Zf_p, Zf.data(), Nz)) || (rc = pad(&h->Zi_p, Zi.data(), Nz))))
       return rc;
+    // k_yz_plane's z fragments: lane l of row tile mo at step 4 mt + r holds
+    // Mat[16 mo + l%16][16 mt + 4 (l/16) + r] (zero past Nz)
+    // (always 4 row tiles x 16 steps, zero past Nz: the kernel's loads are
+    // unconditional)
+    auto quad = [&](void** dst, const double* src) -> int {
+      std::vector<double> m((size_t)4 * 64 * 16, 0.0);
+      for (int mo = 0; mo < 4; ++mo)
+        for (int l = 0; l < 64; ++l)
+          for (int mt = 0; mt < 4; ++mt)
+            for (int r = 0; r < 4; ++r) {
+              const int row = 16 * mo + (l & 15), col = 16 * mt + 4 * (l >> 4) + r;
+              if (row < Nz && col < Nz)
+                m[((size_t)mo * 64 + l) * 16 + 4 * mt + r] = src[(size_t)row * Nz + col];
+            }
+      return upload_cast<float>(dst, m.data(), m.size());
+    };
+    // (with the FFT x transform, its tile kernel takes Nx <= DT_NMAX)
+    if (g->dtype == FW_F32 && Ny >= 16 && Ny <= 128 && Nz <= 64 &&
+        (!h->fft_x || Nx <= DT_NMAX) &&
+        ((rc = quad(&h->ZfQ, Zf.data())) || (rc = quad(&h->ZiQ, Zi.data()))))
+      return rc;
   }
   FW_CHECK_HIP(hipMalloc((void**)&h->lx, Nx * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->ly, Ny * sizeof(double)));
@@ -1793,12 +2268,26 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   const size_t n = h->n;
   FW_CHECK_HIP(hipMalloc((void**)&h->x64, n * 8));
   FW_CHECK_HIP(hipMalloc((void**)&h->b64, n * 8));
-  void** vecs[] = {&h->r, &h->z, &h->p, &h->q, &h->d, &h->w1, &h->w2, &h->p2};
+  void** vecs[] = {&h->r, &h->z, &h->p, &h->q, &h->d, &h->p2};
   for (void** v : vecs) FW_CHECK_HIP(hipMalloc(v, n * h->esz));
+  // the work buffers hold the transposed layout (ldt >= Nz Ny per x
+  // frequency) when k_yz_plane can run
+  const size_t nw = h->ZfQ ? (size_t)work_ldt(h) * Nx : n;
+  if (h->ZfQ && dct_tile_shm(Nx) > 65536) {
+    const void* tk[] = {(const void*)k_dct4_tile<DCT_PLAIN, false>,
+                        (const void*)k_dct4_tile<DCT_PLAIN, true>,
+                        (const void*)k_dct4_tile<DCT_UPDATE, false>,
+                        (const void*)k_dct4_tile<DCT_DOT, true>};
+    for (const void* k : tk)
+      FW_CHECK_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)dct_tile_shm(Nx)));
+  }
+  FW_CHECK_HIP(hipMalloc(&h->w1, nw * h->esz));
+  FW_CHECK_HIP(hipMalloc(&h->w2, nw * h->esz));
   FW_CHECK_HIP(hipMalloc((void**)&h->part, PJ_RED * sizeof(double)));
   {
     const int ndct = (g->nz * g->ny + 3) / 4;   // row-transform workgroups
-    const size_t np = ndct > PJ_RED ? ndct : PJ_RED;
+    const size_t np = ndct > PJ_MV_MAX ? ndct : PJ_MV_MAX;
     FW_CHECK_HIP(hipMalloc((void**)&h->Ppq, 3 * np * sizeof(double)));
     h->Prr = h->Ppq + np;
     h->Prz = h->Ppq + 2 * np;
@@ -1818,7 +2307,7 @@ int fw_proj_destroy(void* handle) {
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
                   h->x64, h->b64, h->r, h->z, h->p, h->p2, h->q, h->d, h->w1,
                   h->w2, h->part, h->S, h->twx, h->Ppq, h->irho, h->Yf_p, h->Yi_p,
-                  h->Zf_p, h->Zi_p, h->Pqz, h->rs, h->qf};
+                  h->Zf_p, h->Zi_p, h->ZfQ, h->ZiQ, h->Pqz, h->rs, h->qf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (h->S_host) (void)hipHostFree(h->S_host);
@@ -1947,6 +2436,17 @@ int fw_proj_precondition(void* handle, const void* r, void* z, void* stream) {
   hipStream_t s = (hipStream_t)stream;
   return h->dtype == FW_F64 ? precond_apply<double>(h, (const double*)r, (double*)z, s)
                             : precond_apply<float>(h, (const float*)r, (float*)z, s);
+}
+
+int fw_proj_set_tiling(void* handle, int yz_plane, int matvec_blocks, int* yz_active) {
+  ProjHandle* h = (ProjHandle*)handle;
+  FW_REQUIRE(h, "proj_set_tiling: null handle");
+  FW_REQUIRE(matvec_blocks == 0 || (matvec_blocks >= 64 && matvec_blocks <= PJ_MV_MAX),
+             "proj_set_tiling: matvec_blocks must be 0 (default) or 64..%d", PJ_MV_MAX);
+  h->use_plane = yz_plane ? 1 : 0;
+  h->mv_blocks = matvec_blocks ? matvec_blocks : PJ_MV_DEFAULT;
+  if (yz_active) *yz_active = h->use_plane && h->ZfQ ? 1 : 0;
+  return FW_OK;
 }
 
 int fw_proj_set_preconditioner(void* handle, int enabled) {

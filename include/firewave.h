@@ -239,6 +239,18 @@ int fw_proj_matvec(void* handle, const void* rho, const void* x, void* y,
 /* z = SeparableLaplacian3D.solve(r) (fft_poisson_3d.py:184-228). */
 int fw_proj_precondition(void* handle, const void* r, void* z, void* stream);
 
+/* Launch shapes of the inner PCG (no counterpart in the reference, whose
+ * solve is NumPy/SciPy calls); results are identical up to float32
+ * rounding of the preconditioner.  yz_plane = 1 (the default) runs the
+ * preconditioner's y and z transforms, all four products per x frequency,
+ * in one pass (k_yz_plane: float32, 16 <= Ny <= 128, Nz <= 64), 0 the
+ * per-direction column transforms; *yz_active (may be NULL) says whether
+ * the one-pass kernel runs.  matvec_blocks: workgroups of the fused
+ * operator launch (64..4096; 0 = 1024).  The autotuner
+ * (firewebapp_amd/autotune.py) picks both per grid. */
+#define FW_PROJ_MV_MAX 4096
+int fw_proj_set_tiling(void* handle, int yz_plane, int matvec_blocks, int* yz_active);
+
 /* Preconditioner of the PCG: FW_PRECOND_SEPARABLE (default; the reference's
  * SeparableLaplacian3D, fft_poisson_3d.py:184-228), FW_PRECOND_NONE, or
  * FW_PRECOND_JACOBI (diag(A)^-1; not in the reference, a comparison row for

@@ -184,6 +184,79 @@ def test_separable_fp32_matrix_cores(shape):
     assert e1 <= 1e-4 and e2 <= 1e-4
 
 
+@pytest.mark.parametrize("shape", [(48, 64, 256), (40, 32, 64), (32, 32, 128),
+                                   (13, 24, 37), (64, 128, 512), (20, 16, 192)],
+                         ids=["256x64x48", "64x32x40", "128x32x32", "ragged",
+                              "512x128x64", "192x16x20"])
+def test_yz_plane_matches_column_transforms(shape):
+    """The one-pass y-z kernel (k_yz_plane, per x frequency) against the
+    per-direction column transforms and the float64 separable inverse:
+    production shapes, the sweep's Nz = 40 (row padding), Ny = 24 (column
+    padding), Nx = 37 / 192 (the dense x transform)."""
+    import ctypes as C
+    import torch
+    from firewebapp_amd import _lib
+    from firewebapp_amd.device import Marshal, ptr, stream_ptr
+    fp, _ = _mods()
+    Nz, Ny, Nx = shape
+    dz, da, db = _grid_z(Nz)
+    s = fp.SeparableLaplacian3D(Nz, Ny, Nx, 0.1, 0.1, dz, da, db, 1.0e-6)
+    r = np.random.default_rng(9).standard_normal(shape)
+    R = torch.as_tensor(r.astype(np.float32), device="cuda")
+    h = s._handle(Marshal(R))
+    out = {}
+    for on in (1, 0):
+        act = C.c_int()
+        _lib.call("fw_proj_set_tiling", h, on, 0, C.byref(act))
+        assert act.value == on, (on, act.value)
+        Z = torch.empty_like(R)
+        _lib.call("fw_proj_precondition", h, ptr(R), ptr(Z), stream_ptr())
+        out[on] = Z.double().cpu().numpy()
+    p64 = s.solve(r)
+    e_col, e_plane = relerr(out[0], p64), relerr(out[1], p64)
+    e_pair = relerr(out[1], out[0])
+    print(f"{shape}: plane vs fp64 {e_plane:.2e}, columns vs fp64 {e_col:.2e}, "
+          f"plane vs columns {e_pair:.2e}", flush=True)
+    assert e_plane <= 1e-4 and e_pair <= 1e-4
+    assert e_plane <= 2.0 * e_col + 1e-6
+
+
+def test_tilings_reach_the_same_residual():
+    """Every launch shape of fw_proj_set_tiling solves the same float32
+    system to the float64 1e-8 residual in the same number of iterations
+    (within one: the preconditioner differs by float32 rounding only), at a
+    shape where the one-pass y-z kernel runs (Ny = 16)."""
+    import ctypes as C
+    import torch
+    from firewebapp_amd import _lib
+    from firewebapp_amd.device import ptr, stream_ptr
+    _, pj = _mods()
+    dz, da, db, rho, *_ = _fire_case(Nz=24, Ny=16, Nx=64, seed=13)
+    sh = rho.shape
+    sol = pj.ProjectionSolver3D(*sh, 0.1, 0.1, dz, da, db, cg_rtol=1e-6)
+    h = sol._handle(torch.float32)
+    b = np.random.default_rng(2).standard_normal(sh)
+    B = torch.as_tensor(b, device="cuda")
+    R = torch.as_tensor(rho.astype(np.float32), device="cuda")
+    coef = OP.coefficients(rho.astype(np.float32).astype(np.float64), 0.1, 0.1, dz, da, db)
+    its = {}
+    for plane, mvb in ((0, 0), (1, 0), (1, 64), (1, 333), (1, 4096)):
+        act = C.c_int()
+        _lib.call("fw_proj_set_tiling", h, plane, mvb, C.byref(act))
+        assert act.value == plane
+        X = torch.zeros_like(B)
+        it, rel = C.c_int(), C.c_double()
+        _lib.call("fw_proj_solve", h, ptr(R), ptr(B), ptr(X), 1e-8, C.byref(it),
+                  C.byref(rel), stream_ptr())
+        true_rel = np.linalg.norm(b - OP.matvec(coef, X.cpu().numpy())) / np.linalg.norm(b)
+        assert true_rel <= 1.05e-8, (plane, mvb, true_rel)
+        its[(plane, mvb)] = it.value
+    print("iterations per tiling", its, flush=True)
+    assert max(its.values()) - min(its.values()) <= 1
+    with pytest.raises(_lib.FirewaveError):
+        _lib.call("fw_proj_set_tiling", h, 1, 5000, None)
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_rhs_accessor_and_preconditioner_modes(dtype):
     """fw_proj_rhs returns the system of the last projection, (div - target)/dt
