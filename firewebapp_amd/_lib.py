@@ -26,6 +26,7 @@ FW_E_HIP = -5
 FW_F32 = 0
 FW_F64 = 1
 FW_REDUCE_BLOCKS = 256
+FW_STEP_SCALARS = 8       # fw_spread_fields.step_scalars doubles
 
 
 class FirewaveNotBuilt(RuntimeError):

@@ -47,7 +47,9 @@ k_edc(size_t n, const T* __restrict__ rho, T* __restrict__ Tg,
       T* __restrict__ Yf_io, T* __restrict__ YO2_io,
       const T* __restrict__ kturb, const T* __restrict__ eturb,
       const T* __restrict__ Yw_in, T* __restrict__ omega_out,
-      const int* __restrict__ summary, EdcParams P) {
+      const int* __restrict__ summary, EdcParams P, const double* __restrict__ sb) {
+  P.dt = sb_get(sb, SB_DT_SUB, P.dt);
+  P.relax = sb_get(sb, SB_EDC_RELAX, P.relax);
   __shared__ int wmax[4];
   __shared__ unsigned char own_sh[256];
   __shared__ int s_carry;
@@ -174,7 +176,7 @@ template <typename T>
 static int edc_impl(const fw_grid* g, const void* rho, void* Tg, void* Yf,
                     void* YO2, const void* kt, const void* et, const void* Yw,
                     void* omega, int* work, const EdcParams& P, bool summary_ready,
-                    hipStream_t s) {
+                    hipStream_t s, const double* sb) {
   const size_t n = (size_t)g->nx * g->ny * g->nz;
   const int nseg = (int)((n + EDC_SEG - 1) / EDC_SEG);
   int* summary = work;
@@ -185,20 +187,27 @@ static int edc_impl(const fw_grid* g, const void* rho, void* Tg, void* Yf,
   }
   hipLaunchKernelGGL(k_edc<T>, dim3(nseg), dim3(256), 0, s, n, (const T*)rho,
                      (T*)Tg, (T*)Yf, (T*)YO2, (const T*)kt, (const T*)et,
-                     (const T*)Yw, (T*)omega, (const int*)summary, P);
+                     (const T*)Yw, (T*)omega, (const int*)summary, P, sb);
   FW_CHECK_LAUNCH("edc");
   return FW_OK;
+}
+
+// The relaxation factor of the wet-bulb clamp for dt (formed on the host,
+// also for the step scalar block).
+double edc_relax(double dt, int n_substeps) {
+  const int nsub = n_substeps > 1 ? n_substeps : 1;
+  return 1.0 - exp(-(dt / nsub) / 0.5);
 }
 
 int edc_launch(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel, void* Y_O2,
                const void* k_turb, const void* eps_turb, const void* Y_H2O, void* omega_out,
                double chi_rad, double cp_g, double dt, int n_substeps, int extinction_enable,
-               double s_stoich, double hoc_J, int* work, bool summary_ready, hipStream_t s) {
-  const int nsub = n_substeps > 1 ? n_substeps : 1;
-  EdcParams P{chi_rad, cp_g, dt, s_stoich, hoc_J, 1.0 - exp(-(dt / nsub) / 0.5), n_substeps,
+               double s_stoich, double hoc_J, int* work, bool summary_ready, hipStream_t s,
+               const double* sb) {
+  EdcParams P{chi_rad, cp_g, dt, s_stoich, hoc_J, edc_relax(dt, n_substeps), n_substeps,
               extinction_enable ? 1 : 0};
   return FW_DISPATCH(g, edc_impl, g, rho, T_g, Y_fuel, Y_O2, k_turb, eps_turb, Y_H2O,
-                     omega_out, work, P, summary_ready, s);
+                     omega_out, work, P, summary_ready, s, sb);
 }
 
 }  // namespace fw

@@ -27,16 +27,47 @@ struct Launch {
 // driver: Q_comb from the array, or (Q_comb == nullptr) formed from the
 // drip-torch gas share and q_rad_gas / dz with the solid share subtracted
 // from Q_pyro in the source bed cells first.
+// Device-resident step scalars of the sub-step loop (a graph-captured loop
+// replays the same kernel arguments every step, so the values that change
+// from step to step are read from this block instead).  Each slot holds the
+// float64 value the host formed; a kernel given a null block uses its
+// by-value argument.
+enum { SB_DT_SUB = 0, SB_EDC_RELAX = 1, SB_DSC = 2, SB_DPR = 3, SB_SLOTS = 8 };
+__device__ __forceinline__ double sb_get(const double* sb, int slot, double v) {
+  return sb ? sb[slot] : v;
+}
+
 // The species turbulent-diffusion pass; with edc_summary (Nx % 256 == 0) it
 // also leaves the EDC segment summaries of its output in edc_summary.
+// sb (may be null): dt_sub from sb[SB_DSC].
 int turb3_launch(const fw_grid* g, const void* const* f, void* const* fo, const void* nu_t,
                  double sc_t, double dt_sub, const double* lo, const double* hi,
-                 int* edc_summary, hipStream_t s);
+                 int* edc_summary, hipStream_t s, const double* sb = nullptr);
 // fw_edc; summary_ready: the summaries of the current species are in work.
+// sb (may be null): dt from sb[SB_DT_SUB], the relaxation from sb[SB_EDC_RELAX].
 int edc_launch(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel, void* Y_O2,
                const void* k_turb, const void* eps_turb, const void* Y_H2O, void* omega_out,
                double chi_rad, double cp_g, double dt, int n_substeps, int extinction_enable,
-               double s_stoich, double hoc_J, int* work, bool summary_ready, hipStream_t s);
+               double s_stoich, double hoc_J, int* work, bool summary_ready, hipStream_t s,
+               const double* sb = nullptr);
+double edc_relax(double dt, int n_substeps);
+// fw_species_transport3; sb (may be null): dt from sb[SB_DT_SUB] (and no
+// kernel timer events).
+int species3_launch(const fw_grid* g, const void* const* Y, void* const* Yo, const void* rho,
+                    const void* u, const void* v, const void* w, const void* Sp,
+                    const void* Sd, double dt, double D, const double* y_in,
+                    const double* y_max, hipStream_t s, const double* sb);
+// fw_turbulent_diffusion_pass; sb (may be null): dt_sub from sb[SB_DPR].
+int turb_pass_launch(const fw_grid* g, const void* f, void* fo, const void* nu_t, double sc_t,
+                     double dt_sub, double lo, double hi, hipStream_t s, const double* sb);
+// fw_gas_update_solid_conduction; sb (may be null): dt_sub from sb[SB_DT_SUB].
+int gas_solid_launch(const fw_grid* g, void* T_g, const void* T_s, void* T_s_out,
+                     const void* alpha_s, const void* rho, const void* Y_H2O,
+                     const void* S_pyro_part, const void* S_dry_part, const void* Q_g_conv,
+                     const void* q_rad_gas, void* Q_comb_out, int n_z_bed, int i_src0,
+                     int i_src1, double drip_gas, double dt_sub, double T_amb,
+                     double k_solid, double rho_solid, double cp_solid, double clip_lo,
+                     double clip_hi, hipStream_t s, const double* sb);
 int coupling_launch(const fw_grid* g, void* T_g, void* T_s, const void* rho, const void* u,
                     const void* v, const void* w, const void* alpha_s, double sigma_sav,
                     const void* q_rad_in, void* Q_pyro, const void* Q_comb,

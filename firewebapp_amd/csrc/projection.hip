@@ -1344,17 +1344,16 @@ k_dct4_rows(int nrows, int N, int logM, const T* __restrict__ tw,
 //   TIN = true:  Wt (in) -> rows (out, stride N); modes DCT_PLAIN, DCT_DOT.
 // The fused work (DCT_UPDATE / DCT_DOT) is that of k_dct4_rows, one partial
 // per workgroup.
-// 512 threads, two rows per wavefront; N <= DT_NMAX, so the tile moves in at
-// most DT_NMAX / 32 elements per thread, all issued before any is used.
-#define DT_ROWS 16
-#define DT_THREADS 512
-#define DT_RPW (DT_ROWS / (DT_THREADS / 64))
+// DT_ROWS (8 or 16) rows per workgroup, two per wavefront (32 DT_ROWS
+// threads); N <= DT_NMAX, so the tile moves in at most DT_NMAX / 32 elements
+// per thread, all issued before any is used.
 #define DT_NMAX 512
-template <int MODE, bool TIN>
-__global__ void __launch_bounds__(DT_THREADS)
+template <int MODE, bool TIN, int DT_ROWS>
+__global__ void __launch_bounds__(32 * DT_ROWS)
 k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
             const float* __restrict__ in, float* __restrict__ out, int ldt,
             DctFuse<float> f) {
+  constexpr int DT_THREADS = 32 * DT_ROWS, DT_RPW = 2;
   extern __shared__ float sm[];
   __shared__ double shr[DT_THREADS / 64];
   const int M = N >> 1, LX = N + 1;       // LX: conflict-free tile row
@@ -1370,6 +1369,50 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
   float alpha = 0.0f;
   if constexpr (MODE == DCT_UPDATE) {
     if (f.S[S_DONE] != 0.0) return;   // uniform: nothing moves once converged
+  }
+  // Every global load of the workgroup is issued first, ahead of the
+  // partial sums (which hold __syncthreads) and of the LDS staging, so one
+  // memory latency covers the launch: the tile's columns of Wt (TIN:
+  // consecutive threads take consecutive rows), then the wavefront's rows
+  // rl = wv DT_RPW + rr (complex point n = lane + 64 u, entries 2n and
+  // N-1-2n).
+  float t[NE];
+  if constexpr (TIN) {
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      const int e = threadIdx.x + u * DT_THREADS;
+      const int i = e / DT_ROWS, row = row0 + (e - i * DT_ROWS);
+      t[u] = i < N && row < nrows ? in[(size_t)i * ldt + row] : 0.0f;
+    }
+  }
+  constexpr int NI = DT_NMAX / 128;
+  float va[DT_RPW][NI], vb[DT_RPW][NI], qa[DT_RPW][NI], qb[DT_RPW][NI];
+  float pa[DT_RPW][NI], pb[DT_RPW][NI], da[DT_RPW][NI], db[DT_RPW][NI];
+#pragma unroll
+  for (int rr = 0; rr < DT_RPW; ++rr) {
+    const int row = row0 + wv * DT_RPW + rr;
+    const size_t rb = (size_t)row * N;
+#pragma unroll
+    for (int u = 0; u < NI; ++u) {
+      const int n = lane + 64 * u;
+      if (row < nrows && n < M) {
+        const size_t e0 = rb + 2 * n, e1 = rb + N - 1 - 2 * n;
+        if constexpr (MODE == DCT_UPDATE) {
+          va[rr][u] = f.r[e0]; vb[rr][u] = f.r[e1];
+          qa[rr][u] = f.q[e0]; qb[rr][u] = f.q[e1];
+          pa[rr][u] = f.p[e0]; pb[rr][u] = f.p[e1];
+          da[rr][u] = f.d[e0]; db[rr][u] = f.d[e1];
+        } else if constexpr (MODE == DCT_DOT) {   // r for the dot of the output stage
+          qa[rr][u] = f.r[e0];
+          qb[rr][u] = f.r[e1];
+        } else if constexpr (!TIN) {
+          va[rr][u] = in[e0];
+          vb[rr][u] = in[e1];
+        }
+      }
+    }
+  }
+  if constexpr (MODE == DCT_UPDATE) {
     const double rz = sum_partials(f.Prz, f.npz, shr);
     __syncthreads();
     const double pq = sum_partials(f.Ppq, f.npq, shr);
@@ -1395,14 +1438,6 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
     }
   }
   if constexpr (TIN) {
-    // the tile's columns of Wt: consecutive threads take consecutive rows
-    float t[NE];
-#pragma unroll
-    for (int u = 0; u < NE; ++u) {
-      const int e = threadIdx.x + u * DT_THREADS;
-      const int i = e / DT_ROWS, row = row0 + (e - i * DT_ROWS);
-      t[u] = i < N && row < nrows ? in[(size_t)i * ldt + row] : 0.0f;
-    }
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
       const int e = threadIdx.x + u * DT_THREADS;
@@ -1410,37 +1445,15 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
       if (i < N) X[rl * LX + i] = t[u];
     }
     __syncthreads();
-  }
-  // The wavefront's rows rl = wv DT_RPW + rr: every operand of the row
-  // set-up first (complex point n = lane + 64 u, entries 2n and N-1-2n),
-  // then the arithmetic.
-  constexpr int NI = DT_NMAX / 128;
-  float va[DT_RPW][NI], vb[DT_RPW][NI], qa[DT_RPW][NI], qb[DT_RPW][NI];
-  float pa[DT_RPW][NI], pb[DT_RPW][NI], da[DT_RPW][NI], db[DT_RPW][NI];
 #pragma unroll
-  for (int rr = 0; rr < DT_RPW; ++rr) {
-    const int rl = wv * DT_RPW + rr, row = row0 + rl;
-    const size_t rb = (size_t)row * N;
+    for (int rr = 0; rr < DT_RPW; ++rr) {
+      const int rl = wv * DT_RPW + rr;
 #pragma unroll
-    for (int u = 0; u < NI; ++u) {
-      const int n = lane + 64 * u;
-      if (row < nrows && n < M) {
-        const size_t e0 = rb + 2 * n, e1 = rb + N - 1 - 2 * n;
-        if constexpr (MODE == DCT_UPDATE) {
-          va[rr][u] = f.r[e0]; vb[rr][u] = f.r[e1];
-          qa[rr][u] = f.q[e0]; qb[rr][u] = f.q[e1];
-          pa[rr][u] = f.p[e0]; pb[rr][u] = f.p[e1];
-          da[rr][u] = f.d[e0]; db[rr][u] = f.d[e1];
-        } else if constexpr (TIN) {
+      for (int u = 0; u < NI; ++u) {
+        const int n = lane + 64 * u;
+        if (row0 + rl < nrows && n < M) {
           va[rr][u] = X[rl * LX + 2 * n];
           vb[rr][u] = X[rl * LX + N - 1 - 2 * n];
-          if constexpr (MODE == DCT_DOT) {   // r for the dot of the output stage
-            qa[rr][u] = f.r[e0];
-            qb[rr][u] = f.r[e1];
-          }
-        } else {
-          va[rr][u] = in[e0];
-          vb[rr][u] = in[e1];
         }
       }
     }
@@ -1588,6 +1601,7 @@ struct ProjHandle {
   int precond_enabled;
   int use_plane;       // k_yz_plane when available (else the column transforms)
   int mv_blocks;       // workgroups of the fused PCG operator (k_matvec_pz)
+  int dct_rows;        // rows per k_dct4_tile workgroup (8 or 16)
 };
 
 template <typename T>
@@ -1611,7 +1625,26 @@ static void launch_gemm(ProjHandle* h, const T* A, const T* B, T* C, int M,
 // runs, in the row layout otherwise.
 static bool work_t(const ProjHandle* h) { return h->ZfQ && h->use_plane; }
 static int work_ldt(const ProjHandle* h) { return (h->g.nz * h->g.ny + 31) & ~31; }
-static size_t dct_tile_shm(int N) { return (size_t)DT_ROWS * (2 * N + 1) * sizeof(float); }
+static size_t dct_tile_shm(int rows_per_wg, int N) {
+  return (size_t)rows_per_wg * (2 * N + 1) * sizeof(float);
+}
+// workgroups of a k_dct4_tile launch (= its partials) at the handle's tiling
+static int dct_tile_blocks(const ProjHandle* h) {
+  return (h->g.nz * h->g.ny + h->dct_rows - 1) / h->dct_rows;
+}
+template <int MODE, bool TIN>
+static void launch_dct_tile(ProjHandle* h, const float* in, float* out, const DctFuse<float>& f,
+                            hipStream_t s) {
+  const int rows = h->g.nz * h->g.ny, N = h->g.nx, rt = h->dct_rows;
+  const dim3 grid(dct_tile_blocks(h)), block(32 * rt);
+  const size_t shm = dct_tile_shm(rt, N);
+  if (rt == 8)
+    hipLaunchKernelGGL((k_dct4_tile<MODE, TIN, 8>), grid, block, shm, s, rows, N, h->logMx,
+                       (const float*)h->twx, in, out, work_ldt(h), f);
+  else
+    hipLaunchKernelGGL((k_dct4_tile<MODE, TIN, 16>), grid, block, shm, s, rows, N, h->logMx,
+                       (const float*)h->twx, in, out, work_ldt(h), f);
+}
 
 // x transform of every row (r -> w1 forward, w1/w2 -> z inverse): DCT-IV by
 // FFT when available, else the GEMM with B = Ux (forward; Ux is symmetric
@@ -1622,14 +1655,10 @@ static void x_transform(ProjHandle* h, const T* in, T* out, bool inverse, hipStr
   if (work_t(h)) {
     if constexpr (sizeof(T) == 4) {
       const int ldt = work_ldt(h);
-      const dim3 grid((rows + DT_ROWS - 1) / DT_ROWS);
-      const size_t shm = dct_tile_shm(Nx);
       if (h->fft_x && inverse)
-        hipLaunchKernelGGL((k_dct4_tile<DCT_PLAIN, true>), grid, dim3(DT_THREADS), shm, s, rows, Nx,
-                           h->logMx, (const float*)h->twx, in, out, ldt, DctFuse<float>{});
+        launch_dct_tile<DCT_PLAIN, true>(h, in, out, DctFuse<float>{}, s);
       else if (h->fft_x)
-        hipLaunchKernelGGL((k_dct4_tile<DCT_PLAIN, false>), grid, dim3(DT_THREADS), shm, s, rows, Nx,
-                           h->logMx, (const float*)h->twx, in, out, ldt, DctFuse<float>{});
+        launch_dct_tile<DCT_PLAIN, false>(h, in, out, DctFuse<float>{}, s);
       else if (!inverse)   // C[row][i] at i ldt + row
         launch_gemm<T>(h, in, (const T*)h->Ux, out, rows, Nx, Nx, 1, 0, Nx, 1, 0, Nx, 1, 0, 1,
                        ldt, false, s);
@@ -1842,9 +1871,8 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
   // four rows per workgroup; one partial per workgroup either way
   const bool tile = sizeof(T) == 4 && work_t(h);
   const int rows = h->g.nz * h->g.ny;
-  const int ndct = tile ? (rows + DT_ROWS - 1) / DT_ROWS : (rows + 3) / 4;
-  const size_t shm = tile ? dct_tile_shm(h->g.nx) : 4 * (size_t)h->g.nx * sizeof(T);
-  const int ldt = work_ldt(h);
+  const int ndct = tile ? dct_tile_blocks(h) : (rows + 3) / 4;
+  const size_t shm = 4 * (size_t)h->g.nx * sizeof(T);   // k_dct4_rows
   const int npz = fused ? ndct : PJ_RED;
   DctFuse<T> fu{};
   fu.d = d; fu.r = r; fu.p = p; fu.q = q;
@@ -1859,9 +1887,7 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
     constexpr int MODE = decltype(mode)::value;
     if constexpr (sizeof(T) == 4) {
       if (tile) {
-        hipLaunchKernelGGL((k_dct4_tile<MODE, MODE == DCT_DOT>), dim3(ndct), dim3(DT_THREADS), shm, s,
-                           rows, h->g.nx, h->logMx, (const float*)h->twx, (const float*)in,
-                           (float*)out, ldt, f);
+        launch_dct_tile<MODE, MODE == DCT_DOT>(h, (const float*)in, (float*)out, f, s);
         return;
       }
     }
@@ -2145,6 +2171,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   h->learned_every = 4;
   h->use_plane = 1;
   h->mv_blocks = PJ_MV_DEFAULT;
+  h->dct_rows = 16;
   const int Nx = g->nx, Ny = g->ny, Nz = g->nz;
   // host copies of the dz metric for folding sqrt(dz) into the z matrices
   std::vector<double> dz(Nz);
@@ -2273,14 +2300,14 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   // the work buffers hold the transposed layout (ldt >= Nz Ny per x
   // frequency) when k_yz_plane can run
   const size_t nw = h->ZfQ ? (size_t)work_ldt(h) * Nx : n;
-  if (h->ZfQ && dct_tile_shm(Nx) > 65536) {
-    const void* tk[] = {(const void*)k_dct4_tile<DCT_PLAIN, false>,
-                        (const void*)k_dct4_tile<DCT_PLAIN, true>,
-                        (const void*)k_dct4_tile<DCT_UPDATE, false>,
-                        (const void*)k_dct4_tile<DCT_DOT, true>};
+  if (h->ZfQ && dct_tile_shm(16, Nx) > 65536) {
+    const void* tk[] = {(const void*)k_dct4_tile<DCT_PLAIN, false, 16>,
+                        (const void*)k_dct4_tile<DCT_PLAIN, true, 16>,
+                        (const void*)k_dct4_tile<DCT_UPDATE, false, 16>,
+                        (const void*)k_dct4_tile<DCT_DOT, true, 16>};
     for (const void* k : tk)
       FW_CHECK_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)dct_tile_shm(Nx)));
+                                       (int)dct_tile_shm(16, Nx)));
   }
   FW_CHECK_HIP(hipMalloc(&h->w1, nw * h->esz));
   FW_CHECK_HIP(hipMalloc(&h->w2, nw * h->esz));
@@ -2438,13 +2465,17 @@ int fw_proj_precondition(void* handle, const void* r, void* z, void* stream) {
                             : precond_apply<float>(h, (const float*)r, (float*)z, s);
 }
 
-int fw_proj_set_tiling(void* handle, int yz_plane, int matvec_blocks, int* yz_active) {
+int fw_proj_set_tiling(void* handle, int yz_plane, int matvec_blocks, int dct_rows,
+                       int* yz_active) {
   ProjHandle* h = (ProjHandle*)handle;
   FW_REQUIRE(h, "proj_set_tiling: null handle");
   FW_REQUIRE(matvec_blocks == 0 || (matvec_blocks >= 64 && matvec_blocks <= PJ_MV_MAX),
              "proj_set_tiling: matvec_blocks must be 0 (default) or 64..%d", PJ_MV_MAX);
+  FW_REQUIRE(dct_rows == 0 || dct_rows == 8 || dct_rows == 16,
+             "proj_set_tiling: dct_rows must be 0 (default), 8 or 16");
   h->use_plane = yz_plane ? 1 : 0;
   h->mv_blocks = matvec_blocks ? matvec_blocks : PJ_MV_DEFAULT;
+  h->dct_rows = dct_rows ? dct_rows : 16;
   if (yz_active) *yz_active = h->use_plane && h->ZfQ ? 1 : 0;
   return FW_OK;
 }

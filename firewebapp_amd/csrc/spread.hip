@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include "fw_common.h"
+#include "graph.h"
 
 namespace fw {
 
@@ -66,11 +67,158 @@ static const double Q_RAD_MAX = 1.0e5, Q_PULSE = 240000.0, T_CAP = 10000.0;
 static const double T_BOOTSTRAP_S = 2.0;
 static const int N_SUB = 10, REINIT_INTERVAL = 10, REINIT_SUBSTEPS = 5;
 
+__global__ void k_set_step_scalars(double* __restrict__ sb, double dt_sub, double relax,
+                                   double dsc, double dpr) {
+  if (threadIdx.x == 0) {
+    sb[SB_DT_SUB] = dt_sub;
+    sb[SB_EDC_RELAX] = relax;
+    sb[SB_DSC] = dsc;
+    sb[SB_DPR] = dpr;
+  }
+}
+
+// What the sub-step loop of a step depends on besides the fields.
+struct SubstepPlan {
+  int nsc, npr;              // turbulent diffusion passes (species, T_g)
+  double dt_sub, dsc, dpr;   // sub-step and pass lengths
+  double drip_gas, drip_solid;
+  bool fuse_summary;
+};
+
+// The N_SUB chemistry / transport / coupling sub-steps (spread_3d.py:2678-3200)
+// on the fields of f, swapping f's ping-pong pointers as it goes.  launch =
+// false only advances the pointers: the replay of a captured graph of the
+// same launches.  sb (may be null): the kernels read dt_sub, the EDC
+// relaxation and the pass lengths from the step scalar block (the particle
+// path only; the Eulerian coupling takes them by value).
+static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_params* p,
+                    const SubstepPlan& P, const double* sb, bool launch, hipStream_t s) {
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  const int nzb = g->n_z_bed, s0 = p->i_src0, s1 = p->i_src1;
+  const double dt_sub = P.dt_sub;
+  void** Y[3] = {&f->Y_fuel, &f->Y_O2, &f->Y_H2O};
+  const double y_in[3] = {0.0, 0.232, 0.0}, y_max[3] = {1.0, 0.232, 1.0};
+  // the last species diffusion pass of a sub-step leaves the next EDC's
+  // segment summaries (nothing between them changes the species) when the
+  // launch rows are whole 256-cell segments
+  bool summary_ready = false;
+  for (int sub = 0; sub < N_SUB; ++sub) {
+    if (launch)
+      TRY(edc_launch(g, f->rho, f->T_g, f->Y_fuel, f->Y_O2, f->k, f->eps, f->Y_H2O, f->omega,
+                     0.34, 1100.0, dt_sub, 1, 0, 1.3, 17.0e6, (int*)f->edc_work, summary_ready,
+                     s, sb));
+    // species with the dilution sources; the Y_O2 cap of :3059 rides in
+    // y_max, the [0, 1] clip is the kernel's own
+    {
+      const void* yin[3] = {*Y[0], *Y[1], *Y[2]};
+      void* yout[3] = {f->tmp1, f->tmp2, f->tmp3};
+      if (launch)
+        TRY(species3_launch(g, yin, yout, f->rho, f->u, f->v, f->w, f->src[0], f->src[1],
+                            dt_sub, 1.0e-5, y_in, y_max, s, sb));
+      SWAP(*Y[0], f->tmp1);
+      SWAP(*Y[1], f->tmp2);
+      SWAP(*Y[2], f->tmp3);
+    }
+    // turbulent diffusion; the clips of :3082-3084 fold into the last pass
+    // (with no pass they are the identity: each field is already in range)
+    for (int m = 0; m < P.nsc; ++m) {
+      const bool last = m == P.nsc - 1;
+      const double lo[3] = {last ? 0.0 : 1.0, last ? 0.0 : 1.0, last ? 0.0 : 1.0};
+      const double hi[3] = {last ? y_max[0] : 0.0, last ? y_max[1] : 0.0,
+                            last ? y_max[2] : 0.0};
+      const void* fin[3] = {*Y[0], *Y[1], *Y[2]};
+      void* fout[3] = {f->tmp1, f->tmp2, f->tmp3};
+      if (launch)
+        TRY(turb3_launch(g, fin, fout, f->nu_t, SC_T, P.dsc, lo, hi,
+                         P.fuse_summary && last ? (int*)f->edc_work : nullptr, s, sb));
+      SWAP(*Y[0], f->tmp1);
+      SWAP(*Y[1], f->tmp2);
+      SWAP(*Y[2], f->tmp3);
+    }
+    summary_ready = P.fuse_summary;
+    if (p->eulerian) {
+      // drip-torch split, q_rad_gas into Q_comb and the gas-solid coupling
+      // (:2916-2956, :3094-3099, :3148-3156); the solid share accumulates in
+      // Q_pyro once per sub-step, as the reference subtracts it in the loop
+      if (launch) {
+        TRY(coupling_launch(g, f->T_g, f->T_s, f->rho, f->u, f->v, f->w, f->alpha_s,
+                            p->sigma_sav, f->q_rad, f->Q_pyro, nullptr, f->q_rad_gas,
+                            sub == N_SUB - 1 ? f->Q_comb : nullptr, f->eu_m_water, p->eu_L_v,
+                            dt_sub, p->T_amb, 0, p->h_conv_mult, nzb, s0, s1, P.drip_gas,
+                            P.drip_solid, s));
+        TRY(fw_solid_conduction_vertical(g, f->T_s, f->tmp1, f->alpha_s, K_SOLID_GRASS,
+                                         RHO_SOLID, CP_SOLID, dt_sub, p->T_amb, T_CAP, s));
+      }
+    } else if (launch) {
+      // gas-energy update and vertical solid conduction with the T_s clamps
+      // of :3195-3196, in one pass
+      TRY(gas_solid_launch(g, f->T_g, f->T_s, f->tmp1, f->alpha_s, f->rho, f->Y_H2O,
+                           f->src[0], f->src[1], f->src[7], f->q_rad_gas,
+                           sub == N_SUB - 1 ? f->Q_comb : nullptr, nzb, s0, s1, P.drip_gas,
+                           dt_sub, p->T_amb, K_SOLID_GRASS, RHO_SOLID, CP_SOLID, p->T_amb,
+                           T_CAP, s, sb));
+    }
+    SWAP(f->T_s, f->tmp1);
+    // T_g turbulent diffusion, the clamps of :3193-3194 in the last pass
+    for (int m = 0; m < P.npr; ++m) {
+      const bool last = m == P.npr - 1;
+      if (launch)
+        TRY(turb_pass_launch(g, f->T_g, f->tmp1, f->nu_t, PR_T, P.dpr,
+                             last ? p->T_amb : 1.0, last ? T_CAP : 0.0, s, sb));
+      SWAP(f->T_g, f->tmp1);
+    }
+    if (P.npr == 0 && launch) {
+      // the particle path's gas update already floors T_g at T_amb; the
+      // coupling does not
+      const double lo = p->eulerian ? p->T_amb : -INFINITY;
+      if (g->dtype == FW_F64)
+        hipLaunchKernelGGL(k_clamp_max<double>, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                           (double*)f->T_g, T_CAP, lo);
+      else
+        hipLaunchKernelGGL(k_clamp_max<float>, dim3((n + 255) / 256), dim3(256), 0, s, n,
+                           (float*)f->T_g, (float)T_CAP, (float)lo);
+      FW_CHECK_LAUNCH("clamp");
+    }
+  }
+  return FW_OK;
+}
+
+// The graph key of a sub-step loop: the plan and every pointer it touches.
+static std::vector<uint64_t> substep_key(const fw_grid* g, const fw_spread_fields* f,
+                                         const SubstepPlan& P) {
+  return {kw(g), (uint64_t)g->dtype, (uint64_t)P.nsc, (uint64_t)P.npr,
+          (uint64_t)P.fuse_summary, kw(P.drip_gas), kw(P.drip_solid),
+          kw(f->rho), kw(f->T_g), kw(f->T_s), kw(f->Y_fuel), kw(f->Y_O2), kw(f->Y_H2O),
+          kw(f->k), kw(f->eps), kw(f->omega), kw(f->nu_t), kw(f->u), kw(f->v), kw(f->w),
+          kw(f->src[0]), kw(f->src[1]), kw(f->src[7]), kw(f->q_rad_gas), kw(f->Q_comb),
+          kw(f->alpha_s), kw(f->tmp1), kw(f->tmp2), kw(f->tmp3), kw(f->edc_work),
+          kw(f->step_scalars)};
+}
+
 }  // namespace fw
 
 using namespace fw;
 
 extern "C" {
+
+int fw_graphs_create(void** graphs) {
+  FW_REQUIRE(graphs, "graphs_create: null argument");
+  *graphs = new GraphCache();
+  return FW_OK;
+}
+
+int fw_graphs_destroy(void* graphs) {
+  delete (GraphCache*)graphs;
+  return FW_OK;
+}
+
+int fw_graphs_stats(void* graphs, long long* replays, long long* captures) {
+  FW_REQUIRE(graphs, "graphs_stats: null handle");
+  const GraphCache* gc = (const GraphCache*)graphs;
+  if (replays) *replays = gc->hits;
+  if (captures) *captures = gc->captures;
+  return FW_OK;
+}
 
 int fw_spread_front(const fw_grid* g, const fw_spread_fields* f, fw_spread_state* st,
                     void* stream) {
@@ -232,91 +380,29 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
   // N_SUB chemistry / transport / coupling sub-steps (:2678-3200)
   double nu_max = 0.0;
   TRY(fw_nu_t_max_interior(g, f->nu_t, f->nu_work, &nu_max, stream));
-  const double dt_sub = dt / N_SUB;
-  double dsc = 0.0, dpr = 0.0;
-  const int nsc = turb_substeps(nu_max, SC_T, dt_sub, g->dx, g->dy, p->dz_min, &dsc);
-  const int npr = turb_substeps(nu_max, PR_T, dt_sub, g->dx, g->dy, p->dz_min, &dpr);
-  const double drip_gas = (combustion && ign_on) ? (1.0 - F_DRIP_TO_SOLID) * Q_drip : 0.0;
-  void** Y[3] = {&f->Y_fuel, &f->Y_O2, &f->Y_H2O};
-  const double y_in[3] = {0.0, 0.232, 0.0}, y_max[3] = {1.0, 0.232, 1.0};
-  // the last species diffusion pass of a sub-step leaves the next EDC's
-  // segment summaries (nothing between them changes the species) when the
-  // launch rows are whole 256-cell segments
-  const bool fuse_summary = g->nx % EDC_SEG == 0 && nsc > 0;
-  bool summary_ready = false;
-  for (int sub = 0; sub < N_SUB; ++sub) {
-    TRY(edc_launch(g, f->rho, f->T_g, f->Y_fuel, f->Y_O2, f->k, f->eps, f->Y_H2O, f->omega,
-                   0.34, 1100.0, dt_sub, 1, 0, 1.3, 17.0e6, (int*)f->edc_work, summary_ready,
-                   s));
-    // species with the dilution sources; the Y_O2 cap of :3059 rides in
-    // y_max, the [0, 1] clip is the kernel's own
-    {
-      const void* yin[3] = {*Y[0], *Y[1], *Y[2]};
-      void* yout[3] = {f->tmp1, f->tmp2, f->tmp3};
-      TRY(fw_species_transport3(g, yin, yout, f->rho, f->u, f->v, f->w, f->src[0],
-                                f->src[1], dt_sub, 1.0e-5, y_in, y_max, stream));
-      SWAP(*Y[0], f->tmp1);
-      SWAP(*Y[1], f->tmp2);
-      SWAP(*Y[2], f->tmp3);
-    }
-    // turbulent diffusion; the clips of :3082-3084 fold into the last pass
-    // (with no pass they are the identity: each field is already in range)
-    for (int m = 0; m < nsc; ++m) {
-      const bool last = m == nsc - 1;
-      const double lo[3] = {last ? 0.0 : 1.0, last ? 0.0 : 1.0, last ? 0.0 : 1.0};
-      const double hi[3] = {last ? y_max[0] : 0.0, last ? y_max[1] : 0.0,
-                            last ? y_max[2] : 0.0};
-      const void* fin[3] = {*Y[0], *Y[1], *Y[2]};
-      void* fout[3] = {f->tmp1, f->tmp2, f->tmp3};
-      TRY(turb3_launch(g, fin, fout, f->nu_t, SC_T, dsc, lo, hi,
-                       fuse_summary && last ? (int*)f->edc_work : nullptr, s));
-      SWAP(*Y[0], f->tmp1);
-      SWAP(*Y[1], f->tmp2);
-      SWAP(*Y[2], f->tmp3);
-    }
-    summary_ready = fuse_summary;
-    if (p->eulerian) {
-      // drip-torch split, q_rad_gas into Q_comb and the gas-solid coupling
-      // (:2916-2956, :3094-3099, :3148-3156); the solid share accumulates in
-      // Q_pyro once per sub-step, as the reference subtracts it in the loop
-      const double drip_solid = (combustion && ign_on) ? F_DRIP_TO_SOLID * Q_drip : 0.0;
-      TRY(coupling_launch(g, f->T_g, f->T_s, f->rho, f->u, f->v, f->w, f->alpha_s,
-                          p->sigma_sav, f->q_rad, f->Q_pyro, nullptr, f->q_rad_gas,
-                          sub == N_SUB - 1 ? f->Q_comb : nullptr, f->eu_m_water, p->eu_L_v,
-                          dt_sub, p->T_amb, 0, p->h_conv_mult, nzb, s0, s1, drip_gas,
-                          drip_solid, s));
-      TRY(fw_solid_conduction_vertical(g, f->T_s, f->tmp1, f->alpha_s, K_SOLID_GRASS,
-                                       RHO_SOLID, CP_SOLID, dt_sub, p->T_amb, T_CAP, stream));
-    } else {
-      // gas-energy update and vertical solid conduction with the T_s clamps
-      // of :3195-3196, in one pass
-      TRY(fw_gas_update_solid_conduction(g, f->T_g, f->T_s, f->tmp1, f->alpha_s, f->rho,
-                                         f->Y_H2O, f->src[0], f->src[1], f->src[7],
-                                         f->q_rad_gas, sub == N_SUB - 1 ? f->Q_comb : nullptr,
-                                         nzb, s0, s1, drip_gas, dt_sub, p->T_amb,
-                                         K_SOLID_GRASS, RHO_SOLID, CP_SOLID, p->T_amb, T_CAP,
-                                         stream));
-    }
-    SWAP(f->T_s, f->tmp1);
-    // T_g turbulent diffusion, the clamps of :3193-3194 in the last pass
-    for (int m = 0; m < npr; ++m) {
-      const bool last = m == npr - 1;
-      TRY(fw_turbulent_diffusion_pass(g, f->T_g, f->tmp1, f->nu_t, PR_T, dpr,
-                                      last ? p->T_amb : 1.0, last ? T_CAP : 0.0, stream));
-      SWAP(f->T_g, f->tmp1);
-    }
-    if (npr == 0) {
-      // the particle path's gas update already floors T_g at T_amb; the
-      // coupling does not
-      const double lo = p->eulerian ? p->T_amb : -INFINITY;
-      if (g->dtype == FW_F64)
-        hipLaunchKernelGGL(k_clamp_max<double>, dim3((n + 255) / 256), dim3(256), 0, s, n,
-                           (double*)f->T_g, T_CAP, lo);
-      else
-        hipLaunchKernelGGL(k_clamp_max<float>, dim3((n + 255) / 256), dim3(256), 0, s, n,
-                           (float*)f->T_g, (float)T_CAP, (float)lo);
-      FW_CHECK_LAUNCH("clamp");
-    }
+  SubstepPlan plan;
+  plan.dt_sub = dt / N_SUB;
+  plan.dsc = plan.dpr = 0.0;
+  plan.nsc = turb_substeps(nu_max, SC_T, plan.dt_sub, g->dx, g->dy, p->dz_min, &plan.dsc);
+  plan.npr = turb_substeps(nu_max, PR_T, plan.dt_sub, g->dx, g->dy, p->dz_min, &plan.dpr);
+  plan.drip_gas = (combustion && ign_on) ? (1.0 - F_DRIP_TO_SOLID) * Q_drip : 0.0;
+  plan.drip_solid = (combustion && ign_on && p->eulerian) ? F_DRIP_TO_SOLID * Q_drip : 0.0;
+  plan.fuse_summary = g->nx % EDC_SEG == 0 && plan.nsc > 0;
+  if (f->graphs && f->step_scalars && s && !p->eulerian) {
+    // one graph launch for the whole loop: the step's scalars go to the
+    // device block first, the pointers advance as the launches would
+    hipLaunchKernelGGL(k_set_step_scalars, dim3(1), dim3(64), 0, s, f->step_scalars,
+                       plan.dt_sub, edc_relax(plan.dt_sub, 1), plan.dsc, plan.dpr);
+    FW_CHECK_LAUNCH("step scalars");
+    GraphCache* gc = (GraphCache*)f->graphs;
+    const fw_spread_fields entry = *f;
+    TRY(gc->run(substep_key(g, f, plan), s, [&](hipStream_t cs) {
+      fw_spread_fields rec = entry;
+      return substeps(g, &rec, p, plan, entry.step_scalars, true, cs);
+    }));
+    TRY(substeps(g, f, p, plan, nullptr, false, s));
+  } else {
+    TRY(substeps(g, f, p, plan, nullptr, true, s));
   }
   // level set (:3204-3300).  Active: the ahead band of the step's level set
   // and the DOM forward flux into it (:2555-2565, :2591; the intensities and

@@ -105,7 +105,9 @@ k_gas_solid(GridArgs g, T* __restrict__ Tg, const T* __restrict__ Ts,
             const T* __restrict__ Sp, const T* __restrict__ Sd,
             const T* __restrict__ Qgc, const T* __restrict__ qrg,
             T* __restrict__ Qcomb_out, int nzb, int i0, int i1, T drip, T dt_sub,
-            T T_amb, T k_solid, T inv_rho_cp, T ts_lo, T ts_hi) {
+            T T_amb, T k_solid, T inv_rho_cp, T ts_lo, T ts_hi,
+            const double* __restrict__ sb) {
+  if (sb) dt_sub = T(sb[SB_DT_SUB]);
   int i, j, k;
   cell_ijk(i, j, k);
   if (i >= g.nx) return;
@@ -447,11 +449,27 @@ int fw_gas_update_solid_conduction(const fw_grid* g, void* T_g, const void* T_s,
                                    double cp_solid, double T_s_lo, double T_s_hi,
                                    void* stream) {
   FW_GRID(g);
+  return fw::gas_solid_launch(g, T_g, T_s, T_s_out, alpha_s, rho, Y_H2O, S_pyro_part,
+                              S_dry_part, Q_g_conv, q_rad_gas, Q_comb_out, n_z_bed, i_src0,
+                              i_src1, drip_W_m3, dt_sub, T_amb, k_solid, rho_solid, cp_solid,
+                              T_s_lo, T_s_hi, (hipStream_t)stream, nullptr);
+}
+
+}  // extern "C"
+
+namespace fw {
+
+int gas_solid_launch(const fw_grid* g, void* T_g, const void* T_s, void* T_s_out,
+                     const void* alpha_s, const void* rho, const void* Y_H2O,
+                     const void* S_pyro_part, const void* S_dry_part, const void* Q_g_conv,
+                     const void* q_rad_gas, void* Q_comb_out, int n_z_bed, int i_src0,
+                     int i_src1, double drip_W_m3, double dt_sub, double T_amb,
+                     double k_solid, double rho_solid, double cp_solid, double T_s_lo,
+                     double T_s_hi, hipStream_t s, const double* sb) {
   FW_REQUIRE(g && T_g && T_s && T_s_out && T_s != T_s_out && alpha_s && rho && Y_H2O &&
                  S_pyro_part && S_dry_part && Q_g_conv && q_rad_gas,
              "gas_update_solid_conduction: bad argument");
   Launch L = cell_launch(g);
-  hipStream_t s = (hipStream_t)stream;
   const double irc = 1.0 / (rho_solid * cp_solid);
   if (g->dtype == FW_F64)
     hipLaunchKernelGGL(k_gas_solid<double>, L.grid, L.block, 0, s, grid_args(g),
@@ -460,7 +478,7 @@ int fw_gas_update_solid_conduction(const fw_grid* g, void* T_g, const void* T_s,
                        (const double*)S_pyro_part, (const double*)S_dry_part,
                        (const double*)Q_g_conv, (const double*)q_rad_gas, (double*)Q_comb_out,
                        n_z_bed, i_src0, i_src1, drip_W_m3, dt_sub, T_amb, k_solid, irc,
-                       T_s_lo, T_s_hi);
+                       T_s_lo, T_s_hi, sb);
   else
     hipLaunchKernelGGL(k_gas_solid<float>, L.grid, L.block, 0, s, grid_args(g),
                        (float*)T_g, (const float*)T_s, (float*)T_s_out,
@@ -469,10 +487,14 @@ int fw_gas_update_solid_conduction(const fw_grid* g, void* T_g, const void* T_s,
                        (const float*)Q_g_conv, (const float*)q_rad_gas, (float*)Q_comb_out,
                        n_z_bed, i_src0, i_src1, (float)drip_W_m3, (float)dt_sub,
                        (float)T_amb, (float)k_solid, (float)irc, (float)T_s_lo,
-                       (float)T_s_hi);
+                       (float)T_s_hi, sb);
   FW_CHECK_LAUNCH("gas_update_solid_conduction");
   return FW_OK;
 }
+
+}  // namespace fw
+
+extern "C" {
 
 int fw_add_patch(const fw_grid* g, void* field, int k, int i0, int i1, double add,
                  void* stream) {

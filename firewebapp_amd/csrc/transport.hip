@@ -192,7 +192,8 @@ k_species3(GridArgs g, const T* __restrict__ Y1, const T* __restrict__ Y2,
            T* __restrict__ O3, const T* __restrict__ rho, const T* __restrict__ u,
            const T* __restrict__ v, const T* __restrict__ w, const T* __restrict__ Sp,
            const T* __restrict__ Sd, T dt, T D, T in1, T in2, T in3, T mx1, T mx2,
-           T mx3) {
+           T mx3, const double* __restrict__ sb) {
+  if (sb) dt = T(sb[SB_DT_SUB]);
   int i, j, kb;
   cell_ijk(i, j, kb);
   if (i >= g.nx) return;
@@ -312,7 +313,9 @@ __device__ __forceinline__ T turb_cell(const GridArgs& g, int i, int j, int k, s
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_turb_diff(GridArgs g, const T* __restrict__ f, T* __restrict__ fo,
-            const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo, T hi) {
+            const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo, T hi,
+            const double* __restrict__ sb, int sb_slot) {
+  if (sb) dt_sub = T(sb[sb_slot]);
   int i, j, k;
   cell_ijk(i, j, k);
   if (i >= g.nx) return;
@@ -326,7 +329,9 @@ __global__ void __launch_bounds__(256)
 k_turb_diff3(GridArgs g, const T* __restrict__ f1, const T* __restrict__ f2,
              const T* __restrict__ f3, T* __restrict__ o1, T* __restrict__ o2,
              T* __restrict__ o3, const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo1,
-             T hi1, T lo2, T hi2, T lo3, T hi3, int* __restrict__ edc_summary) {
+             T hi1, T lo2, T hi2, T lo3, T hi3, int* __restrict__ edc_summary,
+             const double* __restrict__ sb, int sb_slot) {
+  if (sb) dt_sub = T(sb[sb_slot]);
   int i, j, k;
   cell_ijk(i, j, k);
   if (i >= g.nx) return;
@@ -622,49 +627,59 @@ template <typename T>
 static int species3_impl(const fw_grid* g, const void* const* Y, void* const* Yo,
                          const void* rho, const void* u, const void* v, const void* w,
                          const void* Sp, const void* Sd, double dt, double D,
-                         const double* y_in, const double* y_max, hipStream_t s) {
+                         const double* y_in, const double* y_max, hipStream_t s,
+                         const double* sb) {
   constexpr int CZ = 2;
   Launch L = cell_launch(g);
   L.grid.z = (g->nz + CZ - 1) / CZ;
   // the event-recording launch costs about twice the host time of a plain
-  // one: use it only while a timer asked for the events
-  const KernelEvents ev = timer_next(FW_TIMER_SPECIES3);
+  // one: use it only while a timer asked for the events (never for a launch
+  // with a step scalar block, which a graph capture records)
+  const KernelEvents ev = sb ? KernelEvents{} : timer_next(FW_TIMER_SPECIES3);
   if (ev.start)
     hipExtLaunchKernelGGL((k_species3<T, CZ>), L.grid, L.block, 0, s, ev.start, ev.stop, 0,
                           grid_args(g), (const T*)Y[0],
                           (const T*)Y[1], (const T*)Y[2], (T*)Yo[0], (T*)Yo[1], (T*)Yo[2],
                           (const T*)rho, (const T*)u, (const T*)v, (const T*)w,
                           (const T*)Sp, (const T*)Sd, T(dt), T(D), T(y_in[0]), T(y_in[1]),
-                          T(y_in[2]), T(y_max[0]), T(y_max[1]), T(y_max[2]));
+                          T(y_in[2]), T(y_max[0]), T(y_max[1]), T(y_max[2]), sb);
   else
     hipLaunchKernelGGL((k_species3<T, CZ>), L.grid, L.block, 0, s, grid_args(g),
                        (const T*)Y[0], (const T*)Y[1], (const T*)Y[2], (T*)Yo[0], (T*)Yo[1],
                        (T*)Yo[2], (const T*)rho, (const T*)u, (const T*)v, (const T*)w,
                        (const T*)Sp, (const T*)Sd, T(dt), T(D), T(y_in[0]), T(y_in[1]),
-                       T(y_in[2]), T(y_max[0]), T(y_max[1]), T(y_max[2]));
+                       T(y_in[2]), T(y_max[0]), T(y_max[1]), T(y_max[2]), sb);
   FW_CHECK_LAUNCH("species_transport3");
   return FW_OK;
+}
+
+int species3_launch(const fw_grid* g, const void* const* Y, void* const* Yo, const void* rho,
+                    const void* u, const void* v, const void* w, const void* Sp,
+                    const void* Sd, double dt, double D, const double* y_in,
+                    const double* y_max, hipStream_t s, const double* sb) {
+  return FW_DISPATCH(g, species3_impl, g, Y, Yo, rho, u, v, w, Sp, Sd, dt, D, y_in, y_max, s,
+                     sb);
 }
 
 template <typename T>
 static int turb3_impl(const fw_grid* g, const void* const* f, void* const* fo,
                       const void* nu_t, double sc_t, double dt_sub, const double* lo,
-                      const double* hi, int* edc_summary, hipStream_t s) {
+                      const double* hi, int* edc_summary, hipStream_t s, const double* sb) {
   Launch L = cell_launch(g);
   if (edc_summary && (g->nx % EDC_SEG != 0 || L.block.x != EDC_SEG))
     return fw_set_error(FW_E_CONFIG, "turbulent_diffusion3: the EDC summary needs Nx %% 256 == 0");
   hipLaunchKernelGGL(k_turb_diff3<T>, L.grid, L.block, 0, s, grid_args(g), (const T*)f[0],
                      (const T*)f[1], (const T*)f[2], (T*)fo[0], (T*)fo[1], (T*)fo[2],
                      (const T*)nu_t, T(sc_t), T(dt_sub), T(lo[0]), T(hi[0]), T(lo[1]),
-                     T(hi[1]), T(lo[2]), T(hi[2]), edc_summary);
+                     T(hi[1]), T(lo[2]), T(hi[2]), edc_summary, sb, (int)SB_DSC);
   FW_CHECK_LAUNCH("turbulent_diffusion3");
   return FW_OK;
 }
 
 int turb3_launch(const fw_grid* g, const void* const* f, void* const* fo, const void* nu_t,
                  double sc_t, double dt_sub, const double* lo, const double* hi,
-                 int* edc_summary, hipStream_t s) {
-  return FW_DISPATCH(g, turb3_impl, g, f, fo, nu_t, sc_t, dt_sub, lo, hi, edc_summary, s);
+                 int* edc_summary, hipStream_t s, const double* sb) {
+  return FW_DISPATCH(g, turb3_impl, g, f, fo, nu_t, sc_t, dt_sub, lo, hi, edc_summary, s, sb);
 }
 
 template <typename T>
@@ -683,13 +698,18 @@ static int gas_energy_impl(const fw_grid* g, const void* Tg, void* To,
 template <typename T>
 static int turb_diff_pass_impl(const fw_grid* g, const void* f, void* fo,
                                const void* nu_t, double sc_t, double dt_sub,
-                               double lo, double hi, hipStream_t s) {
+                               double lo, double hi, hipStream_t s, const double* sb) {
   Launch L = cell_launch(g);
   hipLaunchKernelGGL(k_turb_diff<T>, L.grid, L.block, 0, s, grid_args(g),
                      (const T*)f, (T*)fo, (const T*)nu_t, T(sc_t), T(dt_sub),
-                     T(lo), T(hi));
+                     T(lo), T(hi), sb, (int)SB_DPR);
   FW_CHECK_LAUNCH("turbulent_diffusion");
   return FW_OK;
+}
+
+int turb_pass_launch(const fw_grid* g, const void* f, void* fo, const void* nu_t, double sc_t,
+                     double dt_sub, double lo, double hi, hipStream_t s, const double* sb) {
+  return FW_DISPATCH(g, turb_diff_pass_impl, g, f, fo, nu_t, sc_t, dt_sub, lo, hi, s, sb);
 }
 
 template <typename T>
@@ -834,7 +854,8 @@ int fw_species_transport3(const fw_grid* g, const void* const* Y, void* const* Y
     for (int b = 0; b < 3; ++b)
       FW_REQUIRE(Y[a] != Y_out[b], "species_transport3: outputs must not alias inputs");
   return FW_DISPATCH(g, species3_impl, g, Y, Y_out, rho, u, v, w, S_pyro_part,
-                     S_dry_part, dt, D, Y_inlet, y_max, (hipStream_t)stream);
+                     S_dry_part, dt, D, Y_inlet, y_max, (hipStream_t)stream,
+                     (const double*)nullptr);
 }
 
 int fw_turbulent_diffusion_pass3(const fw_grid* g, const void* const* f, void* const* f_out,
@@ -847,7 +868,7 @@ int fw_turbulent_diffusion_pass3(const fw_grid* g, const void* const* f, void* c
     for (int b = 0; b < 3; ++b)
       FW_REQUIRE(f[a] != f_out[b], "turbulent_diffusion3: outputs must not alias inputs");
   return FW_DISPATCH(g, turb3_impl, g, f, f_out, nu_t, sc_t, dt_sub, clip_lo, clip_hi,
-                     (int*)nullptr, (hipStream_t)stream);
+                     (int*)nullptr, (hipStream_t)stream, (const double*)nullptr);
 }
 
 int fw_turbulent_diffusion_pass3_summary(const fw_grid* g, const void* const* f,
@@ -862,7 +883,7 @@ int fw_turbulent_diffusion_pass3_summary(const fw_grid* g, const void* const* f,
     for (int b = 0; b < 3; ++b)
       FW_REQUIRE(f[a] != f_out[b], "turbulent_diffusion3: outputs must not alias inputs");
   return FW_DISPATCH(g, turb3_impl, g, f, f_out, nu_t, sc_t, dt_sub, clip_lo, clip_hi,
-                     edc_summary, (hipStream_t)stream);
+                     edc_summary, (hipStream_t)stream, (const double*)nullptr);
 }
 
 int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
@@ -881,7 +902,7 @@ int fw_turbulent_diffusion_pass(const fw_grid* g, const void* f, void* f_out,
   FW_GRID(g);
   FW_REQUIRE(g && f != f_out, "turbulent_diffusion: f_out must not alias f");
   return FW_DISPATCH(g, turb_diff_pass_impl, g, f, f_out, nu_t, sc_t, dt_sub,
-                     clip_lo, clip_hi, (hipStream_t)stream);
+                     clip_lo, clip_hi, (hipStream_t)stream, (const double*)nullptr);
 }
 
 int fw_nu_t_max_interior(const fw_grid* g, const void* nu_t, double* work,

@@ -31,6 +31,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass, field
 from pathlib import Path
 from typing import Optional
@@ -72,7 +73,7 @@ class FwSpreadFields(C.Structure):
             "q_in_soil", "soil_dz", "soil_d_above", "soil_d_below", "phi", "phi_tmp",
             "phi0", "red_work", "red_out", "red_host", "nu_work", "contrib", "dT_work",
             "counters", "edc_work", "lset_mask", "q_dom_fwd", "v_n", "eu_m_solid",
-            "eu_m_char", "eu_m_water", "Q_pyro")])
+            "eu_m_char", "eu_m_water", "Q_pyro", "graphs", "step_scalars")])
 
 
 class FwSpreadParams(C.Structure):
@@ -556,9 +557,16 @@ class Spread3DSolver:
               "Sd", "Qp", "Qd", "YF", "Qch", "Qsm", "Qgc", "M_local", "T_s_old",
               "tmp1", "tmp2", "tmp3")
 
-    def __init__(self, cfg: Spread3DConfig, grid: Optional[Grid3D] = None):
+    def __init__(self, cfg: Spread3DConfig, grid: Optional[Grid3D] = None,
+                 graphs: Optional[bool] = None):
+        """graphs: replay the step's sub-step loop from captured hipGraphs
+        (default: on unless FIREWAVE_GRAPHS=0); bit-identical to direct
+        launches.  With graphs on and the caller on the default stream, the
+        steps run on a stream of the solver's own, ordered against the
+        caller's stream on entry and exit."""
         self.cfg = cfg
         self.proj = None
+        self._graphs = None
         self.dev = device()
         self.dtype = cfg.dtype
         g = grid or build_grid(cfg)
@@ -574,10 +582,10 @@ class Spread3DSolver:
         self.N = g.Nx * g.Ny * g.Nz
         self.f = {n: torch.zeros(self.shape, dtype=self.dtype, device=self.dev)
                   for n in self.FIELDS}
-        self._setup()
+        self._setup(graphs)
 
     # -- setup (spread_3d.py:1126-1896) --------------------------------------
-    def _setup(self):
+    def _setup(self, graphs):
         cfg, g, f, dev = self.cfg, self.grid, self.f, self.dev
         T_amb = cfg.T_amb
         f64 = dict(dtype=torch.float64, device=dev)
@@ -691,6 +699,17 @@ class Spread3DSolver:
                         ("contrib", self.contrib), ("dT_work", self.dT_work),
                         ("counters", self.counters), ("edc_work", self.edc_work)):
             setattr(fs, name, t.data_ptr())
+        if graphs is None:
+            graphs = os.environ.get("FIREWAVE_GRAPHS", "1") != "0"
+        self._stream = None
+        if graphs:
+            h = C.c_void_p()
+            _lib.call("fw_graphs_create", C.byref(h))
+            self._graphs = h
+            self.step_scalars = torch.zeros(_lib.FW_STEP_SCALARS, **f64)
+            fs.graphs = h.value
+            fs.step_scalars = self.step_scalars.data_ptr()
+            self._stream = torch.cuda.Stream(device=dev)
         # active level set: the band, its forward flux and the front speed,
         # kept across the step for the snapshot writer
         cfg_active = not cfg.level_set_passive
@@ -853,11 +872,20 @@ class Spread3DSolver:
         if self.gust.enabled:
             # the inlet of a step starting at t: the log law times g(t)
             self.u_inlet.copy_(self.u_inlet_base * self.gust.factor(st.t))
+        cur = torch.cuda.current_stream()
+        run = cur
+        if self._stream is not None and cur.cuda_stream == 0:
+            # graph capture needs a stream other than the legacy default one
+            run = self._stream
+            run.wait_stream(cur)
         rc = _lib.lib().fw_spread_step(
             self.dg.ref, self.dg64.ref, C.byref(self.fields), C.byref(self.params),
             C.byref(st), self.proj, self.dom_h, C.byref(self.P.c),
             C.byref(self.ix.c) if self.ix is not None else None,
-            C.byref(self.bed_params) if self.bed_params is not None else None, stream_ptr())
+            C.byref(self.bed_params) if self.bed_params is not None else None,
+            C.c_void_p(run.cuda_stream))
+        if run is not cur:
+            cur.wait_stream(run)
         self._remap()
         if rc == _lib.FW_E_NONFINITE:
             raise RuntimeError(_lib.lib().fw_last_error().decode())
@@ -964,10 +992,23 @@ class Spread3DSolver:
             k_turb_final=h64(f["k"]), eps_turb_final=h64(f["eps"]),
             nu_t_final=h64(f["nu_t"]), n_steps=self.step_no, stop_reason=self.stop_reason)
 
+    def graph_stats(self):
+        """(replays, captures) of the step's graph cache, or None without one."""
+        if self._graphs is None:
+            return None
+        r, c = C.c_longlong(), C.c_longlong()
+        _lib.call("fw_graphs_stats", self._graphs, C.byref(r), C.byref(c))
+        return r.value, c.value
+
     def close(self):
         if self.proj is not None:
             _lib.lib().fw_proj_destroy(self.proj)
             self.proj = None
+        if self._graphs is not None:
+            if self._stream is not None:
+                self._stream.synchronize()
+            _lib.lib().fw_graphs_destroy(self._graphs)
+            self._graphs = None
 
     def __del__(self):
         try:

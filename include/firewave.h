@@ -246,10 +246,12 @@ int fw_proj_precondition(void* handle, const void* r, void* z, void* stream);
  * in one pass (k_yz_plane: float32, 16 <= Ny <= 128, Nz <= 64), 0 the
  * per-direction column transforms; *yz_active (may be NULL) says whether
  * the one-pass kernel runs.  matvec_blocks: workgroups of the fused
- * operator launch (64..4096; 0 = 1024).  The autotuner
- * (firewebapp_amd/autotune.py) picks both per grid. */
+ * operator launch (64..4096; 0 = 1024).  dct_rows: rows per workgroup of
+ * the x transforms that feed the one-pass kernel (8 or 16; 0 = 16).  The
+ * autotuner (firewebapp_amd/autotune.py) picks them per grid. */
 #define FW_PROJ_MV_MAX 4096
-int fw_proj_set_tiling(void* handle, int yz_plane, int matvec_blocks, int* yz_active);
+int fw_proj_set_tiling(void* handle, int yz_plane, int matvec_blocks, int dct_rows,
+                       int* yz_active);
 
 /* Preconditioner of the PCG: FW_PRECOND_SEPARABLE (default; the reference's
  * SeparableLaplacian3D, fft_poisson_3d.py:184-228), FW_PRECOND_NONE, or
@@ -632,7 +634,26 @@ typedef struct fw_spread_fields {
    * the grid-type solid heat sink */
   double *eu_m_solid, *eu_m_char, *eu_m_water;
   void* Q_pyro;
+  /* hipGraph replay of the sub-step loop (may be NULL: direct launches):
+   * graphs from fw_graphs_create, step_scalars FW_STEP_SCALARS device
+   * doubles.  Used only on a stream other than the legacy default stream
+   * and for the Lagrangian bed. */
+  void* graphs;
+  double* step_scalars;
 } fw_spread_fields;
+
+/* Cache of instantiated hipGraphs for one case's step loops (no counterpart
+ * in the reference: its step is interpreted Python).  A loop whose launch
+ * sequence is fixed by its trip counts, flags and field pointers is recorded
+ * once per distinct combination by stream capture and replayed with one
+ * hipGraphLaunch; the per-step scalars it needs are read by its kernels from
+ * fw_spread_fields.step_scalars.  Results are bit-identical to direct
+ * launches.  A cache serves one case (one host thread) at a time. */
+#define FW_STEP_SCALARS 8
+int fw_graphs_create(void** graphs);
+int fw_graphs_destroy(void* graphs);
+/* replays and captures since creation */
+int fw_graphs_stats(void* graphs, long long* replays, long long* captures);
 
 /* Run constants resolved from the deck (spread_3d.py:959-1896). */
 typedef struct fw_spread_params {

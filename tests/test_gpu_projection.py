@@ -207,7 +207,7 @@ def test_yz_plane_matches_column_transforms(shape):
     out = {}
     for on in (1, 0):
         act = C.c_int()
-        _lib.call("fw_proj_set_tiling", h, on, 0, C.byref(act))
+        _lib.call("fw_proj_set_tiling", h, on, 0, 0, C.byref(act))
         assert act.value == on, (on, act.value)
         Z = torch.empty_like(R)
         _lib.call("fw_proj_precondition", h, ptr(R), ptr(Z), stream_ptr())
@@ -240,21 +240,22 @@ def test_tilings_reach_the_same_residual():
     R = torch.as_tensor(rho.astype(np.float32), device="cuda")
     coef = OP.coefficients(rho.astype(np.float32).astype(np.float64), 0.1, 0.1, dz, da, db)
     its = {}
-    for plane, mvb in ((0, 0), (1, 0), (1, 64), (1, 333), (1, 4096)):
+    for plane, mvb, rt in ((0, 0, 0), (1, 0, 0), (1, 64, 8), (1, 333, 16), (1, 4096, 8)):
         act = C.c_int()
-        _lib.call("fw_proj_set_tiling", h, plane, mvb, C.byref(act))
+        _lib.call("fw_proj_set_tiling", h, plane, mvb, rt, C.byref(act))
         assert act.value == plane
         X = torch.zeros_like(B)
         it, rel = C.c_int(), C.c_double()
         _lib.call("fw_proj_solve", h, ptr(R), ptr(B), ptr(X), 1e-8, C.byref(it),
                   C.byref(rel), stream_ptr())
         true_rel = np.linalg.norm(b - OP.matvec(coef, X.cpu().numpy())) / np.linalg.norm(b)
-        assert true_rel <= 1.05e-8, (plane, mvb, true_rel)
-        its[(plane, mvb)] = it.value
+        assert true_rel <= 1.05e-8, (plane, mvb, rt, true_rel)
+        its[(plane, mvb, rt)] = it.value
     print("iterations per tiling", its, flush=True)
     assert max(its.values()) - min(its.values()) <= 1
-    with pytest.raises(_lib.FirewaveError):
-        _lib.call("fw_proj_set_tiling", h, 1, 5000, None)
+    for bad in ((1, 5000, 0), (1, 0, 12)):
+        with pytest.raises(_lib.FirewaveError):
+            _lib.call("fw_proj_set_tiling", h, *bad, None)
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
