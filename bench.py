@@ -26,7 +26,9 @@ steps, inputs resident in HBM.  The JSON line also carries:
   launches) / its mean launch time over the timed steps, measured with HIP
   events that each launch itself records (hipExtLaunchKernelGGL, on the
   stream the kernel is launched on: torch's current stream), against
-  8 TB/s; roofline_species3 the same for the sub-steps' species kernel;
+  8 TB/s; roofline_species3 the same for the sub-steps' species kernel
+  (every 4th launch of 4 steps after the timed ones, run with the graph
+  replay of the sub-step loop off);
   traffic is the HBM bytes per launch from the FETCH_SIZE and WRITE_SIZE
   counters;
 * step_roofline: the whole step against the HBM roofline, with the
@@ -251,20 +253,22 @@ def deck(name, dtype):
     return RomInputs(outdoor_overrides=ovr)
 
 
-# kernels timed with HIP events over the timed steps: (timer id, stride);
-# every SPECIES_STRIDE-th k_species3 launch and every DOM sweep launch
-# records its events (the event-recording launch costs extra host time)
+# kernels timed with HIP events: every DOM sweep launch of the timed steps
+# records its events (the event-recording launch costs extra host time, and
+# a timed launch is never captured into a graph, so the DOM solves of the
+# timed steps run as direct launches); every SPECIES_STRIDE-th k_species3
+# launch of SPECIES_STEPS extra steps after them, with the step's graph
+# replay off (the sub-step loop is otherwise one graph launch)
 TIMER_SPECIES3, TIMER_DOM_SWEEP = 0, 1
-SPECIES_STRIDE = 4
+SPECIES_STRIDE, SPECIES_STEPS = 4, 4
 
 
-def kernel_timers(on):
-    """Start (reset) or stop the HIP-event timing of the k_species3 and
-    k_dom_sweep_tile launches; the events are recorded by the launches
+def kernel_timer(which, stride):
+    """Start (reset) with the given stride, or stop (0), the HIP-event
+    timing of a kernel's launches; the events are recorded by the launches
     themselves, on the stream the kernels run on."""
     from firewebapp_amd import _lib
-    _lib.call("fw_kernel_timer", TIMER_SPECIES3, SPECIES_STRIDE if on else 0)
-    _lib.call("fw_kernel_timer", TIMER_DOM_SWEEP, 1 if on else 0)
+    _lib.call("fw_kernel_timer", which, stride)
 
 
 def kernel_timer_read(which):
@@ -383,7 +387,7 @@ def main(argv=None):
     t_win0 = sv.t
     print(f"# spin-up and warm-up done: {spinup + a.warmup} steps, t = {sv.t:.4f} s",
           file=sys.stderr, flush=True)
-    kernel_timers(True)
+    kernel_timer(TIMER_DOM_SWEEP, 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -396,9 +400,17 @@ def main(argv=None):
     # the DOM intensities among them, are outside torch's allocator)
     free_b, total_b = torch.cuda.mem_get_info()
     hbm_used = total_b - free_b
-    kernel_timers(False)
+    kernel_timer(TIMER_DOM_SWEEP, 0)
     print(f"# timed {a.steps} steps in {wall:.3f} s, t = {t_win0:.4f} .. {sv.t:.4f} s",
           file=sys.stderr, flush=True)
+    graph_stats = sv.graph_stats()
+    graphs, sv.fields.graphs = sv.fields.graphs, None
+    kernel_timer(TIMER_SPECIES3, SPECIES_STRIDE)
+    for i in range(SPECIES_STEPS):
+        sv.step()
+    torch.cuda.synchronize()
+    kernel_timer(TIMER_SPECIES3, 0)
+    sv.fields.graphs = graphs
     esz = 4 if a.dtype == "float32" else 8
     M = sv.dom.M
     TS = 4
@@ -432,6 +444,9 @@ def main(argv=None):
         "roofline_species3": _roofline("k_species3", SPECIES3_FIELDS * esz * cells,
                                        sp_ms / max(sp_n, 1), sp_n, a.workload, a.dtype),
         "cpu_baseline": None,
+        "graphs": {"substep_loop": "one hipGraph launch per step" if graph_stats else "off",
+                   "replays": graph_stats[0] if graph_stats else 0,
+                   "captures": graph_stats[1] if graph_stats else 0},
     }
     line["roofline"]["definition"] = (
         f"{DOM_PASSES_PER_ORDINATE} field passes per ordinate per source iteration "

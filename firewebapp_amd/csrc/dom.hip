@@ -20,6 +20,7 @@
 #include <string.h>
 #include <vector>
 #include "fw_common.h"
+#include "graph.h"
 
 namespace fw {
 
@@ -37,6 +38,7 @@ struct DomHandle {
   void *Ig_out, *Ig_emit, *q_in;  // (Ny, Nx)
   double *part, *part2, *S, *S_host;
   int last_iters;
+  GraphCache* gc;   // captured source iterations (non-default streams, timer off)
 };
 
 template <typename T>
@@ -559,12 +561,12 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
                      (const T*)Tsoil, T(h->eps_ground), I_amb, (T*)h->Ig_emit,
                      (T*)h->Ig_out);
   FW_CHECK_LAUNCH("dom_props");
-  int it = 0;
-  for (; it < h->maxiter; ++it) {
-    // G_prev <- G (swap)
-    void* tmp = h->G;
-    h->G = h->Gp;
-    h->Gp = tmp;
+  // One source iteration: the sweep, G with its change, the ground update
+  // and (after the first) the change's final sum.  Its launches depend only
+  // on the G buffers' order and whether it is the first, so off the legacy
+  // default stream and with the sweep timer off it is replayed from a
+  // captured graph.
+  auto iteration = [&](int first) -> int {
     if (tile_side<T>(h) > 0) {
       TRY_DOM(sweep_tiled<T>(h, I_amb, s));
     } else {
@@ -583,10 +585,34 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
                        (const double*)h->Om, (const double*)h->w, (const T*)h->I,
                        (const T*)h->Ig_emit, (T*)h->Ig_out, (T*)h->q_in,
                        T(h->eps_ground), T(h->relax));
-    FW_CHECK_LAUNCH("dom_G");
-    if (it > 0) {
+    if (!first)
       hipLaunchKernelGGL(k_dom_fin, dim3(1), dim3(256), 0, s, h->part, h->part2,
                          (int)nb(n), h->S);
+    FW_CHECK_LAUNCH("dom_G");
+    return FW_OK;
+  };
+  const bool graph = h->gc && s && !timer_on(FW_TIMER_DOM_SWEEP) && loop_graphs_on();
+  int it = 0;
+  for (; it < h->maxiter; ++it) {
+    // G_prev <- G (swap)
+    void* tmp = h->G;
+    h->G = h->Gp;
+    h->Gp = tmp;
+    // I_amb rides in the key: the launches take it by value
+    if (graph) {
+      // recording: iteration() issues to s, the recording stream meanwhile
+      TRY_DOM(h->gc->run({kw(h->G), kw(h->Gp), (uint64_t)(it == 0), kw((double)I_amb)}, s,
+                         [&](hipStream_t cs) {
+                           const hipStream_t keep = s;
+                           s = cs;
+                           const int r2 = iteration(it == 0);
+                           s = keep;
+                           return r2;
+                         }));
+    } else {
+      TRY_DOM(iteration(it == 0));
+    }
+    if (it > 0) {
       FW_CHECK_HIP(hipMemcpyAsync(h->S_host, h->S, 2 * sizeof(double),
                                   hipMemcpyDeviceToHost, s));
       FW_CHECK_HIP(hipStreamSynchronize(s));
@@ -640,6 +666,7 @@ int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* we
   FW_REQUIRE(g && Omega && weights && handle && M > 0, "dom_create: bad argument");
   FW_REQUIRE(g->nx <= 64 * 32, "dom_create: Nx > 2048 not supported by the row scan");
   DomHandle* h = new DomHandle();
+  h->gc = new GraphCache();
   h->g = *g;
   h->ga = grid_args(g);
   h->M = M;
@@ -683,6 +710,7 @@ int fw_dom_destroy(void* handle) {
   for (void* p : b)
     if (p) (void)hipFree(p);
   if (h->S_host) (void)hipHostFree(h->S_host);
+  delete h->gc;
   delete h;
   return FW_OK;
 }

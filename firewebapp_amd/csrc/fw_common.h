@@ -90,6 +90,9 @@ struct KernelEvents {
 // thread while its timer is on and the launch is one of the timed ones;
 // {null, null} otherwise (timer.hip).
 KernelEvents timer_next(int which);
+// Whether this host thread's timer of kernel `which` is on (launches that
+// record events are not captured into graphs).
+bool timer_on(int which);
 
 inline Launch cell_launch(const fw_grid* g) {
   int bx = 64;

@@ -7,11 +7,15 @@
 // 64-bit words; anything a kernel reads that changes between replays must
 // come from device memory (the step scalar block), never from a captured
 // by-value argument.  The captured sequences contain kernel launches only
-// (no memset, copy or host synchronisation).  Capture needs a stream other
-// than the legacy default stream.  Least recently used entries beyond
-// `capacity` are destroyed.
+// (no memset, copy or host synchronisation).  Recording happens on a private
+// non-blocking stream (capture executes nothing, and a capturing stream that
+// the legacy default stream synchronises with would make other threads'
+// synchronous copies fail); the instantiated graph is then launched on the
+// caller's stream, which must not be the legacy default stream.  Least
+// recently used entries beyond `capacity` are destroyed.
 #pragma once
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 #include <vector>
 
@@ -29,8 +33,12 @@ struct GraphCache {
   uint64_t clock = 0;
   int capacity = 24;
   long long hits = 0, captures = 0;
+  hipStream_t cap = nullptr;   // the recording stream, created on first use
 
-  ~GraphCache() { clear(); }
+  ~GraphCache() {
+    clear();
+    if (cap) (void)hipStreamDestroy(cap);
+  }
 
   void clear() {
     for (Entry& e : entries) (void)hipGraphExecDestroy(e.exec);
@@ -38,7 +46,8 @@ struct GraphCache {
   }
 
   // Launch the graph recorded for key on s, recording it first with
-  // record(s) (which issues the launches to s and returns an FW_* code).
+  // record(cs) (which issues the launches to the stream cs it is given and
+  // returns an FW_* code).
   template <typename F>
   int run(const std::vector<uint64_t>& key, hipStream_t s, F&& record) {
     ++clock;
@@ -49,10 +58,11 @@ struct GraphCache {
         FW_CHECK_HIP(hipGraphLaunch(e.exec, s));
         return FW_OK;
       }
-    FW_CHECK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    const int rc = record(s);
+    if (!cap) FW_CHECK_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    FW_CHECK_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    const int rc = record(cap);
     hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(s, &g);
+    const hipError_t ec = hipStreamEndCapture(cap, &g);
     if (rc) {
       if (g) (void)hipGraphDestroy(g);
       return rc;
@@ -77,6 +87,17 @@ struct GraphCache {
     return FW_OK;
   }
 };
+
+// Whether the PCG and DOM loops replay their iterations from graphs
+// (FIREWAVE_LOOP_GRAPHS=1; read at each solve).  Off by default: at
+// 256 x 64 x 48 a replayed PCG iteration (four kernels) or DOM source
+// iteration measured slower than its direct launches (2.05 against
+// 1.97 ms per step), the launch of a small graph costing more than the
+// dispatches it saves.
+inline bool loop_graphs_on() {
+  const char* e = getenv("FIREWAVE_LOOP_GRAPHS");
+  return e && e[0] == '1';
+}
 
 // Key words for pointers and doubles.
 inline uint64_t kw(const void* p) { return (uint64_t)(uintptr_t)p; }

@@ -22,6 +22,7 @@
 #include <string.h>
 #include <vector>
 #include "fw_common.h"
+#include "graph.h"
 
 namespace fw {
 
@@ -1602,6 +1603,7 @@ struct ProjHandle {
   int use_plane;       // k_yz_plane when available (else the column transforms)
   int mv_blocks;       // workgroups of the fused PCG operator (k_matvec_pz)
   int dct_rows;        // rows per k_dct4_tile workgroup (8 or 16)
+  GraphCache* gc;      // captured PCG batches (fused path, non-default streams)
 };
 
 template <typename T>
@@ -1928,51 +1930,91 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
   int queued = 0;
   const int rd = h->round < 8 ? h->round : 7;
   int batch = h->last_inner[rd] > 0 ? h->last_inner[rd] : 1;
+  // A batch on the fused path is replayed from a captured graph (its
+  // launches depend only on the batch length, whether it opens the round,
+  // the direction buffers' order and the tiling); launch = false then only
+  // advances the direction pointers as the launches would.
+  auto fused_batch = [&](T*& pp, T*& pa, int q0, int count, bool launch) -> int {
+    for (int b = 0; b < count; ++b) {
+      if (q0 + b > 0) {
+        if (launch)
+          hipLaunchKernelGGL(k_matvec_pz<T>, dim3(mvb), dim3(256), 0, s, h->ga, ir, pp, z,
+                             pa, q, (T)h->eps_reg, h->Prz, npz, h->S, h->Ppq);
+        T* t = pp;
+        pp = pa;
+        pa = t;
+      } else if (launch) {
+        hipLaunchKernelGGL(k_matvec_rows<T>, dim3(mvb), dim3(256), 0, s, h->ga, ir, pp, q,
+                           (T)h->eps_reg, h->Ppq);
+      }
+      if (launch) {
+        DctFuse<T> fw = fu;
+        fw.p = pp;
+        fw.Pout = h->Prr;
+        row_transform(update_t{}, (const T*)r, (T*)h->w1, fw);
+        const int rc2 = precond_dot(1);
+        if (rc2) return rc2;
+      }
+    }
+    FW_CHECK_LAUNCH("pcg iteration");
+    return FW_OK;
+  };
+  const bool graph = fused && h->gc && s && loop_graphs_on();
   for (;;) {
     const int nb_it = maxit - queued < batch ? maxit - queued : batch;
     batch = 1;
-    for (int b = 0; b < nb_it; ++b) {
-      if (fused && queued + b > 0) {
-        // the previous iteration's direction update rides in this operator
-        hipLaunchKernelGGL(k_matvec_pz<T>, dim3(mvb), dim3(256), 0, s, h->ga, ir, p, z,
-                           p_alt, q, (T)h->eps_reg, h->Prz, npz, h->S, h->Ppq);
-        T* t = p;
-        p = p_alt;
-        p_alt = t;
-      } else {
-        hipLaunchKernelGGL(k_matvec_rows<T>, dim3(mvb), dim3(256), 0, s, h->ga, ir, p, q,
-                           (T)h->eps_reg, h->Ppq);
-      }
-      if (fused) {
-        DctFuse<T> fw = fu;
-        fw.p = p;
-        fw.Pout = h->Prr;
-        row_transform(update_t{}, (const T*)r, (T*)h->w1, fw);
-        if ((rc = precond_dot(1))) return rc;
-      } else {
-        hipLaunchKernelGGL(k_update_xr2<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
-                           h->Prz, npz, h->Ppq, h->S, h->Prr);
-        // learned mode: the network corrects every learned_every-th
-        // direction (its training inputs are residuals of every stage of a
-        // solve), the others use the separable inverse; the flexible beta
-        // keeps the directions conjugate to the last one
-        const int itn = queued + b + 1;
-        const bool use_net = flex && h->learned_every > 0 && itn % h->learned_every == 0;
-        if ((rc = flex && !use_net ? precond_apply<T>(h, r, z, s) : apply_M<T>(h, r, z, s)))
+    if (graph) {
+      // one graph per iteration kind (opening the round or not, direction
+      // buffers' order): at most four recordings per tiling
+      for (int b = 0; b < nb_it; ++b) {
+        const int q0 = queued + b;
+        const std::vector<uint64_t> key = {kw(p), kw(p_alt), (uint64_t)(q0 == 0),
+                                           (uint64_t)mvb, (uint64_t)h->dct_rows,
+                                           (uint64_t)h->use_plane};
+        T* const p0 = p;
+        T* const pa0 = p_alt;
+        // recording: the launch helpers above issue to s, so s is the
+        // recording stream for the duration
+        if ((rc = h->gc->run(key, s, [&](hipStream_t cs) {
+               const hipStream_t keep = s;
+               s = cs;
+               T* pp = p0;
+               T* pa = pa0;
+               const int r2 = fused_batch(pp, pa, q0, 1, true);
+               s = keep;
+               return r2;
+             })))
           return rc;
-        if (flex)
-          hipLaunchKernelGGL(k_dot2_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, q, z,
-                             h->Prz, h->Pqz, h->Prr, h->S, 1);
-        else
-          hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z,
-                             h->Prz, h->Prr, h->S, 1);
+        if ((rc = fused_batch(p, p_alt, q0, 1, false))) return rc;
       }
-      if (flex)
+    } else if (fused) {
+      if ((rc = fused_batch(p, p_alt, queued, nb_it, true))) return rc;
+    }
+    // the other preconditioners: direct launches
+    for (int b = 0; b < nb_it && !fused; ++b) {
+      hipLaunchKernelGGL(k_matvec_rows<T>, dim3(mvb), dim3(256), 0, s, h->ga, ir, p, q,
+                         (T)h->eps_reg, h->Ppq);
+      hipLaunchKernelGGL(k_update_xr2<T>, dim3(PJ_RED), dim3(256), 0, s, n, d, r, p, q,
+                         h->Prz, npz, h->Ppq, h->S, h->Prr);
+      // learned mode: the network corrects every learned_every-th
+      // direction (its training inputs are residuals of every stage of a
+      // solve), the others use the separable inverse; the flexible beta
+      // keeps the directions conjugate to the last one
+      const int itn = queued + b + 1;
+      const bool use_net = flex && h->learned_every > 0 && itn % h->learned_every == 0;
+      if ((rc = flex && !use_net ? precond_apply<T>(h, r, z, s) : apply_M<T>(h, r, z, s)))
+        return rc;
+      if (flex) {
+        hipLaunchKernelGGL(k_dot2_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, q, z,
+                           h->Prz, h->Pqz, h->Prr, h->S, 1);
         hipLaunchKernelGGL(k_p_update_flex<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Pqz,
                            h->S);
-      else if (!fused)
+      } else {
+        hipLaunchKernelGGL(k_dot_rows<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, r, z,
+                           h->Prz, h->Prr, h->S, 1);
         hipLaunchKernelGGL(k_p_update2<T>, dim3(PJ_RED), dim3(256), 0, s, n, p, z, h->Prz,
                            npz, h->S, 0);
+      }
     }
     queued += nb_it;
     FW_CHECK_LAUNCH("pcg iteration");
@@ -2172,6 +2214,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   h->use_plane = 1;
   h->mv_blocks = PJ_MV_DEFAULT;
   h->dct_rows = 16;
+  h->gc = new GraphCache();
   const int Nx = g->nx, Ny = g->ny, Nz = g->nz;
   // host copies of the dz metric for folding sqrt(dz) into the z matrices
   std::vector<double> dz(Nz);
@@ -2338,6 +2381,7 @@ int fw_proj_destroy(void* handle) {
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (h->S_host) (void)hipHostFree(h->S_host);
+  delete h->gc;
   delete h;
   return FW_OK;
 }

@@ -22,6 +22,10 @@ struct KernelTimer {
 thread_local KernelTimer g_kt[FW_TIMER_KINDS];
 }  // namespace
 
+bool timer_on(int which) {
+  return which >= 0 && which < FW_TIMER_KINDS && g_kt[which].on;
+}
+
 KernelEvents timer_next(int which) {
   KernelEvents e;
   if (which < 0 || which >= FW_TIMER_KINDS) return e;
