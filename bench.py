@@ -41,6 +41,10 @@ steps, inputs resident in HBM.  The JSON line also carries:
   iterative refinement): plain PCG (the separable preconditioner),
   Jacobi-PCG and the learned preconditioner, iterations and wall time per
   system;
+* sweep_c4: the 48-case batched wind x moisture sweep (300 steps each, 4
+  cases in flight), whole-sweep cell-steps/s;
+* autotune: the launch shapes the autotuner picked for the grid, with the
+  timings behind the choice;
 * cpu_baseline: the oracle's float64 NumPy step on one host core over
   steps 1 .. dom_every of the same workload (one DOM solve among them, the
   run's cadence), with the host's CPU model and CPU count (rank 0, N = 1).
@@ -263,6 +267,39 @@ TIMER_SPECIES3, TIMER_DOM_SWEEP = 0, 1
 SPECIES_STRIDE, SPECIES_STEPS = 4, 4
 
 
+def sweep_c4():
+    """The batched sweep: the 48 uneven cases of firewebapp_amd/data/sweep_c4.txt
+    (64-256 x 32-64 x 32-48 cells, U10 1-7 m/s, moisture 2-12 %, two fuel
+    loads, 300 steps each) run 4 at a time, each on its own stream and host
+    thread; the figure is the sum of cells x steps over the wall time of the
+    whole sweep."""
+    import torch
+    from firewebapp_amd.batch import run_batch
+    deck = os.path.join(ROOT, "firewebapp_amd", "data", "sweep_c4.txt")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    recs = run_batch(deck)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total = sum(r["cells"] * r["steps"] for r in recs)
+    sizes = sorted({(r["Nx"], r["Ny"], r["Nz"]) for r in recs})
+    print(f"# sweep: {len(recs)} cases in {wall:.2f} s, {total / wall:.3e} cell-steps/s",
+          file=sys.stderr, flush=True)
+    return {"cases": len(recs), "steps_each": 300, "concurrent": 4, "wall_s": wall,
+            "cell_steps": total, "value": total / wall, "unit": "cell-steps/s",
+            "grids": [list(s) for s in sizes]}
+
+
+def tune_shapes(sv):
+    """The autotuner's choice of launch shapes for the workload's grid,
+    timed on the solver's current state (firewebapp_amd.autotune)."""
+    from firewebapp_amd.autotune import tune
+    c = tune(sv, retune=True)
+    print(f"# autotune ({c['tuning_s']} s): projection {c['proj']}, DOM tile {c['dom_tile']}, "
+          f"species CZ {c['species_cz']}", file=sys.stderr, flush=True)
+    return c
+
+
 def kernel_timer(which, stride):
     """Start (reset) with the given stride, or stop (0), the HIP-event
     timing of a kernel's launches; the events are recorded by the launches
@@ -325,11 +362,15 @@ def cpu_baseline(name, dom_every):
             "step_s": times}
 
 
-def _roofline(kernel, bytes_launch, ms, launches, workload, dtype):
+def _roofline(kernel, bytes_launch, ms, launches, workload, dtype, shape_key=None):
+    """shape_key: the launch shape the counter figure in ROOFLINE_TRAFFIC
+    was measured at, when the kernel's traffic depends on it (the DOM tile
+    side); None when it was not that shape."""
     achieved = bytes_launch / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    traffic = ROOFLINE_TRAFFIC.get((kernel, workload, dtype)) if shape_key is not False else None
     return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": ROOFLINE_TRAFFIC.get((kernel, workload, dtype)),
+            "traffic": traffic,
             "bytes_per_launch": bytes_launch, "mean_launch_ms": ms, "launches": launches}
 
 
@@ -350,6 +391,13 @@ def main(argv=None):
                     help="projection systems for pcg_time_to_1e8 (0 = skip)")
     ap.add_argument("--early-window", type=int, default=1,
                     help="also time the first warmup + steps of a fresh run (0 = skip)")
+    ap.add_argument("--sweep", type=int, default=1,
+                    help="1: also run the 48-case wind x moisture sweep "
+                         "(firewebapp_amd/data/sweep_c4.txt, 300 steps each, 4 cases in "
+                         "flight) and report its whole-sweep cell-steps/s as sweep_c4")
+    ap.add_argument("--tune", type=int, default=1,
+                    help="1: run the autotuner on the spun-up state before the warm-up "
+                         "(untimed) and report its choice; 0: the default launch shapes")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -379,11 +427,16 @@ def main(argv=None):
           f"timed {a.steps} steps", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t_run0 = time.perf_counter()
+    tuned = None
     for i in range(spinup + a.warmup):
         if sv.step() is not None:
             raise RuntimeError(f"run stopped early at step {sv.step_no}: {sv.stop_reason}")
         if (i + 1) % 100 == 0:
             print(f"# step {i + 1}, t = {sv.t:.4f} s", file=sys.stderr, flush=True)
+        if i + 1 == spinup and a.tune:
+            tuned = tune_shapes(sv)
+    if spinup == 0 and a.tune:
+        tuned = tune_shapes(sv)
     t_win0 = sv.t
     print(f"# spin-up and warm-up done: {spinup + a.warmup} steps, t = {sv.t:.4f} s",
           file=sys.stderr, flush=True)
@@ -413,7 +466,7 @@ def main(argv=None):
     sv.fields.graphs = graphs
     esz = 4 if a.dtype == "float32" else 8
     M = sv.dom.M
-    TS = 4
+    TS = int(tuned["dom_tile"]) if tuned else 4     # the sweep's tile side
     sweep_launches_per_iter = -(-g.Ny // TS) + -(-g.Nz // TS) - 1
     dom_bytes = DOM_PASSES_PER_ORDINATE * esz * M * cells / sweep_launches_per_iter
     dom_ms, dom_n = kernel_timer_read(TIMER_DOM_SWEEP)
@@ -440,10 +493,11 @@ def main(argv=None):
                      "value": cells * total_steps / (t1 - t_run0),
                      "ms_per_step": 1e3 * (t1 - t_run0) / total_steps},
         "roofline": _roofline("k_dom_sweep_tile", dom_bytes, dom_ms / max(dom_n, 1), dom_n,
-                              a.workload, a.dtype),
+                              a.workload, a.dtype, shape_key=TS == 4),
         "roofline_species3": _roofline("k_species3", SPECIES3_FIELDS * esz * cells,
                                        sp_ms / max(sp_n, 1), sp_n, a.workload, a.dtype),
         "cpu_baseline": None,
+        "autotune": tuned,
         "graphs": {"substep_loop": "one hipGraph launch per step" if graph_stats else "off",
                    "replays": graph_stats[0] if graph_stats else 0,
                    "captures": graph_stats[1] if graph_stats else 0},
@@ -482,6 +536,8 @@ def main(argv=None):
         sv.close()
         del sv
         torch.cuda.empty_cache()
+    if a.sweep:
+        line["sweep_c4"] = sweep_c4()
     if a.cpu_budget > 0:
         print("# timing the oracle on the host", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(a.workload, cfg.dom_subcycle_every)

@@ -120,12 +120,17 @@ def batch_table(ri: RomInputs) -> List[BatchCase]:
     return out
 
 
-def _run_case(base: RomInputs, case: BatchCase, snapshot_dir=None):
+def _run_case(base: RomInputs, case: BatchCase, snapshot_dir=None, tune=False):
     import torch
     from .spread_3d import Spread3DConfig, Spread3DSolver
     cfg = Spread3DConfig.resolve(case.inputs(base), wind_speed_m_s=case.wind_speed_m_s)
     sv = Spread3DSolver(cfg)
     try:
+        if tune:
+            # the launch shapes of the case's grid (timed once per shape per
+            # process and cached; firewebapp_amd.autotune)
+            from .autotune import tune as _tune
+            _tune(sv)
         torch.cuda.current_stream().synchronize()
         t0 = time.perf_counter()
         sv.run(snapshot_dir=snapshot_dir)
@@ -143,12 +148,13 @@ def _run_case(base: RomInputs, case: BatchCase, snapshot_dir=None):
         sv.close()
 
 
-def run_batch(deck, concurrent=None, snapshot_root=None, on_done=None):
+def run_batch(deck, concurrent=None, snapshot_root=None, on_done=None, tune=False):
     """Run every case of the deck's batch table; returns one record per case
     (table order) with its size, steps, wall time, cell-steps/s, ROS, stop
     reason and Spread3DResult.  ``concurrent`` cases run at once (default
     ``batch.concurrent``, else 4), each on its own stream; the largest cases
-    start first so the tail of the sweep is short."""
+    start first so the tail of the sweep is short.  tune: pick each grid
+    shape's launch shapes with the autotuner before its first case runs."""
     import torch
     base = load_text_input(deck) if not isinstance(deck, RomInputs) else deck
     cases = batch_table(base)
@@ -185,7 +191,7 @@ def run_batch(deck, concurrent=None, snapshot_root=None, on_done=None):
                     if snapshot_root is not None:
                         from pathlib import Path
                         sd = Path(snapshot_root) / f"case_{c.index:04d}"
-                    rec = _run_case(base, c, sd)
+                    rec = _run_case(base, c, sd, tune)
                 except BaseException as e:    # re-raised on the caller's thread
                     with lock:
                         errors.append(e)
@@ -215,6 +221,7 @@ def main(argv=None):
     ap.add_argument("--cases", type=int, default=None, help="run only the first N cases")
     ap.add_argument("--steps", type=int, default=None, help="override batch n_steps")
     ap.add_argument("--out", default=None, help="write the table as JSON")
+    ap.add_argument("--tune", action="store_true", help="autotune each grid shape first")
     a = ap.parse_args(argv)
     base = load_text_input(a.deck)
     if a.steps is not None:
@@ -228,7 +235,7 @@ def main(argv=None):
               f" {r['steps']} steps {r['wall_s']:.2f} s {r['cell_steps_per_s']:.3e} cell-steps/s"
               f" ROS {r['ros_m_s']:.3f} m/s ({r['stop_reason']})", flush=True)
 
-    recs = run_batch(base, concurrent=a.concurrent, on_done=done)
+    recs = run_batch(base, concurrent=a.concurrent, on_done=done, tune=a.tune)
     wall = time.perf_counter() - t0
     total = sum(r["cells"] * r["steps"] for r in recs)
     summary = {"cases": len(recs), "cell_steps": total, "wall_s": wall,

@@ -39,6 +39,7 @@ struct DomHandle {
   double *part, *part2, *S, *S_host;
   int last_iters;
   GraphCache* gc;   // captured source iterations (non-default streams, timer off)
+  int tile;         // requested sweep tile side (0: the default 4)
 };
 
 template <typename T>
@@ -487,9 +488,15 @@ static void launch_tile(DomHandle* h, int D, int tjlo, int ntiles, T I_amb, hipS
 // kernel).  8 x 8 tiles halve the launches but not the time: at 256 x 64 x
 // 48, S8, a source iteration is VALU-throughput-bound across the tiles of a
 // diagonal (13 launches of 29.2 us against 27 of 13.6 us).
+// The handle's requested side (fw_dom_set_tiling: 2, 4 or 8; 0 = 4) is used
+// when its TS x TS rows fit 64 KB of LDS (and, past 512 cells a row, only
+// the side 4 has a kernel); the per-cell arithmetic does not depend on the
+// side, so every side gives the same intensities.
 template <typename T>
 static int tile_side(const DomHandle* h) {
   const size_t row = (size_t)h->g.nx * sizeof(T);
+  const int want = h->tile ? h->tile : 4;
+  if (want != 4 && h->g.nx <= 512 && (size_t)want * want * row <= 65536) return want;
   return 16 * row <= 65536 ? 4 : 0;
 }
 
@@ -513,8 +520,15 @@ static int sweep_tiled_ts(DomHandle* h, T I_amb, hipStream_t s) {
 }
 
 template <typename T>
+int tile_side_of(const DomHandle* h) { return tile_side<T>(h); }
+
+template <typename T>
 static int sweep_tiled(DomHandle* h, T I_amb, hipStream_t s) {
-  return sweep_tiled_ts<T, 4>(h, I_amb, s);
+  switch (tile_side<T>(h)) {
+    case 2: return sweep_tiled_ts<T, 2>(h, I_amb, s);
+    case 8: return sweep_tiled_ts<T, 8>(h, I_amb, s);
+    default: return sweep_tiled_ts<T, 4>(h, I_amb, s);
+  }
 }
 
 // Cells per lane: ceil(Nx / 64) rounded up to a power of two (Nx <= 2048).
@@ -661,6 +675,7 @@ extern "C" {
 int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* weights,
                   double eps_w_ground, int max_source_iter, double tol_source_iter,
                   double omega_relax, double kappa_gas_max, void** handle) {
+  LegacyScope legacy_scope;
   FW_GRID(g);
   FW_GRID_OFFSETS32(g);
   FW_REQUIRE(g && Omega && weights && handle && M > 0, "dom_create: bad argument");
@@ -703,6 +718,7 @@ int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* we
 }
 
 int fw_dom_destroy(void* handle) {
+  LegacyScope legacy_scope;
   DomHandle* h = (DomHandle*)handle;
   if (!h) return FW_OK;
   void* b[] = {h->Om, h->w, h->I, h->G, h->Gp, h->kap, h->kap_s, h->kap_safe, h->B,
@@ -733,6 +749,17 @@ int fw_dom_solve(void* handle, const void* T_s, const void* T_g, const void* alp
              : dom_solve_impl<float>(h, T_s, T_g, alpha_s, omega_comb, sigma_sav, T_amb,
                                      q_rad_solid_out, q_rad_gas_out, T_soil_surface,
                                      q_in_soil_out, Yw, rh, bed_moisture, s);
+}
+
+int fw_dom_set_tiling(void* handle, int tile_side, int* active) {
+  DomHandle* h = (DomHandle*)handle;
+  FW_REQUIRE(h, "dom_set_tiling: null handle");
+  FW_REQUIRE(tile_side == 0 || tile_side == 2 || tile_side == 4 || tile_side == 8,
+             "dom_set_tiling: tile side must be 0 (default), 2, 4 or 8");
+  h->tile = tile_side;
+  if (active)
+    *active = h->g.dtype == FW_F64 ? tile_side_of<double>(h) : tile_side_of<float>(h);
+  return FW_OK;
 }
 
 int fw_dom_stats(void* handle, int* iters) {

@@ -27,6 +27,22 @@ struct Launch {
 // driver: Q_comb from the array, or (Q_comb == nullptr) formed from the
 // drip-torch gas share and q_rad_gas / dz with the solid share subtracted
 // from Q_pyro in the source bed cells first.
+// HIP refuses a synchronous copy or set on the legacy default stream while
+// any stream of the process records a graph.  Graph recording (GraphCache)
+// holds this lock exclusively; the entry points that issue such copies
+// (handle creation and release) hold it shared, so on concurrent host
+// threads the two never overlap.  Scopes do not nest.
+void record_lock(bool exclusive);
+void record_unlock(bool exclusive);
+struct LegacyScope {
+  LegacyScope() { record_lock(false); }
+  ~LegacyScope() { record_unlock(false); }
+};
+struct RecordScope {
+  RecordScope() { record_lock(true); }
+  ~RecordScope() { record_unlock(true); }
+};
+
 // Device-resident step scalars of the sub-step loop (a graph-captured loop
 // replays the same kernel arguments every step, so the values that change
 // from step to step are read from this block instead).  Each slot holds the

@@ -58,22 +58,25 @@ struct GraphCache {
         FW_CHECK_HIP(hipGraphLaunch(e.exec, s));
         return FW_OK;
       }
-    if (!cap) FW_CHECK_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-    FW_CHECK_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-    const int rc = record(cap);
-    hipGraph_t g = nullptr;
-    const hipError_t ec = hipStreamEndCapture(cap, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    if (ec != hipSuccess || !g)
-      return fw_set_error(FW_E_HIP, "graph capture: %s", hipGetErrorString(ec));
     hipGraphExec_t ex = nullptr;
-    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ei != hipSuccess)
-      return fw_set_error(FW_E_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+    {
+      RecordScope lock;   // no legacy-stream copies elsewhere meanwhile
+      if (!cap) FW_CHECK_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      FW_CHECK_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+      const int rc = record(cap);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(cap, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (ec != hipSuccess || !g)
+        return fw_set_error(FW_E_HIP, "graph capture: %s", hipGetErrorString(ec));
+      const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess)
+        return fw_set_error(FW_E_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+    }
     ++captures;
     if ((int)entries.size() >= capacity) {
       size_t lru = 0;

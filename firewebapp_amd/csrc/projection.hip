@@ -2193,6 +2193,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
                    const double* Uz, const double* lam_x, const double* lam_y,
                    const double* lam_z, double eps_reg, double cg_rtol,
                    int maxiter, void** handle) {
+  LegacyScope legacy_scope;
   FW_GRID(g);
   FW_GRID_OFFSETS32(g);
   FW_REQUIRE(g && handle && Ux && Uy && Uz && lam_x && lam_y && lam_z,
@@ -2372,6 +2373,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
 }
 
 int fw_proj_destroy(void* handle) {
+  LegacyScope legacy_scope;
   ProjHandle* h = (ProjHandle*)handle;
   if (!h) return FW_OK;
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,

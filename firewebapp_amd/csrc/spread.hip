@@ -208,6 +208,7 @@ int fw_graphs_create(void** graphs) {
 }
 
 int fw_graphs_destroy(void* graphs) {
+  LegacyScope legacy_scope;
   delete (GraphCache*)graphs;
   return FW_OK;
 }

@@ -9,6 +9,9 @@
 // (the reference double-buffers through dY/du temporaries, so the results
 // are identical) and every expression keeps the reference's operation order
 // so that the FP64 instantiation reproduces the reference bit-for-bit.
+#include <map>
+#include <mutex>
+
 #include "fw_common.h"
 
 namespace fw {
@@ -623,13 +626,58 @@ static int species_impl(const fw_grid* g, const void* Y, void* Yo,
 }
 
 
+// Cells per thread of k_species3 per grid shape (fw_tune_species_cz; 2 when
+// the shape has no entry).  Every CZ computes each cell with the same
+// arithmetic, so the choice changes speed only.
+namespace {
+struct ShapeKey {
+  int nx, ny, nz, dtype;
+  bool operator<(const ShapeKey& o) const {
+    if (nx != o.nx) return nx < o.nx;
+    if (ny != o.ny) return ny < o.ny;
+    if (nz != o.nz) return nz < o.nz;
+    return dtype < o.dtype;
+  }
+};
+std::mutex g_cz_mu;
+std::map<ShapeKey, int> g_cz;
+}  // namespace
+
+static int species_cz(const fw_grid* g) {
+  std::lock_guard<std::mutex> lock(g_cz_mu);
+  const auto it = g_cz.find(ShapeKey{g->nx, g->ny, g->nz, g->dtype});
+  return it == g_cz.end() ? 2 : it->second;
+}
+
+template <typename T, int CZ>
+static int species3_cz(const fw_grid* g, const void* const* Y, void* const* Yo,
+                       const void* rho, const void* u, const void* v, const void* w,
+                       const void* Sp, const void* Sd, double dt, double D,
+                       const double* y_in, const double* y_max, hipStream_t s,
+                       const double* sb);
+
 template <typename T>
 static int species3_impl(const fw_grid* g, const void* const* Y, void* const* Yo,
                          const void* rho, const void* u, const void* v, const void* w,
                          const void* Sp, const void* Sd, double dt, double D,
                          const double* y_in, const double* y_max, hipStream_t s,
                          const double* sb) {
-  constexpr int CZ = 2;
+  switch (species_cz(g)) {
+    case 1:
+      return species3_cz<T, 1>(g, Y, Yo, rho, u, v, w, Sp, Sd, dt, D, y_in, y_max, s, sb);
+    case 4:
+      return species3_cz<T, 4>(g, Y, Yo, rho, u, v, w, Sp, Sd, dt, D, y_in, y_max, s, sb);
+    default:
+      return species3_cz<T, 2>(g, Y, Yo, rho, u, v, w, Sp, Sd, dt, D, y_in, y_max, s, sb);
+  }
+}
+
+template <typename T, int CZ>
+static int species3_cz(const fw_grid* g, const void* const* Y, void* const* Yo,
+                       const void* rho, const void* u, const void* v, const void* w,
+                       const void* Sp, const void* Sd, double dt, double D,
+                       const double* y_in, const double* y_max, hipStream_t s,
+                       const double* sb) {
   Launch L = cell_launch(g);
   L.grid.z = (g->nz + CZ - 1) / CZ;
   // the event-recording launch costs about twice the host time of a plain
@@ -839,6 +887,17 @@ int fw_species_transport_dilution(const fw_grid* g, const void* Y, void* Y_out,
   return FW_DISPATCH(g, species_impl, g, Y, Y_out, rho, u, v, w, nullptr,
                      S_pyro_part, S_dry_part, kind, dt, D, Y_inlet, y_max,
                      (hipStream_t)stream);
+}
+
+int fw_tune_species_cz(const fw_grid* g, int cz) {
+  FW_REQUIRE(g, "tune_species_cz: null grid");
+  FW_REQUIRE(cz == 0 || cz == 1 || cz == 2 || cz == 4,
+             "tune_species_cz: cells per thread must be 0 (default), 1, 2 or 4");
+  std::lock_guard<std::mutex> lock(g_cz_mu);
+  const ShapeKey key{g->nx, g->ny, g->nz, g->dtype};
+  if (cz == 0) g_cz.erase(key);
+  else g_cz[key] = cz;
+  return FW_OK;
 }
 
 int fw_species_transport3(const fw_grid* g, const void* const* Y, void* const* Y_out,

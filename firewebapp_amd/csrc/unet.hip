@@ -956,6 +956,7 @@ int fw_net_nparams(void) {
 }
 
 int fw_net_create(int nx, int ny, int nz, const float* theta_host, void** handle) {
+  LegacyScope legacy_scope;
   FW_REQUIRE(handle && theta_host, "net_create: null argument");
   FW_REQUIRE(nx >= 8 && ny >= 8 && nz >= 8 && nx % 8 == 0 && ny % 8 == 0 && nz % 8 == 0,
              "net_create: grid dims must be multiples of 8 (got %d x %d x %d)", nx, ny, nz);
@@ -989,6 +990,7 @@ int fw_net_create(int nx, int ny, int nz, const float* theta_host, void** handle
 }
 
 int fw_net_destroy(void* handle) {
+  LegacyScope legacy_scope;
   NetHandle* h = (NetHandle*)handle;
   if (!h) return FW_OK;
   void* bufs[] = {h->theta, h->grad, h->m,   h->v,   h->wr,  h->wpack, h->e0,  h->pe0,
@@ -1003,6 +1005,7 @@ int fw_net_destroy(void* handle) {
 }
 
 int fw_net_get_params(void* handle, float* theta_host) {
+  LegacyScope legacy_scope;
   NetHandle* h = (NetHandle*)handle;
   FW_REQUIRE(h && theta_host, "net_get_params: null argument");
   FW_CHECK_HIP(hipMemcpy(theta_host, h->theta, h->nparam * sizeof(float),
@@ -1021,6 +1024,7 @@ int fw_net_forward(void* handle, const void* r, const void* q, void* c_out, void
 }
 
 int fw_net_set_params(void* handle, const float* theta_host) {
+  LegacyScope legacy_scope;
   NetHandle* h = (NetHandle*)handle;
   FW_REQUIRE(h && theta_host, "net_set_params: null argument");
   FW_CHECK_HIP(hipMemcpy(h->theta, theta_host, h->nparam * sizeof(float),

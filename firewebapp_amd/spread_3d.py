@@ -560,10 +560,13 @@ class Spread3DSolver:
     def __init__(self, cfg: Spread3DConfig, grid: Optional[Grid3D] = None,
                  graphs: Optional[bool] = None):
         """graphs: replay the step's sub-step loop from captured hipGraphs
-        (default: on unless FIREWAVE_GRAPHS=0); bit-identical to direct
-        launches.  With graphs on and the caller on the default stream, the
-        steps run on a stream of the solver's own, ordered against the
-        caller's stream on entry and exit."""
+        (default: off unless FIREWAVE_GRAPHS=1); bit-identical to direct
+        launches, and on MI355X no faster (256 x 64 x 48: 1.972 against
+        1.970 ms per step; the 48-case sweep: 10.9 against 10.0 s, the
+        recordings of every case's launch sequences costing more than the
+        dispatches saved).  With graphs on and the caller on the default
+        stream, the steps run on a stream of the solver's own, ordered
+        against the caller's stream on entry and exit."""
         self.cfg = cfg
         self.proj = None
         self._graphs = None
@@ -700,7 +703,7 @@ class Spread3DSolver:
                         ("counters", self.counters), ("edc_work", self.edc_work)):
             setattr(fs, name, t.data_ptr())
         if graphs is None:
-            graphs = os.environ.get("FIREWAVE_GRAPHS", "1") != "0"
+            graphs = os.environ.get("FIREWAVE_GRAPHS", "0") == "1"
         self._stream = None
         if graphs:
             h = C.c_void_p()

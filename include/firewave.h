@@ -94,6 +94,12 @@ int fw_species_transport3(const fw_grid* g, const void* const* Y, void* const* Y
                           double D, const double* Y_inlet, const double* y_max,
                           void* stream);
 
+/* Vertically adjacent cells per thread of fw_species_transport3 for grids of
+ * g's shape and type (no counterpart in the reference): 1, 2 or 4; 0 restores
+ * the default 2.  Every choice computes each cell identically.  Process
+ * wide; picked per shape by the autotuner (firewebapp_amd/autotune.py). */
+int fw_tune_species_cz(const fw_grid* g, int cz);
+
 /* _adv_gas_energy (spread_3d.py:3827-3897): MUSCL advection with inlet ghost
  * T_amb + alpha_th FV diffusion, then clamp to [T_amb, 1e4]; out of place. */
 int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
@@ -313,6 +319,14 @@ int fw_dom_solve(void* handle, const void* T_s, const void* T_g,
 
 /* Source iterations of the last solve. */
 int fw_dom_stats(void* handle, int* iters);
+
+/* Tile side of the sweep (no counterpart in the reference, whose sweep is a
+ * Numba loop): 2, 4 or 8 rows square per workgroup (0 = 4), used when those
+ * rows fit 64 KB of LDS (and only 4 past 512 cells a row).  The intensities
+ * are the same for every side.  *active (may be NULL): the side a float32 /
+ * float64 handle of this grid then uses (0: the per-row kernel).  Picked
+ * per grid by the autotuner (firewebapp_amd/autotune.py). */
+int fw_dom_set_tiling(void* handle, int tile_side, int* active);
 
 /* Device pointer of I_set (M, Nz, Ny, Nx) in the grid's type (read by
  * flame_front_3d.compute_q_dom_fwd_at_band, flame_front_3d.py:594-630). */
