@@ -53,6 +53,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -257,14 +258,26 @@ def deck(name, dtype):
     return RomInputs(outdoor_overrides=ovr)
 
 
-# kernels timed with HIP events: every DOM sweep launch of the timed steps
-# records its events (the event-recording launch costs extra host time, and
-# a timed launch is never captured into a graph, so the DOM solves of the
-# timed steps run as direct launches); every SPECIES_STRIDE-th k_species3
+# kernels timed with HIP events: every dom_stride()-th DOM sweep launch of
+# the timed steps records its events (event-recording launches slow the
+# step: by 0.19 ms per step at 256x64x48 when every sweep launch records,
+# so one launch in DOM_STRIDE_MIN or more records; a timed
+# launch is never captured into a graph); every SPECIES_STRIDE-th k_species3
 # launch of SPECIES_STEPS extra steps after them, with the step's graph
 # replay off (the sub-step loop is otherwise one graph launch)
 TIMER_SPECIES3, TIMER_DOM_SWEEP = 0, 1
 SPECIES_STRIDE, SPECIES_STEPS = 4, 4
+DOM_STRIDE_MIN = 8
+
+
+def dom_stride(launches_per_iteration):
+    """The DOM timer's stride: the first one from DOM_STRIDE_MIN that is
+    coprime with the sweep launches per source iteration, so the sampled
+    launches cycle through every anti-diagonal position (their sizes differ)."""
+    s = DOM_STRIDE_MIN
+    while math.gcd(s, launches_per_iteration) != 1:
+        s += 1
+    return s
 
 
 def sweep_c4():
@@ -440,7 +453,9 @@ def main(argv=None):
     t_win0 = sv.t
     print(f"# spin-up and warm-up done: {spinup + a.warmup} steps, t = {sv.t:.4f} s",
           file=sys.stderr, flush=True)
-    kernel_timer(TIMER_DOM_SWEEP, 1)
+    TS = int(tuned["dom_tile"]) if tuned else 4     # the sweep's tile side
+    sweep_launches_per_iter = -(-g.Ny // TS) + -(-g.Nz // TS) - 1
+    kernel_timer(TIMER_DOM_SWEEP, dom_stride(sweep_launches_per_iter))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -466,8 +481,6 @@ def main(argv=None):
     sv.fields.graphs = graphs
     esz = 4 if a.dtype == "float32" else 8
     M = sv.dom.M
-    TS = int(tuned["dom_tile"]) if tuned else 4     # the sweep's tile side
-    sweep_launches_per_iter = -(-g.Ny // TS) + -(-g.Nz // TS) - 1
     dom_bytes = DOM_PASSES_PER_ORDINATE * esz * M * cells / sweep_launches_per_iter
     dom_ms, dom_n = kernel_timer_read(TIMER_DOM_SWEEP)
     sp_ms, sp_n = kernel_timer_read(TIMER_SPECIES3)
@@ -505,6 +518,8 @@ def main(argv=None):
     line["roofline"]["definition"] = (
         f"{DOM_PASSES_PER_ORDINATE} field passes per ordinate per source iteration "
         f"x {M} ordinates x {cells} cells x {esz} B / {sweep_launches_per_iter} launches")
+    line["roofline"]["sampled"] = (f"every {dom_stride(sweep_launches_per_iter)}th sweep "
+                                   f"launch of the timed steps")
     bpc = step_bytes_per_cell(esz, 10, M, cfg.dom_subcycle_every, sv.n_particles, cells)
     ach = bpc * line["value"] / 1e9
     line["step_roofline"] = {

@@ -40,6 +40,10 @@ struct DomHandle {
   int last_iters;
   GraphCache* gc;   // captured source iterations (non-default streams, timer off)
   int tile;         // requested sweep tile side (0: the default 4)
+  // the stream a step's DOM solve runs on beside the step (dom_fork), and
+  // the events that order it; created on first use
+  hipStream_t side;
+  hipEvent_t ev_fork, ev_join;
 };
 
 template <typename T>
@@ -666,6 +670,33 @@ k_dom_fwd(size_t n, int M, const double* __restrict__ Om, const double* __restri
   out[c] = s;
 }
 
+int dom_fork(void* dom, hipStream_t s, hipStream_t* side) {
+  DomHandle* h = (DomHandle*)dom;
+  FW_REQUIRE(h && side, "dom_fork: null argument");
+  if (!h->side) {
+    LegacyScope legacy_scope;
+    FW_CHECK_HIP(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    FW_CHECK_HIP(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    FW_CHECK_HIP(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+  }
+  FW_CHECK_HIP(hipEventRecord(h->ev_fork, s));
+  FW_CHECK_HIP(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+  *side = h->side;
+  return FW_OK;
+}
+
+int dom_join_record(void* dom) {
+  DomHandle* h = (DomHandle*)dom;
+  FW_CHECK_HIP(hipEventRecord(h->ev_join, h->side));
+  return FW_OK;
+}
+
+int dom_join(void* dom, hipStream_t s) {
+  DomHandle* h = (DomHandle*)dom;
+  FW_CHECK_HIP(hipStreamWaitEvent(s, h->ev_join, 0));
+  return FW_OK;
+}
+
 }  // namespace fw
 
 using namespace fw;
@@ -727,6 +758,12 @@ int fw_dom_destroy(void* handle) {
     if (p) (void)hipFree(p);
   if (h->S_host) (void)hipHostFree(h->S_host);
   delete h->gc;
+  if (h->side) {
+    (void)hipStreamSynchronize(h->side);
+    (void)hipEventDestroy(h->ev_fork);
+    (void)hipEventDestroy(h->ev_join);
+    (void)hipStreamDestroy(h->side);
+  }
   delete h;
   return FW_OK;
 }

@@ -109,6 +109,19 @@ KernelEvents timer_next(int which);
 // Whether this host thread's timer of kernel `which` is on (launches that
 // record events are not captured into graphs).
 bool timer_on(int which);
+// The calling thread's timers, for a worker thread that launches on its
+// behalf to adopt (timer_adopt(null) returns the worker to its own); the two
+// threads must not launch timed kernels of one kind at the same time.
+void* timer_share();
+void timer_adopt(void* timers_of_another_thread);
+
+// The DOM solve on the handle's own stream, concurrent with the caller's
+// stream (dom.hip): dom_fork orders the side stream after the work already
+// on s and returns it; dom_join_record marks the end of the side stream's
+// work; dom_join makes s wait for it.
+int dom_fork(void* dom, hipStream_t s, hipStream_t* side);
+int dom_join_record(void* dom);
+int dom_join(void* dom, hipStream_t s);
 
 inline Launch cell_launch(const fw_grid* g) {
   int bx = 64;

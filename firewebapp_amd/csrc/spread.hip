@@ -10,7 +10,9 @@
 // each evolve or reinitialisation, and the exits.  Ping-pong fields are swapped by
 // exchanging pointers inside fw_spread_fields; the caller re-reads them.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
+#include <thread>
 
 #include "fw_common.h"
 #include "graph.h"
@@ -58,6 +60,58 @@ static int turb_substeps(double nu_t_max, double sc_t, double dt, double dx, dou
     int _rc = (call);        \
     if (_rc) return _rc;     \
   } while (0)
+
+// The step's DOM solve, run beside drag .. k-epsilon on the DOM handle's own
+// stream.  The solve waits on the host for each source iteration's residual,
+// so it runs from a worker thread and the step's thread keeps issuing the
+// flow work meanwhile.  Its inputs (T_s, T_g, alpha_s, omega, Y_H2O, rho,
+// M_local, T_soil) are final once the bed has stepped, and nothing before
+// the soil conduction reads what it writes (q_rad, q_rad_gas, q_in_soil and
+// the intensities), so the results are those of the solve issued in line
+// after k-epsilon.  FIREWAVE_DOM_BESIDE=0 issues it in line instead.
+static bool dom_beside_on() {
+  const char* e = getenv("FIREWAVE_DOM_BESIDE");
+  return !(e && e[0] == '0');
+}
+
+struct DomBeside {
+  std::thread th;
+  void* dom = nullptr;
+  int rc = FW_OK;
+  char msg[1024] = "";
+
+  template <typename Solve>
+  int start(void* dom_handle, hipStream_t s, Solve solve) {
+    hipStream_t side = nullptr;
+    int dev = 0;
+    FW_CHECK_HIP(hipGetDevice(&dev));
+    TRY(dom_fork(dom_handle, s, &side));
+    dom = dom_handle;
+    void* timers = timer_share();
+    th = std::thread([this, dev, side, timers, solve]() {
+      timer_adopt(timers);
+      int e = hipSetDevice(dev) == hipSuccess
+                  ? solve(side)
+                  : fw_set_error(FW_E_HIP, "spread_step: DOM worker cannot select device %d",
+                                 dev);
+      if (e == FW_OK) e = dom_join_record(dom);
+      if (e != FW_OK) snprintf(msg, sizeof(msg), "%s", fw_last_error());
+      rc = e;
+      timer_adopt(nullptr);
+    });
+    return FW_OK;
+  }
+  // s waits for the solve; the worker's error becomes this thread's
+  int join(hipStream_t s) {
+    if (!th.joinable()) return FW_OK;
+    th.join();
+    if (rc != FW_OK) return fw_set_error(rc, "%s", msg);
+    return dom_join(dom, s);
+  }
+  ~DomBeside() {
+    if (th.joinable()) th.join();
+  }
+};
 
 static const double SC_T = 0.7, PR_T = 0.85;
 static const double K_SOLID_GRASS = 0.20;     // solid_conduction_3d.py:52
@@ -328,6 +382,22 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                                      K_SOLID_GRASS, p->rho_solid_true, p->cp_solid, nzb, dt,
                                      f->dT_work, stream));
   }
+  // DOM every dom_every steps (:2455-2505), issued here beside the flow work
+  // or in line after k-epsilon
+  const bool dom_now = st->step_no % (p->dom_every > 0 ? p->dom_every : 1) == 0;
+  const bool beside = dom_now && dom_beside_on();
+  const fw_spread_fields fd = *f;   // the solve's buffers: the flow work swaps others
+  auto dom_solve = [dom, fd, p](hipStream_t ds) {
+    return fw_dom_solve(dom, fd.T_s, fd.T_g, fd.alpha_s, fd.omega, p->sigma_sav, p->T_amb,
+                        fd.q_rad, fd.q_rad_gas, fd.T_soil, fd.q_in_soil, fd.Y_H2O, fd.rho,
+                        p->eulerian ? nullptr : fd.M_local, ds);
+  };
+  DomBeside dom_side;
+  if (dom_now) {
+    // the particles' moisture scales kappa_solid; the Eulerian bed has none
+    if (!p->eulerian) TRY(fw_bed_aggregate(g, particles, index, nullptr, f->M_local, stream));
+    if (beside) TRY(dom_side.start(dom, s, dom_solve));
+  }
   // drag, tentative momentum (:2217-2277)
   TRY(fw_drag_force(g, f->u, f->v, f->w, f->rho, f->alpha_s, p->sigma_sav, p->canopy_C_d,
                     f->Fx, f->Fy, f->Fz, stream));
@@ -361,14 +431,9 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                    p->canopy_beta_p, p->canopy_beta_d, stream));
   SWAP(f->k, f->tmp1);
   SWAP(f->eps, f->tmp2);
-  // DOM every dom_every steps, soil every step (:2455-2510)
-  if (st->step_no % (p->dom_every > 0 ? p->dom_every : 1) == 0) {
-    // the particles' moisture scales kappa_solid; the Eulerian bed has none
-    if (!p->eulerian) TRY(fw_bed_aggregate(g, particles, index, nullptr, f->M_local, stream));
-    TRY(fw_dom_solve(dom, f->T_s, f->T_g, f->alpha_s, f->omega, p->sigma_sav, p->T_amb,
-                     f->q_rad, f->q_rad_gas, f->T_soil, f->q_in_soil, f->Y_H2O, f->rho,
-                     p->eulerian ? nullptr : f->M_local, stream));
-  }
+  // the DOM's results, then the soil every step (:2505-2510)
+  if (beside) TRY(dom_side.join(s));
+  else if (dom_now) TRY(dom_solve(s));
   TRY(fw_soil_conduction(g, p->n_soil, f->T_soil, f->q_in_soil, f->soil_dz,
                          f->soil_d_above, f->soil_d_below, dt, p->soil_k, p->soil_rho,
                          p->soil_cp, p->soil_eps, p->T_amb, stream));

@@ -20,16 +20,25 @@ struct KernelTimer {
   hipEvent_t* ev = nullptr;   // 2 * CAP events, created on first use
 };
 thread_local KernelTimer g_kt[FW_TIMER_KINDS];
+// a worker thread launching on behalf of another thread uses that thread's
+// timers (timer_adopt)
+thread_local KernelTimer* g_kt_use = nullptr;
+KernelTimer* timers() { return g_kt_use ? g_kt_use : g_kt; }
 }  // namespace
 
+void* timer_share() { return timers(); }
+void timer_adopt(void* timers_of_another_thread) {
+  g_kt_use = (KernelTimer*)timers_of_another_thread;
+}
+
 bool timer_on(int which) {
-  return which >= 0 && which < FW_TIMER_KINDS && g_kt[which].on;
+  return which >= 0 && which < FW_TIMER_KINDS && timers()[which].on;
 }
 
 KernelEvents timer_next(int which) {
   KernelEvents e;
   if (which < 0 || which >= FW_TIMER_KINDS) return e;
-  KernelTimer& t = g_kt[which];
+  KernelTimer& t = timers()[which];
   if (!t.on || t.used >= KernelTimer::CAP) return e;
   if ((t.seen++ % t.stride) != 0) return e;
   e.start = t.ev[2 * t.used];
