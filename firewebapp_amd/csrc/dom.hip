@@ -232,12 +232,29 @@ k_dom_sweep_diag(GridArgs g, int d, int jlo, int M, const double* __restrict__ O
 // block is stored, so the wrap of small Ny still reads the previous
 // iteration.  One reciprocal per cell: a = ax r, b = (...) r, I = num r with
 // r = 1 / (kappa + ax + ay + az).
+//
+// Everything of a row that does not depend on its inflows is computed once,
+// before the first stage, while the other rows' loads are in flight: the
+// reciprocals r, kappa B, and the multiplicative half of the row's affine
+// scan (the lane's fold product A and its value at each scan level, which
+// the additive half multiplies by).  A stage is then the additive half only:
+// the inflow terms, the lane's fold of b, six DPP steps and the
+// re-evaluation.  The arithmetic is the same operation for operation
+// as folding and scanning (A, Bc) together, so the intensities are too.
+// The PER cells of one lane, consecutive in memory, as one aligned vector:
+// a row whose length is 64 PER moves in 16-byte (or 8-byte) accesses that
+// cover whole cache lines, instead of PER strided scalar accesses.
+template <typename T, int N>
+struct alignas(N * sizeof(T) >= 16 ? 16 : N * sizeof(T)) LaneCells {
+  T v[N];
+};
+
 template <typename T, int PER, bool FULL, int TS>
 __global__ void __launch_bounds__(64 * TS)
 k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ Om,
                  T* __restrict__ Iset, const T* __restrict__ kap,
                  const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb) {
-  extern __shared__ unsigned char smem_raw[];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sh = reinterpret_cast<T*>(smem_raw);   // TS*TS rows in sweep order: slot (w, kk)
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -263,26 +280,41 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
     ok[q] = FULL || sq < Nx;
     ix[q] = (uint32_t)(fwd ? sq : Nx - 1 - sq);
   }
-  // global inputs read up front, so each stage is pure arithmetic: kappa and
-  // B of the column's TS rows, the y inflow of the column's rows (w == 0)
-  // and the z inflow of its first row (kk == 0)
-  T Iyg[TS][PER], Iz[PER], kvs[TS][PER], bvs[TS][PER];
+  // whole rows (FULL): the lane's cells are x in [i0, i0 + PER), sweep
+  // position q at element q (forward) or PER - 1 - q (backward)
+  using LC = LaneCells<T, PER>;
+  const uint32_t i0 = (uint32_t)(fwd ? lane * PER : Nx - PER * (lane + 1));
+  auto load_cells = [&](const T* p, uint32_t rowoff, T* out) {
+    if constexpr (FULL) {
+      const LC c = at(reinterpret_cast<const LC*>(p), (rowoff + i0) * E);
+#pragma unroll
+      for (int q = 0; q < PER; ++q) out[q] = fwd ? c.v[q] : c.v[PER - 1 - q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+        if (ok[q]) out[q] = at(p, (rowoff + ix[q]) * E);
+    }
+  };
+  // global inputs read up front: kappa and B of the column's TS rows, the y
+  // inflow of the column's rows (w == 0), the z inflow of its first row
+  // (kk == 0) and each row's z coefficient.  A stage then issues no global
+  // load (on gfx9 the wait for a load also waits for every earlier store).
+  T Iyg[TS][PER], Iz[PER], kB[TS][PER], rc[TS][PER], azs[TS];
+  T Al[TS][7];   // per row: A at the six scan levels, then the exclusive prefix
   if (col) {
+    T kvs[TS][PER];
 #pragma unroll
     for (int kk = 0; kk < TS; ++kk) {
       const int kp = TS * tk + kk;
+      azs[kk] = T(0);
       if (kp < Nz) {
         const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
+        azs[kk] = T(fabs(mu) * g.inv_dz[k]);
         const uint32_t row = ((uint32_t)k * Ny + j) * Nx;
         const uint32_t rowy = ((uint32_t)k * Ny + jy) * Nx;
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          if (ok[q]) {
-            kvs[kk][q] = at(kap, (row + ix[q]) * E);
-            bvs[kk][q] = at(B, (row + ix[q]) * E);
-            if (w == 0) Iyg[kk][q] = at((const T*)In, (rowy + ix[q]) * E);
-          }
-        }
+        load_cells(kap, row, kvs[kk]);
+        load_cells(B, row, kB[kk]);
+        if (w == 0) load_cells(In, rowy, Iyg[kk]);
       }
     }
     const int kp0 = TS * tk;
@@ -290,64 +322,130 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
     const bool zb = mu >= 0.0 ? (k0 == 0) : (k0 == Nz - 1);
     const int kz = mu >= 0.0 ? k0 - 1 : k0 + 1;
     const uint32_t rowz = zb ? 0u : ((uint32_t)kz * Ny + j) * Nx;
+    if (!zb) {
+      load_cells(In, rowz, Iz);
+    } else if (mu >= 0.0) {
+      load_cells(Ig_out, (uint32_t)j * Nx, Iz);
+    } else {
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      if (ok[q])
-        Iz[q] = zb ? (mu >= 0.0 ? at(Ig_out, ((uint32_t)j * Nx + ix[q]) * E) : I_amb)
-                   : at((const T*)In, (rowz + ix[q]) * E);
+      for (int q = 0; q < PER; ++q) Iz[q] = I_amb;
+    }
+    // the inflow-independent part of each row
+#pragma unroll
+    for (int kk = 0; kk < TS; ++kk) {
+      if (TS * tk + kk < Nz) {
+        T A = T(1);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          if (ok[q]) {
+            const T kv = kvs[kk][q];
+            rc[kk][q] = recip(kv + ax + ay + azs[kk]);
+            kB[kk][q] = kv * kB[kk][q];
+            const T a = ax * rc[kk][q];
+            A = a * A;
+          }
+        }
+        Al[kk][0] = A;
+        A = A * dpp_mov<0x111, 0xF>(T(1), A);   // row_shr:1
+        Al[kk][1] = A;
+        A = A * dpp_mov<0x112, 0xF>(T(1), A);   // row_shr:2
+        Al[kk][2] = A;
+        A = A * dpp_mov<0x114, 0xF>(T(1), A);   // row_shr:4
+        Al[kk][3] = A;
+        A = A * dpp_mov<0x118, 0xF>(T(1), A);   // row_shr:8
+        Al[kk][4] = A;
+        A = A * dpp_mov<0x142, 0xA>(T(1), A);   // row_bcast:15 into rows 1, 3
+        Al[kk][5] = A;
+        A = A * dpp_mov<0x143, 0xC>(T(1), A);   // row_bcast:31 into rows 2, 3
+        Al[kk][6] = dpp_mov<0x138, 0xF>(T(1), A);   // wave_shr:1: exclusive prefix
+      }
     }
   }
-  __syncthreads();   // every global inflow is read before any row is stored
-  for (int st = 0; st < 2 * TS - 1; ++st) {
-    const int kk = st - w;
-    const int kp = TS * tk + kk;
-    if (col && kk >= 0 && kk < TS && kp < Nz) {
+  // every global inflow is read before any row is stored: the wait for all
+  // of the wavefront's loads (s_waitcnt vmcnt(0); the compiler's wait
+  // tracking sees it, so no later wait holds a stage on its own stores),
+  // then the workgroup barrier
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  // Stage st ends at the workgroup's (st + 1)-th barrier, and wavefront w
+  // computes its row r at stage w + r: it passes w barriers, then does its
+  // rows in order with a barrier after each, then passes the rest, so every
+  // wavefront passes 2 TS - 1 barriers.  The row loop is unrolled, so each
+  // row's registers are addressed statically.
+  for (int b = 0; b < w; ++b) __syncthreads();
+#pragma unroll
+  for (int r = 0; r < TS; ++r) {
+    const int kp = TS * tk + r;
+    if (col && kp < Nz) {
       const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
-      const T az = T(fabs(mu) * g.inv_dz[k]);
       const uint32_t row = ((uint32_t)k * Ny + j) * Nx;
-      const T* shy = sh + ((w - 1) * TS + kk) * Nx;   // wavefront w - 1's row
-      T* sho = sh + (w * TS + kk) * Nx;
-      T num0[PER], rc[PER];
-      T A = T(1), Bc = T(0);
+      const T* shy = sh + ((w - 1) * TS + r) * Nx;   // wavefront w - 1's row
+      T* sho = sh + (w * TS + r) * Nx;
+      const T az = azs[r];
+      T num0[PER], Iy[PER];
+      if (w == 0) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) Iy[q] = Iyg[r][q];
+      } else if constexpr (FULL) {
+        const LC c = *reinterpret_cast<const LC*>(shy + lane * PER);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) Iy[q] = c.v[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+          if (ok[q]) Iy[q] = shy[lane * PER + q];
+      }
+      T Bc = T(0);
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
         if (ok[q]) {
-          const T kv = kvs[0][q];
-          const T Iy = w == 0 ? Iyg[0][q] : shy[lane * PER + q];
-          rc[q] = recip(kv + ax + ay + az);
-          num0[q] = kv * bvs[0][q] + ay * Iy + az * Iz[q];
-          const T a = ax * rc[q];
-          const T b = num0[q] * rc[q];
+          num0[q] = kB[r][q] + ay * Iy[q] + az * Iz[q];
+          const T a = ax * rc[r][q];
+          const T b = num0[q] * rc[r][q];
           Bc = a * Bc + b;
-          A = a * A;
         }
       }
-      T eA, eB;
-      affine_scan64(A, Bc, eA, eB);
-      T Iprev = eA * I_amb + eB;
+      // the additive half of the affine scan: x -> A x + Bc composed with
+      // the earlier lane's map; lanes without one add A * 0
+      Bc = Al[r][0] * dpp_mov<0x111, 0xF>(T(0), Bc) + Bc;
+      Bc = Al[r][1] * dpp_mov<0x112, 0xF>(T(0), Bc) + Bc;
+      Bc = Al[r][2] * dpp_mov<0x114, 0xF>(T(0), Bc) + Bc;
+      Bc = Al[r][3] * dpp_mov<0x118, 0xF>(T(0), Bc) + Bc;
+      Bc = Al[r][4] * dpp_mov<0x142, 0xA>(T(0), Bc) + Bc;
+      Bc = Al[r][5] * dpp_mov<0x143, 0xC>(T(0), Bc) + Bc;
+      const T eB = dpp_mov<0x138, 0xF>(T(0), Bc);
+      T Iprev = Al[r][6] * I_amb + eB;
 #pragma unroll
       for (int q = 0; q < PER; ++q) {
         if (ok[q]) {
-          const T v = (num0[q] + ax * Iprev) * rc[q];
+          const T v = (num0[q] + ax * Iprev) * rc[r][q];
           Iz[q] = v;                      // z inflow of the next row up
-          sho[lane * PER + q] = v;        // y inflow of wavefront w + 1
-          at(In, (row + ix[q]) * E) = v;
           Iprev = v;
         }
       }
-      // the wavefront's rows run in order kk = 0..TS-1: shift the prefetched
-      // inputs down so the next row's sit in slot 0 (static register indices)
-#pragma unroll
-      for (int r = 0; r < TS - 1; ++r)
+      // the row to LDS (y inflow of wavefront w + 1) and to the intensities
+      if constexpr (FULL) {
+        LC c, m;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-          kvs[r][q] = kvs[r + 1][q];
-          bvs[r][q] = bvs[r + 1][q];
-          Iyg[r][q] = Iyg[r + 1][q];
+          c.v[q] = Iz[q];
+          m.v[q] = fwd ? Iz[q] : Iz[PER - 1 - q];
         }
+        *reinterpret_cast<LC*>(sho + lane * PER) = c;
+        at(reinterpret_cast<LC*>(In), (row + i0) * E) = m;
+      } else {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          if (ok[q]) {
+            sho[lane * PER + q] = Iz[q];
+            at(In, (row + ix[q]) * E) = Iz[q];
+          }
+        }
+      }
     }
     __syncthreads();
   }
+  for (int b = w; b < TS - 1; ++b) __syncthreads();
 }
 
 // G = sum_n w_n I_n (ordinates summed in order), one cell per thread; per
@@ -504,21 +602,45 @@ static int tile_side(const DomHandle* h) {
   return 16 * row <= 65536 ? 4 : 0;
 }
 
+// Whether a (cells per lane, tile side) pair has a kernel: the LDS rows of an
+// 8 x 8 tile fit 64 KB only for rows of <= 1 KB (float32 Nx <= 256,
+// float64 Nx <= 128), which is what tile_side allows; the other pairs are
+// never launched and are not instantiated.
+template <typename T, int PER, int TS>
+constexpr bool tile_kernel() {
+  return TS != 8 || PER * sizeof(T) <= 16;
+}
+
 // One source iteration of the tiled sweep (cells per lane as below; a float64
 // row of more than 512 cells does not fit the LDS and never gets here).
 template <typename T, int TS>
 static int sweep_tiled_ts(DomHandle* h, T I_amb, hipStream_t s) {
   const int Nx = h->g.nx;
   const int TJ = (h->g.ny + TS - 1) / TS, TK = (h->g.nz + TS - 1) / TS;
+  const int per = Nx <= 64 ? 1 : Nx <= 128 ? 2 : Nx <= 256 ? 4 : Nx <= 512 ? 8 : 16;
+  auto go = [&](auto PC, int D, int tjlo, int nt) -> int {
+    constexpr int PER = decltype(PC)::value;
+    if constexpr (tile_kernel<T, PER, TS>() && (PER < 16 || (sizeof(T) == 4 && TS == 4))) {
+      launch_tile<T, PER, TS>(h, D, tjlo, nt, I_amb, s);
+      return FW_OK;
+    } else {
+      return fw_set_error(FW_E_CONFIG, "dom sweep: no %dx%d tile kernel for rows of %d cells",
+                          TS, TS, Nx);
+    }
+  };
   for (int D = 0; D <= TJ + TK - 2; ++D) {
     const int tjlo = D - (TK - 1) > 0 ? D - (TK - 1) : 0;
     const int tjhi = D < TJ - 1 ? D : TJ - 1;
     const int nt = tjhi - tjlo + 1;
-    if (Nx <= 64) launch_tile<T, 1, TS>(h, D, tjlo, nt, I_amb, s);
-    else if (Nx <= 128) launch_tile<T, 2, TS>(h, D, tjlo, nt, I_amb, s);
-    else if (Nx <= 256) launch_tile<T, 4, TS>(h, D, tjlo, nt, I_amb, s);
-    else if (Nx <= 512) launch_tile<T, 8, TS>(h, D, tjlo, nt, I_amb, s);
-    else if constexpr (sizeof(T) == 4 && TS == 4) launch_tile<T, 16, TS>(h, D, tjlo, nt, I_amb, s);
+    int rc;
+    switch (per) {
+      case 1: rc = go(std::integral_constant<int, 1>{}, D, tjlo, nt); break;
+      case 2: rc = go(std::integral_constant<int, 2>{}, D, tjlo, nt); break;
+      case 4: rc = go(std::integral_constant<int, 4>{}, D, tjlo, nt); break;
+      case 8: rc = go(std::integral_constant<int, 8>{}, D, tjlo, nt); break;
+      default: rc = go(std::integral_constant<int, 16>{}, D, tjlo, nt); break;
+    }
+    if (rc) return rc;
   }
   return FW_OK;
 }
