@@ -1337,7 +1337,7 @@ k_dct4_rows(int nrows, int N, int logM, const T* __restrict__ tw,
 // The same DCT-IV with the preconditioner's work buffer in the transposed
 // layout Wt[i][row] (row = k Ny + j, leading dimension ldt), the layout
 // k_yz_plane reads one x frequency i at a time as a contiguous block.
-// Float32 only.  A workgroup transforms DT_ROWS consecutive rows (four per
+// Float32 only.  A workgroup transforms DT_ROWS consecutive rows (two per
 // wavefront) and moves them between the row layout and Wt through an LDS
 // tile, so every global access covers DT_ROWS consecutive rows of one
 // frequency (64-byte runs) instead of one element per cache line.
@@ -1345,6 +1345,11 @@ k_dct4_rows(int nrows, int N, int logM, const T* __restrict__ tw,
 //   TIN = true:  Wt (in) -> rows (out, stride N); modes DCT_PLAIN, DCT_DOT.
 // The fused work (DCT_UPDATE / DCT_DOT) is that of k_dct4_rows, one partial
 // per workgroup.
+// Memory order: the row-layout vectors move in whole rows, lane l taking
+// elements l + 64 u (every access a run of 256 bytes); the FFT's pairing of
+// elements 2n and N - 1 - 2n is made in LDS (the rows are staged in the
+// tile buffer).  The twiddle table is copied to LDS once per workgroup, so
+// no global load sits between the FFT stages.
 // DT_ROWS (8 or 16) rows per workgroup, two per wavefront (32 DT_ROWS
 // threads); N <= DT_NMAX, so the tile moves in at most DT_NMAX / 32 elements
 // per thread, all issued before any is used.
@@ -1359,13 +1364,14 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
   __shared__ double shr[DT_THREADS / 64];
   const int M = N >> 1, LX = N + 1;       // LX: conflict-free tile row
   float* A = sm;                          // DT_ROWS x N: the complex FFT rows
-  float* X = sm + DT_ROWS * N;            // DT_ROWS x LX: the transposed tile
+  float* X = sm + DT_ROWS * N;            // DT_ROWS x LX: tile / row staging
+  float* TW = X + DT_ROWS * LX;           // the 5 M twiddles
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int NE = DT_NMAX * DT_ROWS / DT_THREADS;   // tile elements per thread
+  constexpr int NL = DT_NMAX / 64;                     // row elements per lane
+  constexpr int NT = (5 * (DT_NMAX / 2) + DT_THREADS - 1) / DT_THREADS;
   const int row0 = blockIdx.x * DT_ROWS;
-  const float* pre = tw;
-  const float* post = tw + 2 * M;
-  const float* W = tw + 4 * M;
+  const int ntw = 5 * M;
   double acc = 0.0;
   float alpha = 0.0f;
   if constexpr (MODE == DCT_UPDATE) {
@@ -1373,10 +1379,15 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
   }
   // Every global load of the workgroup is issued first, ahead of the
   // partial sums (which hold __syncthreads) and of the LDS staging, so one
-  // memory latency covers the launch: the tile's columns of Wt (TIN:
-  // consecutive threads take consecutive rows), then the wavefront's rows
-  // rl = wv DT_RPW + rr (complex point n = lane + 64 u, entries 2n and
-  // N-1-2n).
+  // memory latency covers the launch: the twiddles, the tile's columns of
+  // Wt (TIN: consecutive threads take consecutive rows), then the
+  // wavefront's rows rl = wv DT_RPW + rr, element e = lane + 64 u.
+  float twv[NT];
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    const int e = threadIdx.x + u * DT_THREADS;
+    twv[u] = e < ntw ? tw[e] : 0.0f;
+  }
   float t[NE];
   if constexpr (TIN) {
 #pragma unroll
@@ -1386,29 +1397,24 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
       t[u] = i < N && row < nrows ? in[(size_t)i * ldt + row] : 0.0f;
     }
   }
-  constexpr int NI = DT_NMAX / 128;
-  float va[DT_RPW][NI], vb[DT_RPW][NI], qa[DT_RPW][NI], qb[DT_RPW][NI];
-  float pa[DT_RPW][NI], pb[DT_RPW][NI], da[DT_RPW][NI], db[DT_RPW][NI];
+  float va[DT_RPW][NL], qa[DT_RPW][NL], pa[DT_RPW][NL], da[DT_RPW][NL];
 #pragma unroll
   for (int rr = 0; rr < DT_RPW; ++rr) {
     const int row = row0 + wv * DT_RPW + rr;
     const size_t rb = (size_t)row * N;
 #pragma unroll
-    for (int u = 0; u < NI; ++u) {
-      const int n = lane + 64 * u;
-      if (row < nrows && n < M) {
-        const size_t e0 = rb + 2 * n, e1 = rb + N - 1 - 2 * n;
+    for (int u = 0; u < NL; ++u) {
+      const int e = lane + 64 * u;
+      if (row < nrows && e < N) {
         if constexpr (MODE == DCT_UPDATE) {
-          va[rr][u] = f.r[e0]; vb[rr][u] = f.r[e1];
-          qa[rr][u] = f.q[e0]; qb[rr][u] = f.q[e1];
-          pa[rr][u] = f.p[e0]; pb[rr][u] = f.p[e1];
-          da[rr][u] = f.d[e0]; db[rr][u] = f.d[e1];
+          va[rr][u] = f.r[rb + e];
+          qa[rr][u] = f.q[rb + e];
+          pa[rr][u] = f.p[rb + e];
+          da[rr][u] = f.d[rb + e];
         } else if constexpr (MODE == DCT_DOT) {   // r for the dot of the output stage
-          qa[rr][u] = f.r[e0];
-          qb[rr][u] = f.r[e1];
+          qa[rr][u] = f.r[rb + e];
         } else if constexpr (!TIN) {
-          va[rr][u] = in[e0];
-          vb[rr][u] = in[e1];
+          va[rr][u] = in[rb + e];
         }
       }
     }
@@ -1438,6 +1444,12 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
       __syncthreads();
     }
   }
+  // twiddles and the input rows into LDS
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    const int e = threadIdx.x + u * DT_THREADS;
+    if (e < ntw) TW[e] = twv[u];
+  }
   if constexpr (TIN) {
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
@@ -1445,40 +1457,42 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
       const int i = e / DT_ROWS, rl = e - i * DT_ROWS;
       if (i < N) X[rl * LX + i] = t[u];
     }
-    __syncthreads();
+  } else {
 #pragma unroll
     for (int rr = 0; rr < DT_RPW; ++rr) {
-      const int rl = wv * DT_RPW + rr;
+      const int rl = wv * DT_RPW + rr, row = row0 + rl;
+      const size_t rb = (size_t)row * N;
 #pragma unroll
-      for (int u = 0; u < NI; ++u) {
-        const int n = lane + 64 * u;
-        if (row0 + rl < nrows && n < M) {
-          va[rr][u] = X[rl * LX + 2 * n];
-          vb[rr][u] = X[rl * LX + N - 1 - 2 * n];
+      for (int u = 0; u < NL; ++u) {
+        const int e = lane + 64 * u;
+        if (row < nrows && e < N) {
+          float v = va[rr][u];
+          if constexpr (MODE == DCT_UPDATE) {
+            f.d[rb + e] = da[rr][u] + alpha * pa[rr][u];
+            v = v - alpha * qa[rr][u];
+            f.r[rb + e] = v;
+            acc += (double)v * (double)v;
+          }
+          X[rl * LX + e] = v;
         }
       }
     }
   }
+  __syncthreads();
+  const float* pre = TW;
+  const float* post = TW + 2 * M;
+  const float* W = TW + 4 * M;
+  // pre-twiddle of the pairs (2n, N - 1 - 2n) into bit-reversed order
 #pragma unroll
   for (int rr = 0; rr < DT_RPW; ++rr) {
     const int rl = wv * DT_RPW + rr, row = row0 + rl;
-    const size_t rb = (size_t)row * N;
     float* a = A + rl * N;
+    const float* x = X + rl * LX;
 #pragma unroll
-    for (int u = 0; u < NI; ++u) {
+    for (int u = 0; u < NL / 2; ++u) {
       const int n = lane + 64 * u;
       if (row < nrows && n < M) {
-        float re = va[rr][u], im = vb[rr][u];
-        if constexpr (MODE == DCT_UPDATE) {
-          const size_t e0 = rb + 2 * n, e1 = rb + N - 1 - 2 * n;
-          f.d[e0] = da[rr][u] + alpha * pa[rr][u];
-          f.d[e1] = db[rr][u] + alpha * pb[rr][u];
-          re = re - alpha * qa[rr][u];
-          im = im - alpha * qb[rr][u];
-          f.r[e0] = re;
-          f.r[e1] = im;
-          acc += (double)re * (double)re + (double)im * (double)im;
-        }
+        const float re = x[2 * n], im = x[N - 1 - 2 * n];
         const float pr = pre[2 * n], pi = pre[2 * n + 1];
         const int r = (int)(__brev((unsigned)n) >> (32 - logM));
         a[2 * r] = re * pr - im * pi;
@@ -1510,35 +1524,45 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
     }
     __syncthreads();
   }
+  // post-twiddle into the staging rows (memory order)
   const float sc = (float)sqrt(2.0 / N);
 #pragma unroll
   for (int rr = 0; rr < DT_RPW; ++rr) {
     const int rl = wv * DT_RPW + rr, row = row0 + rl;
     if (row >= nrows) break;
-    const size_t rb = (size_t)row * N;
     const float* a = A + rl * N;
-    const double dzk = MODE == DCT_DOT ? f.dz[row / f.ny] : 0.0;
 #pragma unroll
-    for (int u = 0; u < NI; ++u) {
+    for (int u = 0; u < NL / 2; ++u) {
       const int kk = lane + 64 * u;
       if (kk >= M) break;
       const float vr = a[2 * kk], vi = a[2 * kk + 1];
       const float pr = post[2 * kk], pi = post[2 * kk + 1];
       const float y0 = (vr * pr - vi * pi) * sc;
       const float y1 = -((vr * pi + vi * pr) * sc);
-      if constexpr (TIN) {
-        out[rb + 2 * kk] = y0;
-        out[rb + N - 1 - 2 * kk] = y1;
-      } else {
-        X[rl * LX + 2 * kk] = y0;
-        X[rl * LX + N - 1 - 2 * kk] = y1;
-      }
-      if constexpr (MODE == DCT_DOT)
-        acc += dzk * ((double)qa[rr][u] * (double)y0 + (double)qb[rr][u] * (double)y1);
+      X[rl * LX + 2 * kk] = y0;
+      X[rl * LX + N - 1 - 2 * kk] = y1;
     }
   }
-  if constexpr (!TIN) {
-    __syncthreads();
+  __syncthreads();
+  if constexpr (TIN) {
+    // whole rows out, and the dot with r in the same order
+#pragma unroll
+    for (int rr = 0; rr < DT_RPW; ++rr) {
+      const int rl = wv * DT_RPW + rr, row = row0 + rl;
+      if (row >= nrows) break;
+      const size_t rb = (size_t)row * N;
+      const double dzk = MODE == DCT_DOT ? f.dz[row / f.ny] : 0.0;
+#pragma unroll
+      for (int u = 0; u < NL; ++u) {
+        const int e = lane + 64 * u;
+        if (e < N) {
+          const float y = X[rl * LX + e];
+          out[rb + e] = y;
+          if constexpr (MODE == DCT_DOT) acc += dzk * ((double)qa[rr][u] * (double)y);
+        }
+      }
+    }
+  } else {
 #pragma unroll
     for (int u = 0; u < NE; ++u) {
       const int e = threadIdx.x + u * DT_THREADS;
@@ -1548,10 +1572,11 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
   }
   if constexpr (MODE != DCT_PLAIN) {
     __syncthreads();
-    const double t = block_sum_wg(acc, shr);
-    if (threadIdx.x == 0) f.Pout[blockIdx.x] = t;
+    const double tt = block_sum_wg(acc, shr);
+    if (threadIdx.x == 0) f.Pout[blockIdx.x] = tt;
   }
 }
+
 
 // ---------------------------------------------------------------------------
 // handle
@@ -1628,7 +1653,8 @@ static void launch_gemm(ProjHandle* h, const T* A, const T* B, T* C, int M,
 static bool work_t(const ProjHandle* h) { return h->ZfQ && h->use_plane; }
 static int work_ldt(const ProjHandle* h) { return (h->g.nz * h->g.ny + 31) & ~31; }
 static size_t dct_tile_shm(int rows_per_wg, int N) {
-  return (size_t)rows_per_wg * (2 * N + 1) * sizeof(float);
+  // FFT rows, tile rows (N + 1), twiddles (5 N / 2)
+  return ((size_t)rows_per_wg * (2 * N + 1) + (size_t)5 * (N / 2)) * sizeof(float);
 }
 // workgroups of a k_dct4_tile launch (= its partials) at the handle's tiling
 static int dct_tile_blocks(const ProjHandle* h) {
