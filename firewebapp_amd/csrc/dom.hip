@@ -25,6 +25,14 @@
 namespace fw {
 
 
+// The solve's device state (DomHandle::S): the largest |G - G_prev| and the
+// largest G of the last completed iteration, the converged flag (set by
+// k_dom_fin; every kernel of a source iteration returns at once when it is
+// set, so iterations can be enqueued in batches and the host reads the
+// state once per batch), the count of completed iterations (its parity
+// names the G buffer the next iteration writes) and the non-finite flag.
+enum { DS_DIFF = 0, DS_GMAX, DS_DONE, DS_ITERS, DS_NAN, DS_N };
+
 struct DomHandle {
   fw_grid g;
   GridArgs ga;
@@ -36,7 +44,7 @@ struct DomHandle {
   std::vector<double> Om_h, w_h;
   void *I, *G, *Gp, *kap, *kap_s, *kap_safe, *B;
   void *Ig_out, *Ig_emit, *q_in;  // (Ny, Nx)
-  double *part, *part2, *S, *S_host;
+  double *part, *part2, *S, *S_host;   // S: DS_N solve-state slots
   int last_iters;
   GraphCache* gc;   // captured source iterations (non-default streams, timer off)
   int tile;         // requested sweep tile side (0: the default 4)
@@ -82,8 +90,13 @@ k_dom_props(size_t n, const T* __restrict__ Ts, const T* __restrict__ Tg,
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_dom_ground_init(int ncol, const T* __restrict__ Tsoil, T eps, T I_amb,
-                  T* __restrict__ emit, T* __restrict__ out) {
+                  T* __restrict__ emit, T* __restrict__ out, double* __restrict__ S) {
   const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c == 0) {   // a new solve: not converged, no iteration done
+    S[DS_DONE] = 0.0;
+    S[DS_ITERS] = 0.0;
+    S[DS_NAN] = 0.0;
+  }
   if (c >= ncol) return;
   T e;
   if (Tsoil) {
@@ -152,7 +165,9 @@ template <typename T, int PER>
 __global__ void __launch_bounds__(256)
 k_dom_sweep_diag(GridArgs g, int d, int jlo, int M, const double* __restrict__ Om,
                  T* __restrict__ Iset, const T* __restrict__ kap,
-                 const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb) {
+                 const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb,
+                 const double* __restrict__ S) {
+  if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const int lane = threadIdx.x & 63;
   const int n = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -253,7 +268,9 @@ template <typename T, int PER, bool FULL, int TS>
 __global__ void __launch_bounds__(64 * TS)
 k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ Om,
                  T* __restrict__ Iset, const T* __restrict__ kap,
-                 const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb) {
+                 const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb,
+                 const double* __restrict__ S) {
+  if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sh = reinterpret_cast<T*>(smem_raw);   // TS*TS rows in sweep order: slot (w, kk)
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
@@ -453,8 +470,13 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ w,
-        T* __restrict__ G, const T* __restrict__ Gp, double* __restrict__ pdiff,
-        double* __restrict__ pmax) {
+        T* __restrict__ G0, T* __restrict__ G1, double* __restrict__ pdiff,
+        double* __restrict__ pmax, const double* __restrict__ S) {
+  if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
+  // this iteration writes G0 or G1 by its parity; the other holds G_prev
+  const bool odd = ((long long)S[DS_ITERS] & 1) != 0;
+  T* __restrict__ G = odd ? G1 : G0;
+  const T* __restrict__ Gp = odd ? G0 : G1;
   __shared__ double sh1[256], sh2[256];
   double md = 0.0, mg = -1e300;
   const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -495,11 +517,26 @@ __global__ void __launch_bounds__(256)
 k_dom_ground(int ncol, size_t n, int M, const double* __restrict__ Om,
              const double* __restrict__ w, const T* __restrict__ Iset,
              const T* __restrict__ emit, T* __restrict__ Ig_out,
-             T* __restrict__ q_in, T eps, T relax) {
+             T* __restrict__ q_in, T eps, T relax, const double* __restrict__ S) {
+  if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= ncol) return;
+  // the downward ordinates summed in order; their intensities are loaded
+  // eight at a time ahead of the sums, so the loads overlap
   T q = T(0);
-  for (int m = 0; m < M; ++m) {
+  int m = 0;
+  for (; m + 8 <= M; m += 8) {
+    T iv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      iv[u] = Om[3 * (m + u) + 2] < 0.0 ? Iset[(size_t)(m + u) * n + c] : T(0);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const double mu = Om[3 * (m + u) + 2];
+      if (mu < 0.0) q += T(w[m + u]) * T(fabs(mu)) * iv[u];
+    }
+  }
+  for (; m < M; ++m) {
     const double mu = Om[3 * m + 2];
     if (mu < 0.0) q += T(w[m]) * T(fabs(mu)) * Iset[(size_t)m * n + c];
   }
@@ -537,7 +574,8 @@ __global__ void k_scale_copy(int ncol, const T* __restrict__ a, T s,
 
 __global__ void __launch_bounds__(256)
 k_dom_fin(const double* __restrict__ pd, const double* __restrict__ pm, int np,
-          double* __restrict__ S) {
+          double* __restrict__ S, int first, double tol) {
+  if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
   __shared__ double a[256], b[256];
   const int t = threadIdx.x;
   double x = 0.0, y = -1e300;
@@ -555,7 +593,23 @@ k_dom_fin(const double* __restrict__ pd, const double* __restrict__ pm, int np,
     }
     __syncthreads();
   }
-  if (t == 0) { S[0] = a[0]; S[1] = b[0]; }
+  if (t == 0) {
+    // the iteration is complete; from the second on, the change of G
+    // relative to its largest value decides convergence
+    S[DS_ITERS] += 1.0;
+    if (!first) {
+      S[DS_DIFF] = a[0];
+      S[DS_GMAX] = b[0];
+      const double gmax = b[0] > 1e-12 ? b[0] : 1e-12;
+      const double err = a[0] / gmax;
+      if (!(err == err)) {
+        S[DS_NAN] = 1.0;
+        S[DS_DONE] = 1.0;
+      } else if (err < tol) {
+        S[DS_DONE] = 1.0;
+      }
+    }
+  }
 }
 
 static inline unsigned nb(size_t n) { return (unsigned)((n + 255) / 256); }
@@ -579,11 +633,11 @@ static void launch_tile(DomHandle* h, int D, int tjlo, int ntiles, T I_amb, hipS
   if (ev.start)
     hipExtLaunchKernelGGL(kern, grid, dim3(64 * TS), shm, s, ev.start, ev.stop, 0, h->ga, D,
                           tjlo, h->M, (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
-                          (const T*)h->B, (const T*)h->Ig_out, I_amb);
+                          (const T*)h->B, (const T*)h->Ig_out, I_amb, (const double*)h->S);
   else
     hipLaunchKernelGGL(kern, grid, dim3(64 * TS), shm, s, h->ga, D, tjlo, h->M,
                        (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
-                       (const T*)h->B, (const T*)h->Ig_out, I_amb);
+                       (const T*)h->B, (const T*)h->Ig_out, I_amb, (const double*)h->S);
 }
 
 // Tile side: 4 x 4 rows when their LDS rows fit 64 KB, else 0 (the per-row
@@ -668,17 +722,17 @@ static int launch_sweep(DomHandle* h, int d, int jlo, int nrows, T I_amb, hipStr
   const T* B = (const T*)h->B;
   const T* Ig = (const T*)h->Ig_out;
   if (Nx <= 64)
-    hipLaunchKernelGGL((k_dom_sweep_diag<T, 1>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 1>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb, h->S);
   else if (Nx <= 128)
-    hipLaunchKernelGGL((k_dom_sweep_diag<T, 2>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 2>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb, h->S);
   else if (Nx <= 256)
-    hipLaunchKernelGGL((k_dom_sweep_diag<T, 4>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 4>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb, h->S);
   else if (Nx <= 512)
-    hipLaunchKernelGGL((k_dom_sweep_diag<T, 8>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 8>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb, h->S);
   else if (Nx <= 1024)
-    hipLaunchKernelGGL((k_dom_sweep_diag<T, 16>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 16>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb, h->S);
   else
-    hipLaunchKernelGGL((k_dom_sweep_diag<T, 32>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb);
+    hipLaunchKernelGGL((k_dom_sweep_diag<T, 32>), grid, blk, 0, s, h->ga, d, jlo, M, Om, I, kap, B, Ig, I_amb, h->S);
   return FW_OK;
 }
 
@@ -699,13 +753,13 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
                      (T*)h->kap_safe, (T*)h->B);
   hipLaunchKernelGGL(k_dom_ground_init<T>, dim3(nb(ncol)), dim3(256), 0, s, ncol,
                      (const T*)Tsoil, T(h->eps_ground), I_amb, (T*)h->Ig_emit,
-                     (T*)h->Ig_out);
+                     (T*)h->Ig_out, h->S);
   FW_CHECK_LAUNCH("dom_props");
   // One source iteration: the sweep, G with its change, the ground update
-  // and (after the first) the change's final sum.  Its launches depend only
-  // on the G buffers' order and whether it is the first, so off the legacy
-  // default stream and with the sweep timer off it is replayed from a
-  // captured graph.
+  // and the change's final sum with the convergence test.  Every kernel of
+  // it returns at once once the solve has converged.  Its launches depend
+  // only on whether it is the first, so off the legacy default stream and
+  // with the sweep timer off it is replayed from a captured graph.
   auto iteration = [&](int first) -> int {
     if (tile_side<T>(h) > 0) {
       TRY_DOM(sweep_tiled<T>(h, I_amb, s));
@@ -719,52 +773,52 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
     }
     FW_CHECK_LAUNCH("dom_sweep");
     hipLaunchKernelGGL(k_dom_G<T>, dim3(nb(n)), dim3(256), 0, s, n, M,
-                       (const T*)h->I, (const double*)h->w, (T*)h->G,
-                       (const T*)h->Gp, h->part, h->part2);
+                       (const T*)h->I, (const double*)h->w, (T*)h->G, (T*)h->Gp,
+                       h->part, h->part2, (const double*)h->S);
     hipLaunchKernelGGL(k_dom_ground<T>, dim3(nb(ncol)), dim3(256), 0, s, ncol, n, M,
                        (const double*)h->Om, (const double*)h->w, (const T*)h->I,
                        (const T*)h->Ig_emit, (T*)h->Ig_out, (T*)h->q_in,
-                       T(h->eps_ground), T(h->relax));
-    if (!first)
-      hipLaunchKernelGGL(k_dom_fin, dim3(1), dim3(256), 0, s, h->part, h->part2,
-                         (int)nb(n), h->S);
+                       T(h->eps_ground), T(h->relax), (const double*)h->S);
+    hipLaunchKernelGGL(k_dom_fin, dim3(1), dim3(256), 0, s, h->part, h->part2,
+                       (int)nb(n), h->S, first, h->tol);
     FW_CHECK_LAUNCH("dom_G");
     return FW_OK;
   };
   const bool graph = h->gc && s && !timer_on(FW_TIMER_DOM_SWEEP) && loop_graphs_on();
-  int it = 0;
-  for (; it < h->maxiter; ++it) {
-    // G_prev <- G (swap)
-    void* tmp = h->G;
-    h->G = h->Gp;
-    h->Gp = tmp;
-    // I_amb rides in the key: the launches take it by value
-    if (graph) {
-      // recording: iteration() issues to s, the recording stream meanwhile
-      TRY_DOM(h->gc->run({kw(h->G), kw(h->Gp), (uint64_t)(it == 0), kw((double)I_amb)}, s,
-                         [&](hipStream_t cs) {
-                           const hipStream_t keep = s;
-                           s = cs;
-                           const int r2 = iteration(it == 0);
-                           s = keep;
-                           return r2;
-                         }));
-    } else {
-      TRY_DOM(iteration(it == 0));
-    }
-    if (it > 0) {
-      FW_CHECK_HIP(hipMemcpyAsync(h->S_host, h->S, 2 * sizeof(double),
-                                  hipMemcpyDeviceToHost, s));
-      FW_CHECK_HIP(hipStreamSynchronize(s));
-      double gmax = h->S_host[1] > 1e-12 ? h->S_host[1] : 1e-12;
-      double err = h->S_host[0] / gmax;
-      if (!(err == err)) return fw_set_error(FW_E_NONFINITE, "dom: non-finite G");
-      if (err < h->tol) { ++it; break; }
-    }
+  auto run_iteration = [&](int first) -> int {
+    if (!graph) return iteration(first);
+    // I_amb rides in the key: the launches take it by value; recording:
+    // iteration() issues to s, the recording stream meanwhile
+    return h->gc->run({(uint64_t)first, kw((double)I_amb)}, s, [&](hipStream_t cs) {
+      const hipStream_t keep = s;
+      s = cs;
+      const int r2 = iteration(first);
+      s = keep;
+      return r2;
+    });
+  };
+  // Iterations in batches with one host read per batch: one fewer than the
+  // previous solve's count first (the count changes slowly from solve to
+  // solve, so the batch seldom runs past convergence; the first iteration
+  // never converges, so at least two), then one at a time.  The device flag
+  // stops every kernel after the converging iteration, so the result is that
+  // of a one-by-one loop.
+  int queued = 0;
+  int batch = h->last_iters > 3 ? h->last_iters - 1 : 2;
+  for (;;) {
+    for (int b = 0; b < batch && queued < h->maxiter; ++b, ++queued)
+      TRY_DOM(run_iteration(queued == 0));
+    FW_CHECK_HIP(hipMemcpyAsync(h->S_host, h->S, DS_N * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+    FW_CHECK_HIP(hipStreamSynchronize(s));
+    if (h->S_host[DS_NAN] != 0.0) return fw_set_error(FW_E_NONFINITE, "dom: non-finite G");
+    if (h->S_host[DS_DONE] != 0.0 || queued >= h->maxiter) break;
+    batch = 1;
   }
-  h->last_iters = it;
+  h->last_iters = (int)h->S_host[DS_ITERS];
   hipLaunchKernelGGL(k_dom_out<T>, dim3(nb(n)), dim3(256), 0, s, h->ga, n,
-                     (const T*)h->G, (const T*)h->kap, (const T*)h->kap_s,
+                     (const T*)(h->last_iters > 0 && ((h->last_iters - 1) & 1) ? h->Gp : h->G),
+                     (const T*)h->kap, (const T*)h->kap_s,
                      (const T*)h->kap_safe, (const T*)h->B, (T*)qs, (T*)qg);
   if (q_soil)
     hipLaunchKernelGGL(k_scale_copy<T>, dim3(nb(ncol)), dim3(256), 0, s, ncol,
@@ -864,8 +918,8 @@ int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* we
   const size_t npart = (n + 255) / 256;
   FW_CHECK_HIP(hipMalloc((void**)&h->part, npart * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->part2, npart * sizeof(double)));
-  FW_CHECK_HIP(hipMalloc((void**)&h->S, 4 * sizeof(double)));
-  FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, 4 * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->S, DS_N * sizeof(double)));
+  FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, DS_N * sizeof(double)));
   *handle = h;
   return FW_OK;
 }

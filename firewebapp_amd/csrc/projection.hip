@@ -551,14 +551,21 @@ k_residual64(GridArgs g, size_t n, const TR* __restrict__ rho,
              const double* __restrict__ b, const double* __restrict__ x,
              T* __restrict__ r_t, double eps_reg, double* __restrict__ part) {
   __shared__ double sh[256];
+  (void)n;
+  // contiguous runs of rows per workgroup in the XCD-aware order, so the
+  // y and z neighbour rows of the stencil mostly come from the same L2
+  int r0, r1;
+  row_range(g.ny * g.nz, r0, r1);
   double acc = 0.0;
-  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * 256) {
-    Cell e = cell_of(g, c);
-    double ax = apply_A<double, double, TR>(g, rho, x, e.k, e.j, e.i, c, eps_reg);
-    double r = b[c] - ax;
-    r_t[c] = (T)r;
-    acc += r * r;
+  for (int row = r0; row < r1; ++row) {
+    const int k = row / g.ny, j = row - k * g.ny;
+    for (int i = threadIdx.x; i < g.nx; i += 256) {
+      const size_t c = ((size_t)k * g.ny + j) * g.nx + i;
+      const double ax = apply_A<double, double, TR>(g, rho, x, k, j, i, c, eps_reg);
+      const double r = b[c] - ax;
+      r_t[c] = (T)r;
+      acc += r * r;
+    }
   }
   block_sum_store(acc, sh, part);
 }
