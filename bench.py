@@ -53,7 +53,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -125,8 +124,12 @@ DOM_PASSES_PER_ORDINATE = 5
 SPECIES3_FIELDS = 12
 # HBM bytes per launch from the FETCH_SIZE + WRITE_SIZE counters scaled by a
 # 256 MiB copy (float32): profiles/step_256x64x48_s8_hbm_traffic.md
-ROOFLINE_TRAFFIC = {("k_species3", "spread3d_256x64x48_s8", "float32"): 44060018,
-                    ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32"): 18228147}
+# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
+# counter runs summarised under profiles/; the DOM sweep's depend on its
+# tile side (the last key)
+ROOFLINE_TRAFFIC = {("k_species3", "spread3d_256x64x48_s8", "float32", 0): 44060018,
+                    ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 4): 18176445,
+                    ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 8): 36213937}
 
 
 # Phase-fused compulsory traffic of one outer step in field-passes (each
@@ -258,26 +261,14 @@ def deck(name, dtype):
     return RomInputs(outdoor_overrides=ovr)
 
 
-# kernels timed with HIP events: every dom_stride()-th DOM sweep launch of
-# the timed steps records its events (event-recording launches slow the
-# step: by 0.19 ms per step at 256x64x48 when every sweep launch records,
-# so one launch in DOM_STRIDE_MIN or more records; a timed
-# launch is never captured into a graph); every SPECIES_STRIDE-th k_species3
-# launch of SPECIES_STEPS extra steps after them, with the step's graph
-# replay off (the sub-step loop is otherwise one graph launch)
+# Kernels timed with HIP events, in dom_every extra steps after the timed
+# ones (so no event-recording launch slows the timed steps): every DOM sweep
+# launch of the one DOM solve among them, with the solve issued in line
+# (beside the flow work it shares the GPU with other kernels, which is not
+# the kernel's own rate), and every SPECIES_STRIDE-th k_species3 launch; the
+# step's graph replay is off meanwhile (a timed launch is never captured)
 TIMER_SPECIES3, TIMER_DOM_SWEEP = 0, 1
-SPECIES_STRIDE, SPECIES_STEPS = 4, 4
-DOM_STRIDE_MIN = 8
-
-
-def dom_stride(launches_per_iteration):
-    """The DOM timer's stride: the first one from DOM_STRIDE_MIN that is
-    coprime with the sweep launches per source iteration, so the sampled
-    launches cycle through every anti-diagonal position (their sizes differ)."""
-    s = DOM_STRIDE_MIN
-    while math.gcd(s, launches_per_iteration) != 1:
-        s += 1
-    return s
+SPECIES_STRIDE = 4
 
 
 def sweep_c4():
@@ -375,12 +366,12 @@ def cpu_baseline(name, dom_every):
             "step_s": times}
 
 
-def _roofline(kernel, bytes_launch, ms, launches, workload, dtype, shape_key=None):
-    """shape_key: the launch shape the counter figure in ROOFLINE_TRAFFIC
-    was measured at, when the kernel's traffic depends on it (the DOM tile
-    side); None when it was not that shape."""
+def _roofline(kernel, bytes_launch, ms, launches, workload, dtype, shape_key=0):
+    """shape_key: the launch shape the kernel ran at, when its traffic
+    depends on it (the DOM tile side; 0 otherwise); traffic is None when no
+    counter figure exists for that shape."""
     achieved = bytes_launch / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    traffic = ROOFLINE_TRAFFIC.get((kernel, workload, dtype)) if shape_key is not False else None
+    traffic = ROOFLINE_TRAFFIC.get((kernel, workload, dtype, shape_key))
     return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
@@ -411,6 +402,9 @@ def main(argv=None):
     ap.add_argument("--tune", type=int, default=1,
                     help="1: run the autotuner on the spun-up state before the warm-up "
                          "(untimed) and report its choice; 0: the default launch shapes")
+    ap.add_argument("--dom-tile", type=int, default=0, choices=(0, 2, 4, 8),
+                    help="DOM sweep tile side to use after the autotuner (0: the tuner's "
+                         "choice, or 4 with --tune 0); the counter runs fix it")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (the "
                          "known-size kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -454,8 +448,15 @@ def main(argv=None):
     print(f"# spin-up and warm-up done: {spinup + a.warmup} steps, t = {sv.t:.4f} s",
           file=sys.stderr, flush=True)
     TS = int(tuned["dom_tile"]) if tuned else 4     # the sweep's tile side
+    if a.dom_tile:
+        import ctypes as C
+        from firewebapp_amd import _lib
+        act = C.c_int()
+        _lib.call("fw_dom_set_tiling", sv.dom_h, a.dom_tile, C.byref(act))
+        TS = act.value
+        if tuned:
+            tuned["dom_tile"] = TS
     sweep_launches_per_iter = -(-g.Ny // TS) + -(-g.Nz // TS) - 1
-    kernel_timer(TIMER_DOM_SWEEP, dom_stride(sweep_launches_per_iter))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -468,16 +469,24 @@ def main(argv=None):
     # the DOM intensities among them, are outside torch's allocator)
     free_b, total_b = torch.cuda.mem_get_info()
     hbm_used = total_b - free_b
-    kernel_timer(TIMER_DOM_SWEEP, 0)
     print(f"# timed {a.steps} steps in {wall:.3f} s, t = {t_win0:.4f} .. {sv.t:.4f} s",
           file=sys.stderr, flush=True)
     graph_stats = sv.graph_stats()
     graphs, sv.fields.graphs = sv.fields.graphs, None
+    beside = os.environ.get("FIREWAVE_DOM_BESIDE")
+    os.environ["FIREWAVE_DOM_BESIDE"] = "0"
     kernel_timer(TIMER_SPECIES3, SPECIES_STRIDE)
-    for i in range(SPECIES_STEPS):
+    kernel_timer(TIMER_DOM_SWEEP, 1)
+    timing_steps = [sv.step_no + 1, sv.step_no + cfg.dom_subcycle_every]
+    for i in range(cfg.dom_subcycle_every):
         sv.step()
     torch.cuda.synchronize()
     kernel_timer(TIMER_SPECIES3, 0)
+    kernel_timer(TIMER_DOM_SWEEP, 0)
+    if beside is None:
+        del os.environ["FIREWAVE_DOM_BESIDE"]
+    else:
+        os.environ["FIREWAVE_DOM_BESIDE"] = beside
     sv.fields.graphs = graphs
     esz = 4 if a.dtype == "float32" else 8
     M = sv.dom.M
@@ -506,7 +515,7 @@ def main(argv=None):
                      "value": cells * total_steps / (t1 - t_run0),
                      "ms_per_step": 1e3 * (t1 - t_run0) / total_steps},
         "roofline": _roofline("k_dom_sweep_tile", dom_bytes, dom_ms / max(dom_n, 1), dom_n,
-                              a.workload, a.dtype, shape_key=TS == 4),
+                              a.workload, a.dtype, shape_key=TS),
         "roofline_species3": _roofline("k_species3", SPECIES3_FIELDS * esz * cells,
                                        sp_ms / max(sp_n, 1), sp_n, a.workload, a.dtype),
         "cpu_baseline": None,
@@ -518,8 +527,9 @@ def main(argv=None):
     line["roofline"]["definition"] = (
         f"{DOM_PASSES_PER_ORDINATE} field passes per ordinate per source iteration "
         f"x {M} ordinates x {cells} cells x {esz} B / {sweep_launches_per_iter} launches")
-    line["roofline"]["sampled"] = (f"every {dom_stride(sweep_launches_per_iter)}th sweep "
-                                   f"launch of the timed steps")
+    line["roofline"]["sampled"] = (f"every sweep launch of the DOM solve in steps "
+                                   f"{timing_steps[0]}-{timing_steps[1]} (after the timed "
+                                   f"steps; the solve in line)")
     bpc = step_bytes_per_cell(esz, 10, M, cfg.dom_subcycle_every, sv.n_particles, cells)
     ach = bpc * line["value"] / 1e9
     line["step_roofline"] = {

@@ -2,7 +2,8 @@
 whose step is interpreted Python).
 
 For a solver's grid it times the candidates of each tunable launch shape on
-the solver's current state and applies the fastest:
+the solver's current state and applies the fastest (the default unless
+another is faster by more than MARGIN, 5 %):
 
 * the projection's inner PCG (``fw_proj_set_tiling``): the one-pass y-z
   preconditioner kernel or the column transforms, the operator's workgroup
@@ -42,6 +43,11 @@ from . import _lib
 PROJ_CANDIDATES = ((1, 1024, 16), (1, 512, 16), (1, 2048, 16), (1, 1024, 8), (0, 512, 16))
 DOM_CANDIDATES = (4, 2, 8)
 CZ_CANDIDATES = (2, 1, 4)
+
+# a candidate replaces the default only when it is faster by more than this
+# fraction, so shapes within the timing noise of each other do not flip from
+# run to run
+MARGIN = 0.05
 
 _lock = threading.Lock()
 _memo: dict = {}     # in-process results by key (batch cases of one shape)
@@ -183,6 +189,14 @@ def _tune_species(sv, reps):
     return out
 
 
+def _pick(times: dict, default):
+    """The fastest candidate, or the default when none beats it by MARGIN."""
+    best = min(times, key=times.get)
+    if default in times and times[best] > (1.0 - MARGIN) * times[default]:
+        return default
+    return best
+
+
 def apply(sv, choice: dict):
     """Set a tuning result on the solver's handles."""
     _lib.call("fw_proj_set_tiling", sv.proj, *choice["proj"], None)
@@ -208,9 +222,9 @@ def tune(sv, path: Optional[Path] = None, reps: int = 3, retune: bool = False) -
         dom = _tune_dom(sv, reps)
         cz = _tune_species(sv, reps)
         choice = {
-            "proj": list(min(proj, key=proj.get)),
-            "dom_tile": int(min(dom, key=dom.get)),
-            "species_cz": int(min(cz, key=cz.get)),
+            "proj": list(_pick(proj, PROJ_CANDIDATES[0])),
+            "dom_tile": int(_pick(dom, DOM_CANDIDATES[0])),
+            "species_cz": int(_pick(cz, CZ_CANDIDATES[0])),
             "ms": {"proj_solve": {"%d/%d/%d" % k: round(v, 4) for k, v in proj.items()},
                    "dom_per_iteration": {str(k): round(v, 4) for k, v in dom.items()},
                    "species3_launch": {str(k): round(v, 5) for k, v in cz.items()}},
