@@ -159,6 +159,42 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
   return block_sum_wg(a, sh);
 }
 
+// Two such sums at once (their loads in flight together, one barrier), each
+// in the same order as sum_partials; sh holds two doubles per wavefront.
+__device__ __forceinline__ void sum_partials2(const double* __restrict__ p1, int n1,
+                                              const double* __restrict__ p2, int n2,
+                                              double* sh, double& s1, double& s2) {
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n1; i += blockDim.x) a += p1[i];
+  for (int i = threadIdx.x; i < n2; i += blockDim.x) b += p2[i];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  const int nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[threadIdx.x >> 6] = a;
+    sh[nw + (threadIdx.x >> 6)] = b;
+  }
+  __syncthreads();
+  double x = sh[0], y = sh[nw];
+  for (int q = 1; q < nw; ++q) {
+    x += sh[q];
+    y += sh[nw + q];
+  }
+  s1 = x;
+  s2 = y;
+}
+
+// Orders a wavefront's LDS accesses before and after it, without a
+// workgroup barrier: for data that only this wavefront writes and reads (a
+// wavefront's LDS operations complete in issue order).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // The inner (Krylov) operator: A x at cell (i, j, k) in precision T from a
 // precomputed 1 / max(rho, 0.01) field, with the metric reciprocals
 // multiplied in.  It equals apply_A up to rounding; the refinement's float64
@@ -1368,7 +1404,7 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
             DctFuse<float> f) {
   constexpr int DT_THREADS = 32 * DT_ROWS, DT_RPW = 2;
   extern __shared__ float sm[];
-  __shared__ double shr[DT_THREADS / 64];
+  __shared__ double shr[2 * (DT_THREADS / 64)];
   const int M = N >> 1, LX = N + 1;       // LX: conflict-free tile row
   float* A = sm;                          // DT_ROWS x N: the complex FFT rows
   float* X = sm + DT_ROWS * N;            // DT_ROWS x LX: tile / row staging
@@ -1427,9 +1463,8 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
     }
   }
   if constexpr (MODE == DCT_UPDATE) {
-    const double rz = sum_partials(f.Prz, f.npz, shr);
-    __syncthreads();
-    const double pq = sum_partials(f.Ppq, f.npq, shr);
+    double rz, pq;
+    sum_partials2(f.Prz, f.npz, f.Ppq, f.npq, shr, rz, pq);
     __syncthreads();
     const double ad = rz / pq;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1489,68 +1524,76 @@ k_dct4_tile(int nrows, int N, int logM, const float* __restrict__ tw,
   const float* pre = TW;
   const float* post = TW + 2 * M;
   const float* W = TW + 4 * M;
-  // pre-twiddle of the pairs (2n, N - 1 - 2n) into bit-reversed order
+  // pre-twiddle of the pairs (2n, N - 1 - 2n) into bit-reversed order.
+  // From here to the post-twiddle a wavefront touches only its own rows of
+  // A (and of X on the input side), so a wavefront-level ordering of its
+  // LDS accesses stands in for the workgroup barriers; the complex values
+  // move as 8-byte pairs.
+  const float2* pre2 = reinterpret_cast<const float2*>(pre);
+  const float2* W2 = reinterpret_cast<const float2*>(W);
 #pragma unroll
   for (int rr = 0; rr < DT_RPW; ++rr) {
     const int rl = wv * DT_RPW + rr, row = row0 + rl;
-    float* a = A + rl * N;
+    float2* a = reinterpret_cast<float2*>(A + rl * N);
     const float* x = X + rl * LX;
 #pragma unroll
     for (int u = 0; u < NL / 2; ++u) {
       const int n = lane + 64 * u;
       if (row < nrows && n < M) {
         const float re = x[2 * n], im = x[N - 1 - 2 * n];
-        const float pr = pre[2 * n], pi = pre[2 * n + 1];
+        const float2 pw = pre2[n];
         const int r = (int)(__brev((unsigned)n) >> (32 - logM));
-        a[2 * r] = re * pr - im * pi;
-        a[2 * r + 1] = re * pi + im * pr;
+        a[r] = make_float2(re * pw.x - im * pw.y, re * pw.y + im * pw.x);
       }
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   for (int st = 0; st < logM; ++st) {
     const int half = 1 << st;
     for (int rr = 0; rr < DT_RPW; ++rr) {
       const int rl = wv * DT_RPW + rr;
       if (row0 + rl >= nrows) break;
-      float* a = A + rl * N;
+      float2* a = reinterpret_cast<float2*>(A + rl * N);
       for (int b = lane; b < (M >> 1); b += 64) {
         const int pos = b & (half - 1);
         const int i1 = ((b >> st) << (st + 1)) + pos;
         const int i2 = i1 + half;
         const int t = pos << (logM - 1 - st);
-        const float wr = W[2 * t], wi = W[2 * t + 1];
-        const float xr = a[2 * i2], xi = a[2 * i2 + 1];
-        const float tr = wr * xr - wi * xi, ti = wr * xi + wi * xr;
-        const float ur = a[2 * i1], ui = a[2 * i1 + 1];
-        a[2 * i1] = ur + tr;
-        a[2 * i1 + 1] = ui + ti;
-        a[2 * i2] = ur - tr;
-        a[2 * i2 + 1] = ui - ti;
+        const float2 wv2 = W2[t];
+        const float2 xv = a[i2];
+        const float tr = wv2.x * xv.x - wv2.y * xv.y, ti = wv2.x * xv.y + wv2.y * xv.x;
+        const float2 uv = a[i1];
+        a[i1] = make_float2(uv.x + tr, uv.y + ti);
+        a[i2] = make_float2(uv.x - tr, uv.y - ti);
       }
     }
-    __syncthreads();
+    wave_lds_sync();
   }
   // post-twiddle into the staging rows (memory order)
   const float sc = (float)sqrt(2.0 / N);
+  const float2* post2 = reinterpret_cast<const float2*>(post);
 #pragma unroll
   for (int rr = 0; rr < DT_RPW; ++rr) {
     const int rl = wv * DT_RPW + rr, row = row0 + rl;
     if (row >= nrows) break;
-    const float* a = A + rl * N;
+    const float2* a = reinterpret_cast<const float2*>(A + rl * N);
 #pragma unroll
     for (int u = 0; u < NL / 2; ++u) {
       const int kk = lane + 64 * u;
       if (kk >= M) break;
-      const float vr = a[2 * kk], vi = a[2 * kk + 1];
-      const float pr = post[2 * kk], pi = post[2 * kk + 1];
+      const float2 av = a[kk], pv = post2[kk];
+      const float vr = av.x, vi = av.y;
+      const float pr = pv.x, pi = pv.y;
       const float y0 = (vr * pr - vi * pi) * sc;
       const float y1 = -((vr * pi + vi * pr) * sc);
       X[rl * LX + 2 * kk] = y0;
       X[rl * LX + N - 1 - 2 * kk] = y1;
     }
   }
-  __syncthreads();
+  // the row output reads the wavefront's own rows; the transposed tile
+  // output reads every row of the workgroup
+  if constexpr (TIN) wave_lds_sync();
+  else __syncthreads();
   if constexpr (TIN) {
     // whole rows out, and the dot with r in the same order
 #pragma unroll
