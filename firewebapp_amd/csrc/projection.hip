@@ -255,6 +255,86 @@ __device__ __forceinline__ T apply_A_irho(const GridArgs& g, const T* __restrict
   return apply_A_irho_f<T>(g, ir, [x](uint32_t o) { return at(x, o); }, i, k, n, eps_reg);
 }
 
+// The operator's seven points of one field, read ahead of their use.
+template <typename T>
+struct Pts7 {
+  T c, xm, xp, ym, yp, zm, zp;
+};
+
+// Byte offsets of cell (i, j, k)'s stencil points with the missing x and z
+// neighbours of the boundary cells replaced by the cell itself (always a
+// valid address; apply_A_pts never uses those values).
+template <typename T>
+__device__ __forceinline__ Nbr nbr_offsets_in(const GridArgs& g, int i, int j, int k) {
+  Nbr n = nbr_offsets<T>(g, i, j, k);
+  if (i == 0) n.xm1 = n.c;
+  if (i == g.nx - 1) n.xp1 = n.c;
+  if (k == 0) n.zm1 = n.c;
+  if (k == g.nz - 1) n.zp1 = n.c;
+  return n;
+}
+
+template <typename T>
+__device__ __forceinline__ Pts7<T> load_pts(const T* __restrict__ f, const Nbr& n) {
+  Pts7<T> v;
+  v.c = at(f, n.c);
+  v.xm = at(f, n.xm1);
+  v.xp = at(f, n.xp1);
+  v.ym = at(f, n.ym1);
+  v.yp = at(f, n.yp1);
+  v.zm = at(f, n.zm1);
+  v.zp = at(f, n.zp1);
+  return v;
+}
+
+// apply_A_irho_f on preloaded point values: the same operations in the same
+// order.
+template <typename T>
+__device__ __forceinline__ T apply_A_pts(const GridArgs& g, const Pts7<T>& ir,
+                                         const Pts7<T>& x, int i, int k, T eps_reg) {
+  const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
+  const T irc = ir.c, xc = x.c;
+  const T idx2 = T(g.inv_dx2), idy2 = T(g.inv_dy2);
+  T diag = eps_reg;
+  T acc = T(0);
+  if (Nx > 1) {
+    if (i < Nx - 1) {
+      const T cxp = T(0.5) * (irc + ir.xp) * idx2;
+      diag -= cxp;
+      acc += cxp * x.xp;
+    } else {
+      diag -= T(2) * irc * idx2;
+    }
+    if (i > 0) {
+      const T cxm = T(0.5) * (ir.xm + irc) * idx2;
+      diag -= cxm;
+      acc += cxm * x.xm;
+    }
+  }
+  if (Ny > 1) {
+    const T cyp = T(0.5) * (irc + ir.yp) * idy2;
+    const T cym = T(0.5) * (ir.ym + irc) * idy2;
+    diag -= cyp + cym;
+    acc += cyp * x.yp + cym * x.ym;
+  }
+  if (Nz > 1) {
+    const T idz = T(g.inv_dz[k]);
+    if (k < Nz - 1) {
+      const T czp = T(0.5) * (irc + ir.zp) * (idz * T(g.inv_da[k]));
+      diag -= czp;
+      acc += czp * x.zp;
+    } else {
+      diag -= T(2) * irc * idz * idz;
+    }
+    if (k > 0) {
+      const T czm = T(0.5) * (ir.zm + irc) * (idz * T(g.inv_db[k]));
+      diag -= czm;
+      acc += czm * x.zm;
+    }
+  }
+  return diag * xc + acc;
+}
+
 struct Cell {
   int i, j, k;
 };
@@ -373,25 +453,63 @@ k_matvec_pz(GridArgs g, const T* __restrict__ ir, const T* __restrict__ p,
             double* __restrict__ Ppq) {
   __shared__ double sh[4];
   if (S[S_DONE] != 0.0) return;
+  int r0, r1;
+  row_range(g.ny * g.nz, r0, r1);
+  // items: (row, cell chunk) in row order, one cell per thread each; the
+  // first MVI items' stencil points of ir, z and p are loaded before the
+  // sum of the <r, z> partials (beta), so one memory latency covers both
+  const int cpr = (g.nx + 255) >> 8;
+  const int nit = (r1 - r0) * cpr;
+  constexpr int MVI = sizeof(T) == 4 ? 4 : 2;
+  Pts7<T> vi[MVI], vz[MVI], vp[MVI];
+#pragma unroll
+  for (int it = 0; it < MVI; ++it) {
+    if (it < nit) {
+      const int row = r0 + it / cpr, i = threadIdx.x + 256 * (it % cpr);
+      if (i < g.nx) {
+        const int k = row / g.ny, j = row - k * g.ny;
+        const Nbr nb = nbr_offsets_in<T>(g, i, j, k);
+        vi[it] = load_pts(ir, nb);
+        vz[it] = load_pts(z, nb);
+        vp[it] = load_pts(p, nb);
+      }
+    }
+  }
   const double bd = sum_partials(Prz, npz, sh) / S[S_RZ];
   __syncthreads();   // sh is reused by the block sum below
   if (blockIdx.x == 0 && threadIdx.x == 0) S[S_BETA] = bd;
   const T beta = (T)bd;
-  const auto pz = [p, z, beta](uint32_t o) { return at(z, o) + beta * at(p, o); };
-  int r0, r1;
-  row_range(g.ny * g.nz, r0, r1);
+  // p+ = z + beta p at one point
+  auto pz = [beta](T zv, T pv) { return zv + beta * pv; };
   double acc = 0.0;
-  for (int row = r0; row < r1; ++row) {
+  auto item = [&](int it, const Pts7<T>& ri, const Pts7<T>& zz, const Pts7<T>& pp) {
+    const int row = r0 + it / cpr, i = threadIdx.x + 256 * (it % cpr);
+    if (i >= g.nx) return;
     const int k = row / g.ny, j = row - k * g.ny;
-    const double dzk = g.dz[k];
-    for (int i = threadIdx.x; i < g.nx; i += 256) {
-      const Nbr nb = nbr_offsets<T>(g, i, j, k);
-      const T y = apply_A_irho_f<T>(g, ir, pz, i, k, nb, eps_reg);
-      const T pc = pz(nb.c);
-      at(q, nb.c) = y;
-      at(pn, nb.c) = pc;
-      acc += dzk * (double)pc * (double)y;
-    }
+    const uint32_t c = (((uint32_t)k * (uint32_t)g.ny + (uint32_t)j) * (uint32_t)g.nx +
+                        (uint32_t)i) * (uint32_t)sizeof(T);
+    Pts7<T> x;
+    x.c = pz(zz.c, pp.c);
+    x.xm = pz(zz.xm, pp.xm);
+    x.xp = pz(zz.xp, pp.xp);
+    x.ym = pz(zz.ym, pp.ym);
+    x.yp = pz(zz.yp, pp.yp);
+    x.zm = pz(zz.zm, pp.zm);
+    x.zp = pz(zz.zp, pp.zp);
+    const T y = apply_A_pts<T>(g, ri, x, i, k, eps_reg);
+    at(q, c) = y;
+    at(pn, c) = x.c;
+    acc += g.dz[k] * (double)x.c * (double)y;
+  };
+#pragma unroll
+  for (int it = 0; it < MVI; ++it)
+    if (it < nit) item(it, vi[it], vz[it], vp[it]);
+  for (int it = MVI; it < nit; ++it) {
+    const int row = r0 + it / cpr, i = threadIdx.x + 256 * (it % cpr);
+    if (i >= g.nx) continue;
+    const int k = row / g.ny, j = row - k * g.ny;
+    const Nbr nb = nbr_offsets_in<T>(g, i, j, k);
+    item(it, load_pts(ir, nb), load_pts(z, nb), load_pts(p, nb));
   }
   const double t = block_sum_wg(acc, sh);
   if (threadIdx.x == 0) Ppq[blockIdx.x] = t;
@@ -1136,6 +1254,18 @@ k_yz_plane(int ldt, int Ny, int Nz, int ky4, const float* __restrict__ Yfp,
 #pragma unroll
     for (int u = 0; u < 4; ++u) za[mo][u] = p[u];
   }
+  // the spectral divide's eigenvalue sums, loaded with the operands (a load
+  // between the products would hold them on its latency)
+  const int jc = 16 * wv + (lane & 15);
+  const double sd = ly[jc < Ny ? jc : Ny - 1] + lx[blockIdx.x] - eps;
+  double lzv[YZ_MT][4];
+#pragma unroll
+  for (int mo = 0; mo < YZ_MT; ++mo)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = 16 * mo + 4 * (lane >> 4) + r;
+      lzv[mo][r] = m < Nz ? lz[m] : 0.0;
+    }
 #pragma unroll
   for (int i = 0; i < YZ_MT * 4; ++i) pl[(4 * i + kt) * S + jt] = v[i];
   __syncthreads();
@@ -1171,7 +1301,6 @@ k_yz_plane(int ldt, int Ny, int Nz, int ky4, const float* __restrict__ Yfp,
   }
   __syncthreads();   // P is no longer read: C may overwrite it
   // B = D^-1 (Zf A)
-  const double sd = ly[j0 + c < Ny ? j0 + c : Ny - 1] + lx[blockIdx.x] - eps;
 #pragma unroll
   for (int mo = 0; mo < YZ_MT; ++mo) {
     if (mo >= MT) continue;
@@ -1187,7 +1316,7 @@ k_yz_plane(int ldt, int Ny, int Nz, int ky4, const float* __restrict__ Yfp,
     for (int r = 0; r < 4; ++r) {
       const int m = 16 * mo + 4 * q + r;
       // rows past Nz stay zero: the inverse's zero-padded A columns meet them
-      acc[r] = m < Nz ? acc[r] * recip((float)(lz[m] + sd)) : 0.0f;
+      acc[r] = m < Nz ? acc[r] * recip((float)(lzv[mo][r] + sd)) : 0.0f;
     }
     acc2[mo] = acc;
   }
