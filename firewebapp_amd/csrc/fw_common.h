@@ -123,6 +123,10 @@ int dom_fork(void* dom, hipStream_t s, hipStream_t* side);
 int dom_join_record(void* dom);
 int dom_join(void* dom, hipStream_t s);
 
+// Per-workgroup maxima of the interior nu_t into work[FW_REDUCE_BLOCKS]
+// (transport.hip), without the host read of fw_nu_t_max_interior.
+int nu_t_max_partials(const fw_grid* g, const void* nu_t, double* work, hipStream_t s);
+
 inline Launch cell_launch(const fw_grid* g) {
   int bx = 64;
   while (bx < g->nx && bx < 256) bx *= 2;

@@ -131,6 +131,66 @@ __global__ void k_set_step_scalars(double* __restrict__ sb, double dt_sub, doubl
   }
 }
 
+// The largest interior nu_t, for the sub-step plan, read without a stall:
+// the partial maxima are reduced on the device into red_out[0] (scratch until
+// the end-of-step reduction) and copied into the pinned red_host[0] behind an
+// event as soon as k-epsilon has written nu_t; the host waits on the event
+// only when it plans the sub-steps, after issuing the work in between.
+__global__ void __launch_bounds__(256)
+k_max_partials(const double* __restrict__ part, int np, double* __restrict__ out) {
+  __shared__ double sh[4];
+  double m = 0.0;   // the host's fold over the partials started from 0 as well
+  for (int b = threadIdx.x; b < np; b += 256) {
+    const double x = part[b];
+    if (x > m) m = x;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_down(m, off, 64);
+    if (o > m) m = o;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = sh[0];
+    for (int q = 1; q < 4; ++q)
+      if (sh[q] > a) a = sh[q];
+    out[0] = a;
+  }
+}
+
+struct NuMaxRead {
+  hipEvent_t ev = nullptr;
+  int dev = -1;
+};
+static thread_local NuMaxRead g_numax;
+
+static int nu_max_issue(const fw_grid* g, fw_spread_fields* f, hipStream_t s) {
+  int dev = 0;
+  FW_CHECK_HIP(hipGetDevice(&dev));
+  if (g_numax.ev && g_numax.dev != dev) {
+    (void)hipEventDestroy(g_numax.ev);
+    g_numax.ev = nullptr;
+  }
+  if (!g_numax.ev) {
+    FW_CHECK_HIP(hipEventCreateWithFlags(&g_numax.ev, hipEventDisableTiming));
+    g_numax.dev = dev;
+  }
+  TRY(nu_t_max_partials(g, f->nu_t, f->nu_work, s));
+  hipLaunchKernelGGL(k_max_partials, dim3(1), dim3(256), 0, s, (const double*)f->nu_work,
+                     FW_REDUCE_BLOCKS, f->red_out);
+  FW_CHECK_LAUNCH("nu_t max");
+  FW_CHECK_HIP(hipMemcpyAsync(f->red_host, f->red_out, sizeof(double), hipMemcpyDeviceToHost,
+                              s));
+  FW_CHECK_HIP(hipEventRecord(g_numax.ev, s));
+  return FW_OK;
+}
+
+static int nu_max_wait(const fw_spread_fields* f, double* nu_max) {
+  FW_CHECK_HIP(hipEventSynchronize(g_numax.ev));
+  *nu_max = f->red_host[0];
+  return FW_OK;
+}
+
 // What the sub-step loop of a step depends on besides the fields.
 struct SubstepPlan {
   int nsc, npr;              // turbulent diffusion passes (species, T_g)
@@ -431,6 +491,9 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                    p->canopy_beta_p, p->canopy_beta_d, stream));
   SWAP(f->k, f->tmp1);
   SWAP(f->eps, f->tmp2);
+  // the sub-step plan's max nu_t (nothing writes nu_t between here and the
+  // sub-steps), read back while the GPU runs the work below
+  TRY(nu_max_issue(g, f, s));
   // the DOM's results, then the soil every step (:2505-2510)
   if (beside) TRY(dom_side.join(s));
   else if (dom_now) TRY(dom_solve(s));
@@ -445,7 +508,7 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
   SWAP(f->T_g, f->tmp1);
   // N_SUB chemistry / transport / coupling sub-steps (:2678-3200)
   double nu_max = 0.0;
-  TRY(fw_nu_t_max_interior(g, f->nu_t, f->nu_work, &nu_max, stream));
+  TRY(nu_max_wait(f, &nu_max));
   SubstepPlan plan;
   plan.dt_sub = dt / N_SUB;
   plan.dsc = plan.dpr = 0.0;

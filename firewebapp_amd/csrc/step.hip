@@ -204,11 +204,14 @@ k_step_reduce(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict
               const T* __restrict__ om, const T* __restrict__ YF,
               const T* __restrict__ alpha, double T_ign, double T_front, int nzb,
               double* __restrict__ part) {
-  __shared__ double sh[NRED][256];
+  __shared__ double sh[NRED][4];
   double r[NRED];
   for (int q = 0; q < NRED; ++q) r[q] = -1e300;
   r[5] = 0.0; r[11] = 0.0; r[12] = 0.0; r[13] = -1.0;
   const size_t plane = (size_t)g.nx * g.ny;
+  // unrolled so the loads of several cells are in flight together (one
+  // wavefront per SIMD at this launch size)
+#pragma unroll 4
   for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
        c += (size_t)gridDim.x * 256) {
     const double au = fabs((double)u[c]), av = fabs((double)v[c]), aw = fabs((double)w[c]);
@@ -236,19 +239,29 @@ k_step_reduce(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict
       r[13] = i > r[13] ? i : r[13];
     }
   }
+  // maxima and counts (exact in any order) within each wavefront by
+  // shuffles, then over the four wavefronts
   const int t = threadIdx.x;
-  for (int q = 0; q < NRED; ++q) sh[q][t] = r[q];
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (t < s)
-      for (int q = 0; q < NRED; ++q) {
-        if (q == 11 || q == 12) sh[q][t] += sh[q][t + s];
-        else if (sh[q][t + s] > sh[q][t]) sh[q][t] = sh[q][t + s];
-      }
-    __syncthreads();
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int q = 0; q < NRED; ++q) {
+      const double o = __shfl_down(r[q], off, 64);
+      if (q == 11 || q == 12) r[q] += o;
+      else if (o > r[q]) r[q] = o;
+    }
   }
-  if (t == 0)
-    for (int q = 0; q < NRED; ++q) part[(size_t)q * gridDim.x + blockIdx.x] = sh[q][0];
+  if ((t & 63) == 0)
+    for (int q = 0; q < NRED; ++q) sh[q][t >> 6] = r[q];
+  __syncthreads();
+  if (t < NRED) {
+    double a = sh[t][0];
+    for (int wq = 1; wq < (int)(blockDim.x >> 6); ++wq) {
+      const double o = sh[t][wq];
+      if (t == 11 || t == 12) a += o;
+      else if (o > a) a = o;
+    }
+    part[(size_t)t * gridDim.x + blockIdx.x] = a;
+  }
 }
 
 __global__ void __launch_bounds__(256)

@@ -769,6 +769,10 @@ static int nu_t_max_impl(const fw_grid* g, const void* nu_t, double* work,
   return FW_OK;
 }
 
+int nu_t_max_partials(const fw_grid* g, const void* nu_t, double* work, hipStream_t s) {
+  return FW_DISPATCH(g, nu_t_max_impl, g, nu_t, work, s);
+}
+
 template <typename T>
 static int solid_cond_impl(const fw_grid* g, const void* Ts, void* To,
                            const void* alpha, double k_solid, double rho_solid,
@@ -969,7 +973,7 @@ int fw_nu_t_max_interior(const fw_grid* g, const void* nu_t, double* work,
   FW_GRID(g);
   FW_REQUIRE(g && work && out_host, "nu_t_max_interior: null pointer");
   hipStream_t s = (hipStream_t)stream;
-  int rc = FW_DISPATCH(g, nu_t_max_impl, g, nu_t, work, s);
+  int rc = nu_t_max_partials(g, nu_t, work, s);
   if (rc) return rc;
   double h[FW_REDUCE_BLOCKS];
   FW_CHECK_HIP(hipMemcpyAsync(h, work, sizeof(h), hipMemcpyDeviceToHost, s));
