@@ -735,6 +735,22 @@ k_sumsq(size_t n, const double* __restrict__ a, double* __restrict__ part) {
   block_sum_store(acc, sh, part);
 }
 
+// The warm start from the last two solutions: x <- 2 x - xp, xp <- x.
+__global__ void __launch_bounds__(256)
+k_extrapolate(size_t n, double* __restrict__ x, double* __restrict__ xp) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= n) return;
+  const double t = x[c];
+  x[c] = 2.0 * t - xp[c];
+  xp[c] = t;
+}
+
+__global__ void __launch_bounds__(256)
+k_copy64(size_t n, const double* __restrict__ x, double* __restrict__ y) {
+  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (c < n) y[c] = x[c];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_x64_add(size_t n, double* __restrict__ x, const T* __restrict__ d) {
@@ -888,15 +904,23 @@ k_div_rhs(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict__ v
           const T* __restrict__ target, double inv_dt, double* __restrict__ b,
           double* __restrict__ part) {
   __shared__ double sh[256];
+  (void)n;
+  // contiguous runs of rows in the XCD-aware order (the stencil's y and z
+  // neighbour rows mostly in the same L2), two rows' loads in flight
+  int r0, r1;
+  row_range(g.ny * g.nz, r0, r1);
   double acc = 0.0;
-  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * 256) {
-    Cell e = cell_of(g, c);
-    double d = div_at<T>(g, u, v, w, u_inlet, e.k, e.j, e.i, c);
-    if (target) d = d - (double)target[c];
-    double r = d * inv_dt;
-    if (b) b[c] = r;
-    acc += r * r;
+#pragma unroll 2
+  for (int row = r0; row < r1; ++row) {
+    const int k = row / g.ny, j = row - k * g.ny;
+    for (int i = threadIdx.x; i < g.nx; i += 256) {
+      const size_t c = ((size_t)k * g.ny + j) * g.nx + i;
+      double d = div_at<T>(g, u, v, w, u_inlet, k, j, i, c);
+      if (target) d = d - (double)target[c];
+      const double r = d * inv_dt;
+      if (b) b[c] = r;
+      acc += r * r;
+    }
   }
   block_sum_store(acc, sh, part);
 }
@@ -919,14 +943,20 @@ k_div_check(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict__
             const T* __restrict__ w, const T* __restrict__ u_inlet,
             const T* __restrict__ target, double* __restrict__ part) {
   __shared__ double sh[256];
+  (void)n;
+  int r0, r1;
+  row_range(g.ny * g.nz, r0, r1);
   double m = 0.0;
-  for (size_t c = (size_t)blockIdx.x * 256 + threadIdx.x; c < n;
-       c += (size_t)gridDim.x * 256) {
-    Cell e = cell_of(g, c);
-    double d = div_at<T>(g, u, v, w, u_inlet, e.k, e.j, e.i, c);
-    if (target) d -= (double)target[c];
-    d = fabs(d);
-    if (d > m || d != d) m = (d != d) ? __builtin_inf() : d;
+#pragma unroll 2
+  for (int row = r0; row < r1; ++row) {
+    const int k = row / g.ny, j = row - k * g.ny;
+    for (int i = threadIdx.x; i < g.nx; i += 256) {
+      const size_t c = ((size_t)k * g.ny + j) * g.nx + i;
+      double d = div_at<T>(g, u, v, w, u_inlet, k, j, i, c);
+      if (target) d -= (double)target[c];
+      d = fabs(d);
+      if (d > m || d != d) m = (d != d) ? __builtin_inf() : d;
+    }
   }
   block_max_store(m, sh, part);
 }
@@ -1801,6 +1831,11 @@ struct ProjHandle {
   float* rs;           // r / RMS(r), the network's input
   float* qf;           // the network's density feature of the running solve
   bool have_prev;
+  // float32 grids: the solution of the projection before last (x64p) and
+  // whether it is there; the warm start is then extrapolated linearly from
+  // the last two solutions
+  double* x64p;
+  bool have_prev2;
   int last_iters, last_outer;
   double last_rel, last_div_max;
   int precond_enabled;
@@ -2360,6 +2395,20 @@ static int project_impl(ProjHandle* h, void* u, void* v, void* w,
   if (!h->have_prev) {
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64);
     h->have_prev = true;
+    h->have_prev2 = false;
+  } else if (sizeof(T) == 4) {
+    // float32 (the production path): the pressure moves smoothly from step
+    // to step, and 2 p_{n-1} - p_{n-2} starts the solve about half as far
+    // from the solution as p_{n-1} (one to two PCG iterations of 14 fewer
+    // at 256x64x48); the solve still runs to the same tolerance.  The
+    // float64 build keeps the reference's warm start p_{n-1}.
+    if (h->have_prev2) {
+      hipLaunchKernelGGL(k_extrapolate, dim3(nblk(n)), dim3(256), 0, s, n, h->x64, h->x64p);
+    } else {
+      hipLaunchKernelGGL(k_copy64, dim3(nblk(n)), dim3(256), 0, s, n, (const double*)h->x64,
+                         h->x64p);
+      h->have_prev2 = true;
+    }
   }
   // the warm start's residual, decided on together with ||b|| on the device
   hipLaunchKernelGGL((k_residual64<T, T>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
@@ -2543,6 +2592,8 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   FW_CHECK_HIP(hipMemcpy(h->lz, lam_z, Nz * sizeof(double), hipMemcpyHostToDevice));
   const size_t n = h->n;
   FW_CHECK_HIP(hipMalloc((void**)&h->x64, n * 8));
+  FW_CHECK_HIP(hipMalloc((void**)&h->x64p, n * 8));
+  h->have_prev2 = false;
   FW_CHECK_HIP(hipMalloc((void**)&h->b64, n * 8));
   void** vecs[] = {&h->r, &h->z, &h->p, &h->q, &h->d, &h->p2};
   for (void** v : vecs) FW_CHECK_HIP(hipMalloc(v, n * h->esz));
@@ -2582,7 +2633,7 @@ int fw_proj_destroy(void* handle) {
   ProjHandle* h = (ProjHandle*)handle;
   if (!h) return FW_OK;
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
-                  h->x64, h->b64, h->r, h->z, h->p, h->p2, h->q, h->d, h->w1,
+                  h->x64, h->x64p, h->b64, h->r, h->z, h->p, h->p2, h->q, h->d, h->w1,
                   h->w2, h->part, h->S, h->twx, h->Ppq, h->irho, h->Yf_p, h->Yi_p,
                   h->Zf_p, h->Zi_p, h->ZfQ, h->ZiQ, h->Pqz, h->rs, h->qf};
   for (void* b : bufs)
@@ -2628,6 +2679,7 @@ int fw_proj_set_pressure(void* handle, const double* p_in, void* stream) {
   FW_CHECK_HIP(hipMemcpyAsync(h->x64, p_in, h->n * 8, hipMemcpyDeviceToDevice,
                               (hipStream_t)stream));
   h->have_prev = true;
+  h->have_prev2 = false;   // a new history starts from this pressure
   return FW_OK;
 }
 
@@ -2757,6 +2809,7 @@ int fw_proj_solve(void* handle, const void* rho, const double* b, double* x,
                      S_BB, F_STORE);
   FW_CHECK_LAUNCH("solve setup");
   h->have_prev = true;
+  h->have_prev2 = false;
   int rc = h->dtype == FW_F64
                ? solve_refined<double, double>(h, (const double*)rho, rtol, false, s)
                : solve_refined<float, float>(h, (const float*)rho, rtol, false, s);
