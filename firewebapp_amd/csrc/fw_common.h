@@ -56,9 +56,14 @@ __device__ __forceinline__ double sb_get(const double* sb, int slot, double v) {
 // The species turbulent-diffusion pass; with edc_summary (Nx % 256 == 0) it
 // also leaves the EDC segment summaries of its output in edc_summary.
 // sb (may be null): dt_sub from sb[SB_DSC].
+struct GasSolidParams;
+// gs (may be null): the gas energy update and vertical solid conduction of
+// gas_solid_launch fused into this pass, with the new Y_H2O (fo[2]) of each
+// cell; gs->T_s_out must not alias any field the pass reads or writes.
 int turb3_launch(const fw_grid* g, const void* const* f, void* const* fo, const void* nu_t,
                  double sc_t, double dt_sub, const double* lo, const double* hi,
-                 int* edc_summary, hipStream_t s, const double* sb = nullptr);
+                 int* edc_summary, hipStream_t s, const double* sb = nullptr,
+                 const GasSolidParams* gs = nullptr);
 // fw_edc; summary_ready: the summaries of the current species are in work.
 // sb (may be null): dt from sb[SB_DT_SUB], the relaxation from sb[SB_EDC_RELAX].
 int edc_launch(const fw_grid* g, const void* rho, void* T_g, void* Y_fuel, void* Y_O2,

@@ -16,6 +16,7 @@
 
 #include "fw_common.h"
 #include "graph.h"
+#include "gas_solid.h"
 
 namespace fw {
 
@@ -233,6 +234,35 @@ static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_param
       SWAP(*Y[1], f->tmp2);
       SWAP(*Y[2], f->tmp3);
     }
+    // the particle path's gas update and solid conduction (below) ride in
+    // the last species diffusion pass when there is one: pointwise on the
+    // cell, they read its new Y_H2O; the solid temperature goes to T_s_old
+    // (free scratch during the sub-steps), which no fused workgroup reads
+    const bool fuse_gs = !p->eulerian && P.nsc > 0;
+    GasSolidParams gsp{};
+    if (fuse_gs) {
+      gsp.T_g = f->T_g;
+      gsp.T_s = f->T_s;
+      gsp.T_s_out = f->T_s_old;
+      gsp.alpha_s = f->alpha_s;
+      gsp.rho = f->rho;
+      gsp.S_pyro_part = f->src[0];
+      gsp.S_dry_part = f->src[1];
+      gsp.Q_g_conv = f->src[7];
+      gsp.q_rad_gas = f->q_rad_gas;
+      gsp.Q_comb_out = sub == N_SUB - 1 ? f->Q_comb : nullptr;
+      gsp.n_z_bed = nzb;
+      gsp.i_src0 = s0;
+      gsp.i_src1 = s1;
+      gsp.drip_W_m3 = P.drip_gas;
+      gsp.dt_sub = dt_sub;
+      gsp.T_amb = p->T_amb;
+      gsp.k_solid = K_SOLID_GRASS;
+      gsp.rho_solid = RHO_SOLID;
+      gsp.cp_solid = CP_SOLID;
+      gsp.T_s_lo = p->T_amb;
+      gsp.T_s_hi = T_CAP;
+    }
     // turbulent diffusion; the clips of :3082-3084 fold into the last pass
     // (with no pass they are the identity: each field is already in range)
     for (int m = 0; m < P.nsc; ++m) {
@@ -244,7 +274,8 @@ static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_param
       void* fout[3] = {f->tmp1, f->tmp2, f->tmp3};
       if (launch)
         TRY(turb3_launch(g, fin, fout, f->nu_t, SC_T, P.dsc, lo, hi,
-                         P.fuse_summary && last ? (int*)f->edc_work : nullptr, s, sb));
+                         P.fuse_summary && last ? (int*)f->edc_work : nullptr, s, sb,
+                         fuse_gs && last ? &gsp : nullptr));
       SWAP(*Y[0], f->tmp1);
       SWAP(*Y[1], f->tmp2);
       SWAP(*Y[2], f->tmp3);
@@ -263,7 +294,7 @@ static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_param
         TRY(fw_solid_conduction_vertical(g, f->T_s, f->tmp1, f->alpha_s, K_SOLID_GRASS,
                                          RHO_SOLID, CP_SOLID, dt_sub, p->T_amb, T_CAP, s));
       }
-    } else if (launch) {
+    } else if (launch && !fuse_gs) {
       // gas-energy update and vertical solid conduction with the T_s clamps
       // of :3195-3196, in one pass
       TRY(gas_solid_launch(g, f->T_g, f->T_s, f->tmp1, f->alpha_s, f->rho, f->Y_H2O,
@@ -272,7 +303,8 @@ static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_param
                            dt_sub, p->T_amb, K_SOLID_GRASS, RHO_SOLID, CP_SOLID, p->T_amb,
                            T_CAP, s, sb));
     }
-    SWAP(f->T_s, f->tmp1);
+    if (fuse_gs) SWAP(f->T_s, f->T_s_old);
+    else SWAP(f->T_s, f->tmp1);
     // T_g turbulent diffusion, the clamps of :3193-3194 in the last pass
     for (int m = 0; m < P.npr; ++m) {
       const bool last = m == P.npr - 1;
@@ -305,7 +337,8 @@ static std::vector<uint64_t> substep_key(const fw_grid* g, const fw_spread_field
           kw(f->rho), kw(f->T_g), kw(f->T_s), kw(f->Y_fuel), kw(f->Y_O2), kw(f->Y_H2O),
           kw(f->k), kw(f->eps), kw(f->omega), kw(f->nu_t), kw(f->u), kw(f->v), kw(f->w),
           kw(f->src[0]), kw(f->src[1]), kw(f->src[7]), kw(f->q_rad_gas), kw(f->Q_comb),
-          kw(f->alpha_s), kw(f->tmp1), kw(f->tmp2), kw(f->tmp3), kw(f->edc_work),
+          kw(f->alpha_s), kw(f->tmp1), kw(f->tmp2), kw(f->tmp3), kw(f->T_s_old),
+          kw(f->edc_work),
           kw(f->step_scalars)};
 }
 

@@ -3,6 +3,7 @@
 // chain of whole-array NumPy expressions of the reference with one pass,
 // keeping the reference's per-element operation order.
 #include "fw_common.h"
+#include "gas_solid.h"
 
 namespace fw {
 
@@ -99,65 +100,14 @@ k_gas_update(GridArgs g, T* __restrict__ Tg, const T* __restrict__ Ts,
 // conduction, as in the reference order.
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_gas_solid(GridArgs g, T* __restrict__ Tg, const T* __restrict__ Ts,
-            T* __restrict__ Ts_out, const T* __restrict__ alpha,
-            const T* __restrict__ rho, const T* __restrict__ Yw,
-            const T* __restrict__ Sp, const T* __restrict__ Sd,
-            const T* __restrict__ Qgc, const T* __restrict__ qrg,
-            T* __restrict__ Qcomb_out, int nzb, int i0, int i1, T drip, T dt_sub,
-            T T_amb, T k_solid, T inv_rho_cp, T ts_lo, T ts_hi,
+k_gas_solid(GridArgs g, GasSolidArgs<T> a, const T* __restrict__ Yw,
             const double* __restrict__ sb) {
-  if (sb) dt_sub = T(sb[SB_DT_SUB]);
+  const T dt_sub = sb ? T(sb[SB_DT_SUB]) : a.dt_sub;
   int i, j, k;
   cell_ijk(i, j, k);
   if (i >= g.nx) return;
   const size_t c = cidx(g, k, j, i);
-  const size_t sz = (size_t)g.nx * g.ny;
-  const T tsc = Ts[c];
-  // gas energy (spread_3d.py:2916-2957, :3104-3105, :3129-3146)
-  T Q = T(0);
-  if (k < nzb && i >= i0 && i < i1) Q += drip;
-  Q += qrg[c] / T(g.dz[k]);
-  if (Qcomb_out) Qcomb_out[c] = Q;
-  const T y = Yw[c];
-  const T cp = (T(1) - y) * T(1100.0) + y * T(2000.0);
-  const T tg = Tg[c];
-  T da = tg - tsc;
-  da = da > T(0) ? da : T(0);
-  T db = tg - T(373.15);
-  db = db > T(0) ? db : T(0);
-  const T debit = (Sp[c] * da + Sd[c] * db) * T(2000.0);
-  const T r = rho[c];
-  const T inv = dt_sub / ((r > T(1.0e-3) ? r : T(1.0e-3)) * cp);
-  T nv = tg + (Q - Qgc[c] - debit) * inv;
-  nv = (nv > T_amb || nv != nv) ? nv : T_amb;
-  Tg[c] = nv;
-  // solid conduction
-  const T a = alpha[c];
-  T ns = tsc;
-  if (a > T(0)) {
-    T flux_up = T(0), flux_dn = T(0);
-    if (k < g.nz - 1) {
-      const T aa = alpha[c + sz];
-      if (aa > T(0)) {
-        const T a_eff = T(2) * a * aa / (a + aa);
-        flux_up = k_solid * a_eff * (Ts[c + sz] - tsc) / T(g.d_above[k]);
-      }
-    }
-    if (k > 0) {
-      const T ab = alpha[c - sz];
-      if (ab > T(0)) {
-        const T a_eff = T(2) * a * ab / (a + ab);
-        flux_dn = k_solid * a_eff * (Ts[c - sz] - tsc) / T(g.d_below[k]);
-      }
-    }
-    const T src = (flux_up + flux_dn) / T(g.dz[k]);
-    const T dT = dt_sub * src * inv_rho_cp / a;
-    ns = tsc + dT;
-  }
-  ns = ns > ts_hi ? ts_hi : ns;
-  ns = ns < ts_lo ? ts_lo : ns;
-  Ts_out[c] = ns;
+  gas_solid_cell<T>(g, i, k, c, a, dt_sub, Yw[c]);
 }
 
 // q_rad[k, :, i0:i1] += add   (spread_3d.py:2528-2532)
@@ -483,24 +433,21 @@ int gas_solid_launch(const fw_grid* g, void* T_g, const void* T_s, void* T_s_out
                  S_pyro_part && S_dry_part && Q_g_conv && q_rad_gas,
              "gas_update_solid_conduction: bad argument");
   Launch L = cell_launch(g);
-  const double irc = 1.0 / (rho_solid * cp_solid);
-  if (g->dtype == FW_F64)
-    hipLaunchKernelGGL(k_gas_solid<double>, L.grid, L.block, 0, s, grid_args(g),
-                       (double*)T_g, (const double*)T_s, (double*)T_s_out,
-                       (const double*)alpha_s, (const double*)rho, (const double*)Y_H2O,
-                       (const double*)S_pyro_part, (const double*)S_dry_part,
-                       (const double*)Q_g_conv, (const double*)q_rad_gas, (double*)Q_comb_out,
-                       n_z_bed, i_src0, i_src1, drip_W_m3, dt_sub, T_amb, k_solid, irc,
-                       T_s_lo, T_s_hi, sb);
-  else
-    hipLaunchKernelGGL(k_gas_solid<float>, L.grid, L.block, 0, s, grid_args(g),
-                       (float*)T_g, (const float*)T_s, (float*)T_s_out,
-                       (const float*)alpha_s, (const float*)rho, (const float*)Y_H2O,
-                       (const float*)S_pyro_part, (const float*)S_dry_part,
-                       (const float*)Q_g_conv, (const float*)q_rad_gas, (float*)Q_comb_out,
-                       n_z_bed, i_src0, i_src1, (float)drip_W_m3, (float)dt_sub,
-                       (float)T_amb, (float)k_solid, (float)irc, (float)T_s_lo,
-                       (float)T_s_hi, sb);
+  if (g->dtype == FW_F64) {
+    const GasSolidArgs<double> a = gas_solid_args<double>(
+        T_g, T_s, T_s_out, alpha_s, rho, S_pyro_part, S_dry_part, Q_g_conv, q_rad_gas,
+        Q_comb_out, n_z_bed, i_src0, i_src1, drip_W_m3, dt_sub, T_amb, k_solid, rho_solid,
+        cp_solid, T_s_lo, T_s_hi);
+    hipLaunchKernelGGL(k_gas_solid<double>, L.grid, L.block, 0, s, grid_args(g), a,
+                       (const double*)Y_H2O, sb);
+  } else {
+    const GasSolidArgs<float> a = gas_solid_args<float>(
+        T_g, T_s, T_s_out, alpha_s, rho, S_pyro_part, S_dry_part, Q_g_conv, q_rad_gas,
+        Q_comb_out, n_z_bed, i_src0, i_src1, drip_W_m3, dt_sub, T_amb, k_solid, rho_solid,
+        cp_solid, T_s_lo, T_s_hi);
+    hipLaunchKernelGGL(k_gas_solid<float>, L.grid, L.block, 0, s, grid_args(g), a,
+                       (const float*)Y_H2O, sb);
+  }
   FW_CHECK_LAUNCH("gas_update_solid_conduction");
   return FW_OK;
 }

@@ -13,6 +13,7 @@
 #include <mutex>
 
 #include "fw_common.h"
+#include "gas_solid.h"
 
 namespace fw {
 
@@ -327,13 +328,17 @@ k_turb_diff(GridArgs g, const T* __restrict__ f, T* __restrict__ fo,
 }
 
 // The same pass for three fields sharing nu_t (the species of a sub-step).
-template <typename T>
+// GS: the sub-step's gas energy update and vertical solid conduction fused
+// in (pointwise on the cell; they read its new Y_H2O, and nothing else this
+// pass writes).
+template <typename T, bool GS>
 __global__ void __launch_bounds__(256)
 k_turb_diff3(GridArgs g, const T* __restrict__ f1, const T* __restrict__ f2,
              const T* __restrict__ f3, T* __restrict__ o1, T* __restrict__ o2,
              T* __restrict__ o3, const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo1,
              T hi1, T lo2, T hi2, T lo3, T hi3, int* __restrict__ edc_summary,
-             const double* __restrict__ sb, int sb_slot) {
+             const double* __restrict__ sb, int sb_slot, GasSolidArgs<T> gsa) {
+  const T dt_gs = sb ? T(sb[SB_DT_SUB]) : gsa.dt_sub;
   if (sb) dt_sub = T(sb[sb_slot]);
   int i, j, k;
   cell_ijk(i, j, k);
@@ -359,6 +364,7 @@ k_turb_diff3(GridArgs g, const T* __restrict__ f1, const T* __restrict__ f2,
       edc_summary[c / EDC_SEG] = edc_own_flag<T>(y3) ? 1 : 0;
     }
   }
+  if constexpr (GS) gas_solid_cell<T>(g, i, k, c, gsa, dt_gs, y3);
 }
 
 // Max of nu_t over interior cells 1<=k<=Nz-2, 1<=i<=Nx-2, all j, starting
@@ -712,22 +718,41 @@ int species3_launch(const fw_grid* g, const void* const* Y, void* const* Yo, con
 template <typename T>
 static int turb3_impl(const fw_grid* g, const void* const* f, void* const* fo,
                       const void* nu_t, double sc_t, double dt_sub, const double* lo,
-                      const double* hi, int* edc_summary, hipStream_t s, const double* sb) {
+                      const double* hi, int* edc_summary, hipStream_t s, const double* sb,
+                      const GasSolidParams* gs) {
   Launch L = cell_launch(g);
   if (edc_summary && (g->nx % EDC_SEG != 0 || L.block.x != EDC_SEG))
     return fw_set_error(FW_E_CONFIG, "turbulent_diffusion3: the EDC summary needs Nx %% 256 == 0");
-  hipLaunchKernelGGL(k_turb_diff3<T>, L.grid, L.block, 0, s, grid_args(g), (const T*)f[0],
-                     (const T*)f[1], (const T*)f[2], (T*)fo[0], (T*)fo[1], (T*)fo[2],
-                     (const T*)nu_t, T(sc_t), T(dt_sub), T(lo[0]), T(hi[0]), T(lo[1]),
-                     T(hi[1]), T(lo[2]), T(hi[2]), edc_summary, sb, (int)SB_DSC);
+  if (gs) {
+    const void* reads[] = {f[0], f[1], f[2], nu_t, gs->T_s, gs->alpha_s, gs->rho,
+                           gs->S_pyro_part, gs->S_dry_part, gs->Q_g_conv, gs->q_rad_gas};
+    const void* writes[] = {fo[0], fo[1], fo[2], gs->T_g, gs->Q_comb_out};
+    FW_REQUIRE(gs->T_g && gs->T_s && gs->T_s_out, "turbulent_diffusion3: bad gas-solid fields");
+    for (const void* q : reads)
+      FW_REQUIRE(q != gs->T_s_out, "turbulent_diffusion3: T_s_out aliases an input");
+    for (const void* q : writes)
+      FW_REQUIRE(q != gs->T_s_out, "turbulent_diffusion3: T_s_out aliases an output");
+    hipLaunchKernelGGL((k_turb_diff3<T, true>), L.grid, L.block, 0, s, grid_args(g),
+                       (const T*)f[0], (const T*)f[1], (const T*)f[2], (T*)fo[0], (T*)fo[1],
+                       (T*)fo[2], (const T*)nu_t, T(sc_t), T(dt_sub), T(lo[0]), T(hi[0]),
+                       T(lo[1]), T(hi[1]), T(lo[2]), T(hi[2]), edc_summary, sb, (int)SB_DSC,
+                       gas_solid_args<T>(*gs));
+  } else {
+    hipLaunchKernelGGL((k_turb_diff3<T, false>), L.grid, L.block, 0, s, grid_args(g),
+                       (const T*)f[0], (const T*)f[1], (const T*)f[2], (T*)fo[0], (T*)fo[1],
+                       (T*)fo[2], (const T*)nu_t, T(sc_t), T(dt_sub), T(lo[0]), T(hi[0]),
+                       T(lo[1]), T(hi[1]), T(lo[2]), T(hi[2]), edc_summary, sb, (int)SB_DSC,
+                       GasSolidArgs<T>{});
+  }
   FW_CHECK_LAUNCH("turbulent_diffusion3");
   return FW_OK;
 }
 
 int turb3_launch(const fw_grid* g, const void* const* f, void* const* fo, const void* nu_t,
                  double sc_t, double dt_sub, const double* lo, const double* hi,
-                 int* edc_summary, hipStream_t s, const double* sb) {
-  return FW_DISPATCH(g, turb3_impl, g, f, fo, nu_t, sc_t, dt_sub, lo, hi, edc_summary, s, sb);
+                 int* edc_summary, hipStream_t s, const double* sb, const GasSolidParams* gs) {
+  return FW_DISPATCH(g, turb3_impl, g, f, fo, nu_t, sc_t, dt_sub, lo, hi, edc_summary, s, sb,
+                     gs);
 }
 
 template <typename T>
@@ -931,7 +956,8 @@ int fw_turbulent_diffusion_pass3(const fw_grid* g, const void* const* f, void* c
     for (int b = 0; b < 3; ++b)
       FW_REQUIRE(f[a] != f_out[b], "turbulent_diffusion3: outputs must not alias inputs");
   return FW_DISPATCH(g, turb3_impl, g, f, f_out, nu_t, sc_t, dt_sub, clip_lo, clip_hi,
-                     (int*)nullptr, (hipStream_t)stream, (const double*)nullptr);
+                     (int*)nullptr, (hipStream_t)stream, (const double*)nullptr,
+                     (const GasSolidParams*)nullptr);
 }
 
 int fw_turbulent_diffusion_pass3_summary(const fw_grid* g, const void* const* f,
@@ -946,7 +972,8 @@ int fw_turbulent_diffusion_pass3_summary(const fw_grid* g, const void* const* f,
     for (int b = 0; b < 3; ++b)
       FW_REQUIRE(f[a] != f_out[b], "turbulent_diffusion3: outputs must not alias inputs");
   return FW_DISPATCH(g, turb3_impl, g, f, f_out, nu_t, sc_t, dt_sub, clip_lo, clip_hi,
-                     edc_summary, (hipStream_t)stream, (const double*)nullptr);
+                     edc_summary, (hipStream_t)stream, (const double*)nullptr,
+                     (const GasSolidParams*)nullptr);
 }
 
 int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
