@@ -20,6 +20,7 @@
 //   type; all dot products accumulate in float64 with a fixed-order
 //   (deterministic) two-level reduction.
 #include <string.h>
+#include <functional>
 #include <vector>
 #include "fw_common.h"
 #include "graph.h"
@@ -35,7 +36,8 @@ namespace fw {
 // ||b||; S_IT counts the iterations taken.  The iteration kernels leave
 // the Krylov state untouched once S_DONE is set, so iterations can be
 // enqueued in batches and the host reads S_DONE once per batch.
-enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_DONE, S_IT, S_TOL, S_SIG, S_NS };
+enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_DONE, S_IT, S_TOL, S_SIG, S_DMAX,
+       S_NS };
 // what a final sum is stored as (k_finalize)
 enum { F_STORE = 0, F_ALPHA = 1, F_BETA = 2, F_RR = 3 };
 
@@ -758,6 +760,13 @@ k_x64_add(size_t n, double* __restrict__ x, const T* __restrict__ d) {
   if (c < n) x[c] += (double)d[c];
 }
 
+// The solve's scalars say it has converged with b != 0: the correction and
+// the divergence check issued ahead of the host's read of that decision run
+// only then (with b = 0 the host zeroes the pressure first).
+__device__ __forceinline__ bool solve_done(const double* __restrict__ S) {
+  return S[S_DONE] == 3.0 && S[S_BB] != 0.0;
+}
+
 // Set-up of a refinement round on the device, from ||b||^2 (S_BB) and
 // ||b - A x||^2 (S_TMP) as the last finalize left them: S_DONE = 3 when the
 // solve has converged (||r|| <= rtol ||b||, or b = 0), 2 on a NaN norm,
@@ -860,8 +869,9 @@ k_finalize(const double* __restrict__ part, int np, double* __restrict__ S,
 
 __global__ void __launch_bounds__(256)
 k_finalize_max(const double* __restrict__ part, int np, double* __restrict__ S,
-               int slot) {
+               int slot, int gated) {
   __shared__ double sh[256];
+  if (gated && !solve_done(S)) return;
   const int t = threadIdx.x;
   double a = 0.0;
   for (int b = t; b < np; b += 256) a = part[b] > a ? part[b] : a;
@@ -941,9 +951,11 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_div_check(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict__ v,
             const T* __restrict__ w, const T* __restrict__ u_inlet,
-            const T* __restrict__ target, double* __restrict__ part) {
+            const T* __restrict__ target, double* __restrict__ part,
+            const double* __restrict__ gate) {
   __shared__ double sh[256];
   (void)n;
+  if (gate && !solve_done(gate)) return;
   int r0, r1;
   row_range(g.ny * g.nz, r0, r1);
   double m = 0.0;
@@ -966,9 +978,10 @@ k_div_check(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict__
 template <typename T, typename TR>
 __global__ void __launch_bounds__(256)
 k_correct(GridArgs g, size_t n, T* u, T* v, T* w, const double* __restrict__ p,
-          const TR* __restrict__ rho, double dt) {
+          const TR* __restrict__ rho, double dt, const double* __restrict__ gate) {
   const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (c >= n) return;
+  if (gate && !solve_done(gate)) return;
   Cell e = cell_of(g, c);
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const size_t sy = Nx, sz = (size_t)Nx * Ny;
@@ -1836,6 +1849,7 @@ struct ProjHandle {
   // the last two solutions
   double* x64p;
   bool have_prev2;
+  bool tail_ran;       // the last solve_refined ran its gated tail
   int last_iters, last_outer;
   double last_rel, last_div_max;
   int precond_enabled;
@@ -2282,9 +2296,14 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
 // before anything else is enqueued, so a converged solve queues no idle
 // round.  Iterates, iteration counts and results are those of the
 // read-every-round loop.
+//
+// tail (may be null) enqueues work gated on the solve's convergence
+// (solve_done): it is issued before the read that follows the last expected
+// round's residual, so a solve that converges there costs that one read.
+// h->tail_ran says whether the gated work ran.
 template <typename T, typename TR>
 static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_residual,
-                         hipStream_t s) {
+                         hipStream_t s, const std::function<void()>* tail = nullptr) {
   const size_t n = h->n;
   const double inner_floor = (sizeof(T) == 4) ? 2e-5 : 0.0;
   const int expected_rounds = h->last_outer;
@@ -2310,6 +2329,7 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
     residual();
   FW_CHECK_LAUNCH("residual64");
   bool known = false;   // S_host holds the outcome of the last k_round_init
+  h->tail_ran = false;
   for (;;) {
     if (known) {
       if (h->S_host[S_DONE] == 4.0)
@@ -2361,7 +2381,13 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
     if (outer < expected_rounds && total < h->maxiter && outer < 20) {
       known = false;   // the next round's first batch read will tell
     } else {
+      if (tail) {
+        (*tail)();
+        FW_CHECK_LAUNCH("gated tail");
+      }
       if ((rc = read_scalars(h, s))) return rc;
+      // the gate read the scalars this read returns (the tail writes none)
+      h->tail_ran = tail && h->S_host[S_DONE] == 3.0 && h->S_host[S_BB] != 0.0;
     }
   }
   const double bnorm = sqrt(h->S_host[S_BB]);
@@ -2414,17 +2440,26 @@ static int project_impl(ProjHandle* h, void* u, void* v, void* w,
   hipLaunchKernelGGL((k_residual64<T, T>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
                      (const T*)rho, h->b64, h->x64, (T*)h->r, h->eps_reg, h->part);
   FW_CHECK_LAUNCH("residual64");
-  int rc = solve_refined<T, T>(h, (const T*)rho, h->rtol, true, s);
+  // the correction and the divergence check (max into S_DMAX), gated on the
+  // device's convergence decision when issued ahead of the host's read
+  auto correct = [&](const double* gate) {
+    hipLaunchKernelGGL((k_correct<T, T>), dim3(nblk(n)), dim3(256), 0, s, h->ga, n,
+                       (T*)u, (T*)v, (T*)w, (const double*)h->x64, (const T*)rho, dt, gate);
+    hipLaunchKernelGGL(k_div_check<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
+                       (const T*)u, (const T*)v, (const T*)w, (const T*)u_inlet,
+                       (const T*)target, h->part, gate);
+    hipLaunchKernelGGL(k_finalize_max, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
+                       S_DMAX, gate ? 1 : 0);
+  };
+  const std::function<void()> gated = [&]() { correct(h->S); };
+  int rc = solve_refined<T, T>(h, (const T*)rho, h->rtol, true, s, &gated);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_correct<T, T>), dim3(nblk(n)), dim3(256), 0, s, h->ga, n,
-                     (T*)u, (T*)v, (T*)w, (const double*)h->x64, (const T*)rho, dt);
-  hipLaunchKernelGGL(k_div_check<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
-                     (const T*)u, (const T*)v, (const T*)w, (const T*)u_inlet,
-                     (const T*)target, h->part);
-  hipLaunchKernelGGL(k_finalize_max, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_TMP);
-  FW_CHECK_LAUNCH("correct");
-  if ((rc = read_scalars(h, s))) return rc;
-  h->last_div_max = h->S_host[S_TMP];
+  if (!h->tail_ran) {
+    correct(nullptr);
+    FW_CHECK_LAUNCH("correct");
+    if ((rc = read_scalars(h, s))) return rc;
+  }
+  h->last_div_max = h->S_host[S_DMAX];
   return FW_OK;
 }
 
