@@ -37,7 +37,7 @@ namespace fw {
 // the Krylov state untouched once S_DONE is set, so iterations can be
 // enqueued in batches and the host reads S_DONE once per batch.
 enum { S_RZ = 0, S_PQ, S_ALPHA, S_BETA, S_RR, S_BB, S_TMP, S_DONE, S_IT, S_TOL, S_SIG, S_DMAX,
-       S_NS };
+       S_ITL, S_SPEC, S_NS };
 // what a final sum is stored as (k_finalize)
 enum { F_STORE = 0, F_ALPHA = 1, F_BETA = 2, F_RR = 3 };
 
@@ -705,9 +705,11 @@ template <typename T, typename TR>
 __global__ void __launch_bounds__(256)
 k_residual64(GridArgs g, size_t n, const TR* __restrict__ rho,
              const double* __restrict__ b, const double* __restrict__ x,
-             T* __restrict__ r_t, double eps_reg, double* __restrict__ part) {
+             T* __restrict__ r_t, double eps_reg, double* __restrict__ part,
+             const double* __restrict__ gate) {
   __shared__ double sh[256];
   (void)n;
+  if (gate && gate[S_DONE] != 1.0) return;   // k_finalize_round's round-end gate
   // contiguous runs of rows per workgroup in the XCD-aware order, so the
   // y and z neighbour rows of the stencil mostly come from the same L2
   int r0, r1;
@@ -755,7 +757,9 @@ k_copy64(size_t n, const double* __restrict__ x, double* __restrict__ y) {
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_x64_add(size_t n, double* __restrict__ x, const T* __restrict__ d) {
+k_x64_add(size_t n, double* __restrict__ x, const T* __restrict__ d,
+          const double* __restrict__ gate) {
+  if (gate && gate[S_DONE] != 1.0) return;   // k_finalize_round's round-end gate
   const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (c < n) x[c] += (double)d[c];
 }
@@ -777,6 +781,7 @@ __device__ __forceinline__ bool solve_done(const double* __restrict__ S) {
 __device__ __forceinline__ void round_init(double* __restrict__ S, double rtol, double floor) {
   const double bnorm = sqrt(S[S_BB]), rnorm = sqrt(S[S_TMP]);
   const double tol_abs = rtol * bnorm;
+  S[S_ITL] = S[S_IT];   // the ended round's inner iterations
   S[S_IT] = 0.0;
   if (!(bnorm == bnorm)) {
     S[S_DONE] = 4.0;
@@ -802,11 +807,20 @@ __global__ void k_round_init(double* __restrict__ S, double rtol, double floor) 
 
 // k_finalize's fixed-order sum of the residual partials into S_TMP, then the
 // round set-up above, in one launch.
+//
+// Round-end gate: a round's end (x64 += d, the float64 residual, this
+// finalize) may be enqueued behind an iteration batch before the host knows
+// whether the inner solve has converged.  With a gate those kernels run only
+// when it has (S_DONE = 1), and this one then leaves `tag` in S_SPEC so the
+// host's next read tells whether the round ended.
 __global__ void __launch_bounds__(256)
 k_finalize_round(const double* __restrict__ part, int np, double* __restrict__ S, double rtol,
-                 double floor) {
+                 double floor, int gated, double tag) {
   __shared__ double sh[256];
   const int t = threadIdx.x;
+  // uniform; every thread reads it before the reduction's barriers, thread 0
+  // writes S after them
+  if (gated && S[S_DONE] != 1.0) return;
   double a = 0.0;
   for (int b = t; b < np; b += 256) a += part[b];
   sh[t] = a;
@@ -818,6 +832,7 @@ k_finalize_round(const double* __restrict__ part, int np, double* __restrict__ S
   if (t == 0) {
     S[S_TMP] = sh[0];
     round_init(S, rtol, floor);
+    if (gated) S[S_SPEC] = tag;
   }
 }
 
@@ -1850,6 +1865,7 @@ struct ProjHandle {
   double* x64p;
   bool have_prev2;
   bool tail_ran;       // the last solve_refined ran its gated tail
+  long long spec_tag;  // tags of gated round ends (k_finalize_round)
   int last_iters, last_outer;
   double last_rel, last_div_max;
   int precond_enabled;
@@ -2105,8 +2121,16 @@ static int read_scalars(ProjHandle* h, hipStream_t s) {
 // Inner PCG on A d = r (type T); r holds the right-hand side on entry and
 // the recursively updated residual on exit.  Stops when ||r||_2 <= S_TOL,
 // the inner tolerance k_round_init left on the device.
+//
+// round_end (may be null) enqueues the round's end behind every batch,
+// gated on the inner convergence (k_finalize_round's round-end gate), before
+// the batch's read; *ended then says whether it ran, and *iters is the
+// round's iteration count either way.
 template <typename T, typename TR>
-static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStream_t s) {
+static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStream_t s,
+                     const std::function<void(double)>* round_end = nullptr,
+                     bool* ended = nullptr) {
+  if (ended) *ended = false;
   const size_t n = h->n;
   T* r = (T*)h->r;
   T* z = (T*)h->z;
@@ -2272,8 +2296,20 @@ static int pcg_inner(ProjHandle* h, const TR* rho, int maxit, int* iters, hipStr
     }
     queued += nb_it;
     FW_CHECK_LAUNCH("pcg iteration");
+    double tag = 0.0;
+    if (round_end) {
+      tag = (double)++h->spec_tag;
+      (*round_end)(tag);
+    }
     if ((rc = read_scalars(h, s))) return rc;
     if (h->S_host[S_DONE] == 2.0) return fw_set_error(FW_E_NONFINITE, "projection: NaN residual");
+    if (tag != 0.0 && h->S_host[S_SPEC] == tag) {
+      // the round ended on the device: S_IT was reset, S_ITL holds its count
+      *ended = true;
+      *iters = (int)h->S_host[S_ITL];
+      h->last_inner[rd] = *iters;
+      return FW_OK;
+    }
     if (h->S_host[S_DONE] != 0.0 || queued >= maxit) break;
   }
   *iters = (int)h->S_host[S_IT];
@@ -2318,13 +2354,13 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
   }
   auto residual = [&]() {
     hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n, rho,
-                       h->b64, h->x64, (T*)h->r, h->eps_reg, h->part);
+                       h->b64, h->x64, (T*)h->r, h->eps_reg, h->part, (const double*)nullptr);
     hipLaunchKernelGGL(k_finalize_round, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
-                       rtol, inner_floor);
+                       rtol, inner_floor, 0, 0.0);
   };
   if (have_residual)
     hipLaunchKernelGGL(k_finalize_round, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
-                       rtol, inner_floor);
+                       rtol, inner_floor, 0, 0.0);
   else
     residual();
   FW_CHECK_LAUNCH("residual64");
@@ -2363,8 +2399,30 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
     const bool learned = h->precond_enabled == FW_PRECOND_LEARNED;
     int budget = h->maxiter - total;
     if (learned && budget > h->learned_budget) budget = h->learned_budget;
-    if ((rc = pcg_inner<T, TR>(h, rho, budget, &it, s))) return rc;
+    // from the last expected round on, the round's end and the tail go
+    // behind the first iteration batch, gated, so that one read covers the
+    // batch, the residual and the tail when the batch converges
+    const bool spec = !learned && outer + 1 >= expected_rounds;
+    const std::function<void(double)> round_end = [&](double tag) {
+      hipLaunchKernelGGL(k_x64_add<T>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64,
+                         (const T*)h->d, (const double*)h->S);
+      hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n, rho,
+                         h->b64, h->x64, (T*)h->r, h->eps_reg, h->part, (const double*)h->S);
+      hipLaunchKernelGGL(k_finalize_round, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
+                         rtol, inner_floor, 1, tag);
+      if (tail) (*tail)();
+    };
+    bool ended = false;
+    if ((rc = pcg_inner<T, TR>(h, rho, budget, &it, s, spec ? &round_end : nullptr, &ended)))
+      return rc;
     known = true;
+    if (spec) h->tail_ran = tail && h->S_host[S_DONE] == 3.0 && h->S_host[S_BB] != 0.0;
+    if (ended) {
+      FW_CHECK_LAUNCH("gated round end");
+      total += it;
+      ++outer;
+      continue;   // S_host holds the round's end: the loop's top decides
+    }
     if (h->S_host[S_DONE] == 4.0)
       return fw_set_error(FW_E_NONFINITE, "projection: NaN divergence");
     if (h->S_host[S_DONE] == 3.0) break;   // converged before this round
@@ -2375,7 +2433,7 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
     total += it;
     ++outer;
     hipLaunchKernelGGL(k_x64_add<T>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64,
-                       (const T*)h->d);
+                       (const T*)h->d, (const double*)nullptr);
     residual();
     FW_CHECK_LAUNCH("x64_add");
     if (outer < expected_rounds && total < h->maxiter && outer < 20) {
@@ -2438,7 +2496,8 @@ static int project_impl(ProjHandle* h, void* u, void* v, void* w,
   }
   // the warm start's residual, decided on together with ||b|| on the device
   hipLaunchKernelGGL((k_residual64<T, T>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
-                     (const T*)rho, h->b64, h->x64, (T*)h->r, h->eps_reg, h->part);
+                     (const T*)rho, h->b64, h->x64, (T*)h->r, h->eps_reg, h->part,
+                     (const double*)nullptr);
   FW_CHECK_LAUNCH("residual64");
   // the correction and the divergence check (max into S_DMAX), gated on the
   // device's convergence decision when issued ahead of the host's read
@@ -2658,6 +2717,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   FW_CHECK_HIP(hipMalloc((void**)&h->Pqz, PJ_RED * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->S, S_NS * sizeof(double)));
   FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, S_NS * sizeof(double)));
+  FW_CHECK_HIP(hipMemset(h->S, 0, S_NS * sizeof(double)));   // S_SPEC matches no tag
   FW_CHECK_HIP(hipMemset(h->x64, 0, n * 8));
   *handle = h;
   return FW_OK;
