@@ -459,9 +459,8 @@ def main(argv=None):
     sweep_launches_per_iter = -(-g.Ny // TS) + -(-g.Nz // TS) - 1
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(a.steps):
-        if sv.step() is not None:
-            raise RuntimeError(f"run stopped early: {sv.stop_reason}")
+    if sv.advance(a.steps) is not None:
+        raise RuntimeError(f"run stopped early: {sv.stop_reason}")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     wall = t1 - t0
@@ -548,8 +547,7 @@ def main(argv=None):
             sv.step()
         torch.cuda.synchronize()
         te0, tw0 = time.perf_counter(), sv.t
-        for i in range(a.steps):
-            sv.step()
+        sv.advance(a.steps)
         torch.cuda.synchronize()
         we = time.perf_counter() - te0
         line["early_window"] = {"timed_steps": [a.warmup + 1, a.warmup + a.steps],

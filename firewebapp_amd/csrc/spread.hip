@@ -655,4 +655,20 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
   return FW_OK;
 }
 
+int fw_spread_run(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
+                  const fw_spread_params* p, fw_spread_state* st, void* proj, void* dom,
+                  const fw_bed_particles* particles, const fw_bed_index* index,
+                  fw_bed_params* bed, void* stream, int max_steps, fw_spread_state* hist,
+                  int* n_done) {
+  FW_REQUIRE(n_done && (max_steps <= 0 || hist), "spread_run: null argument");
+  *n_done = 0;
+  for (int i = 0; i < max_steps; ++i) {
+    TRY(fw_spread_step(g, g64, f, p, st, proj, dom, particles, index, bed, stream));
+    hist[i] = *st;
+    *n_done = i + 1;
+    if (st->stop != 0) break;
+  }
+  return FW_OK;
+}
+
 }  // extern "C"

@@ -875,24 +875,61 @@ class Spread3DSolver:
         if self.gust.enabled:
             # the inlet of a step starting at t: the log law times g(t)
             self.u_inlet.copy_(self.u_inlet_base * self.gust.factor(st.t))
+        rc = self._native("fw_spread_step")
+        if rc == _lib.FW_E_NONFINITE:
+            raise RuntimeError(_lib.lib().fw_last_error().decode())
+        _lib.check(rc, "fw_spread_step")
+        return self._after_step(st)
+
+    def advance(self, n):
+        """Up to n outer steps in one native call (fw_spread_run): the
+        reference's driver loop without a return to Python between steps.
+        Diagnostics, front points and exits are those of n calls of step();
+        returns the stop reason or None.  With gusts the inlet of each step is
+        set from Python, so the steps go one by one."""
+        if n <= 0:
+            return None
+        if self.gust.enabled:
+            for _ in range(n):
+                r = self.step()
+                if r is not None:
+                    return r
+            return None
+        hist = (FwSpreadState * n)()
+        done = C.c_int(0)
+        rc = self._native("fw_spread_run", C.c_int(n), hist, C.byref(done))
+        reason = None
+        for i in range(done.value):
+            reason = self._after_step(hist[i])
+            if reason is not None:
+                break
+        if rc == _lib.FW_E_NONFINITE:
+            raise RuntimeError(_lib.lib().fw_last_error().decode())
+        _lib.check(rc, "fw_spread_run")
+        return reason
+
+    def _native(self, name, *extra):
+        """Call fw_spread_step / fw_spread_run on the step stream, then re-read
+        which buffer holds which field."""
         cur = torch.cuda.current_stream()
         run = cur
         if self._stream is not None and cur.cuda_stream == 0:
             # graph capture needs a stream other than the legacy default one
             run = self._stream
             run.wait_stream(cur)
-        rc = _lib.lib().fw_spread_step(
+        rc = getattr(_lib.lib(), name)(
             self.dg.ref, self.dg64.ref, C.byref(self.fields), C.byref(self.params),
-            C.byref(st), self.proj, self.dom_h, C.byref(self.P.c),
+            C.byref(self.state), self.proj, self.dom_h, C.byref(self.P.c),
             C.byref(self.ix.c) if self.ix is not None else None,
             C.byref(self.bed_params) if self.bed_params is not None else None,
-            C.c_void_p(run.cuda_stream))
+            C.c_void_p(run.cuda_stream), *extra)
         if run is not cur:
             cur.wait_stream(run)
         self._remap()
-        if rc == _lib.FW_E_NONFINITE:
-            raise RuntimeError(_lib.lib().fw_last_error().decode())
-        _lib.check(rc, "fw_spread_step")
+        return rc
+
+    def _after_step(self, st):
+        """The exits, front points and diagnostics of one step's state."""
         if st.stop in (1, 2):
             self.stop_reason = _STOPS[st.stop]
             return self.stop_reason
@@ -921,7 +958,12 @@ class Spread3DSolver:
             writer = SnapshotWriter(self, cfg.wind_speed_m_s)
             last_t, idx = -1.0e30, 0
         try:
-            while True:
+            while writer is None and n_target is not None and self.step_no < n_target:
+                # no snapshots: the steps run in native calls of at most 1000
+                # steps, back to Python only at an exit
+                if self.advance(min(n_target - self.step_no, 1000)) is not None:
+                    break
+            while writer is not None or n_target is None:
                 if n_target is not None:
                     if self.step_no >= n_target:
                         break

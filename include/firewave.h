@@ -723,6 +723,17 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
                    void* dom, const fw_bed_particles* particles,
                    const fw_bed_index* index, fw_bed_params* bed, void* stream);
 
+/* Up to max_steps calls of fw_spread_step in one call, the driver loop of
+ * run_3d_spread (spread_3d.py:1897-3683) without a return to the caller
+ * between steps: hist[i] receives the state after step i; the loop stops
+ * after the first step that sets st->stop or returns an error (that step's
+ * error is returned; *n_done counts the steps that completed). */
+int fw_spread_run(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
+                  const fw_spread_params* p, fw_spread_state* st, void* proj,
+                  void* dom, const fw_bed_particles* particles,
+                  const fw_bed_index* index, fw_bed_params* bed, void* stream,
+                  int max_steps, fw_spread_state* hist, int* n_done);
+
 /* The end-of-step reduction into st->red (with_sources = 0 skips S_pyro and
  * Q_comb: the state before the first step). */
 int fw_spread_reduce(const fw_grid* g, const fw_spread_fields* f,

@@ -147,6 +147,26 @@ def test_integration_fp32_within_tolerance():
         assert err < 1e-4, key
 
 
+def test_advance_matches_step():
+    """fw_spread_run (advance: many steps per native call) against one
+    fw_spread_step per call: the same steps, diagnostics, front points and
+    fields, bit for bit (float32 build, DOM solves at steps 5, 10, ...)."""
+    _, a = _solver(DECK, wind_speed_m_s=4.0)
+    _, b = _solver(DECK, wind_speed_m_s=4.0)
+    for _ in range(33):
+        assert a.step() is None
+    assert b.advance(12) is None
+    assert b.advance(21) is None
+    assert b.step_no == a.step_no == 33
+    assert a.diag == b.diag
+    assert (a.front_t, a.front_x) == (b.front_t, b.front_x)
+    sa, sb = a.state_numpy(), b.state_numpy()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    a.close()
+    b.close()
+
+
 # ---- level set and glue kernels ---------------------------------------------
 
 @pytest.mark.parametrize("c", cases("flame_front"), ids=case_ids("flame_front"))
