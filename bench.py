@@ -113,14 +113,19 @@ def wind(name):
 CONFIG_STEPS = {"spread3d_256x64x48_s8": 500, "spread3d_512x128x64_s8": 1000,
                 "spread3d_128x32x32_s4": 200}
 
-# The roofline kernels.  k_dom_sweep_tile (the DOM's tiled sweep, the largest
-# kernel-time share of the step) moves, by the phase-fused count, 5 field
-# passes per ordinate per source iteration (kappa, B, the upwind inflow, the
-# intensity written, its accumulation into G), spread over the sweep's
-# Ny/4 + Nz/4 - 1 launches of a source iteration.  k_species3 reads Y_fuel,
-# Y_O2, Y_H2O, rho, u, v, w, S_pyro part, S_dry part and writes the three
-# species: 12 fields per cell per launch.
-DOM_PASSES_PER_ORDINATE = 5
+# The roofline kernels.  The DOM sweep (k_dom_sweep_flow: one launch per
+# source iteration; k_dom_sweep_tile: Ny/TS + Nz/TS - 1 launches) must, per
+# source iteration, write every ordinate's intensities once and read its
+# inflow rows, the one row in TS in y and in z that crosses a block edge
+# (2/TS of a field); kappa and B are read once for all ordinates, since they
+# are shared (further reads of them are the caches' work).  k_species3 reads
+# Y_fuel, Y_O2, Y_H2O, rho, u, v, w, S_pyro part, S_dry part and writes the
+# three species: 12 fields per cell per launch.
+def dom_sweep_bytes(esz, M, cells, ts):
+    """Compulsory bytes of one source iteration's sweep."""
+    return esz * cells * (M * (1.0 + 2.0 / ts) + 2.0)
+
+
 SPECIES3_FIELDS = 12
 # HBM bytes per launch from the FETCH_SIZE + WRITE_SIZE counters scaled by a
 # 256 MiB copy (float32): profiles/step_256x64x48_s8_hbm_traffic.md
@@ -456,7 +461,11 @@ def main(argv=None):
         TS = act.value
         if tuned:
             tuned["dom_tile"] = TS
-    sweep_launches_per_iter = -(-g.Ny // TS) + -(-g.Nz // TS) - 1
+    import ctypes as C
+    from firewebapp_amd import _lib
+    nl = C.c_int()
+    _lib.call("fw_dom_sweep_launches", sv.dom_h, C.byref(nl))
+    sweep_launches_per_iter = nl.value
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if sv.advance(a.steps) is not None:
@@ -489,7 +498,7 @@ def main(argv=None):
     sv.fields.graphs = graphs
     esz = 4 if a.dtype == "float32" else 8
     M = sv.dom.M
-    dom_bytes = DOM_PASSES_PER_ORDINATE * esz * M * cells / sweep_launches_per_iter
+    dom_bytes = dom_sweep_bytes(esz, M, cells, TS) / sweep_launches_per_iter
     dom_ms, dom_n = kernel_timer_read(TIMER_DOM_SWEEP)
     sp_ms, sp_n = kernel_timer_read(TIMER_SPECIES3)
     total_steps = spinup + a.warmup + a.steps
@@ -513,8 +522,9 @@ def main(argv=None):
         "full_run": {"steps": total_steps,
                      "value": cells * total_steps / (t1 - t_run0),
                      "ms_per_step": 1e3 * (t1 - t_run0) / total_steps},
-        "roofline": _roofline("k_dom_sweep_tile", dom_bytes, dom_ms / max(dom_n, 1), dom_n,
-                              a.workload, a.dtype, shape_key=TS),
+        "roofline": _roofline("k_dom_sweep_flow" if sweep_launches_per_iter == 1
+                              else "k_dom_sweep_tile", dom_bytes, dom_ms / max(dom_n, 1),
+                              dom_n, a.workload, a.dtype, shape_key=TS),
         "roofline_species3": _roofline("k_species3", SPECIES3_FIELDS * esz * cells,
                                        sp_ms / max(sp_n, 1), sp_n, a.workload, a.dtype),
         "cpu_baseline": None,
@@ -524,8 +534,9 @@ def main(argv=None):
                    "captures": graph_stats[1] if graph_stats else 0},
     }
     line["roofline"]["definition"] = (
-        f"{DOM_PASSES_PER_ORDINATE} field passes per ordinate per source iteration "
-        f"x {M} ordinates x {cells} cells x {esz} B / {sweep_launches_per_iter} launches")
+        f"per source iteration: {M} ordinates x (1 field written + 2/{TS} of a field of "
+        f"inflow rows read) + kappa and B once, x {cells} cells x {esz} B, "
+        f"/ {sweep_launches_per_iter} launches")
     line["roofline"]["sampled"] = (f"every sweep launch of the DOM solve in steps "
                                    f"{timing_steps[0]}-{timing_steps[1]} (after the timed "
                                    f"steps; the solve in line)")

@@ -18,6 +18,7 @@
 // across solves (warm start, dom_3d.py:282).
 #include <math.h>
 #include <string.h>
+#include <stdlib.h>
 #include <vector>
 #include "fw_common.h"
 #include "graph.h"
@@ -31,7 +32,29 @@ namespace fw {
 // set, so iterations can be enqueued in batches and the host reads the
 // state once per batch), the count of completed iterations (its parity
 // names the G buffer the next iteration writes) and the non-finite flag.
-enum { DS_DIFF = 0, DS_GMAX, DS_DONE, DS_ITERS, DS_NAN, DS_N };
+enum { DS_DIFF = 0, DS_GMAX, DS_DONE, DS_ITERS, DS_NAN, DS_EPOCH, DS_STALL, DS_N };
+
+// The dataflow sweep (k_dom_sweep_flow): per XCD a queue counter and a
+// give-up word, then per (ordinate, block) the epoch its block was stored in.
+// Each XCD's words, and each ordinate's flags, sit in 256-byte lines of
+// their own: no cache line is written from two XCDs' L2s.
+constexpr int FLOW_XCD = 8;        // XCDs of an MI355X (the XCC_ID register's range)
+constexpr int FLOW_LINE = 64;      // ints per 256-byte line
+__host__ __device__ constexpr int flow_ctr(int x) { return FLOW_LINE * x; }
+__host__ __device__ constexpr int flow_gave_up(int x) { return FLOW_LINE * x + 1; }
+constexpr int FLOW_MAXD = 127;     // block diagonals a plan holds
+constexpr int FLOW_SPIN = 1 << 20; // polls before a workgroup gives up (about 1 s)
+struct FlowPlan {
+  int TJ, TK, ndiag;
+  int fstride;              // ints per ordinate's flags (TJ TK rounded up to a line)
+  int cum[FLOW_MAXD + 1];   // blocks on the diagonals before D
+};
+// rows the dataflow sweep takes: 64 PER cells (PER 1 .. 8) whose 4 x 4
+// blocks, with the kernel's own two words, fit 64 KB of LDS
+static inline bool flow_rows(int nx, size_t esz) {
+  const bool pow2 = nx == 64 || nx == 128 || nx == 256 || nx == 512;
+  return pow2 && 16 * (size_t)nx * esz < 65536;
+}
 
 struct DomHandle {
   fw_grid g;
@@ -52,6 +75,12 @@ struct DomHandle {
   // the events that order it; created on first use
   hipStream_t side;
   hipEvent_t ev_fork, ev_join;
+  // dataflow sweep: queue counters and give-up words (a line per XCD), the
+  // blocks' epochs (a run of lines per ordinate); flow: 1 in use, 0 not built
+  // for this grid, -1 switched off
+  int *flow_q, *flow_flags;
+  int flow;
+  FlowPlan plan;
 };
 
 template <typename T>
@@ -90,12 +119,17 @@ k_dom_props(size_t n, const T* __restrict__ Ts, const T* __restrict__ Tg,
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_dom_ground_init(int ncol, const T* __restrict__ Tsoil, T eps, T I_amb,
-                  T* __restrict__ emit, T* __restrict__ out, double* __restrict__ S) {
+                  T* __restrict__ emit, T* __restrict__ out, double* __restrict__ S,
+                  int* __restrict__ flow_q) {
   const int c = blockIdx.x * 256 + threadIdx.x;
   if (c == 0) {   // a new solve: not converged, no iteration done
     S[DS_DONE] = 0.0;
     S[DS_ITERS] = 0.0;
     S[DS_NAN] = 0.0;
+    S[DS_STALL] = 0.0;
+    S[DS_EPOCH] += 1.0;   // the first sweep's epoch
+    if (flow_q)
+      for (int x = 0; x < FLOW_XCD; ++x) flow_q[flow_ctr(x)] = flow_q[flow_gave_up(x)] = 0;
   }
   if (c >= ncol) return;
   T e;
@@ -264,19 +298,42 @@ struct alignas(N * sizeof(T) >= 16 ? 16 : N * sizeof(T)) LaneCells {
   T v[N];
 };
 
-template <typename T, int PER, bool FULL, int TS>
-__global__ void __launch_bounds__(64 * TS)
-k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ Om,
-                 T* __restrict__ Iset, const T* __restrict__ kap,
-                 const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb,
-                 const double* __restrict__ S) {
-  if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sh = reinterpret_cast<T*>(smem_raw);   // TS*TS rows in sweep order: slot (w, kk)
+// v = *(V*)(base + off) read at device scope (the vector L1 cache is not
+// consulted): rows another workgroup of the same XCD stored during the
+// running kernel come from that XCD's L2, where the stores went
+template <typename V>
+__device__ __forceinline__ V load_l2(const V* base, uint32_t off) {
+  const char* p = reinterpret_cast<const char*>(base) + off;
+  V v;
+  if constexpr (sizeof(V) % 8 == 0) {
+    unsigned long long u[sizeof(V) / 8];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(V) / 8); ++i)
+      u[i] = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p) + i,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_memcpy(&v, u, sizeof(V));
+  } else {
+    static_assert(sizeof(V) == 4, "4- or 8k-byte values");
+    const unsigned u = __hip_atomic_load(reinterpret_cast<const unsigned*>(p),
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_memcpy(&v, &u, 4);
+  }
+  return v;
+}
+
+// One TS x TS block (tj, tk) of ordinate n, the body of both sweep kernels.
+// FLOW (the dataflow kernel): the block's inflow-independent part first,
+// then wait() (every thread; false: give up), then the inflows, read from
+// L2; without FLOW every input is read up front.
+template <typename T, int PER, bool FULL, int TS, bool FLOW, typename Wait>
+__device__ __forceinline__ bool sweep_block(const GridArgs& g, int n, int tj, int tk,
+                                            const double* __restrict__ Om,
+                                            T* __restrict__ Iset, const T* __restrict__ kap,
+                                            const T* __restrict__ B,
+                                            const T* __restrict__ Ig_out, T I_amb, T* sh,
+                                            Wait&& wait) {
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int n = blockIdx.y;
-  const int tj = tjlo + blockIdx.x, tk = D - tj;
   const int jp = TS * tj + w;
   const bool col = jp < Ny;                 // uniform per wavefront
   const double xi = Om[3 * n], eta = Om[3 * n + 1], mu = Om[3 * n + 2];
@@ -312,12 +369,42 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
         if (ok[q]) out[q] = at(p, (rowoff + ix[q]) * E);
     }
   };
+  // the inflow rows: from L2 in the dataflow kernel
+  auto load_inflow = [&](const T* p, uint32_t rowoff, T* out) {
+    if constexpr (!FLOW) {
+      load_cells(p, rowoff, out);
+    } else if constexpr (FULL) {
+      const LC c = load_l2(reinterpret_cast<const LC*>(p), (rowoff + i0) * E);
+#pragma unroll
+      for (int q = 0; q < PER; ++q) out[q] = fwd ? c.v[q] : c.v[PER - 1 - q];
+    } else {
+#pragma unroll
+      for (int q = 0; q < PER; ++q)
+        if (ok[q]) out[q] = load_l2(p, (rowoff + ix[q]) * E);
+    }
+  };
   // global inputs read up front: kappa and B of the column's TS rows, the y
   // inflow of the column's rows (w == 0), the z inflow of its first row
   // (kk == 0) and each row's z coefficient.  A stage then issues no global
   // load (on gfx9 the wait for a load also waits for every earlier store).
   T Iyg[TS][PER], Iz[PER], kB[TS][PER], rc[TS][PER], azs[TS];
   T Al[TS][7];   // per row: A at the six scan levels, then the exclusive prefix
+  // the z inflow of the column's first row: the block below, the ground
+  // (upward ordinates) or the top boundary
+  auto load_z_inflow = [&]() {
+    const int kp0 = TS * tk;
+    const int k0 = mu >= 0.0 ? kp0 : Nz - 1 - kp0;
+    const bool zb = mu >= 0.0 ? (k0 == 0) : (k0 == Nz - 1);
+    const int kz = mu >= 0.0 ? k0 - 1 : k0 + 1;
+    if (!zb) {
+      load_inflow(In, ((uint32_t)kz * Ny + j) * Nx, Iz);
+    } else if (mu >= 0.0) {
+      load_cells(Ig_out, (uint32_t)j * Nx, Iz);
+    } else {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) Iz[q] = I_amb;
+    }
+  };
   if (col) {
     T kvs[TS][PER];
 #pragma unroll
@@ -328,25 +415,13 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
         const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
         azs[kk] = T(fabs(mu) * g.inv_dz[k]);
         const uint32_t row = ((uint32_t)k * Ny + j) * Nx;
-        const uint32_t rowy = ((uint32_t)k * Ny + jy) * Nx;
         load_cells(kap, row, kvs[kk]);
         load_cells(B, row, kB[kk]);
-        if (w == 0) load_cells(In, rowy, Iyg[kk]);
+        if constexpr (!FLOW)
+          if (w == 0) load_cells(In, ((uint32_t)k * Ny + jy) * Nx, Iyg[kk]);
       }
     }
-    const int kp0 = TS * tk;
-    const int k0 = mu >= 0.0 ? kp0 : Nz - 1 - kp0;
-    const bool zb = mu >= 0.0 ? (k0 == 0) : (k0 == Nz - 1);
-    const int kz = mu >= 0.0 ? k0 - 1 : k0 + 1;
-    const uint32_t rowz = zb ? 0u : ((uint32_t)kz * Ny + j) * Nx;
-    if (!zb) {
-      load_cells(In, rowz, Iz);
-    } else if (mu >= 0.0) {
-      load_cells(Ig_out, (uint32_t)j * Nx, Iz);
-    } else {
-#pragma unroll
-      for (int q = 0; q < PER; ++q) Iz[q] = I_amb;
-    }
+    if constexpr (!FLOW) load_z_inflow();
     // the inflow-independent part of each row
 #pragma unroll
     for (int kk = 0; kk < TS; ++kk) {
@@ -376,6 +451,21 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
         A = A * dpp_mov<0x143, 0xC>(T(1), A);   // row_bcast:31 into rows 2, 3
         Al[kk][6] = dpp_mov<0x138, 0xF>(T(1), A);   // wave_shr:1: exclusive prefix
       }
+    }
+  }
+  if constexpr (FLOW) {
+    // the blocks this one reads from have been stored, then its inflows
+    if (!wait()) return false;
+    if (col) {
+#pragma unroll
+      for (int kk = 0; kk < TS; ++kk) {
+        const int kp = TS * tk + kk;
+        if (w == 0 && kp < Nz) {
+          const int k = mu >= 0.0 ? kp : Nz - 1 - kp;
+          load_inflow(In, ((uint32_t)k * Ny + jy) * Nx, Iyg[kk]);
+        }
+      }
+      load_z_inflow();
     }
   }
   // every global inflow is read before any row is stored: the wait for all
@@ -463,6 +553,22 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
     __syncthreads();
   }
   for (int b = w; b < TS - 1; ++b) __syncthreads();
+  return true;
+}
+
+template <typename T, int PER, bool FULL, int TS>
+__global__ void __launch_bounds__(64 * TS)
+k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ Om,
+                 T* __restrict__ Iset, const T* __restrict__ kap,
+                 const T* __restrict__ B, const T* __restrict__ Ig_out, T I_amb,
+                 const double* __restrict__ S) {
+  if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
+  (void)M;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sh = reinterpret_cast<T*>(smem_raw);   // TS*TS rows in sweep order: slot (w, kk)
+  const int tj = tjlo + blockIdx.x;
+  sweep_block<T, PER, FULL, TS, false>(g, blockIdx.y, tj, D - tj, Om, Iset, kap, B, Ig_out,
+                                       I_amb, sh, [] { return true; });
 }
 
 // G = sum_n w_n I_n (ordinates summed in order), one cell per thread; per
@@ -574,7 +680,8 @@ __global__ void k_scale_copy(int ncol, const T* __restrict__ a, T s,
 
 __global__ void __launch_bounds__(256)
 k_dom_fin(const double* __restrict__ pd, const double* __restrict__ pm, int np,
-          double* __restrict__ S, int first, double tol) {
+          double* __restrict__ S, int first, double tol, int* __restrict__ flow_q,
+          int flow_blocks, int M) {
   if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
   __shared__ double a[256], b[256];
   const int t = threadIdx.x;
@@ -609,6 +716,122 @@ k_dom_fin(const double* __restrict__ pd, const double* __restrict__ pm, int np,
         S[DS_DONE] = 1.0;
       }
     }
+    if (flow_q) {
+      // the dataflow sweep of this iteration: every XCD's queue handed out
+      // all of its blocks and no workgroup gave up; then the next sweep's
+      // queues and epoch
+      bool stalled = false;
+      for (int x = 0; x < FLOW_XCD; ++x) {
+        const int items = ((M - x + FLOW_XCD - 1) / FLOW_XCD) * flow_blocks;
+        if (flow_q[flow_ctr(x)] < items || flow_q[flow_gave_up(x)] != 0) stalled = true;
+      }
+      if (stalled) {
+        S[DS_STALL] = 1.0;
+        S[DS_DONE] = 1.0;
+      }
+      for (int x = 0; x < FLOW_XCD; ++x) flow_q[flow_ctr(x)] = 0;
+      S[DS_EPOCH] += 1.0;
+    }
+  }
+}
+
+// XCD (XCC) of the calling wavefront: the XCC_ID hardware register
+__device__ __forceinline__ int xcc_id() {
+  // s_getreg_b32 hwreg(HW_REG_XCC_ID = 20, offset 0, 4 bits)
+  return __builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & (FLOW_XCD - 1);
+}
+
+// Dataflow sweep: one launch per source iteration instead of one per block
+// diagonal.  Ordinate n belongs to XCD n % 8, and the blocks of the
+// ordinates of one XCD form a queue in diagonal order (its counter).
+// A workgroup reads the XCD it runs on, takes the next block of that XCD's
+// queue, does its inflow-independent work, waits until the blocks it reads
+// from (y and z neighbours, one diagonal earlier) have stored their rows,
+// sweeps the block and publishes it: the epoch of this sweep into the
+// block's flag once its wavefronts' stores are complete.
+//
+// Producer and consumer share the XCD and so its L2: stores pass the
+// write-through L1 to the L2, and the flags and inflow rows are read at
+// device scope (past the L1), so no L2 write-back or invalidation is
+// needed.  A block waits only on blocks earlier in its queue, which
+// workgroups already running have taken, so the sweep cannot deadlock
+// however many workgroups are resident.  Every wait is bounded: after
+// FLOW_SPIN polls (or once another workgroup of the XCD has given up) a
+// workgroup raises the XCD's give-up word and leaves; k_dom_fin turns that,
+// or a queue not handed out in full, into DS_STALL.
+template <typename T, int PER, int TS>
+__global__ void __launch_bounds__(64 * TS)
+k_dom_sweep_flow(GridArgs g, int M, const double* __restrict__ Om, T* __restrict__ Iset,
+                 const T* __restrict__ kap, const T* __restrict__ B,
+                 const T* __restrict__ Ig_out, T I_amb, const double* __restrict__ S,
+                 int* __restrict__ flow_q, int* __restrict__ flags, FlowPlan plan) {
+  if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sh = reinterpret_cast<T*>(smem_raw);
+  __shared__ int s_item, s_ok;
+  const int x = xcc_id();
+  const int epoch = (int)S[DS_EPOCH];
+  const int nord = (M - x + FLOW_XCD - 1) / FLOW_XCD;   // ordinates x, x + 8, ...
+  const int items = nord * plan.cum[plan.ndiag];
+  const int TJ = plan.TJ;
+  int D = 0;
+  for (;;) {
+    if (threadIdx.x == 0)
+      s_item = __hip_atomic_fetch_add(&flow_q[flow_ctr(x)], 1, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();   // s_item read by all before the next take
+    if (it >= items) return;
+    while (it >= nord * plan.cum[D + 1]) ++D;   // items come in increasing order
+    const int nt = plan.cum[D + 1] - plan.cum[D];
+    const int r = it - nord * plan.cum[D];
+    const int o = r / nt;
+    const int tjlo = D - (plan.TK - 1) > 0 ? D - (plan.TK - 1) : 0;
+    const int tj = tjlo + (r - o * nt), tk = D - tj;
+    const int n = x + FLOW_XCD * o;
+    int* fl = flags + (size_t)n * plan.fstride;
+    auto wait = [&]() -> bool {
+      if (threadIdx.x == 0) {
+        // one load per poll (the y flag, then the z flag), the give-up word
+        // every 256 polls: fewer requests queue in the L2 ahead of the
+        // blocks' own loads and stores (0.32 -> 0.23 ms per source iteration
+        // at 256 x 64 x 48, S8, against two flags and the word every poll)
+        int ok = 1;
+        bool ry = tj == 0, rz = tk == 0;
+        for (int spin = 0;; ++spin) {
+          if (!ry)
+            ry = __hip_atomic_load(&fl[tk * TJ + tj - 1], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT) >= epoch;
+          else if (!rz)
+            rz = __hip_atomic_load(&fl[(tk - 1) * TJ + tj], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT) >= epoch;
+          if (ry && rz) break;
+          if (spin >= FLOW_SPIN ||
+              ((spin & 255) == 255 && __hip_atomic_load(&flow_q[flow_gave_up(x)],
+                                                        __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+            __hip_atomic_store(&flow_q[flow_gave_up(x)], 1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            ok = 0;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+        s_ok = ok;
+      }
+      __syncthreads();
+      return s_ok != 0;
+    };
+    if (!sweep_block<T, PER, true, TS, true>(g, n, tj, tk, Om, Iset, kap, B, Ig_out, I_amb, sh,
+                                             wait))
+      return;   // uniform: s_ok
+    // every wavefront's stores of the block have reached the L2 (on gfx9
+    // vmcnt counts stores too), then the block's flag
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&fl[tk * TJ + tj], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -702,8 +925,48 @@ static int sweep_tiled_ts(DomHandle* h, T I_amb, hipStream_t s) {
 template <typename T>
 int tile_side_of(const DomHandle* h) { return tile_side<T>(h); }
 
+// Whether the dataflow sweep runs this handle's source iterations: built for
+// the grid (whole rows of 64 PER cells, 4 x 4 blocks in LDS, a plan that
+// fits), not switched off, and the tile side 4 in use.
+template <typename T>
+bool flow_active(const DomHandle* h) {
+  return h->flow == 1 && tile_side<T>(h) == 4 && flow_rows(h->g.nx, sizeof(T));
+}
+
+// One source iteration's sweep as one dataflow launch: 128 workgroups per
+// XCD (four per CU), each taking blocks from its XCD's queue.
+template <typename T>
+static int sweep_flow(DomHandle* h, T I_amb, hipStream_t s) {
+  const int Nx = h->g.nx;
+  const size_t shm = 16 * (size_t)Nx * sizeof(T);
+  const dim3 grid(FLOW_XCD * 128);
+  const KernelEvents ev = timer_next(FW_TIMER_DOM_SWEEP);
+  auto go = [&](auto kern) {
+    if (ev.start)
+      hipExtLaunchKernelGGL(kern, grid, dim3(256), shm, s, ev.start, ev.stop, 0, h->ga, h->M,
+                            (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
+                            (const T*)h->B, (const T*)h->Ig_out, I_amb, (const double*)h->S,
+                            h->flow_q, h->flow_flags, h->plan);
+    else
+      hipLaunchKernelGGL(kern, grid, dim3(256), shm, s, h->ga, h->M, (const double*)h->Om,
+                         (T*)h->I, (const T*)h->kap_safe, (const T*)h->B,
+                         (const T*)h->Ig_out, I_amb, (const double*)h->S, h->flow_q,
+                         h->flow_flags, h->plan);
+  };
+  switch (Nx) {
+    case 64: go(k_dom_sweep_flow<T, 1, 4>); break;
+    case 128: go(k_dom_sweep_flow<T, 2, 4>); break;
+    case 256: go(k_dom_sweep_flow<T, 4, 4>); break;
+    case 512: go(k_dom_sweep_flow<T, 8, 4>); break;
+    default: return fw_set_error(FW_E_CONFIG, "dom: no dataflow sweep for rows of %d cells", Nx);
+  }
+  FW_CHECK_LAUNCH("dom_sweep_flow");
+  return FW_OK;
+}
+
 template <typename T>
 static int sweep_tiled(DomHandle* h, T I_amb, hipStream_t s) {
+  if (flow_active<T>(h)) return sweep_flow<T>(h, I_amb, s);
   switch (tile_side<T>(h)) {
     case 2: return sweep_tiled_ts<T, 2>(h, I_amb, s);
     case 8: return sweep_tiled_ts<T, 8>(h, I_amb, s);
@@ -753,7 +1016,7 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
                      (T*)h->kap_safe, (T*)h->B);
   hipLaunchKernelGGL(k_dom_ground_init<T>, dim3(nb(ncol)), dim3(256), 0, s, ncol,
                      (const T*)Tsoil, T(h->eps_ground), I_amb, (T*)h->Ig_emit,
-                     (T*)h->Ig_out, h->S);
+                     (T*)h->Ig_out, h->S, h->flow_q);
   FW_CHECK_LAUNCH("dom_props");
   // One source iteration: the sweep, G with its change, the ground update
   // and the change's final sum with the convergence test.  Every kernel of
@@ -780,7 +1043,9 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
                        (const T*)h->Ig_emit, (T*)h->Ig_out, (T*)h->q_in,
                        T(h->eps_ground), T(h->relax), (const double*)h->S);
     hipLaunchKernelGGL(k_dom_fin, dim3(1), dim3(256), 0, s, h->part, h->part2,
-                       (int)nb(n), h->S, first, h->tol);
+                       (int)nb(n), h->S, first, h->tol,
+                       flow_active<T>(h) ? h->flow_q : (int*)nullptr, h->plan.cum[h->plan.ndiag],
+                       M);
     FW_CHECK_LAUNCH("dom_G");
     return FW_OK;
   };
@@ -812,6 +1077,11 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
                                 hipMemcpyDeviceToHost, s));
     FW_CHECK_HIP(hipStreamSynchronize(s));
     if (h->S_host[DS_NAN] != 0.0) return fw_set_error(FW_E_NONFINITE, "dom: non-finite G");
+    if (h->S_host[DS_STALL] != 0.0) {
+      h->flow = -1;   // the per-diagonal launches from now on
+      return fw_set_error(FW_E_HIP, "dom: the dataflow sweep stalled (switched off for this "
+                                    "handle; the solve's intensities are invalid)");
+    }
     if (h->S_host[DS_DONE] != 0.0 || queued >= h->maxiter) break;
     batch = 1;
   }
@@ -919,7 +1189,28 @@ int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* we
   FW_CHECK_HIP(hipMalloc((void**)&h->part, npart * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->part2, npart * sizeof(double)));
   FW_CHECK_HIP(hipMalloc((void**)&h->S, DS_N * sizeof(double)));
+  FW_CHECK_HIP(hipMemset(h->S, 0, DS_N * sizeof(double)));
   FW_CHECK_HIP(hipHostMalloc((void**)&h->S_host, DS_N * sizeof(double)));
+  // the dataflow sweep's plan and state: 4 x 4 blocks of whole rows
+  const int TJ = (g->ny + 3) / 4, TK = (g->nz + 3) / 4;
+  if (flow_rows(g->nx, h->esz) && TJ + TK - 1 <= FLOW_MAXD) {
+    h->plan.TJ = TJ;
+    h->plan.TK = TK;
+    h->plan.ndiag = TJ + TK - 1;
+    h->plan.cum[0] = 0;
+    for (int D = 0; D < h->plan.ndiag; ++D) {
+      const int lo = D - (TK - 1) > 0 ? D - (TK - 1) : 0, hi = D < TJ - 1 ? D : TJ - 1;
+      h->plan.cum[D + 1] = h->plan.cum[D] + hi - lo + 1;
+    }
+    h->plan.fstride = (TJ * TK + FLOW_LINE - 1) / FLOW_LINE * FLOW_LINE;
+    const size_t nflags = (size_t)M * h->plan.fstride;
+    FW_CHECK_HIP(hipMalloc((void**)&h->flow_q, FLOW_LINE * FLOW_XCD * sizeof(int)));
+    FW_CHECK_HIP(hipMemset(h->flow_q, 0, FLOW_LINE * FLOW_XCD * sizeof(int)));
+    FW_CHECK_HIP(hipMalloc((void**)&h->flow_flags, nflags * sizeof(int)));
+    FW_CHECK_HIP(hipMemset(h->flow_flags, 0, nflags * sizeof(int)));
+    const char* e = getenv("FIREWAVE_DOM_FLOW");
+    h->flow = (e && e[0] == '0') ? -1 : 1;
+  }
   *handle = h;
   return FW_OK;
 }
@@ -929,7 +1220,8 @@ int fw_dom_destroy(void* handle) {
   DomHandle* h = (DomHandle*)handle;
   if (!h) return FW_OK;
   void* b[] = {h->Om, h->w, h->I, h->G, h->Gp, h->kap, h->kap_s, h->kap_safe, h->B,
-               h->Ig_out, h->Ig_emit, h->q_in, h->part, h->part2, h->S};
+               h->Ig_out, h->Ig_emit, h->q_in, h->part, h->part2, h->S, h->flow_q,
+               h->flow_flags};
   for (void* p : b)
     if (p) (void)hipFree(p);
   if (h->S_host) (void)hipHostFree(h->S_host);
@@ -972,6 +1264,29 @@ int fw_dom_set_tiling(void* handle, int tile_side, int* active) {
   h->tile = tile_side;
   if (active)
     *active = h->g.dtype == FW_F64 ? tile_side_of<double>(h) : tile_side_of<float>(h);
+  return FW_OK;
+}
+
+int fw_dom_set_flow(void* handle, int enable, int* active) {
+  DomHandle* h = (DomHandle*)handle;
+  FW_REQUIRE(h, "dom_set_flow: null handle");
+  if (h->flow != 0) h->flow = enable ? 1 : -1;
+  if (active)
+    *active = h->g.dtype == FW_F64 ? (int)flow_active<double>(h) : (int)flow_active<float>(h);
+  return FW_OK;
+}
+
+int fw_dom_sweep_launches(void* handle, int* launches) {
+  DomHandle* h = (DomHandle*)handle;
+  FW_REQUIRE(h && launches, "dom_sweep_launches: null argument");
+  const int ts = h->g.dtype == FW_F64 ? tile_side_of<double>(h) : tile_side_of<float>(h);
+  const bool fl = h->g.dtype == FW_F64 ? flow_active<double>(h) : flow_active<float>(h);
+  if (fl)
+    *launches = 1;
+  else if (ts > 0)
+    *launches = (h->g.ny + ts - 1) / ts + (h->g.nz + ts - 1) / ts - 1;
+  else
+    *launches = h->g.ny + h->g.nz - 1;
   return FW_OK;
 }
 

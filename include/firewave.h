@@ -328,6 +328,18 @@ int fw_dom_stats(void* handle, int* iters);
  * per grid by the autotuner (firewebapp_amd/autotune.py). */
 int fw_dom_set_tiling(void* handle, int tile_side, int* active);
 
+/* Dataflow sweep on or off (no counterpart in the reference): one launch per
+ * source iteration in which each 4 x 4 block waits on the two blocks it
+ * reads from, instead of one launch per block diagonal.  Built for rows of
+ * 64, 128, 256 or 512 cells (float64: up to 256) and used with tile side 4; on
+ * by default (environment FIREWAVE_DOM_FLOW=0: off).  The intensities are
+ * the same either way.  *active (may be NULL): whether it now runs. */
+int fw_dom_set_flow(void* handle, int enable, int* active);
+
+/* Sweep launches per source iteration with the current settings (1 for the
+ * dataflow sweep). */
+int fw_dom_sweep_launches(void* handle, int* launches);
+
 /* Device pointer of I_set (M, Nz, Ny, Nx) in the grid's type (read by
  * flame_front_3d.compute_q_dom_fwd_at_band, flame_front_3d.py:594-630). */
 int fw_dom_intensity(void* handle, void** I_set);

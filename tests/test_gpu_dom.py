@@ -112,3 +112,47 @@ def test_dom_sweep_shapes_vs_oracle(shape, dtype):
         assert gpu.last_iters == ref.last_iters
     for a, b in zip((back(gs), back(gg), back(gsoil)), (qs, qg, qsoil)):
         assert relerr(a, b) <= tol
+
+
+@pytest.mark.parametrize("shape,dtype", [((11, 6, 256), np.float32),    # ragged blocks
+                                         ((13, 9, 128), np.float64),
+                                         ((8, 3, 64), np.float32),      # wrap inside a block
+                                         ((48, 64, 256), np.float32)])  # the 256x64x48 grid
+def test_dom_dataflow_sweep_matches_launches(shape, dtype):
+    """The dataflow sweep (one launch per source iteration, blocks waiting on
+    their neighbours' flags) against the per-diagonal launches: the same
+    intensities, fluxes and iteration counts, bit for bit, over two solves
+    (the second warm-started), S8."""
+    import ctypes as C
+    import torch
+    from firewebapp_amd import _lib
+    rng = np.random.default_rng(11)
+    nz, ny, nx = shape
+    dz = 0.03 + 0.07 * rng.random(nz)
+    alpha = np.zeros(shape)
+    alpha[:2] = 0.002 + 0.01 * rng.random((2, ny, nx))
+    T_s = 300.0 + 800.0 * rng.random(shape) * (alpha > 0)
+    T_g = 300.0 + 1200.0 * rng.random(shape)
+    om = 5e-3 * rng.random(shape)
+    Tsoil = 300.0 + 50.0 * rng.random((ny, nx))
+    tdt = torch.float32 if dtype == np.float32 else torch.float64
+    cv = lambda a: torch.as_tensor(a, dtype=tdt, device="cuda")
+    runs = []
+    for flow in (1, 0):
+        s = _solver(*shape, 0.12, 0.1, dz, dz, dz, "periodic", 8)
+        act = C.c_int(-1)
+        _lib.call("fw_dom_set_flow", s._handle(tdt), flow, C.byref(act))
+        assert act.value == flow
+        outs = []
+        for rep in range(2):
+            qs, qg = cv(np.zeros(shape)), cv(np.zeros(shape))
+            qsoil = cv(np.zeros((ny, nx)))
+            s.solve(cv(T_s), cv(T_g * (1.0 + 0.05 * rep)), cv(alpha), cv(om), 2000.0, 300.0,
+                    qs, qg, cv(Tsoil), qsoil)
+            torch.cuda.synchronize()
+            outs.append((s.last_iters, qs.cpu(), qg.cpu(), qsoil.cpu(), s.I_set.cpu()))
+        runs.append(outs)
+    for a, b in zip(*runs):
+        assert a[0] == b[0]
+        for x, y in zip(a[1:], b[1:]):
+            assert torch.equal(x, y)
