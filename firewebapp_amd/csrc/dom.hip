@@ -572,12 +572,17 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
 }
 
 // G = sum_n w_n I_n (ordinates summed in order), one cell per thread; per
-// workgroup partial max |G - G_prev| and max G
+// workgroup partial max |G - G_prev| and max G.  The cells of the ground
+// layer (c < ncol) also update the ground from the same loads (dom_3d.py's
+// q_in_ground / I_ground_out step): q_in = sum over the downward ordinates of
+// w |mu| I, in order, and I_ground_out relaxed toward emit + (1 - eps) q_in / pi.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ w,
         T* __restrict__ G0, T* __restrict__ G1, double* __restrict__ pdiff,
-        double* __restrict__ pmax, const double* __restrict__ S) {
+        double* __restrict__ pmax, const double* __restrict__ S,
+        const double* __restrict__ Om, int ncol, const T* __restrict__ emit,
+        T* __restrict__ Ig_out, T* __restrict__ q_in, T eps, T relax) {
   if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
   // this iteration writes G0 or G1 by its parity; the other holds G_prev
   const bool odd = ((long long)S[DS_ITERS] & 1) != 0;
@@ -587,8 +592,14 @@ k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ 
   double md = 0.0, mg = -1e300;
   const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (c < n) {
-    T s = T(0);
+    T s = T(0), qg = T(0);
     const T* p = Iset + c;
+    const bool ground = c < (size_t)ncol;
+    // the ground sum's term of ordinate m (downward ordinates only)
+    auto down = [&](int m, T iv) {
+      const double mu = Om[3 * m + 2];
+      if (mu < 0.0) qg += T(w[m]) * T(fabs(mu)) * iv;
+    };
     int q = 0;
     for (; q + 4 <= M; q += 4) {
       const T i0 = p[(size_t)q * n], i1 = p[(size_t)(q + 1) * n];
@@ -597,9 +608,24 @@ k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ 
       s += T(w[q + 1]) * i1;
       s += T(w[q + 2]) * i2;
       s += T(w[q + 3]) * i3;
+      if (ground) {
+        down(q, i0);
+        down(q + 1, i1);
+        down(q + 2, i2);
+        down(q + 3, i3);
+      }
     }
-    for (; q < M; ++q) s += T(w[q]) * p[(size_t)q * n];
+    for (; q < M; ++q) {
+      const T iv = p[(size_t)q * n];
+      s += T(w[q]) * iv;
+      if (ground) down(q, iv);
+    }
     G[c] = s;
+    if (ground) {
+      q_in[c] = qg;
+      const T Inew = emit[c] + (T(1) - eps) / T(M_PI) * qg;
+      Ig_out[c] = relax * Inew + (T(1) - relax) * Ig_out[c];
+    }
     const double dd = fabs((double)s - (double)Gp[c]);
     if (dd > md || dd != dd) md = dd;
     if ((double)s > mg) mg = (double)s;
@@ -616,39 +642,6 @@ k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ 
     __syncthreads();
   }
   if (t == 0) { pdiff[blockIdx.x] = sh1[0]; pmax[blockIdx.x] = sh2[0]; }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256)
-k_dom_ground(int ncol, size_t n, int M, const double* __restrict__ Om,
-             const double* __restrict__ w, const T* __restrict__ Iset,
-             const T* __restrict__ emit, T* __restrict__ Ig_out,
-             T* __restrict__ q_in, T eps, T relax, const double* __restrict__ S) {
-  if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncol) return;
-  // the downward ordinates summed in order; their intensities are loaded
-  // eight at a time ahead of the sums, so the loads overlap
-  T q = T(0);
-  int m = 0;
-  for (; m + 8 <= M; m += 8) {
-    T iv[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u)
-      iv[u] = Om[3 * (m + u) + 2] < 0.0 ? Iset[(size_t)(m + u) * n + c] : T(0);
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const double mu = Om[3 * (m + u) + 2];
-      if (mu < 0.0) q += T(w[m + u]) * T(fabs(mu)) * iv[u];
-    }
-  }
-  for (; m < M; ++m) {
-    const double mu = Om[3 * m + 2];
-    if (mu < 0.0) q += T(w[m]) * T(fabs(mu)) * Iset[(size_t)m * n + c];
-  }
-  q_in[c] = q;
-  const T Inew = emit[c] + (T(1) - eps) / T(M_PI) * q;
-  Ig_out[c] = relax * Inew + (T(1) - relax) * Ig_out[c];
 }
 
 template <typename T>
@@ -1018,8 +1011,8 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
                      (const T*)Tsoil, T(h->eps_ground), I_amb, (T*)h->Ig_emit,
                      (T*)h->Ig_out, h->S, h->flow_q);
   FW_CHECK_LAUNCH("dom_props");
-  // One source iteration: the sweep, G with its change, the ground update
-  // and the change's final sum with the convergence test.  Every kernel of
+  // One source iteration: the sweep, G with its change and the ground
+  // update, and the change's final sum with the convergence test.  Every kernel of
   // it returns at once once the solve has converged.  Its launches depend
   // only on whether it is the first, so off the legacy default stream and
   // with the sweep timer off it is replayed from a captured graph.
@@ -1037,11 +1030,9 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
     FW_CHECK_LAUNCH("dom_sweep");
     hipLaunchKernelGGL(k_dom_G<T>, dim3(nb(n)), dim3(256), 0, s, n, M,
                        (const T*)h->I, (const double*)h->w, (T*)h->G, (T*)h->Gp,
-                       h->part, h->part2, (const double*)h->S);
-    hipLaunchKernelGGL(k_dom_ground<T>, dim3(nb(ncol)), dim3(256), 0, s, ncol, n, M,
-                       (const double*)h->Om, (const double*)h->w, (const T*)h->I,
-                       (const T*)h->Ig_emit, (T*)h->Ig_out, (T*)h->q_in,
-                       T(h->eps_ground), T(h->relax), (const double*)h->S);
+                       h->part, h->part2, (const double*)h->S, (const double*)h->Om, ncol,
+                       (const T*)h->Ig_emit, (T*)h->Ig_out, (T*)h->q_in, T(h->eps_ground),
+                       T(h->relax));
     hipLaunchKernelGGL(k_dom_fin, dim3(1), dim3(256), 0, s, h->part, h->part2,
                        (int)nb(n), h->S, first, h->tol,
                        flow_active<T>(h) ? h->flow_q : (int*)nullptr, h->plan.cum[h->plan.ndiag],
