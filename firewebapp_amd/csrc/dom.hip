@@ -80,6 +80,7 @@ struct DomHandle {
   // for this grid, -1 switched off
   int *flow_q, *flow_flags;
   int flow;
+  int flow_wg;      // dataflow sweep workgroups per XCD
   FlowPlan plan;
 };
 
@@ -932,7 +933,7 @@ template <typename T>
 static int sweep_flow(DomHandle* h, T I_amb, hipStream_t s) {
   const int Nx = h->g.nx;
   const size_t shm = 16 * (size_t)Nx * sizeof(T);
-  const dim3 grid(FLOW_XCD * 128);
+  const dim3 grid(FLOW_XCD * h->flow_wg);
   const KernelEvents ev = timer_next(FW_TIMER_DOM_SWEEP);
   auto go = [&](auto kern) {
     if (ev.start)
@@ -1201,6 +1202,10 @@ int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* we
     FW_CHECK_HIP(hipMemset(h->flow_flags, 0, nflags * sizeof(int)));
     const char* e = getenv("FIREWAVE_DOM_FLOW");
     h->flow = (e && e[0] == '0') ? -1 : 1;
+    const char* wg = getenv("FIREWAVE_DOM_FLOW_WG");
+    h->flow_wg = wg ? atoi(wg) : 128;
+    if (h->flow_wg < 8) h->flow_wg = 8;
+    if (h->flow_wg > 512) h->flow_wg = 512;
   }
   *handle = h;
   return FW_OK;
