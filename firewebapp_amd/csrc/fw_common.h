@@ -79,8 +79,13 @@ int species3_launch(const fw_grid* g, const void* const* Y, void* const* Yo, con
                     const void* Sd, double dt, double D, const double* y_in,
                     const double* y_max, hipStream_t s, const double* sb);
 // fw_turbulent_diffusion_pass; sb (may be null): dt_sub from sb[SB_DPR].
+// edc (may be null): the EDC update of the new field (the temperature) fused
+// into the pass, one 256-cell segment per workgroup (needs Nx % 256 == 0 and
+// the species' segment summaries in edc->summary).
+struct EdcFuseArgs;
 int turb_pass_launch(const fw_grid* g, const void* f, void* fo, const void* nu_t, double sc_t,
-                     double dt_sub, double lo, double hi, hipStream_t s, const double* sb);
+                     double dt_sub, double lo, double hi, hipStream_t s, const double* sb,
+                     const EdcFuseArgs* edc = nullptr);
 // fw_gas_update_solid_conduction; sb (may be null): dt_sub from sb[SB_DT_SUB].
 int gas_solid_launch(const fw_grid* g, void* T_g, const void* T_s, void* T_s_out,
                      const void* alpha_s, const void* rho, const void* Y_H2O,

@@ -17,6 +17,7 @@
 #include "fw_common.h"
 #include "graph.h"
 #include "gas_solid.h"
+#include "edc_cell.h"
 
 namespace fw {
 
@@ -72,6 +73,13 @@ static int turb_substeps(double nu_t_max, double sc_t, double dt, double dx, dou
 // after k-epsilon.  FIREWAVE_DOM_BESIDE=0 issues it in line instead.
 static bool dom_beside_on() {
   const char* e = getenv("FIREWAVE_DOM_BESIDE");
+  return !(e && e[0] == '0');
+}
+
+// The next sub-step's EDC fused into the temperature pass (same results as
+// its own launch); FIREWAVE_FUSE_EDC=0 launches it on its own.
+static bool fuse_edc_on() {
+  const char* e = getenv("FIREWAVE_FUSE_EDC");
   return !(e && e[0] == '0');
 }
 
@@ -217,8 +225,17 @@ static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_param
   // segment summaries (nothing between them changes the species) when the
   // launch rows are whole 256-cell segments
   bool summary_ready = false;
+  // the EDC of sub-step sub + 1 rides in sub-step sub's last temperature
+  // pass when that pass exists and the species pass left the summaries:
+  // pointwise per cell (and per segment for the carried flag), it reads the
+  // new temperature and the species, which nothing in between changes
+  const bool fuse_edc = P.fuse_summary && P.npr > 0 && fuse_edc_on();
+  const EdcFuseArgs edc_args{f->rho, f->k, f->eps, nullptr, nullptr, nullptr, f->omega,
+                             (const int*)f->edc_work,
+                             edc_params(0.34, 1100.0, dt_sub, 1, 0, 1.3, 17.0e6)};
+  bool edc_done = false;
   for (int sub = 0; sub < N_SUB; ++sub) {
-    if (launch)
+    if (launch && !edc_done)
       TRY(edc_launch(g, f->rho, f->T_g, f->Y_fuel, f->Y_O2, f->k, f->eps, f->Y_H2O, f->omega,
                      0.34, 1100.0, dt_sub, 1, 0, 1.3, 17.0e6, (int*)f->edc_work, summary_ready,
                      s, sb));
@@ -306,13 +323,20 @@ static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_param
     if (fuse_gs) SWAP(f->T_s, f->T_s_old);
     else SWAP(f->T_s, f->tmp1);
     // T_g turbulent diffusion, the clamps of :3193-3194 in the last pass
+    const bool edc_next = fuse_edc && sub < N_SUB - 1;
     for (int m = 0; m < P.npr; ++m) {
       const bool last = m == P.npr - 1;
+      EdcFuseArgs ea = edc_args;
+      ea.Y_fuel = f->Y_fuel;
+      ea.Y_O2 = f->Y_O2;
+      ea.Y_H2O = f->Y_H2O;
       if (launch)
         TRY(turb_pass_launch(g, f->T_g, f->tmp1, f->nu_t, PR_T, P.dpr,
-                             last ? p->T_amb : 1.0, last ? T_CAP : 0.0, s, sb));
+                             last ? p->T_amb : 1.0, last ? T_CAP : 0.0, s, sb,
+                             edc_next && last ? &ea : nullptr));
       SWAP(f->T_g, f->tmp1);
     }
+    edc_done = edc_next;
     if (P.npr == 0 && launch) {
       // the particle path's gas update already floors T_g at T_amb; the
       // coupling does not
@@ -333,7 +357,7 @@ static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_param
 static std::vector<uint64_t> substep_key(const fw_grid* g, const fw_spread_fields* f,
                                          const SubstepPlan& P) {
   return {kw(g), (uint64_t)g->dtype, (uint64_t)P.nsc, (uint64_t)P.npr,
-          (uint64_t)P.fuse_summary, kw(P.drip_gas), kw(P.drip_solid),
+          (uint64_t)P.fuse_summary, (uint64_t)fuse_edc_on(), kw(P.drip_gas), kw(P.drip_solid),
           kw(f->rho), kw(f->T_g), kw(f->T_s), kw(f->Y_fuel), kw(f->Y_O2), kw(f->Y_H2O),
           kw(f->k), kw(f->eps), kw(f->omega), kw(f->nu_t), kw(f->u), kw(f->v), kw(f->w),
           kw(f->src[0]), kw(f->src[1]), kw(f->src[7]), kw(f->q_rad_gas), kw(f->Q_comb),

@@ -14,6 +14,7 @@
 
 #include "fw_common.h"
 #include "gas_solid.h"
+#include "edc_cell.h"
 
 namespace fw {
 
@@ -325,6 +326,30 @@ k_turb_diff(GridArgs g, const T* __restrict__ f, T* __restrict__ fo,
   if (i >= g.nx) return;
   const size_t c = cidx(g, k, j, i);
   fo[c] = turb_cell<T>(g, i, j, k, c, f, nu_t[c] / sc_t, dt_sub, lo, hi);
+}
+
+// The temperature pass with the next EDC update of the same cell fused in:
+// the workgroup is one whole 256-cell segment (256 threads along x, Nx a
+// multiple of 256, checked by the host, so no thread returns early), the
+// diffused temperature goes straight into the EDC update and its result to
+// fo.  The species' segment summaries come from the species pass.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_turb_diff_edc(GridArgs g, const T* __restrict__ f, T* __restrict__ fo,
+                const T* __restrict__ nu_t, T sc_t, T dt_sub, T lo, T hi,
+                const double* __restrict__ sb, int sb_slot, size_t n,
+                const T* __restrict__ rho, T* __restrict__ Yf, T* __restrict__ YO2,
+                const T* __restrict__ kt, const T* __restrict__ et,
+                const T* __restrict__ Yw, T* __restrict__ omega,
+                const int* __restrict__ summary, EdcParams P) {
+  if (sb) dt_sub = T(sb[sb_slot]);
+  P.dt = sb_get(sb, SB_DT_SUB, P.dt);
+  P.relax = sb_get(sb, SB_EDC_RELAX, P.relax);
+  int i, j, k;
+  cell_ijk(i, j, k);
+  const size_t c = cidx(g, k, j, i);
+  const T tn = turb_cell<T>(g, i, j, k, c, f, nu_t[c] / sc_t, dt_sub, lo, hi);
+  edc_segment<T>(n, (int)(c / EDC_SEG), tn, rho, fo, Yf, YO2, kt, et, Yw, omega, summary, P);
 }
 
 // The same pass for three fields sharing nu_t (the species of a sub-step).
@@ -771,18 +796,32 @@ static int gas_energy_impl(const fw_grid* g, const void* Tg, void* To,
 template <typename T>
 static int turb_diff_pass_impl(const fw_grid* g, const void* f, void* fo,
                                const void* nu_t, double sc_t, double dt_sub,
-                               double lo, double hi, hipStream_t s, const double* sb) {
+                               double lo, double hi, hipStream_t s, const double* sb,
+                               const EdcFuseArgs* e) {
   Launch L = cell_launch(g);
-  hipLaunchKernelGGL(k_turb_diff<T>, L.grid, L.block, 0, s, grid_args(g),
-                     (const T*)f, (T*)fo, (const T*)nu_t, T(sc_t), T(dt_sub),
-                     T(lo), T(hi), sb, (int)SB_DPR);
+  if (e) {
+    FW_REQUIRE(g->nx % EDC_SEG == 0 && L.block.x == EDC_SEG && e->summary,
+               "turbulent_diffusion: the fused EDC needs whole 256-cell rows and summaries");
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    hipLaunchKernelGGL(k_turb_diff_edc<T>, L.grid, L.block, 0, s, grid_args(g),
+                       (const T*)f, (T*)fo, (const T*)nu_t, T(sc_t), T(dt_sub), T(lo), T(hi),
+                       sb, (int)SB_DPR, n, (const T*)e->rho, (T*)e->Y_fuel, (T*)e->Y_O2,
+                       (const T*)e->k_turb, (const T*)e->eps_turb, (const T*)e->Y_H2O,
+                       (T*)e->omega_out, e->summary, e->P);
+  } else {
+    hipLaunchKernelGGL(k_turb_diff<T>, L.grid, L.block, 0, s, grid_args(g),
+                       (const T*)f, (T*)fo, (const T*)nu_t, T(sc_t), T(dt_sub),
+                       T(lo), T(hi), sb, (int)SB_DPR);
+  }
   FW_CHECK_LAUNCH("turbulent_diffusion");
   return FW_OK;
 }
 
 int turb_pass_launch(const fw_grid* g, const void* f, void* fo, const void* nu_t, double sc_t,
-                     double dt_sub, double lo, double hi, hipStream_t s, const double* sb) {
-  return FW_DISPATCH(g, turb_diff_pass_impl, g, f, fo, nu_t, sc_t, dt_sub, lo, hi, s, sb);
+                     double dt_sub, double lo, double hi, hipStream_t s, const double* sb,
+                     const EdcFuseArgs* edc) {
+  return FW_DISPATCH(g, turb_diff_pass_impl, g, f, fo, nu_t, sc_t, dt_sub, lo, hi, s, sb,
+                     edc);
 }
 
 template <typename T>
@@ -992,7 +1031,8 @@ int fw_turbulent_diffusion_pass(const fw_grid* g, const void* f, void* f_out,
   FW_GRID(g);
   FW_REQUIRE(g && f != f_out, "turbulent_diffusion: f_out must not alias f");
   return FW_DISPATCH(g, turb_diff_pass_impl, g, f, f_out, nu_t, sc_t, dt_sub,
-                     clip_lo, clip_hi, (hipStream_t)stream, (const double*)nullptr);
+                     clip_lo, clip_hi, (hipStream_t)stream, (const double*)nullptr,
+                     (const EdcFuseArgs*)nullptr);
 }
 
 int fw_nu_t_max_interior(const fw_grid* g, const void* nu_t, double* work,

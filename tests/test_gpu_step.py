@@ -356,3 +356,26 @@ def test_fused_substep_kernels_match_single(dtype):
               ptr(alpha), ptr(rho), ptr(Yw), ptr(Sp), ptr(Sd), ptr(Qgc), ptr(qrg), ptr(Qc2),
               nzb, 2, 7, 6.0e4, 1e-4, 300.0, 0.2, 500.0, 1300.0, 300.0, 1.0e4, stream_ptr())
     assert torch.equal(Tg1, Tg2) and torch.equal(Qc1, Qc2) and torch.equal(Ts1, Ts2)
+
+
+def test_fused_edc_matches_own_launch(monkeypatch):
+    """The next sub-step's EDC fused into the temperature diffusion pass
+    against its own launch (FIREWAVE_FUSE_EDC=0), at 256x64x48 (whole
+    256-cell rows, the case the fusion takes): 12 steps, fields, particles
+    and diagnostics bit for bit, float32 and float64."""
+    import bench
+    from firewebapp_amd.spread_3d import Spread3DConfig, Spread3DSolver
+    name = "spread3d_256x64x48_s8"
+    for dtype in ("float32", "float64"):
+        states = []
+        for fuse in ("1", "0"):
+            monkeypatch.setenv("FIREWAVE_FUSE_EDC", fuse)
+            cfg = Spread3DConfig.resolve(bench.deck(name, dtype), wind_speed_m_s=bench.wind(name))
+            sv = Spread3DSolver(cfg)
+            assert sv.advance(12) is None
+            states.append((sv.state_numpy(), dict(sv.diag)))
+            sv.close()
+        (sa, da), (sb, db) = states
+        assert da == db, dtype
+        for k in sa:
+            assert np.array_equal(sa[k], sb[k]), (dtype, k)
