@@ -1054,14 +1054,15 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
       return r2;
     });
   };
-  // Iterations in batches with one host read per batch: one fewer than the
-  // previous solve's count first (the count changes slowly from solve to
-  // solve, so the batch seldom runs past convergence; the first iteration
-  // never converges, so at least two), then one at a time.  The device flag
-  // stops every kernel after the converging iteration, so the result is that
-  // of a one-by-one loop.
+  // Iterations in batches with one host read per batch: the previous solve's
+  // count first (the count changes slowly from solve to solve; the first
+  // iteration never converges, so at least two), then one at a time.  The
+  // device flag stops every kernel after the converging iteration, so the
+  // result is that of a one-by-one loop.  An iteration queued past
+  // convergence costs its launches, which return at once (a few us); one
+  // queued short costs a read and the host's turnaround.
   int queued = 0;
-  int batch = h->last_iters > 3 ? h->last_iters - 1 : 2;
+  int batch = h->last_iters > 2 ? h->last_iters : 2;
   for (;;) {
     for (int b = 0; b < batch && queued < h->maxiter; ++b, ++queued)
       TRY_DOM(run_iteration(queued == 0));
