@@ -128,13 +128,17 @@ def dom_sweep_bytes(esz, M, cells, ts):
 
 SPECIES3_FIELDS = 12
 # HBM bytes per launch from the FETCH_SIZE + WRITE_SIZE counters scaled by a
-# 256 MiB copy (float32): profiles/step_256x64x48_s8_hbm_traffic.md
-# HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated) from the
-# counter runs summarised under profiles/; the DOM sweep's depend on its
-# tile side (the last key)
-ROOFLINE_TRAFFIC = {("k_species3", "spread3d_256x64x48_s8", "float32", 0): 44060018,
-                    ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 4): 18176445,
-                    ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 8): 36213937}
+# 256 MiB copy (float32), from the counter runs summarised in
+# profiles/step_256x64x48_s8_hbm_traffic.md and
+# profiles/step_512x128x64_s8_kernel_roofline.md; the DOM sweep's depend on
+# its kernel and tile side (the last key)
+ROOFLINE_TRAFFIC = {
+    ("k_species3", "spread3d_256x64x48_s8", "float32", 0): 43978577,
+    ("k_species3", "spread3d_512x128x64_s8", "float32", 0): 354541293,
+    ("k_dom_sweep_flow", "spread3d_256x64x48_s8", "float32", 4): 485769553,
+    ("k_dom_sweep_flow", "spread3d_512x128x64_s8", "float32", 4): 3674013840,
+    ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 4): 18176445,
+    ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 8): 36213937}
 
 
 # Phase-fused compulsory traffic of one outer step in field-passes (each

@@ -46,8 +46,10 @@ k_edc(size_t n, const T* __restrict__ rho, T* __restrict__ Tg,
   P.dt = sb_get(sb, SB_DT_SUB, P.dt);
   P.relax = sb_get(sb, SB_EDC_RELAX, P.relax);
   const size_t c = (size_t)blockIdx.x * EDC_SEG + threadIdx.x;
-  edc_segment<T>(n, (int)blockIdx.x, c < n ? Tg[c] : T(0), rho, Tg, Yf_io, YO2_io, kturb,
-                 eturb, Yw_in, omega_out, summary, P);
+  const bool valid = c < n;
+  const EdcCellIn<T> in = edc_load<T>(valid, c, rho, Yf_io, YO2_io, kturb, eturb, Yw_in);
+  edc_segment<T>(n, (int)blockIdx.x, valid ? Tg[c] : T(0), in, Tg, Yf_io, YO2_io, omega_out,
+                 summary, P);
 }
 
 // summary_ready: the segment summaries of the current Y_F, Y_O2, Y_H2O are

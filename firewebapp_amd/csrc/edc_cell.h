@@ -33,13 +33,34 @@ struct EdcFuseArgs {
   EdcParams P;
 };
 
+// A cell's EDC inputs besides its temperature, loaded by the caller (so a
+// fused kernel can issue them with its own loads); past n: the neutral values.
 template <typename T>
-__device__ __forceinline__ void edc_segment(size_t n, int seg, T Tg_in,
-                                            const T* __restrict__ rho, T* __restrict__ Tg_out,
-                                            T* __restrict__ Yf_io, T* __restrict__ YO2_io,
-                                            const T* __restrict__ kturb,
-                                            const T* __restrict__ eturb,
-                                            const T* __restrict__ Yw_in,
+struct EdcCellIn {
+  T Yf, YO2, Yw, rho, k, eps;
+};
+
+template <typename T>
+__device__ __forceinline__ EdcCellIn<T> edc_load(bool valid, size_t c, const T* __restrict__ rho,
+                                                 const T* __restrict__ Yf,
+                                                 const T* __restrict__ YO2,
+                                                 const T* __restrict__ kturb,
+                                                 const T* __restrict__ eturb,
+                                                 const T* __restrict__ Yw) {
+  EdcCellIn<T> a;
+  a.Yf = valid ? Yf[c] : T(0);
+  a.YO2 = valid ? YO2[c] : T(0);
+  a.Yw = valid ? Yw[c] : T(0);
+  a.rho = valid ? rho[c] : T(1);
+  a.k = valid ? kturb[c] : T(1);
+  a.eps = valid ? eturb[c] : T(1);
+  return a;
+}
+
+template <typename T>
+__device__ __forceinline__ void edc_segment(size_t n, int seg, T Tg_in, const EdcCellIn<T>& in,
+                                            T* __restrict__ Tg_out, T* __restrict__ Yf_io,
+                                            T* __restrict__ YO2_io,
                                             T* __restrict__ omega_out,
                                             const int* __restrict__ summary,
                                             const EdcParams& P) {
@@ -50,13 +71,13 @@ __device__ __forceinline__ void edc_segment(size_t n, int seg, T Tg_in,
   const size_t c = (size_t)seg * EDC_SEG + t;
   const bool valid = c < n;
   // every load of the cell is issued before the workgroup exchange below
-  T Yf = valid ? Yf_io[c] : T(0);
-  T YO2 = valid ? YO2_io[c] : T(0);
-  const T Yw = valid ? Yw_in[c] : T(0);
+  T Yf = in.Yf;
+  T YO2 = in.YO2;
+  const T Yw = in.Yw;
   T Tgv = Tg_in;
-  const T rho_i = valid ? rho[c] : T(1);
-  T k_t = valid ? kturb[c] : T(1);
-  T e_t = valid ? eturb[c] : T(1);
+  const T rho_i = in.rho;
+  T k_t = in.k;
+  T e_t = in.eps;
   const bool resets = valid && edc_resets<T>(Yf, YO2);
   own_sh[t] = edc_own_flag<T>(Yw) ? 1 : 0;
   // nearest preceding resetting cell of the segment: in my wavefront (ballot

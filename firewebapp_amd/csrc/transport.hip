@@ -348,8 +348,10 @@ k_turb_diff_edc(GridArgs g, const T* __restrict__ f, T* __restrict__ fo,
   int i, j, k;
   cell_ijk(i, j, k);
   const size_t c = cidx(g, k, j, i);
+  // the EDC inputs issued with the stencil's loads
+  const EdcCellIn<T> in = edc_load<T>(true, c, rho, Yf, YO2, kt, et, Yw);
   const T tn = turb_cell<T>(g, i, j, k, c, f, nu_t[c] / sc_t, dt_sub, lo, hi);
-  edc_segment<T>(n, (int)(c / EDC_SEG), tn, rho, fo, Yf, YO2, kt, et, Yw, omega, summary, P);
+  edc_segment<T>(n, (int)(c / EDC_SEG), tn, in, fo, Yf, YO2, omega, summary, P);
 }
 
 // The same pass for three fields sharing nu_t (the species of a sub-step).
