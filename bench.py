@@ -134,9 +134,9 @@ SPECIES3_FIELDS = 12
 # its kernel and tile side (the last key)
 ROOFLINE_TRAFFIC = {
     ("k_species3", "spread3d_256x64x48_s8", "float32", 0): 43978577,
-    ("k_species3", "spread3d_512x128x64_s8", "float32", 0): 354541293,
+    ("k_species3", "spread3d_512x128x64_s8", "float32", 0): 354394735,
     ("k_dom_sweep_flow", "spread3d_256x64x48_s8", "float32", 4): 485769553,
-    ("k_dom_sweep_flow", "spread3d_512x128x64_s8", "float32", 4): 3674013840,
+    ("k_dom_sweep_flow", "spread3d_512x128x64_s8", "float32", 4): 3673308680,
     ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 4): 18176445,
     ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 8): 36213937}
 
