@@ -739,22 +739,6 @@ k_sumsq(size_t n, const double* __restrict__ a, double* __restrict__ part) {
   block_sum_store(acc, sh, part);
 }
 
-// The warm start from the last two solutions: x <- 2 x - xp, xp <- x.
-__global__ void __launch_bounds__(256)
-k_extrapolate(size_t n, double* __restrict__ x, double* __restrict__ xp) {
-  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (c >= n) return;
-  const double t = x[c];
-  x[c] = 2.0 * t - xp[c];
-  xp[c] = t;
-}
-
-__global__ void __launch_bounds__(256)
-k_copy64(size_t n, const double* __restrict__ x, double* __restrict__ y) {
-  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (c < n) y[c] = x[c];
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_x64_add(size_t n, double* __restrict__ x, const T* __restrict__ d,
@@ -815,9 +799,22 @@ __global__ void k_round_init(double* __restrict__ S, double rtol, double floor) 
 // host's next read tells whether the round ended.
 __global__ void __launch_bounds__(256)
 k_finalize_round(const double* __restrict__ part, int np, double* __restrict__ S, double rtol,
-                 double floor, int gated, double tag) {
+                 double floor, int gated, double tag, const double* __restrict__ part_b) {
   __shared__ double sh[256];
   const int t = threadIdx.x;
+  if (part_b) {
+    // ||b||^2 into S_BB first, k_finalize's fixed-order sum (F_STORE)
+    double a = 0.0;
+    for (int b = t; b < np; b += 256) a += part_b[b];
+    sh[t] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if (t < w) sh[t] += sh[t + w];
+      __syncthreads();
+    }
+    if (t == 0) S[S_BB] = sh[0];
+    __syncthreads();
+  }
   // uniform; every thread reads it before the reduction's barriers, thread 0
   // writes S after them
   if (gated && S[S_DONE] != 1.0) return;
@@ -927,7 +924,8 @@ __global__ void __launch_bounds__(256)
 k_div_rhs(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict__ v,
           const T* __restrict__ w, const T* __restrict__ u_inlet,
           const T* __restrict__ target, double inv_dt, double* __restrict__ b,
-          double* __restrict__ part) {
+          double* __restrict__ part, double* __restrict__ x, double* __restrict__ xp,
+          int xmode) {
   __shared__ double sh[256];
   (void)n;
   // contiguous runs of rows in the XCD-aware order (the stencil's y and z
@@ -945,6 +943,15 @@ k_div_rhs(GridArgs g, size_t n, const T* __restrict__ u, const T* __restrict__ v
       const double r = d * inv_dt;
       if (b) b[c] = r;
       acc += r * r;
+      // the warm start's pointwise update on the same cells: xmode 2
+      // extrapolates x <- 2 x - xp, xp <- x; 1 keeps x in xp
+      if (xmode == 2) {
+        const double t = x[c];
+        x[c] = 2.0 * t - xp[c];
+        xp[c] = t;
+      } else if (xmode == 1) {
+        xp[c] = x[c];
+      }
     }
   }
   block_sum_store(acc, sh, part);
@@ -1847,7 +1854,7 @@ struct ProjHandle {
   double *x64, *b64;
   void *r, *z, *p, *q, *d, *w1, *w2;
   void* p2;   // second direction buffer of the fused update (k_matvec_pz)
-  double *part, *S, *S_host;
+  double *part, *part_b, *S, *S_host;   // part_b: b's partial norms (k_div_rhs)
   double *Ppq, *Prr, *Prz;  // partials of the PCG iteration's reductions
   void* irho;          // 1 / max(rho, 0.01) of the running solve (type T)
   void* net;           // learned correction (unet.hip), or null
@@ -2356,11 +2363,12 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
     hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n, rho,
                        h->b64, h->x64, (T*)h->r, h->eps_reg, h->part, (const double*)nullptr);
     hipLaunchKernelGGL(k_finalize_round, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
-                       rtol, inner_floor, 0, 0.0);
+                       rtol, inner_floor, 0, 0.0, (const double*)nullptr);
   };
+  // have_residual: the caller left b's partial norms in part_b as well
   if (have_residual)
     hipLaunchKernelGGL(k_finalize_round, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
-                       rtol, inner_floor, 0, 0.0);
+                       rtol, inner_floor, 0, 0.0, (const double*)h->part_b);
   else
     residual();
   FW_CHECK_LAUNCH("residual64");
@@ -2409,7 +2417,7 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
       hipLaunchKernelGGL((k_residual64<T, TR>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n, rho,
                          h->b64, h->x64, (T*)h->r, h->eps_reg, h->part, (const double*)h->S);
       hipLaunchKernelGGL(k_finalize_round, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S,
-                         rtol, inner_floor, 1, tag);
+                         rtol, inner_floor, 1, tag, (const double*)nullptr);
       if (tail) (*tail)();
     };
     bool ended = false;
@@ -2471,29 +2479,27 @@ static int project_impl(ProjHandle* h, void* u, void* v, void* w,
                         const void* rho, double dt, const void* target,
                         const void* u_inlet, hipStream_t s) {
   const size_t n = h->n;
-  hipLaunchKernelGGL(k_div_rhs<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
-                     (const T*)u, (const T*)v, (const T*)w, (const T*)u_inlet,
-                     (const T*)target, 1.0 / dt, h->b64, h->part);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, h->part, PJ_RED, h->S, S_BB, F_STORE);
-  FW_CHECK_LAUNCH("div_rhs");
+  // the warm start: float32 (the production path) starts from
+  // 2 p_{n-1} - p_{n-2}: the pressure moves smoothly from step to step, and
+  // that starts the solve about half as far from the solution as p_{n-1}
+  // (one to two PCG iterations of 14 fewer at 256x64x48); the solve still
+  // runs to the same tolerance.  The float64 build keeps the reference's
+  // warm start p_{n-1}.  The pointwise update rides in the divergence pass
+  // (xmode 2: extrapolate, 1: keep p_{n-1} for the next step).
+  int xmode = 0;
   if (!h->have_prev) {
     hipLaunchKernelGGL(k_zero<double>, dim3(nblk(n)), dim3(256), 0, s, n, h->x64);
     h->have_prev = true;
     h->have_prev2 = false;
   } else if (sizeof(T) == 4) {
-    // float32 (the production path): the pressure moves smoothly from step
-    // to step, and 2 p_{n-1} - p_{n-2} starts the solve about half as far
-    // from the solution as p_{n-1} (one to two PCG iterations of 14 fewer
-    // at 256x64x48); the solve still runs to the same tolerance.  The
-    // float64 build keeps the reference's warm start p_{n-1}.
-    if (h->have_prev2) {
-      hipLaunchKernelGGL(k_extrapolate, dim3(nblk(n)), dim3(256), 0, s, n, h->x64, h->x64p);
-    } else {
-      hipLaunchKernelGGL(k_copy64, dim3(nblk(n)), dim3(256), 0, s, n, (const double*)h->x64,
-                         h->x64p);
-      h->have_prev2 = true;
-    }
+    xmode = h->have_prev2 ? 2 : 1;
+    h->have_prev2 = true;
   }
+  // b and its partial norms (summed into S_BB by the first round set-up)
+  hipLaunchKernelGGL(k_div_rhs<T>, dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
+                     (const T*)u, (const T*)v, (const T*)w, (const T*)u_inlet,
+                     (const T*)target, 1.0 / dt, h->b64, h->part_b, h->x64, h->x64p, xmode);
+  FW_CHECK_LAUNCH("div_rhs");
   // the warm start's residual, decided on together with ||b|| on the device
   hipLaunchKernelGGL((k_residual64<T, T>), dim3(PJ_RED), dim3(256), 0, s, h->ga, n,
                      (const T*)rho, h->b64, h->x64, (T*)h->r, h->eps_reg, h->part,
@@ -2706,6 +2712,7 @@ int fw_proj_create(const fw_grid* g, const double* Ux, const double* Uy,
   FW_CHECK_HIP(hipMalloc(&h->w1, nw * h->esz));
   FW_CHECK_HIP(hipMalloc(&h->w2, nw * h->esz));
   FW_CHECK_HIP(hipMalloc((void**)&h->part, PJ_RED * sizeof(double)));
+  FW_CHECK_HIP(hipMalloc((void**)&h->part_b, PJ_RED * sizeof(double)));
   {
     const int ndct = (g->nz * g->ny + 3) / 4;   // row-transform workgroups
     const size_t np = ndct > PJ_MV_MAX ? ndct : PJ_MV_MAX;
@@ -2729,7 +2736,7 @@ int fw_proj_destroy(void* handle) {
   if (!h) return FW_OK;
   void* bufs[] = {h->Ux, h->UxT, h->Yf, h->Yi, h->Zf, h->Zi, h->lx, h->ly, h->lz,
                   h->x64, h->x64p, h->b64, h->r, h->z, h->p, h->p2, h->q, h->d, h->w1,
-                  h->w2, h->part, h->S, h->twx, h->Ppq, h->irho, h->Yf_p, h->Yi_p,
+                  h->w2, h->part, h->part_b, h->S, h->twx, h->Ppq, h->irho, h->Yf_p, h->Yi_p,
                   h->Zf_p, h->Zi_p, h->ZfQ, h->ZiQ, h->Pqz, h->rs, h->qf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
