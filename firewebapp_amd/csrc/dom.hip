@@ -577,7 +577,7 @@ k_dom_sweep_tile(GridArgs g, int D, int tjlo, int M, const double* __restrict__ 
 // layer (c < ncol) also update the ground from the same loads (dom_3d.py's
 // q_in_ground / I_ground_out step): q_in = sum over the downward ordinates of
 // w |mu| I, in order, and I_ground_out relaxed toward emit + (1 - eps) q_in / pi.
-template <typename T>
+template <typename T, int V>
 __global__ void __launch_bounds__(256)
 k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ w,
         T* __restrict__ G0, T* __restrict__ G1, double* __restrict__ pdiff,
@@ -591,45 +591,54 @@ k_dom_G(size_t n, int M, const T* __restrict__ Iset, const double* __restrict__ 
   const T* __restrict__ Gp = odd ? G0 : G1;
   __shared__ double sh1[256], sh2[256];
   double md = 0.0, mg = -1e300;
-  const size_t c = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (c < n) {
-    T s = T(0), qg = T(0);
-    const T* p = Iset + c;
-    const bool ground = c < (size_t)ncol;
-    // the ground sum's term of ordinate m (downward ordinates only)
-    auto down = [&](int m, T iv) {
-      const double mu = Om[3 * m + 2];
-      if (mu < 0.0) qg += T(w[m]) * T(fabs(mu)) * iv;
+  // V consecutive cells per thread, one V-wide vector per ordinate (the host
+  // launches V > 1 only when n and ncol are multiples of V)
+  using LC = LaneCells<T, V>;
+  const size_t c0 = ((size_t)blockIdx.x * 256 + threadIdx.x) * V;
+  if (c0 < n) {
+    T s[V], qg[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) s[v] = qg[v] = T(0);
+    const bool ground = c0 < (size_t)ncol;
+    // one ordinate's terms, in ordinate order per cell (the ground sum over
+    // the downward ordinates only)
+    auto add = [&](int m, const LC& iv) {
+      const T wm = T(w[m]);
+#pragma unroll
+      for (int v = 0; v < V; ++v) s[v] += wm * iv.v[v];
+      if (ground) {
+        const double mu = Om[3 * m + 2];
+        if (mu < 0.0) {
+#pragma unroll
+          for (int v = 0; v < V; ++v) qg[v] += T(w[m]) * T(fabs(mu)) * iv.v[v];
+        }
+      }
     };
+    const LC* p = reinterpret_cast<const LC*>(Iset + c0);
+    const size_t stride = n / V;   // one ordinate's field, in vectors
     int q = 0;
     for (; q + 4 <= M; q += 4) {
-      const T i0 = p[(size_t)q * n], i1 = p[(size_t)(q + 1) * n];
-      const T i2 = p[(size_t)(q + 2) * n], i3 = p[(size_t)(q + 3) * n];
-      s += T(w[q]) * i0;
-      s += T(w[q + 1]) * i1;
-      s += T(w[q + 2]) * i2;
-      s += T(w[q + 3]) * i3;
+      const LC i0 = p[(size_t)q * stride], i1 = p[(size_t)(q + 1) * stride];
+      const LC i2 = p[(size_t)(q + 2) * stride], i3 = p[(size_t)(q + 3) * stride];
+      add(q, i0);
+      add(q + 1, i1);
+      add(q + 2, i2);
+      add(q + 3, i3);
+    }
+    for (; q < M; ++q) add(q, p[(size_t)q * stride]);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+      const size_t c = c0 + v;
+      G[c] = s[v];
       if (ground) {
-        down(q, i0);
-        down(q + 1, i1);
-        down(q + 2, i2);
-        down(q + 3, i3);
+        q_in[c] = qg[v];
+        const T Inew = emit[c] + (T(1) - eps) / T(M_PI) * qg[v];
+        Ig_out[c] = relax * Inew + (T(1) - relax) * Ig_out[c];
       }
+      const double dd = fabs((double)s[v] - (double)Gp[c]);
+      if (dd > md || dd != dd) md = dd;
+      if ((double)s[v] > mg) mg = (double)s[v];
     }
-    for (; q < M; ++q) {
-      const T iv = p[(size_t)q * n];
-      s += T(w[q]) * iv;
-      if (ground) down(q, iv);
-    }
-    G[c] = s;
-    if (ground) {
-      q_in[c] = qg;
-      const T Inew = emit[c] + (T(1) - eps) / T(M_PI) * qg;
-      Ig_out[c] = relax * Inew + (T(1) - relax) * Ig_out[c];
-    }
-    const double dd = fabs((double)s - (double)Gp[c]);
-    if (dd > md || dd != dd) md = dd;
-    if ((double)s > mg) mg = (double)s;
   }
   const int t = threadIdx.x;
   sh1[t] = md;
@@ -1017,6 +1026,7 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
   // it returns at once once the solve has converged.  Its launches depend
   // only on whether it is the first, so off the legacy default stream and
   // with the sweep timer off it is replayed from a captured graph.
+  const bool vec_G = n % 4 == 0 && ncol % 4 == 0;
   auto iteration = [&](int first) -> int {
     if (tile_side<T>(h) > 0) {
       TRY_DOM(sweep_tiled<T>(h, I_amb, s));
@@ -1029,13 +1039,22 @@ static int dom_solve_impl(DomHandle* h, const void* Ts, const void* Tg,
       }
     }
     FW_CHECK_LAUNCH("dom_sweep");
-    hipLaunchKernelGGL(k_dom_G<T>, dim3(nb(n)), dim3(256), 0, s, n, M,
-                       (const T*)h->I, (const double*)h->w, (T*)h->G, (T*)h->Gp,
-                       h->part, h->part2, (const double*)h->S, (const double*)h->Om, ncol,
-                       (const T*)h->Ig_emit, (T*)h->Ig_out, (T*)h->q_in, T(h->eps_ground),
-                       T(h->relax));
+    // four cells per thread in 16-byte (float) vectors when the fields allow
+    if (vec_G) {
+      hipLaunchKernelGGL((k_dom_G<T, 4>), dim3(nb(n / 4)), dim3(256), 0, s, n, M,
+                         (const T*)h->I, (const double*)h->w, (T*)h->G, (T*)h->Gp,
+                         h->part, h->part2, (const double*)h->S, (const double*)h->Om, ncol,
+                         (const T*)h->Ig_emit, (T*)h->Ig_out, (T*)h->q_in, T(h->eps_ground),
+                         T(h->relax));
+    } else {
+      hipLaunchKernelGGL((k_dom_G<T, 1>), dim3(nb(n)), dim3(256), 0, s, n, M,
+                         (const T*)h->I, (const double*)h->w, (T*)h->G, (T*)h->Gp,
+                         h->part, h->part2, (const double*)h->S, (const double*)h->Om, ncol,
+                         (const T*)h->Ig_emit, (T*)h->Ig_out, (T*)h->q_in, T(h->eps_ground),
+                         T(h->relax));
+    }
     hipLaunchKernelGGL(k_dom_fin, dim3(1), dim3(256), 0, s, h->part, h->part2,
-                       (int)nb(n), h->S, first, h->tol,
+                       (int)nb(vec_G ? n / 4 : n), h->S, first, h->tol,
                        flow_active<T>(h) ? h->flow_q : (int*)nullptr, h->plan.cum[h->plan.ndiag],
                        M);
     FW_CHECK_LAUNCH("dom_G");
