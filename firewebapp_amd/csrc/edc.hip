@@ -74,6 +74,22 @@ static int edc_impl(const fw_grid* g, const void* rho, void* Tg, void* Yf,
   return FW_OK;
 }
 
+template <typename T>
+static int edc_summary_impl(const fw_grid* g, const void* Yf, const void* YO2,
+                            const void* Yw, int* work, hipStream_t s) {
+  const size_t n = (size_t)g->nx * g->ny * g->nz;
+  const int nseg = (int)((n + EDC_SEG - 1) / EDC_SEG);
+  hipLaunchKernelGGL(k_edc_summary<T>, dim3(nseg), dim3(256), 0, s, n, (const T*)Yf,
+                     (const T*)YO2, (const T*)Yw, work);
+  FW_CHECK_LAUNCH("edc_summary");
+  return FW_OK;
+}
+
+int edc_summary_launch(const fw_grid* g, const void* Y_fuel, const void* Y_O2,
+                       const void* Y_H2O, int* work, hipStream_t s) {
+  return FW_DISPATCH(g, edc_summary_impl, g, Y_fuel, Y_O2, Y_H2O, work, s);
+}
+
 // The relaxation factor of the wet-bulb clamp for dt (formed on the host,
 // also for the step scalar block).
 double edc_relax(double dt, int n_substeps) {

@@ -83,6 +83,14 @@ int species3_launch(const fw_grid* g, const void* const* Y, void* const* Yo, con
 // into the pass, one 256-cell segment per workgroup (needs Nx % 256 == 0 and
 // the species' segment summaries in edc->summary).
 struct EdcFuseArgs;
+// fw_gas_energy; edc (may be null): the first sub-step's EDC fused in, as
+// for turb_pass_launch.
+int gas_energy_launch(const fw_grid* g, const void* Tg, void* To, const void* u,
+                      const void* v, const void* w, double dt, double alpha_th,
+                      double T_amb, hipStream_t s, const EdcFuseArgs* edc);
+// The EDC segment summaries of the current species (k_edc_summary).
+int edc_summary_launch(const fw_grid* g, const void* Y_fuel, const void* Y_O2,
+                       const void* Y_H2O, int* work, hipStream_t s);
 int turb_pass_launch(const fw_grid* g, const void* f, void* fo, const void* nu_t, double sc_t,
                      double dt_sub, double lo, double hi, hipStream_t s, const double* sb,
                      const EdcFuseArgs* edc = nullptr);

@@ -206,6 +206,7 @@ struct SubstepPlan {
   double dt_sub, dsc, dpr;   // sub-step and pass lengths
   double drip_gas, drip_solid;
   bool fuse_summary;
+  bool first_edc_done;       // the first sub-step's EDC ran in the gas-energy pass
 };
 
 // The N_SUB chemistry / transport / coupling sub-steps (spread_3d.py:2678-3200)
@@ -236,7 +237,7 @@ static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_param
   const EdcFuseArgs edc_args{f->rho, f->k, f->eps, nullptr, nullptr, nullptr, f->omega,
                              (const int*)f->edc_work,
                              edc_params(0.34, 1100.0, dt_sub, 1, 0, 1.3, 17.0e6)};
-  bool edc_done = false;
+  bool edc_done = P.first_edc_done;
   for (int sub = 0; sub < N_SUB; ++sub) {
     if (launch && !edc_done)
       TRY(edc_launch(g, f->rho, f->T_g, f->Y_fuel, f->Y_O2, f->k, f->eps, f->Y_H2O, f->omega,
@@ -360,7 +361,8 @@ static int substeps(const fw_grid* g, fw_spread_fields* f, const fw_spread_param
 static std::vector<uint64_t> substep_key(const fw_grid* g, const fw_spread_fields* f,
                                          const SubstepPlan& P) {
   return {kw(g), (uint64_t)g->dtype, (uint64_t)P.nsc, (uint64_t)P.npr,
-          (uint64_t)P.fuse_summary, (uint64_t)fuse_edc_on(), kw(P.drip_gas), kw(P.drip_solid),
+          (uint64_t)P.fuse_summary, (uint64_t)fuse_edc_on(), (uint64_t)P.first_edc_done,
+          kw(P.drip_gas), kw(P.drip_solid),
           kw(f->rho), kw(f->T_g), kw(f->T_s), kw(f->Y_fuel), kw(f->Y_O2), kw(f->Y_H2O),
           kw(f->k), kw(f->eps), kw(f->omega), kw(f->nu_t), kw(f->u), kw(f->v), kw(f->w),
           kw(f->src[0]), kw(f->src[1]), kw(f->src[7]), kw(f->q_rad_gas), kw(f->Q_comb),
@@ -563,8 +565,22 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
   // ignition pulse on the cached q_rad (:2528-2532)
   if (ign_on)
     TRY(fw_add_patch(g, f->q_rad, nzb - 1, s0, s1, Q_PULSE * p->ignition_q_mult, stream));
-  // gas-energy advection (:2613-2622)
-  TRY(fw_gas_energy(g, f->T_g, f->tmp1, f->u, f->v, f->w, dt, 2.0e-5, p->T_amb, stream));
+  // gas-energy advection (:2613-2622); with whole 256-cell rows (up to 2^21
+  // cells, as for the later sub-steps' fused EDC) the first sub-step's EDC
+  // rides in the same pass: it reads the cell's new temperature and the
+  // species, which nothing between the two changes
+  const bool first_edc =
+      fuse_edc_on() && g->nx % EDC_SEG == 0 && n <= ((size_t)1 << 21) && f->edc_work;
+  if (first_edc) {
+    TRY(edc_summary_launch(g, f->Y_fuel, f->Y_O2, f->Y_H2O, (int*)f->edc_work, s));
+    const EdcFuseArgs ea{f->rho, f->k, f->eps, f->Y_H2O, f->Y_fuel, f->Y_O2, f->omega,
+                         (const int*)f->edc_work,
+                         edc_params(0.34, 1100.0, dt / N_SUB, 1, 0, 1.3, 17.0e6)};
+    TRY(gas_energy_launch(g, f->T_g, f->tmp1, f->u, f->v, f->w, dt, 2.0e-5, p->T_amb, s,
+                          &ea));
+  } else {
+    TRY(fw_gas_energy(g, f->T_g, f->tmp1, f->u, f->v, f->w, dt, 2.0e-5, p->T_amb, stream));
+  }
   SWAP(f->T_g, f->tmp1);
   // N_SUB chemistry / transport / coupling sub-steps (:2678-3200)
   double nu_max = 0.0;
@@ -577,6 +593,7 @@ int fw_spread_step(const fw_grid* g, const fw_grid* g64, fw_spread_fields* f,
   plan.drip_gas = (combustion && ign_on) ? (1.0 - F_DRIP_TO_SOLID) * Q_drip : 0.0;
   plan.drip_solid = (combustion && ign_on && p->eulerian) ? F_DRIP_TO_SOLID * Q_drip : 0.0;
   plan.fuse_summary = g->nx % EDC_SEG == 0 && plan.nsc > 0;
+  plan.first_edc_done = first_edc;
   if (f->graphs && f->step_scalars && s && !p->eulerian) {
     // one graph launch for the whole loop: the step's scalars go to the
     // device block first, the pointers advance as the launches would

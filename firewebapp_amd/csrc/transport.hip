@@ -215,16 +215,15 @@ k_species3(GridArgs g, const T* __restrict__ Y1, const T* __restrict__ Y2,
 // gas energy: MUSCL advection + alpha_th FV diffusion + clamps
 // (spread_3d.py:3827-3897)
 // ---------------------------------------------------------------------------
+// The new temperature of cell (i, j, k) = c.
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_gas_energy(GridArgs g, const T* __restrict__ Tg, T* __restrict__ To,
-             const T* __restrict__ u, const T* __restrict__ v,
-             const T* __restrict__ w, T dt, T alpha_th, T T_amb) {
-  int i, j, k;
-  cell_ijk(i, j, k);
-  if (i >= g.nx) return;
+__device__ __forceinline__ T gas_energy_cell(const GridArgs& g, int i, int j, int k, size_t c,
+                                             const T* __restrict__ Tg,
+                                             const T* __restrict__ u,
+                                             const T* __restrict__ v,
+                                             const T* __restrict__ w, T dt, T alpha_th,
+                                             T T_amb) {
   const int Nx = g.nx, Ny = g.ny, Nz = g.nz;
-  const size_t c = cidx(g, k, j, i);
   const size_t sy = Nx, sz = (size_t)Nx * Ny;
   const T dx = T(g.dx), dy = T(g.dy);
   const T da = T(g.d_above[k]), db = T(g.d_below[k]);
@@ -279,7 +278,40 @@ k_gas_energy(GridArgs g, const T* __restrict__ Tg, T* __restrict__ To,
   T nv = pc + dT * dt;
   nv = nv < T_amb ? T_amb : nv;  // np.maximum then np.minimum
   nv = nv > T(10000) ? T(10000) : nv;
-  To[c] = nv;
+  return nv;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gas_energy(GridArgs g, const T* __restrict__ Tg, T* __restrict__ To,
+             const T* __restrict__ u, const T* __restrict__ v,
+             const T* __restrict__ w, T dt, T alpha_th, T T_amb) {
+  int i, j, k;
+  cell_ijk(i, j, k);
+  if (i >= g.nx) return;
+  const size_t c = cidx(g, k, j, i);
+  To[c] = gas_energy_cell<T>(g, i, j, k, c, Tg, u, v, w, dt, alpha_th, T_amb);
+}
+
+// The same update with the first sub-step's EDC of the cell fused in, as
+// k_turb_diff_edc does for the later sub-steps (whole 256-cell segments per
+// workgroup, checked by the host; the species' segment summaries from
+// k_edc_summary).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gas_energy_edc(GridArgs g, const T* __restrict__ Tg, T* __restrict__ To,
+                 const T* __restrict__ u, const T* __restrict__ v,
+                 const T* __restrict__ w, T dt, T alpha_th, T T_amb, size_t n,
+                 const T* __restrict__ rho, T* __restrict__ Yf, T* __restrict__ YO2,
+                 const T* __restrict__ kt, const T* __restrict__ et,
+                 const T* __restrict__ Yw, T* __restrict__ omega,
+                 const int* __restrict__ summary, EdcParams P) {
+  int i, j, k;
+  cell_ijk(i, j, k);
+  const size_t c = cidx(g, k, j, i);
+  const EdcCellIn<T> in = edc_load<T>(true, c, rho, Yf, YO2, kt, et, Yw);
+  const T tn = gas_energy_cell<T>(g, i, j, k, c, Tg, u, v, w, dt, alpha_th, T_amb);
+  edc_segment<T>(n, (int)(c / EDC_SEG), tn, in, To, Yf, YO2, omega, summary, P);
 }
 
 // ---------------------------------------------------------------------------
@@ -786,13 +818,30 @@ template <typename T>
 static int gas_energy_impl(const fw_grid* g, const void* Tg, void* To,
                            const void* u, const void* v, const void* w,
                            double dt, double alpha_th, double T_amb,
-                           hipStream_t s) {
+                           hipStream_t s, const EdcFuseArgs* e) {
   Launch L = cell_launch(g);
-  hipLaunchKernelGGL(k_gas_energy<T>, L.grid, L.block, 0, s, grid_args(g),
-                     (const T*)Tg, (T*)To, (const T*)u, (const T*)v,
-                     (const T*)w, T(dt), T(alpha_th), T(T_amb));
+  if (e) {
+    FW_REQUIRE(g->nx % EDC_SEG == 0 && L.block.x == EDC_SEG && e->summary,
+               "gas_energy: the fused EDC needs whole 256-cell rows and summaries");
+    const size_t n = (size_t)g->nx * g->ny * g->nz;
+    hipLaunchKernelGGL(k_gas_energy_edc<T>, L.grid, L.block, 0, s, grid_args(g),
+                       (const T*)Tg, (T*)To, (const T*)u, (const T*)v, (const T*)w, T(dt),
+                       T(alpha_th), T(T_amb), n, (const T*)e->rho, (T*)e->Y_fuel,
+                       (T*)e->Y_O2, (const T*)e->k_turb, (const T*)e->eps_turb,
+                       (const T*)e->Y_H2O, (T*)e->omega_out, e->summary, e->P);
+  } else {
+    hipLaunchKernelGGL(k_gas_energy<T>, L.grid, L.block, 0, s, grid_args(g),
+                       (const T*)Tg, (T*)To, (const T*)u, (const T*)v,
+                       (const T*)w, T(dt), T(alpha_th), T(T_amb));
+  }
   FW_CHECK_LAUNCH("gas_energy");
   return FW_OK;
+}
+
+int gas_energy_launch(const fw_grid* g, const void* Tg, void* To, const void* u,
+                      const void* v, const void* w, double dt, double alpha_th,
+                      double T_amb, hipStream_t s, const EdcFuseArgs* edc) {
+  return FW_DISPATCH(g, gas_energy_impl, g, Tg, To, u, v, w, dt, alpha_th, T_amb, s, edc);
 }
 
 template <typename T>
@@ -1024,7 +1073,7 @@ int fw_gas_energy(const fw_grid* g, const void* T_g, void* T_out,
   FW_REQUIRE(g && T_g != T_out, "gas_energy: T_out must not alias T_g");
   FW_REQUIRE(g->nx >= 2 && g->nz >= 2, "gas_energy: needs Nx, Nz >= 2");
   return FW_DISPATCH(g, gas_energy_impl, g, T_g, T_out, u, v, w, dt, alpha_th,
-                     T_amb, (hipStream_t)stream);
+                     T_amb, (hipStream_t)stream, (const EdcFuseArgs*)nullptr);
 }
 
 int fw_turbulent_diffusion_pass(const fw_grid* g, const void* f, void* f_out,
