@@ -133,10 +133,10 @@ SPECIES3_FIELDS = 12
 # profiles/step_512x128x64_s8_kernel_roofline.md; the DOM sweep's depend on
 # its kernel and tile side (the last key)
 ROOFLINE_TRAFFIC = {
-    ("k_species3", "spread3d_256x64x48_s8", "float32", 0): 43978577,
-    ("k_species3", "spread3d_512x128x64_s8", "float32", 0): 354394735,
-    ("k_dom_sweep_flow", "spread3d_256x64x48_s8", "float32", 4): 485769553,
-    ("k_dom_sweep_flow", "spread3d_512x128x64_s8", "float32", 4): 3673308680,
+    ("k_species3", "spread3d_256x64x48_s8", "float32", 0): 44010989,
+    ("k_species3", "spread3d_512x128x64_s8", "float32", 0): 355120644,
+    ("k_dom_sweep_flow", "spread3d_256x64x48_s8", "float32", 4): 485781011,
+    ("k_dom_sweep_flow", "spread3d_512x128x64_s8", "float32", 4): 3676208589,
     ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 4): 18176445,
     ("k_dom_sweep_tile", "spread3d_256x64x48_s8", "float32", 8): 36213937}
 
