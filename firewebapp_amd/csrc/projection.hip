@@ -2403,7 +2403,7 @@ static int solve_refined(ProjHandle* h, const TR* rho, double rtol, bool have_re
     h->round = outer;
     // the learned mode gets an iteration budget per inner solve; a round
     // that does not converge within it hands the rest of this solve to plain
-    // PCG (the separable preconditioner stays the fallback and the judge)
+    // PCG (the separable preconditioner stays the fallback)
     const bool learned = h->precond_enabled == FW_PRECOND_LEARNED;
     int budget = h->maxiter - total;
     if (learned && budget > h->learned_budget) budget = h->learned_budget;

@@ -1,7 +1,7 @@
 """GPU parity of the pressure projection through the C ABI.
 
 Standards (docs/kerneltest.mjs:577-627 of the reference): operator and
-divergence tight (1e-13 relative here, float64); the projection judged by
+divergence tight (1e-13 relative here, float64); the projection checked by
 max|div(u_new) - target| within 10x the reference's own residual, not
 elementwise (two Krylov paths legitimately stop at different points inside
 the rtol ball).  Beyond the vectors (Ny = 1 only), seeded Ny > 1 cases check

@@ -1,5 +1,5 @@
 """Host setup: grid/mesh against the golden mesh vectors (bit-exact), deck
-parsing rules, and the BASELINE mesh recipes (Nz of the 128x32x32 and
+parsing rules, and the benchmark mesh recipes (Nz of the 128x32x32 and
 256x64x48 configurations)."""
 import numpy as np
 import pytest
