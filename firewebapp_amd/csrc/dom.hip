@@ -81,6 +81,7 @@ struct DomHandle {
   int *flow_q, *flow_flags;
   int flow;
   int flow_wg;      // dataflow sweep workgroups per XCD
+  int flow_spin;    // polls before a waiting workgroup gives up
   FlowPlan plan;
 };
 
@@ -759,15 +760,16 @@ __device__ __forceinline__ int xcc_id() {
 // needed.  A block waits only on blocks earlier in its queue, which
 // workgroups already running have taken, so the sweep cannot deadlock
 // however many workgroups are resident.  Every wait is bounded: after
-// FLOW_SPIN polls (or once another workgroup of the XCD has given up) a
-// workgroup raises the XCD's give-up word and leaves; k_dom_fin turns that,
-// or a queue not handed out in full, into DS_STALL.
+// spin_limit polls (at most FLOW_SPIN), or once another workgroup of the XCD
+// has given up, a workgroup raises the XCD's give-up word and leaves;
+// k_dom_fin turns that, or a queue not handed out in full, into DS_STALL.
 template <typename T, int PER, int TS>
 __global__ void __launch_bounds__(64 * TS)
 k_dom_sweep_flow(GridArgs g, int M, const double* __restrict__ Om, T* __restrict__ Iset,
                  const T* __restrict__ kap, const T* __restrict__ B,
                  const T* __restrict__ Ig_out, T I_amb, const double* __restrict__ S,
-                 int* __restrict__ flow_q, int* __restrict__ flags, FlowPlan plan) {
+                 int* __restrict__ flow_q, int* __restrict__ flags, FlowPlan plan,
+                 int spin_limit) {
   if (S[DS_DONE] != 0.0) return;   // uniform: the solve has converged
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sh = reinterpret_cast<T*>(smem_raw);
@@ -810,7 +812,7 @@ k_dom_sweep_flow(GridArgs g, int M, const double* __restrict__ Om, T* __restrict
             rz = __hip_atomic_load(&fl[(tk - 1) * TJ + tj], __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT) >= epoch;
           if (ry && rz) break;
-          if (spin >= FLOW_SPIN ||
+          if (spin >= spin_limit ||
               ((spin & 255) == 255 && __hip_atomic_load(&flow_q[flow_gave_up(x)],
                                                         __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT) != 0)) {
@@ -949,12 +951,12 @@ static int sweep_flow(DomHandle* h, T I_amb, hipStream_t s) {
       hipExtLaunchKernelGGL(kern, grid, dim3(256), shm, s, ev.start, ev.stop, 0, h->ga, h->M,
                             (const double*)h->Om, (T*)h->I, (const T*)h->kap_safe,
                             (const T*)h->B, (const T*)h->Ig_out, I_amb, (const double*)h->S,
-                            h->flow_q, h->flow_flags, h->plan);
+                            h->flow_q, h->flow_flags, h->plan, h->flow_spin);
     else
       hipLaunchKernelGGL(kern, grid, dim3(256), shm, s, h->ga, h->M, (const double*)h->Om,
                          (T*)h->I, (const T*)h->kap_safe, (const T*)h->B,
                          (const T*)h->Ig_out, I_amb, (const double*)h->S, h->flow_q,
-                         h->flow_flags, h->plan);
+                         h->flow_flags, h->plan, h->flow_spin);
   };
   switch (Nx) {
     case 64: go(k_dom_sweep_flow<T, 1, 4>); break;
@@ -1226,6 +1228,10 @@ int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* we
     h->flow_wg = wg ? atoi(wg) : 128;
     if (h->flow_wg < 8) h->flow_wg = 8;
     if (h->flow_wg > 512) h->flow_wg = 512;
+    // a smaller poll limit makes the give-up path testable from the host
+    const char* sp = getenv("FIREWAVE_DOM_FLOW_SPIN");
+    h->flow_spin = sp ? atoi(sp) : FLOW_SPIN;
+    if (h->flow_spin < 1 || h->flow_spin > FLOW_SPIN) h->flow_spin = FLOW_SPIN;
   }
   *handle = h;
   return FW_OK;

@@ -156,3 +156,55 @@ def test_dom_dataflow_sweep_matches_launches(shape, dtype):
         assert a[0] == b[0]
         for x, y in zip(a[1:], b[1:]):
             assert torch.equal(x, y)
+
+
+def test_dom_dataflow_give_up_path(monkeypatch):
+    """The dataflow sweep's bounded wait, forced from the host: with a poll
+    limit of 1 (FIREWAVE_DOM_FLOW_SPIN) a block whose inflows are not yet
+    stored gives up instead of waiting, every kernel still ends, and the
+    solve returns an error naming the stall; the handle then runs the
+    per-diagonal launches, and its next solve agrees with a fresh solver's to
+    the source-iteration tolerance.  (Should no block ever have to wait, the
+    solve must instead equal the fresh solver's bit for bit.)"""
+    import ctypes as C
+    import torch
+    from firewebapp_amd import _lib
+    rng = np.random.default_rng(3)
+    shape = (48, 64, 256)
+    nz, ny, nx = shape
+    dz = 0.03 + 0.07 * rng.random(nz)
+    alpha = np.zeros(shape)
+    alpha[:2] = 0.002 + 0.01 * rng.random((2, ny, nx))
+    T_s = 300.0 + 800.0 * rng.random(shape) * (alpha > 0)
+    T_g = 300.0 + 1200.0 * rng.random(shape)
+    om = 5e-3 * rng.random(shape)
+    Tsoil = 300.0 + 50.0 * rng.random((ny, nx))
+    cv = lambda a: torch.as_tensor(a, dtype=torch.float32, device="cuda")
+
+    def solve(s):
+        qs, qg, qsoil = cv(np.zeros(shape)), cv(np.zeros(shape)), cv(np.zeros((ny, nx)))
+        s.solve(cv(T_s), cv(T_g), cv(alpha), cv(om), 2000.0, 300.0, qs, qg, cv(Tsoil), qsoil)
+        torch.cuda.synchronize()
+        return qs.cpu(), qg.cpu(), qsoil.cpu()
+
+    ref = solve(_solver(*shape, 0.12, 0.1, dz, dz, dz, "periodic", 8))
+    monkeypatch.setenv("FIREWAVE_DOM_FLOW_SPIN", "1")
+    s = _solver(*shape, 0.12, 0.1, dz, dz, dz, "periodic", 8)
+    h = s._handle(torch.float32)
+    nl = C.c_int()
+    _lib.call("fw_dom_sweep_launches", h, C.byref(nl))
+    assert nl.value == 1
+    try:
+        out = solve(s)
+    except Exception as e:   # the stall, reported as an error of this solve
+        assert "stalled" in str(e)
+        print("dataflow sweep gave up as forced; per-diagonal launches after it", flush=True)
+        _lib.call("fw_dom_sweep_launches", h, C.byref(nl))
+        assert nl.value == 16 + 12 - 1      # per-diagonal launches from now on
+        out = solve(s)
+        for a, b in zip(out, ref):
+            assert float((a - b).abs().max() / b.abs().max()) < 1e-2
+    else:
+        print("no block had to wait", flush=True)
+        for a, b in zip(out, ref):
+            assert torch.equal(a, b)
