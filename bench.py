@@ -17,7 +17,7 @@ then exactly K timed steps ending at the configuration's last step
 (--spinup overrides).  ``value`` = cells x K / wall seconds over those
 steps, inputs resident in HBM.  The JSON line also carries:
 * window: the step numbers and simulated times of the timed window;
-* full_run: the rate over every step of the run (spin-up included);
+* full_run: the rate over every step of the run (spin-up included, the autotuner's time not);
 * early_window: the rate over steps W+1 .. W+K of a fresh run (the
   ignition transient, before the fire develops);
 * roofline: k_dom_sweep_tile, the kernel with the largest share of the
@@ -444,15 +444,32 @@ def main(argv=None):
     torch.cuda.synchronize()
     t_run0 = time.perf_counter()
     tuned = None
-    for i in range(spinup + a.warmup):
-        if sv.step() is not None:
-            raise RuntimeError(f"run stopped early at step {sv.step_no}: {sv.stop_reason}")
-        if (i + 1) % 100 == 0:
-            print(f"# step {i + 1}, t = {sv.t:.4f} s", file=sys.stderr, flush=True)
-        if i + 1 == spinup and a.tune:
-            tuned = tune_shapes(sv)
-    if spinup == 0 and a.tune:
+    tune_s = 0.0   # the autotuner's wall time, kept out of full_run
+
+    def tune():
+        nonlocal tuned, tune_s
+        torch.cuda.synchronize()
+        tt = time.perf_counter()
         tuned = tune_shapes(sv)
+        torch.cuda.synchronize()
+        tune_s += time.perf_counter() - tt
+
+    # the spin-up and warm-up in native calls of up to 100 steps (progress
+    # lines between them), the autotuner after the spin-up
+    done, total = 0, spinup + a.warmup
+    while done < total:
+        chunk = min(100 - done % 100, total - done)
+        if a.tune and done < spinup < done + chunk:
+            chunk = spinup - done
+        if sv.advance(chunk) is not None:
+            raise RuntimeError(f"run stopped early at step {sv.step_no}: {sv.stop_reason}")
+        done += chunk
+        if done % 100 == 0:
+            print(f"# step {done}, t = {sv.t:.4f} s", file=sys.stderr, flush=True)
+        if done == spinup and a.tune:
+            tune()
+    if spinup == 0 and a.tune:
+        tune()
     t_win0 = sv.t
     print(f"# spin-up and warm-up done: {spinup + a.warmup} steps, t = {sv.t:.4f} s",
           file=sys.stderr, flush=True)
@@ -524,8 +541,8 @@ def main(argv=None):
                    "timed_steps": [spinup + a.warmup + 1, total_steps],
                    "config_steps": n_config, "t_start_s": t_win0, "t_end_s": sv.t},
         "full_run": {"steps": total_steps,
-                     "value": cells * total_steps / (t1 - t_run0),
-                     "ms_per_step": 1e3 * (t1 - t_run0) / total_steps},
+                     "value": cells * total_steps / (t1 - t_run0 - tune_s),
+                     "ms_per_step": 1e3 * (t1 - t_run0 - tune_s) / total_steps},
         "roofline": _roofline("k_dom_sweep_flow" if sweep_launches_per_iter == 1
                               else "k_dom_sweep_tile", dom_bytes, dom_ms / max(dom_n, 1),
                               dom_n, a.workload, a.dtype, shape_key=TS),
