@@ -1224,6 +1224,12 @@ int fw_dom_create(const fw_grid* g, int M, const double* Omega, const double* we
     FW_CHECK_HIP(hipMemset(h->flow_flags, 0, nflags * sizeof(int)));
     const char* e = getenv("FIREWAVE_DOM_FLOW");
     h->flow = (e && e[0] == '0') ? -1 : 1;
+    // the queues assume the MI355X's 8 XCDs (a partitioned device shows fewer)
+    int dev = 0, nxcc = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&nxcc, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess ||
+        nxcc != FLOW_XCD)
+      h->flow = 0;   // not available on this device (fw_dom_set_flow cannot turn it on)
     const char* wg = getenv("FIREWAVE_DOM_FLOW_WG");
     h->flow_wg = wg ? atoi(wg) : 128;
     if (h->flow_wg < 8) h->flow_wg = 8;
